@@ -1,0 +1,1250 @@
+// ORACLE — test infrastructure only.
+//
+// A CPU restatement of HPCrack's matrix-free multigrid/PCG hot path (degraded
+// elasticity + AT2 phase field on a voxel hexahedral grid).  It exists to check
+// the HIP implementation in hpcrack_amd/: only tests/, __graft_entry__.smoke()
+// and bench.py's cpu_baseline leg may load it, and only as the checker or as
+// the timed CPU baseline.  Nothing in the product links or calls it.
+//
+// Every routine cites the reference file:line it restates (paths relative to
+// the HPCrack source tree).  Semantics are kept, including the quirks that
+// decide iteration counts: the zero-node rule of gauss_point_MP, squared norms,
+// restarted fixed-count Jacobi-PCG smoothers, the nu2 x nu2 phase-field post
+// smoothing, the frozen f_norm and the unit-load solve.
+//
+// Deliberate, documented differences from the reference:
+//  * the 8x redundant Gauss-point coefficient evaluation
+//    (mechanical_pcg.cpp:132-134) is done once per element: same values;
+//  * element loops run in two z-colours (even element layers, then odd ones)
+//    so the result does not depend on the thread count; the reference's
+//    OpenMP "critical" scatter order is itself unspecified;
+//  * dot products sum per z-plane, then planes in order (deterministic);
+//  * the split flag is computed on the finest level only: the reference also
+//    computes it on coarse levels (material_selection.cpp:138-147) but reads
+//    it only on the finest one (mechanical_pcg.cpp:39-59);
+//  * MP_choice loops use the correct x/y extents (the reference swaps them,
+//    material_selection.cpp:143-146, identical when nx == ny).
+//
+// Parity pins for each function are listed in tests/test_oracle_*.py.
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cstdint>
+#include <vector>
+#include <algorithm>
+#include <stdexcept>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// Shape functions (shape_function.cpp:16-75)
+// ----------------------------------------------------------------------------
+struct Shape {
+  double N[8][8];      // N[g][a]
+  double DN[3][8][8];  // DN[c][g][a], already scaled by 2/h
+};
+
+static const int Cx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+static const int Cy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+static const int Cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+static void dshape_function(Shape& S, double hx, double hy, double hz) {
+  double x[8] = {-1., 1., 1., -1., -1., 1., 1., -1.};
+  double y[8] = {-1., -1., 1., 1., -1., -1., 1., 1.};
+  double z[8] = {-1., -1., -1., -1., 1., 1., 1., 1.};
+  double w[8];
+  double x1 = 0, x2 = 0, y1 = 0, y2 = 0, z1 = 0, z2 = 0, detJ = 0;
+  for (int i = 0; i < 8; i++) {
+    x[i] = x[i] / std::sqrt(3.0);
+    y[i] = y[i] / std::sqrt(3.0);
+    z[i] = z[i] / std::sqrt(3.0);
+    S.N[i][0] = 0.125 * (1 - x[i]) * (1 - y[i]) * (1 - z[i]);
+    S.N[i][1] = 0.125 * (1 + x[i]) * (1 - y[i]) * (1 - z[i]);
+    S.N[i][2] = 0.125 * (1 + x[i]) * (1 + y[i]) * (1 - z[i]);
+    S.N[i][3] = 0.125 * (1 - x[i]) * (1 + y[i]) * (1 - z[i]);
+    S.N[i][4] = 0.125 * (1 - x[i]) * (1 - y[i]) * (1 + z[i]);
+    S.N[i][5] = 0.125 * (1 + x[i]) * (1 - y[i]) * (1 + z[i]);
+    S.N[i][6] = 0.125 * (1 + x[i]) * (1 + y[i]) * (1 + z[i]);
+    S.N[i][7] = 0.125 * (1 - x[i]) * (1 + y[i]) * (1 + z[i]);
+    for (int c = 0; c < 3; c++) {
+      if (c == 0) { x1 = 2; x2 = 0; y1 = y[i]; y2 = y[i]; z1 = z[i]; z2 = z[i]; detJ = 2. / hx; }
+      if (c == 1) { x1 = x[i]; x2 = x[i]; y1 = 2; y2 = 0; z1 = z[i]; z2 = z[i]; detJ = 2. / hy; }
+      if (c == 2) { x1 = x[i]; x2 = x[i]; y1 = y[i]; y2 = y[i]; z1 = 2; z2 = 0; detJ = 2. / hz; }
+      w[0] = 0.125 * (1 - x1) * (1 - y1) * (1 - z1);
+      w[1] = 0.125 * (1 + x2) * (1 - y1) * (1 - z1);
+      w[2] = 0.125 * (1 + x2) * (1 + y2) * (1 - z1);
+      w[3] = 0.125 * (1 - x1) * (1 + y2) * (1 - z1);
+      w[4] = 0.125 * (1 - x1) * (1 - y1) * (1 + z2);
+      w[5] = 0.125 * (1 + x2) * (1 - y1) * (1 + z2);
+      w[6] = 0.125 * (1 + x2) * (1 + y2) * (1 + z2);
+      w[7] = 0.125 * (1 - x1) * (1 + y2) * (1 + z2);
+      for (int j = 0; j < 8; j++) S.DN[c][i][j] = w[j] * detJ;
+    }
+  }
+}
+
+struct Dims {
+  int nx, ny, nz;
+  size_t at(int i, int j, int k) const { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k); }
+  size_t n() const { return (size_t)nx * ny * nz; }
+};
+
+// gauss_point_MP (material_gauss_point.cpp:18-41): trilinear interpolation,
+// but 0 if any of the 8 nodal values is exactly 0.0.
+static inline double gauss_point_MP(const double* MP, const Dims& D, const Shape& S,
+                                    int i, int j, int k, int g) {
+  int a = 1;
+  for (int n = 0; n < 8; n++)
+    if (MP[D.at(Cx[n] + i, Cy[n] + j, Cz[n] + k)] == 0.0) a = 0;
+  double coefg = 0.0;
+  if (a == 1) {
+    double tmp = 0.0;
+    for (int n = 0; n < 8; n++) tmp += MP[D.at(Cx[n] + i, Cy[n] + j, Cz[n] + k)] * S.N[g][n];
+    coefg = tmp;
+  }
+  return coefg;
+}
+
+// K_G_d_g (material_gauss_point.cpp:44-59)
+static inline double K_G_d_g(const double* d, const double* MP, const Dims& D, const Shape& S,
+                             int i, int j, int k, int g, double small_k) {
+  double MP_g = gauss_point_MP(MP, D, S, i, j, k, g);
+  double d_g = gauss_point_MP(d, D, S, i, j, k, g);
+  if (d_g < 0.) d_g = 0.;
+  else if (d_g > 1.) d_g = 1.;
+  double g_d = (1. - small_k) * (1 - d_g) * (1 - d_g) + small_k;
+  return g_d * MP_g;
+}
+
+// epsilon_routine (mechanical_strain.cpp:15-35)
+static inline void epsilon_routine(double eps[3][3], const double* const u[3], const Dims& D,
+                                   const Shape& S, int i, int j, int k, int g) {
+  double uv[3][8];
+  for (int c = 0; c < 3; c++)
+    for (int tf = 0; tf < 8; tf++) uv[c][tf] = u[c][D.at(Cx[tf] + i, Cy[tf] + j, Cz[tf] + k)];
+  for (int c = 0; c < 3; c++)
+    for (int m = 0; m < 3; m++) {
+      double e = 0.0;
+      for (int tf = 0; tf < 8; tf++) {
+        double dc = uv[c][tf], dm = uv[m][tf];
+        e += 0.5 * (S.DN[c][g][tf] * dm + S.DN[m][g][tf] * dc);
+      }
+      eps[c][m] = e;
+    }
+}
+static inline double trace_epsilon(const double e[3][3]) { return e[0][0] + e[1][1] + e[2][2]; }
+// epsilon_dev (mechanical_strain.cpp:47-61)
+static inline double epsilon_dev(const double e[3][3]) {
+  double ddp = 0.0, third = (1.0 / 3.0) * trace_epsilon(e);
+  for (int c = 0; c < 3; c++)
+    for (int m = 0; m < 3; m++) {
+      if (c == m) ddp += (e[c][m] - third) * (e[c][m] - third);
+      else ddp += e[c][m] * e[c][m];
+    }
+  return ddp;
+}
+// Sigma_g (mechanical_stress.cpp:14-27)
+static inline void sigma_g(double s[3][3], const double e[3][3], double K, double G, double tr) {
+  for (int c = 0; c < 3; c++)
+    for (int m = 0; m < 3; m++) {
+      if (c == m) s[c][m] = K * tr + 2.0 * G * (e[c][m] - 1.0 / 3.0 * tr);
+      else s[c][m] = 2.0 * G * e[c][m];
+    }
+}
+// Phi_positive / Phi_negative (strain_history.cpp:134-159)
+static inline double phi_positive(double K, double G, const double e[3][3]) {
+  double tr = trace_epsilon(e), ddp = epsilon_dev(e);
+  if (tr >= 0) return 0.5 * K * tr * tr + G * ddp;
+  return G * ddp;
+}
+static inline double phi_negative(double K, const double e[3][3]) {
+  double tr = trace_epsilon(e);
+  if (tr >= 0) return 0.0;
+  return 0.5 * K * tr * tr;
+}
+// W (strain_history.cpp:73-83)
+static inline double W_energy(const double s[3][3], const double e[3][3]) {
+  double w = 0.;
+  for (int c = 0; c < 3; c++)
+    for (int m = 0; m < 3; m++) w += 0.5 * s[c][m] * e[c][m];
+  return w;
+}
+
+static int n_threads_override = 0;
+static inline int nthreads() {
+#ifdef _OPENMP
+  return n_threads_override > 0 ? n_threads_override : omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// Deterministic reductions: one partial per z-plane, planes summed in order.
+static double plane_sum(const Dims& D, const double* a, const double* b) {
+  std::vector<double> part(D.nz, 0.0);
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+  for (int k = 0; k < D.nz; k++) {
+    double s = 0.0;
+    size_t base = (size_t)k * D.nx * D.ny;
+    for (size_t q = 0; q < (size_t)D.nx * D.ny; q++) s += a[base + q] * b[base + q];
+    part[k] = s;
+  }
+  double ss = 0.0;
+  for (int k = 0; k < D.nz; k++) ss += part[k];
+  return ss;
+}
+
+// Element loop in two z-colours: layers of one colour touch disjoint node
+// planes, so scatter-adds within a colour never collide.
+template <class F>
+static void element_loop_coloured(const Dims& D, F&& body) {
+  for (int colour = 0; colour < 2; colour++) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads())
+    for (int k = colour; k <= D.nz - 2; k += 2)
+      for (int j = 0; j <= D.ny - 2; j++)
+        for (int i = 0; i <= D.nx - 2; i++) body(i, j, k);
+  }
+}
+template <class F>
+static void element_loop(const Dims& D, F&& body) {
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+  for (int k = 0; k <= D.nz - 2; k++)
+    for (int j = 0; j <= D.ny - 2; j++)
+      for (int i = 0; i <= D.nx - 2; i++) body(i, j, k);
+}
+
+// ----------------------------------------------------------------------------
+// Context: one problem, all levels (stack_and_level.h, initialization.cpp)
+// ----------------------------------------------------------------------------
+enum Field {
+  F_U = 0, F_FU = 1, F_RU = 2, F_UOLD = 3, F_P = 4, F_W = 5,
+  F_D = 6, F_FD = 7, F_RD = 8, F_DOLD = 9, F_H = 10,
+  F_K = 11, F_G = 12, F_GC = 13, F_KE = 14, F_GE = 15, F_FLAG = 16,
+  F_DPREV = 17, F_PD = 18, F_WD = 19, F_COUNT = 20
+};
+static int field_ncomp(int f) {
+  switch (f) {
+    case F_U: case F_FU: case F_RU: case F_UOLD: case F_P: case F_W: return 3;
+    default: return 1;
+  }
+}
+
+struct MGp { int nu0, nu1, nu2, gamma, ncy; };
+
+struct Ctx {
+  int maxlevel;
+  double xa, xb, ya, yb, za, zb;
+  std::vector<Dims> dims;
+  std::vector<double> hx, hy, hz;
+  double small_k, lc;
+  MGp mgu, mgd;
+  double tol_u, tol_d_rel, tol_d_abs;
+  // fields[f][level][comp] -> vector
+  std::vector<std::vector<std::vector<std::vector<double>>>> F;
+  // statistics
+  long pcg_u_iters = 0, pcg_d_iters = 0, applies_u = 0, applies_d = 0;
+  int last_cycles_u = 0, last_cycles_d = 0;
+  std::vector<double> hist_u, hist_d;
+  char err[256] = {0};
+
+  double* fld(int f, int l, int c = 0) { return F[f][l][c].data(); }
+};
+
+// get_coordinate (boundary_conditions.cpp:13-17) with one rank per axis
+static inline double coord(double origin, double h, int index) { return origin + index * h; }
+
+// Dirichlet_BD (boundary_conditions.cpp:28-66)
+static void dirichlet_bd(Ctx& C, int l, double* u[3], double Ut) {
+  const Dims& D = C.dims[l];
+  const double xmid = 0.5 * (C.xa + C.xb), ymid = 0.5 * (C.ya + C.yb);
+  for (int k = 0; k < D.nz; k++) {
+    double z = coord(C.za, C.hz[l], k);
+    for (int j = 0; j < D.ny; j++) {
+      double y = coord(C.ya, C.hy[l], j);
+      for (int i = 0; i < D.nx; i++) {
+        double x = coord(C.xa, C.hx[l], i);
+        size_t q = D.at(i, j, k);
+        if (z == C.za && x == xmid && y == C.ya) u[0][q] = 0.0;
+        if (z == C.za && x == C.xb && y == ymid) u[1][q] = 0.0;
+        if (z == C.za) u[2][q] = 0.0;
+        if (z == C.zb) u[2][q] = Ut;
+      }
+    }
+  }
+}
+// Dirichlet_force_u (boundary_conditions.cpp:69-111): f := Fi on Dirichlet DOFs
+static void dirichlet_force_u(Ctx& C, int l, double* f[3], double* const Fi[3]) {
+  const Dims& D = C.dims[l];
+  const double xmid = 0.5 * (C.xa + C.xb), ymid = 0.5 * (C.ya + C.yb);
+  for (int k = 0; k < D.nz; k++) {
+    double z = coord(C.za, C.hz[l], k);
+    for (int j = 0; j < D.ny; j++) {
+      double y = coord(C.ya, C.hy[l], j);
+      for (int i = 0; i < D.nx; i++) {
+        double x = coord(C.xa, C.hx[l], i);
+        size_t q = D.at(i, j, k);
+        if (z == C.za && x == xmid && y == C.ya) f[0][q] = Fi[0][q];
+        if (z == C.za && x == C.xb && y == ymid) f[1][q] = Fi[1][q];
+        if (z == C.za) f[2][q] = Fi[2][q];
+        if (z == C.zb) f[2][q] = Fi[2][q];
+      }
+    }
+  }
+}
+// Dirichlet_residual_u (boundary_conditions.cpp:112-154)
+static void dirichlet_residual_u(Ctx& C, int l, double* r[3]) {
+  const Dims& D = C.dims[l];
+  const double xmid = 0.5 * (C.xa + C.xb), ymid = 0.5 * (C.ya + C.yb);
+  for (int k = 0; k < D.nz; k++) {
+    double z = coord(C.za, C.hz[l], k);
+    for (int j = 0; j < D.ny; j++) {
+      double y = coord(C.ya, C.hy[l], j);
+      for (int i = 0; i < D.nx; i++) {
+        double x = coord(C.xa, C.hx[l], i);
+        size_t q = D.at(i, j, k);
+        if (z == C.za && x == xmid && y == C.ya) r[0][q] = 0.0;
+        if (z == C.za && x == C.xb && y == ymid) r[1][q] = 0.0;
+        if (z == C.za) r[2][q] = 0.0;
+        if (z == C.zb) r[2][q] = 0.0;
+      }
+    }
+  }
+}
+
+// MP_all_GAUSS_Point (mechanical_pcg.cpp:26-61)
+static inline void mp_all_gauss_point(Ctx& C, int l, const Shape& S, int i, int j, int k,
+                                      double Kd[8], double Gd[8]) {
+  const Dims& D = C.dims[l];
+  const double* K = C.fld(F_K, l);
+  const double* G = C.fld(F_G, l);
+  const double* d = C.fld(F_D, l);
+  size_t e = D.at(i, j, k);
+  if (l == C.maxlevel) {
+    const double flag = C.fld(F_FLAG, l)[e];
+    for (int g = 0; g < 8; g++) {
+      double K_g = gauss_point_MP(K, D, S, i, j, k, g);
+      double G_g = gauss_point_MP(G, D, S, i, j, k, g);
+      Kd[g] = (flag == 0) ? K_G_d_g(d, K, D, S, i, j, k, g, C.small_k) : K_g;
+      Gd[g] = (flag == 2) ? G_g : K_G_d_g(d, G, D, S, i, j, k, g, C.small_k);
+    }
+  } else {
+    const double Ke = C.fld(F_KE, l)[e], Ge = C.fld(F_GE, l)[e];
+    for (int g = 0; g < 8; g++) { Kd[g] = Ke; Gd[g] = Ge; }
+  }
+}
+
+// Internal_Force / Internal_Force_element (mechanical_pcg.cpp:114-191)
+static void internal_force(Ctx& C, int l, double* Fi[3], const double* const u[3]) {
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double wg = 0.125 * C.hx[l] * C.hy[l] * C.hz[l];
+  for (int c = 0; c < 3; c++) std::fill(Fi[c], Fi[c] + D.n(), 0.0);
+  element_loop_coloured(D, [&](int i, int j, int k) {
+    double Kd[8], Gd[8];
+    mp_all_gauss_point(C, l, S, i, j, k, Kd, Gd);
+    double sig[8][3][3];
+    for (int g = 0; g < 8; g++) {
+      double eps[3][3], s[3][3];
+      epsilon_routine(eps, u, D, S, i, j, k, g);
+      double tr = trace_epsilon(eps);
+      sigma_g(s, eps, Kd[g], Gd[g], tr);
+      for (int c = 0; c < 3; c++)
+        for (int m = 0; m < 3; m++) sig[g][c][m] = s[c][m];
+    }
+    for (int sf = 0; sf < 8; sf++) {
+      double fu = 0.0, fv = 0.0, fw = 0.0;
+      for (int g = 0; g < 8; g++) {
+        fu += wg * (sig[g][0][0] * S.DN[0][g][sf] + sig[g][1][0] * S.DN[1][g][sf] + sig[g][2][0] * S.DN[2][g][sf]);
+        fv += wg * (sig[g][0][1] * S.DN[0][g][sf] + sig[g][1][1] * S.DN[1][g][sf] + sig[g][2][1] * S.DN[2][g][sf]);
+        fw += wg * (sig[g][0][2] * S.DN[0][g][sf] + sig[g][1][2] * S.DN[1][g][sf] + sig[g][2][2] * S.DN[2][g][sf]);
+      }
+      size_t q = D.at(Cx[sf] + i, Cy[sf] + j, Cz[sf] + k);
+      Fi[0][q] += fu;
+      Fi[1][q] += fv;
+      Fi[2][q] += fw;
+    }
+  });
+  C.applies_u++;
+}
+
+// PreM_u / PreM_u_element (mechanical_pcg.cpp:63-112)
+static void prem_u(Ctx& C, int l, double* preM[3]) {
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double wg = 0.125 * C.hx[l] * C.hy[l] * C.hz[l];
+  for (int c = 0; c < 3; c++) std::fill(preM[c], preM[c] + D.n(), 0.0);
+  element_loop_coloured(D, [&](int i, int j, int k) {
+    double Kd[8], Gd[8];
+    mp_all_gauss_point(C, l, S, i, j, k, Kd, Gd);
+    for (int sf = 0; sf < 8; sf++) {
+      double su = 0.0, sv = 0.0, sw = 0.0;
+      for (int g = 0; g < 8; g++) {
+        const double d0 = S.DN[0][g][sf], d1 = S.DN[1][g][sf], d2 = S.DN[2][g][sf];
+        su += wg * ((Kd[g] - 2. / 3. * Gd[g]) * d0 * d0 + 2. * Gd[g] * d0 * d0 + Gd[g] * (d1 * d1 + d2 * d2));
+        sv += wg * ((Kd[g] - 2. / 3. * Gd[g]) * d1 * d1 + 2. * Gd[g] * d1 * d1 + Gd[g] * (d0 * d0 + d2 * d2));
+        sw += wg * ((Kd[g] - 2. / 3. * Gd[g]) * d2 * d2 + 2. * Gd[g] * d2 * d2 + Gd[g] * (d0 * d0 + d1 * d1));
+      }
+      size_t q = D.at(Cx[sf] + i, Cy[sf] + j, Cz[sf] + k);
+      preM[0][q] += su;
+      preM[1][q] += sv;
+      preM[2][q] += sw;
+    }
+  });
+}
+
+// Fld / Fld_element (phase_field_pcg.cpp:61-103)
+static void fld(Ctx& C, int l, double* Fi, const double* d) {
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double wg = 0.125 * C.hx[l] * C.hy[l] * C.hz[l];
+  const double lc = C.lc;
+  const double* H = C.fld(F_H, l);
+  const double* gc = C.fld(F_GC, l);
+  std::fill(Fi, Fi + D.n(), 0.0);
+  element_loop_coloured(D, [&](int i, int j, int k) {
+    double gcg[8], dv[8];
+    for (int g = 0; g < 8; g++) gcg[g] = gauss_point_MP(gc, D, S, i, j, k, g);
+    for (int tf = 0; tf < 8; tf++) dv[tf] = d[D.at(Cx[tf] + i, Cy[tf] + j, Cz[tf] + k)];
+    const double He = H[D.at(i, j, k)];
+    for (int sf = 0; sf < 8; sf++) {
+      double acc = 0.0;
+      for (int g = 0; g < 8; g++) {
+        const double gc_g = gcg[g];
+        for (int tf = 0; tf < 8; tf++)
+          acc += wg * ((2.0 * He + gc_g / lc) * S.N[g][sf] * S.N[g][tf] +
+                       gc_g * lc * (S.DN[0][g][sf] * S.DN[0][g][tf] + S.DN[1][g][sf] * S.DN[1][g][tf] +
+                                    S.DN[2][g][sf] * S.DN[2][g][tf])) * dv[tf];
+      }
+      Fi[D.at(Cx[sf] + i, Cy[sf] + j, Cz[sf] + k)] += acc;
+    }
+  });
+  C.applies_d++;
+}
+
+// Precondition_M / Md_element (phase_field_pcg.cpp:20-59)
+static void precondition_m(Ctx& C, int l, double* Md) {
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double wg = 0.125 * C.hx[l] * C.hy[l] * C.hz[l];
+  const double lc = C.lc;
+  const double* H = C.fld(F_H, l);
+  const double* gc = C.fld(F_GC, l);
+  std::fill(Md, Md + D.n(), 0.0);
+  element_loop_coloured(D, [&](int i, int j, int k) {
+    double gcg[8];
+    for (int g = 0; g < 8; g++) gcg[g] = gauss_point_MP(gc, D, S, i, j, k, g);
+    const double He = H[D.at(i, j, k)];
+    for (int sf = 0; sf < 8; sf++) {
+      double acc = 0.0;
+      for (int g = 0; g < 8; g++)
+        acc += wg * ((2. * He + gcg[g] / lc) * S.N[g][sf] * S.N[g][sf] +
+                     gcg[g] * lc * (S.DN[0][g][sf] * S.DN[0][g][sf] + S.DN[1][g][sf] * S.DN[1][g][sf] +
+                                    S.DN[2][g][sf] * S.DN[2][g][sf]));
+      Md[D.at(Cx[sf] + i, Cy[sf] + j, Cz[sf] + k)] += acc;
+    }
+  });
+}
+
+// fd_h (phase_field_right_hand_side.cpp:14-47)
+static void fd_h(Ctx& C, int l) {
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double wg = 0.125 * C.hx[l] * C.hy[l] * C.hz[l];
+  double* fd = C.fld(F_FD, l);
+  const double* H = C.fld(F_H, l);
+  std::fill(fd, fd + D.n(), 0.0);
+  element_loop_coloured(D, [&](int i, int j, int k) {
+    const double He = H[D.at(i, j, k)];
+    for (int tf = 0; tf < 8; tf++) {
+      double acc = 0.0;
+      for (int g = 0; g < 8; g++) acc += wg * 2.0 * He * S.N[g][tf];
+      fd[D.at(Cx[tf] + i, Cy[tf] + j, Cz[tf] + k)] += acc;
+    }
+  });
+}
+
+// MP_choice_element (material_selection.cpp:19-45), finest level only
+static void mp_choice(Ctx& C) {
+  const int l = C.maxlevel;
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double* K = C.fld(F_K, l);
+  const double* G = C.fld(F_G, l);
+  const double* u[3] = {C.fld(F_U, l, 0), C.fld(F_U, l, 1), C.fld(F_U, l, 2)};
+  double* flag = C.fld(F_FLAG, l);
+  std::fill(flag, flag + D.n(), 0.0);
+  element_loop(D, [&](int i, int j, int k) {
+    double pp = 0., pn = 0., tr = 0.;
+    for (int g = 0; g < 8; g++) {
+      double K_g = gauss_point_MP(K, D, S, i, j, k, g);
+      double G_g = gauss_point_MP(G, D, S, i, j, k, g);
+      double eps[3][3];
+      epsilon_routine(eps, u, D, S, i, j, k, g);
+      tr += trace_epsilon(eps);
+      pp += phi_positive(K_g, G_g, eps);
+      pn += phi_negative(K_g, eps);
+    }
+    double v = 0;
+    if (tr >= 0.0) v = 0;
+    if (tr < 0.0) v = 1;
+    if (pp < pn) v = 2;
+    flag[D.at(i, j, k)] = v;
+  });
+}
+
+// MP_element / MP_all_element (material_selection.cpp:67-116), finest level
+static void mp_all_element(Ctx& C) {
+  const int l = C.maxlevel;
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double* K = C.fld(F_K, l);
+  const double* G = C.fld(F_G, l);
+  const double* d = C.fld(F_D, l);
+  const double* flag = C.fld(F_FLAG, l);
+  double* Ke = C.fld(F_KE, l);
+  double* Ge = C.fld(F_GE, l);
+  element_loop(D, [&](int i, int j, int k) {
+    double K_e = 0., G_e = 0.;
+    const double fl = flag[D.at(i, j, k)];
+    for (int g = 0; g < 8; g++) {
+      double K_g = gauss_point_MP(K, D, S, i, j, k, g);
+      double G_g = gauss_point_MP(G, D, S, i, j, k, g);
+      if (fl == 0) K_e += K_G_d_g(d, K, D, S, i, j, k, g, C.small_k);
+      else K_e += K_g;
+      if (fl == 2) G_e += G_g;
+      else G_e += K_G_d_g(d, G, D, S, i, j, k, g, C.small_k);
+    }
+    Ke[D.at(i, j, k)] = 0.125 * K_e;
+    Ge[D.at(i, j, k)] = 0.125 * G_e;
+  });
+}
+
+// coarse_voigt / coarse_reuss / VR_all_level / Voigt_all_level
+// (material_coarse_element.cpp:20-118), element-indexed fields
+static void vr_all_level(Ctx& C, int f, bool voigt_only) {
+  for (int l = C.maxlevel; l > 0; l--) {
+    const Dims& Df = C.dims[l];
+    const Dims& Dc = C.dims[l - 1];
+    const double* Hf = C.fld(f, l);
+    double* Hc = C.fld(f, l - 1);
+    std::vector<double> Hv(Dc.n(), 0.0), Hr(Dc.n(), 0.0);
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+    for (int kc = 0; kc <= Dc.nz - 2; kc++)
+      for (int jc = 0; jc <= Dc.ny - 2; jc++)
+        for (int ic = 0; ic <= Dc.nx - 2; ic++) {
+          int x = 2 * ic, y = 2 * jc, z = 2 * kc;
+          size_t q = Dc.at(ic, jc, kc);
+          Hv[q] = 0.125 * (Hf[Df.at(x, y, z)] + Hf[Df.at(x + 1, y, z)] + Hf[Df.at(x, y + 1, z)] +
+                           Hf[Df.at(x + 1, y + 1, z)] + Hf[Df.at(x, y, z + 1)] + Hf[Df.at(x + 1, y, z + 1)] +
+                           Hf[Df.at(x, y + 1, z + 1)] + Hf[Df.at(x + 1, y + 1, z + 1)]);
+          if (!voigt_only)
+            Hr[q] = 8. / (1. / Hf[Df.at(x, y, z)] + 1. / Hf[Df.at(x + 1, y, z)] + 1. / Hf[Df.at(x, y + 1, z)] +
+                          1. / Hf[Df.at(x + 1, y + 1, z)] + 1. / Hf[Df.at(x, y, z + 1)] +
+                          1. / Hf[Df.at(x + 1, y, z + 1)] + 1. / Hf[Df.at(x, y + 1, z + 1)] +
+                          1. / Hf[Df.at(x + 1, y + 1, z + 1)]);
+        }
+    if (voigt_only) {
+      // Voigt_all_level writes the element range in place; the unused last
+      // layer keeps its previous value (material_coarse_element.cpp:106-118)
+      for (int kc = 0; kc <= Dc.nz - 2; kc++)
+        for (int jc = 0; jc <= Dc.ny - 2; jc++)
+          for (int ic = 0; ic <= Dc.nx - 2; ic++) Hc[Dc.at(ic, jc, kc)] = Hv[Dc.at(ic, jc, kc)];
+    } else {
+      // H_c = 0.5*(0.5*(V+R) + sqrt(V*R)) on the whole grid
+      // (material_coarse_element.cpp:94; temporaries of grid.h:251-320)
+      for (size_t q = 0; q < Dc.n(); q++) {
+        double s = Hv[q] + Hr[q];
+        double h = 0.5 * s;
+        double pr = Hv[q] * Hr[q];
+        double sq = std::sqrt(pr);
+        double t = h + sq;
+        Hc[q] = 0.5 * t;
+      }
+    }
+  }
+}
+
+// voight / reuss / coarse_VR (image_input.cpp:144-273), nodal 27-point means
+static void coarse_vr_nodal(Ctx& C, int f) {
+  for (int l = C.maxlevel; l > 0; l--) {
+    const Dims& Df = C.dims[l];
+    const Dims& Dc = C.dims[l - 1];
+    const double* M = C.fld(f, l);
+    double* Mc = C.fld(f, l - 1);
+    const int Xl = Df.nx - 1, Yl = Df.ny - 1, Zl = Df.nz - 1;
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+    for (int kc = 0; kc < Dc.nz; kc++)
+      for (int jc = 0; jc < Dc.ny; jc++)
+        for (int ic = 0; ic < Dc.nx; ic++) {
+          const int x = 2 * ic, y = 2 * jc, z = 2 * kc;
+          const int xm = std::max(x - 1, 0), xp = std::min(x + 1, Xl);
+          const int ym = std::max(y - 1, 0), yp = std::min(y + 1, Yl);
+          const int zm = std::max(z - 1, 0), zp = std::min(z + 1, Zl);
+          // the reference's summation order (image_input.cpp:169-195)
+          const int P[27][3] = {
+              {xm, ym, zm}, {xm, ym, zp}, {xm, yp, zm}, {xm, yp, zp}, {xp, ym, zm}, {xp, ym, zp},
+              {xp, yp, zm}, {xp, yp, zp}, {xm, ym, z},  {xm, yp, z},  {xp, ym, z},  {xp, yp, z},
+              {xm, y, zm},  {xm, y, zp},  {xp, y, zm},  {xp, y, zp},  {x, ym, zm},  {x, ym, zp},
+              {x, yp, zm},  {x, yp, zp},  {x, y, zm},   {x, y, zp},   {x, ym, z},   {x, yp, z},
+              {xm, y, z},   {xp, y, z},   {x, y, z}};
+          double sv = 0.0, sr = 0.0;
+          for (int t = 0; t < 27; t++) {
+            double v = M[Df.at(P[t][0], P[t][1], P[t][2])];
+            sv += v;
+            sr += 1. / v;
+          }
+          double V = 1.0 / 27.0 * sv;
+          double R = 27.0 / sr;
+          Mc[Dc.at(ic, jc, kc)] = 0.5 * (0.5 * (V + R) + std::sqrt(V * R));
+        }
+  }
+}
+
+// solution_injection (multigrid_intergrid.cpp:25-90)
+static void solution_injection(const Dims& Dc, double* uc, const Dims& Df, const double* uf) {
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+  for (int kc = 0; kc < Dc.nz; kc++)
+    for (int jc = 0; jc < Dc.ny; jc++)
+      for (int ic = 0; ic < Dc.nx; ic++) uc[Dc.at(ic, jc, kc)] = uf[Df.at(2 * ic, 2 * jc, 2 * kc)];
+}
+
+// coarsen_f_standard_routine (multigrid_intergrid.cpp:124-342): FAS right-hand
+// side fc = Fic + full-weighting(rf), out-of-range neighbours count as 0.
+static void coarsen_f_standard(const Dims& Dc, double* fc, const Dims& Df, const double* rf,
+                               const double* Fic) {
+  const int Xl = Df.nx - 2, Yl = Df.ny - 2, Zl = Df.nz - 2;
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+  for (int kc = 0; kc < Dc.nz; kc++)
+    for (int jc = 0; jc < Dc.ny; jc++)
+      for (int ic = 0; ic < Dc.nx; ic++) {
+        const int x = 2 * ic, y = 2 * jc, z = 2 * kc;
+        auto R = [&](bool ok, int a, int b, int c) { return ok ? rf[Df.at(a, b, c)] : 0.0; };
+        double r11 = R(!(x < 1 || y < 1 || z < 1), x - 1, y - 1, z - 1);
+        double r12 = R(!(x < 1 || y < 1 || z > Zl), x - 1, y - 1, z + 1);
+        double r13 = R(!(x < 1 || y > Yl || z < 1), x - 1, y + 1, z - 1);
+        double r14 = R(!(x < 1 || y > Yl || z > Zl), x - 1, y + 1, z + 1);
+        double r15 = R(!(x > Xl || y < 1 || z < 1), x + 1, y - 1, z - 1);
+        double r16 = R(!(x > Xl || y < 1 || z > Zl), x + 1, y - 1, z + 1);
+        double r17 = R(!(x > Xl || y > Yl || z < 1), x + 1, y + 1, z - 1);
+        double r18 = R(!(x > Xl || y > Yl || z > Zl), x + 1, y + 1, z + 1);
+        double r21 = R(!(x < 1 || y < 1), x - 1, y - 1, z);
+        double r22 = R(!(x < 1 || y > Yl), x - 1, y + 1, z);
+        double r23 = R(!(x > Xl || y < 1), x + 1, y - 1, z);
+        double r24 = R(!(x > Xl || y > Yl), x + 1, y + 1, z);
+        double r25 = R(!(x < 1 || z < 1), x - 1, y, z - 1);
+        double r26 = R(!(x < 1 || z > Zl), x - 1, y, z + 1);
+        double r27 = R(!(x > Xl || z < 1), x + 1, y, z - 1);
+        double r28 = R(!(x > Xl || z > Zl), x + 1, y, z + 1);
+        double r29 = R(!(y < 1 || z < 1), x, y - 1, z - 1);
+        double r210 = R(!(y < 1 || z > Zl), x, y - 1, z + 1);
+        double r211 = R(!(y > Yl || z < 1), x, y + 1, z - 1);
+        double r212 = R(!(y > Yl || z > Zl), x, y + 1, z + 1);
+        double r41 = R(!(z < 1), x, y, z - 1);
+        double r42 = R(!(z > Zl), x, y, z + 1);
+        double r43 = R(!(y < 1), x, y - 1, z);
+        double r44 = R(!(y > Yl), x, y + 1, z);
+        double r45 = R(!(x < 1), x - 1, y, z);
+        double r46 = R(!(x > Xl), x + 1, y, z);
+        double r8 = rf[Df.at(x, y, z)];
+        size_t q = Dc.at(ic, jc, kc);
+        fc[q] = Fic[q] + 0.125 * (r11 + r12 + r13 + r14 + r15 + r16 + r17 + r18 +
+                                  2.0 * (r21 + r22 + r23 + r24 + r25 + r26 + r27 + r28 + r29 + r210 + r211 + r212) +
+                                  4.0 * (r41 + r42 + r43 + r44 + r45 + r46) + 8.0 * r8);
+      }
+}
+
+// refine_standard_routine (multigrid_intergrid.cpp:416-520): trilinear
+// prolongation of the coarse correction, one writer per fine node.
+static void refine_standard(const Dims& Df, double* uf, const Dims& Dc, const double* uc,
+                            const double* ucold) {
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+  for (int kc = 0; kc < Dc.nz; kc++)
+    for (int jc = 0; jc < Dc.ny; jc++)
+      for (int ic = 0; ic < Dc.nx; ic++) {
+        const int x = 2 * ic, y = 2 * jc, z = 2 * kc;
+        auto E = [&](int a, int b, int c) { size_t q = Dc.at(a, b, c); return uc[q] - ucold[q]; };
+        uf[Df.at(x, y, z)] += E(ic, jc, kc);
+        if (x > 0) uf[Df.at(x - 1, y, z)] += 0.5 * (E(ic, jc, kc) + E(ic - 1, jc, kc));
+        if (y > 0) uf[Df.at(x, y - 1, z)] += 0.5 * (E(ic, jc, kc) + E(ic, jc - 1, kc));
+        if (z > 0) uf[Df.at(x, y, z - 1)] += 0.5 * (E(ic, jc, kc) + E(ic, jc, kc - 1));
+        if (x > 0 && y > 0)
+          uf[Df.at(x - 1, y - 1, z)] += 0.25 * (E(ic, jc, kc) + E(ic, jc - 1, kc) + E(ic - 1, jc, kc) + E(ic - 1, jc - 1, kc));
+        if (x > 0 && z > 0)
+          uf[Df.at(x - 1, y, z - 1)] += 0.25 * (E(ic, jc, kc) + E(ic - 1, jc, kc - 1) + E(ic, jc, kc - 1) + E(ic - 1, jc, kc));
+        if (y > 0 && z > 0)
+          uf[Df.at(x, y - 1, z - 1)] += 0.25 * (E(ic, jc, kc) + E(ic, jc - 1, kc - 1) + E(ic, jc, kc - 1) + E(ic, jc - 1, kc));
+        if (x > 0 && y > 0 && z > 0)
+          uf[Df.at(x - 1, y - 1, z - 1)] +=
+              0.125 * (E(ic, jc, kc) + E(ic, jc - 1, kc) + E(ic - 1, jc, kc) + E(ic - 1, jc - 1, kc) +
+                       E(ic, jc, kc - 1) + E(ic - 1, jc, kc - 1) + E(ic, jc - 1, kc - 1) + E(ic - 1, jc - 1, kc - 1));
+      }
+}
+
+static void set_err(Ctx& C, const char* m) { snprintf(C.err, sizeof(C.err), "%s", m); }
+
+// pcg_u (mechanical_pcg.cpp:223-330)
+static int pcg_u(Ctx& C, int l, int NB, double Ut, int t, double& f_norm) {
+  const Dims& D = C.dims[l];
+  const size_t n = D.n();
+  double* u[3] = {C.fld(F_U, l, 0), C.fld(F_U, l, 1), C.fld(F_U, l, 2)};
+  double* fu[3] = {C.fld(F_FU, l, 0), C.fld(F_FU, l, 1), C.fld(F_FU, l, 2)};
+  double* ru[3] = {C.fld(F_RU, l, 0), C.fld(F_RU, l, 1), C.fld(F_RU, l, 2)};
+  std::vector<double> preM[3], Fi[3], z[3], p[3];
+  for (int c = 0; c < 3; c++) { preM[c].assign(n, 0.); Fi[c].assign(n, 0.); z[c].assign(n, 0.); p[c].assign(n, 0.); }
+  double* Fip[3] = {Fi[0].data(), Fi[1].data(), Fi[2].data()};
+  double* pp[3] = {p[0].data(), p[1].data(), p[2].data()};
+  double* pm[3] = {preM[0].data(), preM[1].data(), preM[2].data()};
+  dirichlet_bd(C, l, u, Ut);
+  internal_force(C, l, Fip, u);
+  if (t == 0 && l == C.maxlevel) {
+    dirichlet_force_u(C, l, fu, Fip);
+    f_norm = 0.;
+    for (int c = 0; c < 3; c++) f_norm += plane_sum(D, fu[c], fu[c]);
+  }
+  for (int c = 0; c < 3; c++)
+    for (size_t q = 0; q < n; q++) ru[c][q] = fu[c][q] - Fi[c][q];
+  dirichlet_residual_u(C, l, ru);
+  prem_u(C, l, pm);
+  for (int c = 0; c < 3; c++)
+    for (size_t q = 0; q < n; q++)
+      if (preM[c][q] == 0.0) { set_err(C, "Division by zero in grid /= operation"); return -5; }
+  double rz_old = 0., beta = 0., alpha = 0.;
+  for (int i = 0; i < NB; i++) {
+    for (int c = 0; c < 3; c++)
+      for (size_t q = 0; q < n; q++) z[c][q] = ru[c][q] / preM[c][q];
+    double rz_new = 0.;
+    for (int c = 0; c < 3; c++) rz_new += plane_sum(D, ru[c], z[c].data());
+    beta = (i == 0) ? 0.0 : rz_new / rz_old;
+    for (int c = 0; c < 3; c++)
+      for (size_t q = 0; q < n; q++) p[c][q] = z[c][q] + beta * p[c][q];
+    internal_force(C, l, Fip, pp);
+    double pw = 0.;
+    for (int c = 0; c < 3; c++) pw += plane_sum(D, pp[c], Fip[c]);
+    alpha = rz_new / pw;
+    for (int c = 0; c < 3; c++)
+      for (size_t q = 0; q < n; q++) {
+        u[c][q] = u[c][q] + alpha * p[c][q];
+        ru[c][q] = ru[c][q] - alpha * Fi[c][q];
+      }
+    dirichlet_residual_u(C, l, ru);
+    rz_old = rz_new;
+    C.pcg_u_iters++;
+  }
+  return 0;
+}
+
+// coarsen_u (multigrid_intergrid.cpp:106-121)
+static void coarsen_u(Ctx& C, int lc_, double Ut) {
+  for (int c = 0; c < 3; c++)
+    solution_injection(C.dims[lc_], C.fld(F_U, lc_, c), C.dims[lc_ + 1], C.fld(F_U, lc_ + 1, c));
+  double* uc[3] = {C.fld(F_U, lc_, 0), C.fld(F_U, lc_, 1), C.fld(F_U, lc_, 2)};
+  dirichlet_bd(C, lc_, uc, Ut);
+  for (int c = 0; c < 3; c++) C.F[F_UOLD][lc_][c] = C.F[F_U][lc_][c];
+}
+
+// coarsen_fu (multigrid_intergrid.cpp:374-413)
+static void coarsen_fu(Ctx& C, int lf) {
+  const int lcz = lf - 1;
+  const Dims& Dc = C.dims[lcz];
+  std::vector<double> Fic[3];
+  for (int c = 0; c < 3; c++) Fic[c].assign(Dc.n(), 0.);
+  double* Fp[3] = {Fic[0].data(), Fic[1].data(), Fic[2].data()};
+  const double* uc[3] = {C.fld(F_U, lcz, 0), C.fld(F_U, lcz, 1), C.fld(F_U, lcz, 2)};
+  internal_force(C, lcz, Fp, uc);
+  for (int c = 0; c < 3; c++)
+    coarsen_f_standard(Dc, C.fld(F_FU, lcz, c), C.dims[lf], C.fld(F_RU, lf, c), Fp[c]);
+}
+
+// refine_u (multigrid_intergrid.cpp:536-549)
+static void refine_u(Ctx& C, int lf, double Ut) {
+  for (int c = 0; c < 3; c++)
+    refine_standard(C.dims[lf], C.fld(F_U, lf, c), C.dims[lf - 1], C.fld(F_U, lf - 1, c), C.fld(F_UOLD, lf - 1, c));
+  double* uf[3] = {C.fld(F_U, lf, 0), C.fld(F_U, lf, 1), C.fld(F_U, lf, 2)};
+  dirichlet_bd(C, lf, uf, Ut);
+}
+
+// cycle_u (mechanical_multigrid.cpp:18-54)
+static int cycle_u(Ctx& C, int l, double Ut, int t, double& f_norm) {
+  int rc;
+  if (l == 0) return pcg_u(C, 0, C.mgu.nu0, Ut, t, f_norm);
+  if ((rc = pcg_u(C, l, C.mgu.nu1, Ut, t, f_norm))) return rc;
+  coarsen_u(C, l - 1, Ut);
+  coarsen_fu(C, l);
+  for (int j = 1; j <= C.mgu.gamma; j++)
+    if ((rc = cycle_u(C, l - 1, Ut, t, f_norm))) return rc;
+  refine_u(C, l, Ut);
+  return pcg_u(C, l, C.mgu.nu2, Ut, t, f_norm);
+}
+
+static double norm3(Ctx& C, int f, int l) {
+  double s = 0.;
+  for (int c = 0; c < 3; c++) s += plane_sum(C.dims[l], C.fld(f, l, c), C.fld(f, l, c));
+  return s;
+}
+
+// u_entire (mechanical_displacement.cpp:25-115)
+static int u_entire(Ctx& C, double Ut, int t, double& f_norm) {
+  const int L = C.maxlevel;
+  C.hist_u.clear();
+  // fresh per-call hierarchies (mechanical_displacement.cpp:32-39)
+  for (int l = 0; l <= L; l++) {
+    for (int c = 0; c < 3; c++) {
+      std::fill(C.F[F_RU][l][c].begin(), C.F[F_RU][l][c].end(), 0.0);
+      std::fill(C.F[F_UOLD][l][c].begin(), C.F[F_UOLD][l][c].end(), 0.0);
+    }
+    std::fill(C.F[F_FLAG][l][0].begin(), C.F[F_FLAG][l][0].end(), 0.0);
+    std::fill(C.F[F_KE][l][0].begin(), C.F[F_KE][l][0].end(), 0.0);
+    std::fill(C.F[F_GE][l][0].begin(), C.F[F_GE][l][0].end(), 0.0);
+  }
+  mp_choice(C);
+  mp_all_element(C);
+  vr_all_level(C, F_KE, false);
+  vr_all_level(C, F_GE, false);
+  // Initial_residual_u (mechanical_pcg.cpp:193-220)
+  {
+    const Dims& D = C.dims[L];
+    std::vector<double> Fi[3];
+    for (int c = 0; c < 3; c++) Fi[c].assign(D.n(), 0.);
+    double* Fp[3] = {Fi[0].data(), Fi[1].data(), Fi[2].data()};
+    const double* u[3] = {C.fld(F_U, L, 0), C.fld(F_U, L, 1), C.fld(F_U, L, 2)};
+    internal_force(C, L, Fp, u);
+    for (int c = 0; c < 3; c++)
+      for (size_t q = 0; q < D.n(); q++) C.F[F_RU][L][c][q] = C.F[F_FU][L][c][q] - Fi[c][q];
+    double* ru[3] = {C.fld(F_RU, L, 0), C.fld(F_RU, L, 1), C.fld(F_RU, L, 2)};
+    dirichlet_residual_u(C, L, ru);
+  }
+  double r_norm = norm3(C, F_RU, L);
+  C.hist_u.push_back(r_norm);
+  double rel = 1.0;
+  if (t >= 1 && f_norm != 0) rel = r_norm / f_norm;
+  int cycles = 0;
+  for (int j = 1; j <= C.mgu.ncy; j++) {
+    if (rel > C.tol_u) {
+      int rc = cycle_u(C, L, Ut, t, f_norm);
+      if (rc) return rc;
+      cycles++;
+      double* ru[3] = {C.fld(F_RU, L, 0), C.fld(F_RU, L, 1), C.fld(F_RU, L, 2)};
+      dirichlet_residual_u(C, L, ru);
+      r_norm = norm3(C, F_RU, L);
+      rel = r_norm / f_norm;
+      C.hist_u.push_back(r_norm);
+    }
+    if (rel <= C.tol_u) break;
+  }
+  C.last_cycles_u = cycles;
+  // externe_force_nodal (mechanical_right_hnd.cpp:18-38)
+  {
+    const Dims& D = C.dims[L];
+    double* u[3] = {C.fld(F_U, L, 0), C.fld(F_U, L, 1), C.fld(F_U, L, 2)};
+    dirichlet_bd(C, L, u, Ut);
+    std::vector<double> Fi[3];
+    for (int c = 0; c < 3; c++) Fi[c].assign(D.n(), 0.);
+    double* Fp[3] = {Fi[0].data(), Fi[1].data(), Fi[2].data()};
+    internal_force(C, L, Fp, u);
+    double* fu[3] = {C.fld(F_FU, L, 0), C.fld(F_FU, L, 1), C.fld(F_FU, L, 2)};
+    dirichlet_force_u(C, L, fu, Fp);
+  }
+  return 0;
+}
+
+// pcg_d (phase_field_pcg.cpp:133-214)
+static int pcg_d(Ctx& C, int l, int NB, double& fd_norm) {
+  const Dims& D = C.dims[l];
+  const size_t n = D.n();
+  double* d = C.fld(F_D, l);
+  double* fd = C.fld(F_FD, l);
+  double* R = C.fld(F_RD, l);
+  std::vector<double> preM(n, 0.), Fi(n, 0.), z(n, 0.), p(n, 0.);
+  precondition_m(C, l, preM.data());
+  fld(C, l, Fi.data(), d);
+  fd_norm = plane_sum(D, fd, fd);
+  for (size_t q = 0; q < n; q++) R[q] = fd[q] - Fi[q];
+  for (size_t q = 0; q < n; q++)
+    if (preM[q] == 0.0) { set_err(C, "Division by zero in grid /= operation"); return -5; }
+  double rz_old = 0., beta = 0., alpha = 0.;
+  for (int i = 0; i < NB; i++) {
+    for (size_t q = 0; q < n; q++) z[q] = R[q] / preM[q];
+    double rz_new = plane_sum(D, R, z.data());
+    beta = (i == 0) ? 0.0 : rz_new / rz_old;
+    for (size_t q = 0; q < n; q++) p[q] = z[q] + beta * p[q];
+    fld(C, l, Fi.data(), p.data());
+    double pw = plane_sum(D, p.data(), Fi.data());
+    alpha = rz_new / pw;
+    for (size_t q = 0; q < n; q++) {
+      d[q] = d[q] + alpha * p[q];
+      R[q] = R[q] - alpha * Fi[q];
+    }
+    rz_old = rz_new;
+    C.pcg_d_iters++;
+  }
+  return 0;
+}
+
+// cycle_d (phase_field_multigrid.cpp:18-44)
+static int cycle_d(Ctx& C, int l, double& fd_norm) {
+  int rc;
+  if (l == 0) return pcg_d(C, 0, C.mgd.nu0, fd_norm);
+  if ((rc = pcg_d(C, l, C.mgd.nu1, fd_norm))) return rc;
+  // coarsen_d (multigrid_intergrid.cpp:92-103)
+  solution_injection(C.dims[l - 1], C.fld(F_D, l - 1), C.dims[l], C.fld(F_D, l));
+  C.F[F_DOLD][l - 1][0] = C.F[F_D][l - 1][0];
+  // coarsen_f_d (multigrid_intergrid.cpp:345-371)
+  {
+    const Dims& Dc = C.dims[l - 1];
+    std::vector<double> Fic(Dc.n(), 0.);
+    fld(C, l - 1, Fic.data(), C.fld(F_D, l - 1));
+    coarsen_f_standard(Dc, C.fld(F_FD, l - 1), C.dims[l], C.fld(F_RD, l), Fic.data());
+  }
+  for (int j = 1; j <= C.mgd.gamma; j++)
+    if ((rc = cycle_d(C, l - 1, fd_norm))) return rc;
+  // refine_d (multigrid_intergrid.cpp:523-533)
+  refine_standard(C.dims[l], C.fld(F_D, l), C.dims[l - 1], C.fld(F_D, l - 1), C.fld(F_DOLD, l - 1));
+  for (int i = 1; i <= C.mgd.nu2; i++)
+    if ((rc = pcg_d(C, l, C.mgd.nu2, fd_norm))) return rc;
+  return 0;
+}
+
+// phase_field (phase_field.cpp:15-57)
+static int phase_field(Ctx& C, int t, double& fd_norm) {
+  (void)t;
+  const int L = C.maxlevel;
+  C.hist_d.clear();
+  for (int l = 0; l <= L; l++) {
+    std::fill(C.F[F_RD][l][0].begin(), C.F[F_RD][l][0].end(), 0.0);
+    std::fill(C.F[F_DOLD][l][0].begin(), C.F[F_DOLD][l][0].end(), 0.0);
+  }
+  const Dims& D = C.dims[L];
+  {
+    std::vector<double> Fi(D.n(), 0.);
+    fld(C, L, Fi.data(), C.fld(F_D, L));
+    double* R = C.fld(F_RD, L);
+    const double* fd = C.fld(F_FD, L);
+    for (size_t q = 0; q < D.n(); q++) R[q] = fd[q] - Fi[q];
+  }
+  double r_norm = plane_sum(D, C.fld(F_RD, L), C.fld(F_RD, L));
+  C.hist_d.push_back(r_norm);
+  double sres = 1.0;
+  int cycles = 0;
+  for (int j = 1; j <= C.mgd.ncy; j++) {
+    if (r_norm > C.tol_d_abs) {
+      if (sres > C.tol_d_rel) {
+        int rc = cycle_d(C, L, fd_norm);
+        if (rc) return rc;
+        cycles++;
+      }
+      r_norm = plane_sum(D, C.fld(F_RD, L), C.fld(F_RD, L));
+      sres = r_norm / fd_norm;
+      C.hist_d.push_back(r_norm);
+      if (sres <= C.tol_d_rel) break;
+    } else {
+      break;
+    }
+  }
+  C.last_cycles_d = cycles;
+  return 0;
+}
+
+// strain_history (strain_history.cpp:162-210), finest level
+static void strain_history(Ctx& C, double u_prop) {
+  const int l = C.maxlevel;
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double* K = C.fld(F_K, l);
+  const double* G = C.fld(F_G, l);
+  const double* gc = C.fld(F_GC, l);
+  const double* u[3] = {C.fld(F_U, l, 0), C.fld(F_U, l, 1), C.fld(F_U, l, 2)};
+  double* H = C.fld(F_H, l);
+  const double lc = C.lc;
+  element_loop(D, [&](int i, int j, int k) {
+    double tmp = 0.;
+    for (int g = 0; g < 8; g++) {
+      double K_g = gauss_point_MP(K, D, S, i, j, k, g);
+      double G_g = gauss_point_MP(G, D, S, i, j, k, g);
+      double gc_g = gauss_point_MP(gc, D, S, i, j, k, g);
+      double eps[3][3];
+      epsilon_routine(eps, u, D, S, i, j, k, g);
+      for (int c = 0; c < 3; c++)
+        for (int m = 0; m < 3; m++) eps[c][m] = u_prop * eps[c][m];
+      tmp += phi_positive(K_g, G_g, eps) - 0.5 * gc_g / lc;
+    }
+    double H_tmp = 0.125 * tmp;
+    size_t q = D.at(i, j, k);
+    if (H[q] < H_tmp) H[q] = H_tmp;
+  });
+}
+
+// U_e (strain_history.cpp:86-131), finest level
+static double U_e(Ctx& C) {
+  const int l = C.maxlevel;
+  const Dims& D = C.dims[l];
+  Shape S;
+  dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
+  const double* K = C.fld(F_K, l);
+  const double* G = C.fld(F_G, l);
+  const double* gc = C.fld(F_GC, l);
+  const double* u[3] = {C.fld(F_U, l, 0), C.fld(F_U, l, 1), C.fld(F_U, l, 2)};
+  std::vector<double> part(D.nz, 1e10);
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+  for (int k = 0; k <= D.nz - 2; k++) {
+    double Ue = 1e10;
+    for (int j = 0; j <= D.ny - 2; j++)
+      for (int i = 0; i <= D.nx - 2; i++)
+        for (int g = 0; g < 8; g++) {
+          double K_g = gauss_point_MP(K, D, S, i, j, k, g);
+          double G_g = gauss_point_MP(G, D, S, i, j, k, g);
+          double gc_g = gauss_point_MP(gc, D, S, i, j, k, g);
+          double eps[3][3], s[3][3];
+          epsilon_routine(eps, u, D, S, i, j, k, g);
+          double tr = trace_epsilon(eps);
+          sigma_g(s, eps, K_g, G_g, tr);
+          double W0 = W_energy(s, eps);
+          double phic = 0.5 * gc_g / C.lc;
+          double ut = std::sqrt(phic / W0);
+          if (ut < Ue) Ue = ut;
+        }
+    part[k] = Ue;
+  }
+  double Ue = 1e10;
+  for (int k = 0; k < D.nz; k++)
+    if (part[k] < Ue) Ue = part[k];
+  return Ue;
+}
+
+}  // namespace
+
+// ============================================================================
+// C API for ctypes (tests, smoke, bench cpu_baseline)
+// ============================================================================
+extern "C" {
+
+void ora_set_threads(int n) { n_threads_override = n; }
+int ora_get_threads() { return nthreads(); }
+
+void ora_shape_functions(double hx, double hy, double hz, double* N64, double* DN192) {
+  Shape S;
+  dshape_function(S, hx, hy, hz);
+  for (int g = 0; g < 8; g++)
+    for (int a = 0; a < 8; a++) N64[g * 8 + a] = S.N[g][a];
+  for (int c = 0; c < 3; c++)
+    for (int g = 0; g < 8; g++)
+      for (int a = 0; a < 8; a++) DN192[(c * 8 + g) * 8 + a] = S.DN[c][g][a];
+}
+
+void* ora_create(int maxlevel, const int* element_nb, const double* domain /*xa,xb,ya,yb,za,zb*/,
+                 double small_k, double lc, const int* mg_u, const int* mg_d, double tol_u,
+                 double tol_d_rel, double tol_d_abs) {
+  Ctx* C = new Ctx();
+  C->maxlevel = maxlevel;
+  C->xa = domain[0]; C->xb = domain[1];
+  C->ya = domain[2]; C->yb = domain[3];
+  C->za = domain[4]; C->zb = domain[5];
+  C->small_k = small_k;
+  C->lc = lc;
+  C->mgu = {mg_u[0], mg_u[1], mg_u[2], mg_u[3], mg_u[4]};
+  C->mgd = {mg_d[0], mg_d[1], mg_d[2], mg_d[3], mg_d[4]};
+  C->tol_u = tol_u;
+  C->tol_d_rel = tol_d_rel;
+  C->tol_d_abs = tol_d_abs;
+  // initialize (initialization.cpp:19-134), one rank per axis
+  double hx = (C->xb - C->xa) / element_nb[0];
+  double hy = (C->yb - C->ya) / element_nb[1];
+  double hz = (C->zb - C->za) / element_nb[2];
+  int ii = element_nb[0], jj = element_nb[1], kk = element_nb[2];
+  for (int l = 0; l <= maxlevel; l++) {
+    C->dims.push_back(Dims{ii + 1, jj + 1, kk + 1});
+    C->hx.push_back(hx); C->hy.push_back(hy); C->hz.push_back(hz);
+    hx *= 0.5; hy *= 0.5; hz *= 0.5;
+    ii *= 2; jj *= 2; kk *= 2;
+  }
+  C->F.resize(F_COUNT);
+  for (int f = 0; f < F_COUNT; f++) {
+    C->F[f].resize(maxlevel + 1);
+    for (int l = 0; l <= maxlevel; l++) {
+      C->F[f][l].resize(field_ncomp(f));
+      for (auto& v : C->F[f][l]) v.assign(C->dims[l].n(), 0.0);
+    }
+  }
+  return C;
+}
+void ora_destroy(void* c) { delete (Ctx*)c; }
+const char* ora_error(void* c) { return ((Ctx*)c)->err; }
+void ora_dims(void* c, int l, int* out3) {
+  Ctx* C = (Ctx*)c;
+  out3[0] = C->dims[l].nx; out3[1] = C->dims[l].ny; out3[2] = C->dims[l].nz;
+}
+void ora_h(void* c, int l, double* out3) {
+  Ctx* C = (Ctx*)c;
+  out3[0] = C->hx[l]; out3[1] = C->hy[l]; out3[2] = C->hz[l];
+}
+double* ora_field(void* c, int f, int l, int comp) { return ((Ctx*)c)->fld(f, l, comp); }
+void ora_stats(void* c, long* out4) {
+  Ctx* C = (Ctx*)c;
+  out4[0] = C->pcg_u_iters; out4[1] = C->pcg_d_iters; out4[2] = C->applies_u; out4[3] = C->applies_d;
+}
+int ora_history(void* c, int which, double* out, int cap) {
+  Ctx* C = (Ctx*)c;
+  const std::vector<double>& h = which == 0 ? C->hist_u : C->hist_d;
+  int n = (int)h.size();
+  for (int q = 0; q < n && q < cap; q++) out[q] = h[q];
+  return n;
+}
+
+// Operator-level entry points: dst = A(src) on level l
+void ora_internal_force(void* c, int l, int src, int dst) {
+  Ctx* C = (Ctx*)c;
+  double* Fi[3] = {C->fld(dst, l, 0), C->fld(dst, l, 1), C->fld(dst, l, 2)};
+  const double* u[3] = {C->fld(src, l, 0), C->fld(src, l, 1), C->fld(src, l, 2)};
+  internal_force(*C, l, Fi, u);
+}
+void ora_prem_u(void* c, int l, int dst) {
+  Ctx* C = (Ctx*)c;
+  double* pm[3] = {C->fld(dst, l, 0), C->fld(dst, l, 1), C->fld(dst, l, 2)};
+  prem_u(*C, l, pm);
+}
+void ora_fld(void* c, int l, int src, int dst) {
+  Ctx* C = (Ctx*)c;
+  fld(*C, l, C->fld(dst, l), C->fld(src, l));
+}
+void ora_precondition_m(void* c, int l, int dst) {
+  Ctx* C = (Ctx*)c;
+  precondition_m(*C, l, C->fld(dst, l));
+}
+void ora_fd_h(void* c, int l) { fd_h(*(Ctx*)c, l); }
+void ora_mp_choice(void* c) { mp_choice(*(Ctx*)c); }
+void ora_mp_all_element(void* c) { mp_all_element(*(Ctx*)c); }
+void ora_vr_all_level(void* c, int f) { vr_all_level(*(Ctx*)c, f, false); }
+void ora_voigt_all_level(void* c, int f) { vr_all_level(*(Ctx*)c, f, true); }
+void ora_coarse_vr_nodal(void* c, int f) { coarse_vr_nodal(*(Ctx*)c, f); }
+void ora_dirichlet_bd(void* c, int l, int f, double Ut) {
+  Ctx* C = (Ctx*)c;
+  double* u[3] = {C->fld(f, l, 0), C->fld(f, l, 1), C->fld(f, l, 2)};
+  dirichlet_bd(*C, l, u, Ut);
+}
+void ora_dirichlet_residual(void* c, int l, int f) {
+  Ctx* C = (Ctx*)c;
+  double* r[3] = {C->fld(f, l, 0), C->fld(f, l, 1), C->fld(f, l, 2)};
+  dirichlet_residual_u(*C, l, r);
+}
+// transfers on arbitrary fields/components
+void ora_injection(void* c, int lcz, int f, int comp) {
+  Ctx* C = (Ctx*)c;
+  solution_injection(C->dims[lcz], C->fld(f, lcz, comp), C->dims[lcz + 1], C->fld(f, lcz + 1, comp));
+}
+void ora_restrict(void* c, int lf, int rf_field, int fic_field, int dst_field, int comp) {
+  Ctx* C = (Ctx*)c;
+  coarsen_f_standard(C->dims[lf - 1], C->fld(dst_field, lf - 1, comp), C->dims[lf],
+                     C->fld(rf_field, lf, comp), C->fld(fic_field, lf - 1, comp));
+}
+void ora_prolong(void* c, int lf, int uf_field, int uc_field, int ucold_field, int comp) {
+  Ctx* C = (Ctx*)c;
+  refine_standard(C->dims[lf], C->fld(uf_field, lf, comp), C->dims[lf - 1], C->fld(uc_field, lf - 1, comp),
+                  C->fld(ucold_field, lf - 1, comp));
+}
+
+int ora_pcg_u(void* c, int l, int NB, double Ut, int t, double* f_norm) {
+  return pcg_u(*(Ctx*)c, l, NB, Ut, t, *f_norm);
+}
+int ora_pcg_d(void* c, int l, int NB, double* fd_norm) { return pcg_d(*(Ctx*)c, l, NB, *fd_norm); }
+int ora_cycle_u(void* c, int l, double Ut, int t, double* f_norm) { return cycle_u(*(Ctx*)c, l, Ut, t, *f_norm); }
+int ora_cycle_d(void* c, int l, double* fd_norm) { return cycle_d(*(Ctx*)c, l, *fd_norm); }
+// Caller-side setup of u_entire kept separate so tests can drive the cycle
+void ora_mech_setup(void* c) {
+  Ctx* C = (Ctx*)c;
+  mp_choice(*C);
+  mp_all_element(*C);
+  vr_all_level(*C, F_KE, false);
+  vr_all_level(*C, F_GE, false);
+}
+int ora_u_entire(void* c, double Ut, int t, double* f_norm, int* cycles) {
+  Ctx* C = (Ctx*)c;
+  int rc = u_entire(*C, Ut, t, *f_norm);
+  if (cycles) *cycles = C->last_cycles_u;
+  return rc;
+}
+int ora_phase_field(void* c, int t, double* fd_norm, int* cycles) {
+  Ctx* C = (Ctx*)c;
+  int rc = phase_field(*C, t, *fd_norm);
+  if (cycles) *cycles = C->last_cycles_d;
+  return rc;
+}
+void ora_strain_history(void* c, double u_prop) { strain_history(*(Ctx*)c, u_prop); }
+double ora_U_e(void* c) { return U_e(*(Ctx*)c); }
+
+// externe_force_sum (mechanical_right_hnd.cpp:40-59) on fu_z, finest level
+double ora_externe_force_sum(void* c) {
+  Ctx* C = (Ctx*)c;
+  const int l = C->maxlevel;
+  const Dims& D = C->dims[l];
+  const double* f = C->fld(F_FU, l, 2);
+  double s = 0.0;
+  for (int j = 0; j < D.ny; j++)
+    for (int i = 0; i < D.nx; i++) s += f[D.at(i, j, D.nz - 1)];
+  return s;
+}
+// delta_d_max / d_max (phase_field_max_search.cpp:17-53); F_DPREV is d_old
+double ora_delta_d_max(void* c) {
+  Ctx* C = (Ctx*)c;
+  const int l = C->maxlevel;
+  const double* d = C->fld(F_D, l);
+  const double* o = C->fld(F_DPREV, l);
+  double m = 0.0;
+  for (size_t q = 0; q < C->dims[l].n(); q++)
+    if (m < d[q] - o[q]) m = d[q] - o[q];
+  return m;
+}
+double ora_d_max(void* c) {
+  Ctx* C = (Ctx*)c;
+  const int l = C->maxlevel;
+  const double* d = C->fld(F_D, l);
+  double m = 0.0;
+  for (size_t q = 0; q < C->dims[l].n(); q++)
+    if (m < d[q]) m = d[q];
+  return m;
+}
+void ora_save_d_old(void* c) {
+  Ctx* C = (Ctx*)c;
+  C->F[F_DPREV][C->maxlevel][0] = C->F[F_D][C->maxlevel][0];
+}
+
+// image_MP (image_input.cpp:120-142): gray levels -> K, G, gc on the finest
+// level through the CSV breakpoints (csv_reader.cpp:74-82), with the
+// reference's integer-division interpolation factor.
+int ora_image_mp(void* c, const unsigned char* img, const int* gray, int ngray, const double* E,
+                 const double* nu, const double* gcv, double voxel_size) {
+  Ctx* C = (Ctx*)c;
+  const int l = C->maxlevel;
+  const size_t n = C->dims[l].n();
+  double* K = C->fld(F_K, l);
+  double* G = C->fld(F_G, l);
+  double* gc = C->fld(F_GC, l);
+  for (size_t q = 0; q < n; q++) {
+    const int x = img[q];
+    int pos = -1;
+    for (int i = 0; i < ngray - 1; ++i)
+      if (x >= gray[i] && x <= gray[i + 1]) { pos = i; break; }
+    if (pos < 0) { set_err(*C, "gray level outside the material table"); return -1; }
+    const int fac = (x - gray[pos]) / (gray[pos + 1] - gray[pos]);
+    double e = E[pos] + fac * (E[pos + 1] - E[pos]);
+    double v = nu[pos] + fac * (nu[pos + 1] - nu[pos]);
+    gc[q] = gcv[pos] + fac * (gcv[pos + 1] - gcv[pos]);
+    e = voxel_size * e;
+    K[q] = 1.0 / (3.0 * (1.0 - 2.0 * v)) * e;
+    G[q] = 1.0 / (2.0 * (1.0 + v)) * e;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,320 @@
+"""Pins the oracle's transfers, coarse coefficients, smoothers, cycles and
+solves against independent matrix formulations.
+
+Reference routines pinned: solution_injection, coarsen_f_standard_routine,
+refine_standard_routine (multigrid_intergrid.cpp:25-520); coarse_voigt,
+coarse_reuss, VR_all_level, Voigt_all_level (material_coarse_element.cpp:20-118);
+voight, reuss, coarse_VR (image_input.cpp:144-273); pcg_u
+(mechanical_pcg.cpp:223-330); pcg_d (phase_field_pcg.cpp:133-214); cycle_u
+(mechanical_multigrid.cpp:18-54); cycle_d (phase_field_multigrid.cpp:18-44);
+u_entire (mechanical_displacement.cpp:25-115); phase_field (phase_field.cpp:15-57).
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle as O
+import textbook as T
+
+DOM = (0.0, 2.0, 0.0, 2.0, -1.0, 1.0)
+
+
+def vflat(a):
+    return np.asarray(a).reshape(-1)
+
+
+def make(maxlevel=1, enb=(2, 2, 2), dom=DOM, **kw):
+    return O.Oracle(maxlevel, enb, dom, **kw)
+
+
+def test_prolongation_is_trilinear_and_restriction_is_its_transpose():
+    o = make(1, (2, 3, 2), (0, 2, 0, 3, 0, 2))
+    dc, df = o.dims(0), o.dims(1)
+    P = T.prolongation(dc)
+    rng = np.random.default_rng(1)
+    uc = rng.standard_normal(dc[::-1])
+    o.set(O.P, 0, np.stack([uc] * 3))
+    o.set(O.UOLD, 0, np.zeros((3,) + dc[::-1]))
+    o.set(O.W, 1, np.zeros((3,) + df[::-1]))
+    o.prolong(1, O.W, O.P, O.UOLD, comp=0)
+    assert np.allclose(vflat(o.field(O.W, 1, 0)), P @ vflat(uc), rtol=1e-15, atol=1e-15)
+    rf = rng.standard_normal(df[::-1])
+    o.set(O.W, 1, np.stack([rf] * 3))
+    o.set(O.P, 0, np.zeros((3,) + dc[::-1]))
+    o.restrict(1, O.W, O.P, O.FU, comp=1)
+    assert np.allclose(vflat(o.field(O.FU, 0, 1)), P.T @ vflat(rf), rtol=1e-14, atol=1e-14)
+    # adjoint identity <R r, v> = <r, P v>
+    lhs = vflat(o.field(O.FU, 0, 1)) @ vflat(uc)
+    assert lhs == pytest.approx(vflat(rf) @ (P @ vflat(uc)), rel=1e-13)
+    o.set(O.W, 1, np.stack([rf] * 3))
+    o.injection(0, O.W, comp=2)
+    assert np.array_equal(vflat(o.field(O.W, 0, 2)), T.injection(dc) @ vflat(rf))
+
+
+def test_prolongation_reproduces_linear_fields():
+    o = make(1, (2, 2, 2))
+    dc, df = o.dims(0), o.dims(1)
+    zc, yc, xc = np.meshgrid(*[np.arange(n) * 2.0 for n in dc[::-1]], indexing="ij")
+    zf, yf, xf = np.meshgrid(*[np.arange(n) * 1.0 for n in df[::-1]], indexing="ij")
+    f = lambda x, y, z: 0.3 * x - 1.2 * y + 0.7 * z + 0.5
+    o.set(O.P, 0, np.stack([f(xc, yc, zc)] * 3))
+    o.set(O.UOLD, 0, np.zeros((3,) + dc[::-1]))
+    o.set(O.W, 1, np.zeros((3,) + df[::-1]))
+    o.prolong(1, O.W, O.P, O.UOLD)
+    assert np.allclose(o.field(O.W, 1, 0), f(xf, yf, zf), atol=1e-14)
+
+
+def test_element_vr_and_voigt_closed_form():
+    o = make(2, (1, 2, 1), (0, 1, 0, 2, 0, 1))
+    rng = np.random.default_rng(3)
+    df = o.dims(2)
+    Hf = rng.uniform(0.5, 4.0, df[::-1])
+    o.set(O.KE, 2, Hf)
+    o.vr_all_level(O.KE)
+    o.set(O.H, 2, Hf)
+    o.voigt_all_level(O.H)
+    d1 = o.dims(1)
+    blocks = Hf[:-1, :-1, :-1].reshape(d1[2] - 1, 2, d1[1] - 1, 2, d1[0] - 1, 2)
+    V = blocks.mean(axis=(1, 3, 5))
+    Rm = 1.0 / (1.0 / blocks).mean(axis=(1, 3, 5))
+    exp = 0.5 * (0.5 * (V + Rm) + np.sqrt(V * Rm))
+    assert np.allclose(o.field(O.KE, 1)[:-1, :-1, :-1], exp, rtol=1e-14)
+    assert np.all(o.field(O.KE, 1)[-1] == 0) and np.all(o.field(O.KE, 1)[:, -1] == 0)
+    assert np.allclose(o.field(O.H, 1)[:-1, :-1, :-1], V, rtol=1e-14)
+    assert np.all(V >= Rm * (1 - 1e-15))
+    # uniform fields are fixed points of both means
+    o.set(O.KE, 2, np.full(df[::-1], 2.5))
+    o.vr_all_level(O.KE)
+    assert np.allclose(o.field(O.KE, 0)[:-1, :-1, :-1], 2.5, rtol=1e-15)
+
+
+def test_nodal_coarse_vr_closed_form():
+    o = make(1, (2, 3, 2), (0, 2, 0, 3, 0, 2))
+    rng = np.random.default_rng(4)
+    df, dc = o.dims(1), o.dims(0)
+    M = rng.uniform(1.0, 9.0, df[::-1])
+    o.set(O.GC, 1, M)
+    o.coarse_vr_nodal(O.GC)
+    pad = np.pad(M, 1, mode="edge")
+    V = np.zeros(dc[::-1])
+    Rs = np.zeros(dc[::-1])
+    for dz in range(3):
+        for dy in range(3):
+            for dx in range(3):
+                win = pad[dz:dz + df[2]:2, dy:dy + df[1]:2, dx:dx + df[0]:2]
+                V += win / 27.0
+                Rs += 1.0 / win
+    R = 27.0 / Rs
+    assert np.allclose(o.field(O.GC, 0), 0.5 * (0.5 * (V + R) + np.sqrt(V * R)), rtol=1e-14)
+
+
+def rotation_mode(o, l):
+    nx, ny, nz = o.dims(l)
+    hx, hy, hz = o.h(l)
+    zz, yy, xx = np.meshgrid(DOM[4] + np.arange(nz) * hz, DOM[2] + np.arange(ny) * hy,
+                             DOM[0] + np.arange(nx) * hx, indexing="ij")
+    return np.stack([-(yy - DOM[2]), xx - DOM[1], np.zeros_like(xx)])
+
+
+def fine_mech(seed, maxlevel=1, enb=(2, 2, 2), dom=DOM, **kw):
+    o = make(maxlevel, enb, dom, **kw)
+    L = maxlevel
+    dims = o.dims(L)
+    K, G, d, fl = T.mesh(dims, seed, zero_frac=0.0, dscale=0.5)
+    o.set(O.K, L, K)
+    o.set(O.G, L, G)
+    o.set(O.D, L, d)
+    o.set(O.FLAG, L, fl)
+    return o, dims, K, G, d, fl
+
+
+def bc_info(o, l, Ut):
+    dims = o.dims(l)
+    h = o.h(l)
+    vals = T.dirichlet(dims, h, (DOM[0], DOM[2], DOM[4]), (DOM[1], DOM[3], DOM[5]), Ut)
+    n = 3 * int(np.prod(dims))
+    fixed = np.array(sorted(vals))
+    free = np.setdiff1d(np.arange(n), fixed)
+    return vals, fixed, free
+
+
+@pytest.mark.parametrize("nb", [1, 3, 7])
+def test_pcg_u_matches_textbook_jacobi_pcg(nb):
+    o, dims, K, G, d, fl = fine_mech(5)
+    L = 1
+    A = T.elastic_matrix(dims, o.h(L), T.fine_coefficients(dims, o.h(L), K, G, d, fl, 1e-5))
+    rng = np.random.default_rng(6)
+    u0 = rng.standard_normal((3,) + dims[::-1]) * 0.1
+    f = rng.standard_normal((3,) + dims[::-1])
+    o.set(O.U, L, u0)
+    o.set(O.FU, L, f)
+    Ut = 0.37
+    o.pcg_u(L, nb, Ut, t=1, f_norm=1.0)
+    vals, fixed, free = bc_info(o, L, Ut)
+    assert len(fixed) == 2 * 25 + 2  # bottom and top faces + 2 pins
+    x0 = vflat(u0).copy()
+    for q, v in vals.items():
+        x0[q] = v
+    x, r = T.jacobi_pcg(A, vflat(f), x0, nb, free)
+    assert np.allclose(vflat(o.vec(O.U, L)), x, rtol=0, atol=1e-12 * np.max(np.abs(x)))
+    assert np.allclose(vflat(o.vec(O.RU, L)), r, rtol=0, atol=1e-12 * np.max(np.abs(r)))
+
+
+def test_pcg_u_at_t0_sets_f_norm_from_dirichlet_reactions():
+    o, dims, K, G, d, fl = fine_mech(8)
+    L = 1
+    A = T.elastic_matrix(dims, o.h(L), T.fine_coefficients(dims, o.h(L), K, G, d, fl, 1e-5))
+    o.set(O.U, L, np.zeros((3,) + dims[::-1]))
+    o.set(O.FU, L, np.zeros((3,) + dims[::-1]))
+    fn = o.pcg_u(L, 2, 1.0, t=0, f_norm=0.0)
+    vals, fixed, free = bc_info(o, L, 1.0)
+    x0 = np.zeros(3 * int(np.prod(dims)))
+    for q, v in vals.items():
+        x0[q] = v
+    react = (A @ x0)[fixed]
+    assert fn == pytest.approx(np.sum(react**2), rel=1e-12)
+    assert np.allclose(vflat(o.vec(O.FU, L))[fixed], react, rtol=1e-12)
+
+
+@pytest.mark.parametrize("nb", [1, 4])
+def test_pcg_d_matches_textbook(nb):
+    o = make(0, (3, 2, 2), (0, 3, 0, 2, 0, 2), lc=1.5)
+    dims = o.dims(0)
+    rng = np.random.default_rng(12)
+    Hn = rng.uniform(0, 2, dims[::-1])
+    gc = rng.uniform(0.5, 2, dims[::-1])
+    d0 = rng.uniform(0, 0.3, dims[::-1])
+    fd = rng.standard_normal(dims[::-1])
+    for f, a in ((O.H, Hn), (O.GC, gc), (O.D, d0), (O.FD, fd)):
+        o.set(f, 0, a)
+    fdn = o.pcg_d(0, nb)
+    assert fdn == pytest.approx(np.sum(fd**2), rel=1e-14)
+    A = T.phase_matrix(dims, o.h(0), Hn, gc, 1.5)
+    x, r = T.jacobi_pcg(A, vflat(fd), vflat(d0).copy(), nb)
+    assert np.allclose(vflat(o.field(O.D, 0)), x, atol=1e-12 * np.max(np.abs(x)))
+    assert np.allclose(vflat(o.field(O.RD, 0)), r, atol=1e-12 * np.max(np.abs(r)))
+
+
+def textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, l):
+    """FAS gamma=1 cycle in matrix form (injection, P, P^T, Jacobi-PCG)."""
+    nu0, nu1, nu2 = nu
+    A = mats[l]
+    vals, free = bcs[l]
+    def setbc(x):
+        for q, v in vals.items():
+            x[q] = v
+    if l == 0:
+        setbc(u[0])
+        u[0], r = T.jacobi_pcg(A, f[0], u[0], nu0, free)
+        return r
+    setbc(u[l])
+    u[l], r = T.jacobi_pcg(A, f[l], u[l], nu1, free)
+    dc = o.dims(l - 1)
+    Inj = T.injection(dc)
+    P = T.prolongation(dc)
+    uc = np.concatenate([Inj @ u[l].reshape(3, -1)[c] for c in range(3)])
+    setbc_c = bcs[l - 1][0]
+    for q, v in setbc_c.items():
+        uc[q] = v
+    uold = uc.copy()
+    u[l - 1] = uc
+    rr = r.reshape(3, -1)
+    f[l - 1] = mats[l - 1] @ uc + np.concatenate([P.T @ rr[c] for c in range(3)])
+    textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, l - 1)
+    corr = (u[l - 1] - uold).reshape(3, -1)
+    u[l] = u[l] + np.concatenate([P @ corr[c] for c in range(3)])
+    setbc(u[l])
+    u[l], r = T.jacobi_pcg(A, f[l], u[l], nu2, free)
+    return r
+
+
+def test_cycle_u_matches_textbook_fas_cycle():
+    o, dims, K, G, d, fl = fine_mech(21, maxlevel=2, enb=(2, 2, 2))
+    L = 2
+    rng = np.random.default_rng(22)
+    u0 = rng.standard_normal((3,) + dims[::-1]) * 0.05
+    o.set(O.U, L, u0)
+    o.mech_setup()  # split flag from u0, element means, coarse VR
+    fl = o.field(O.FLAG, L).copy()
+    mats, bcs = {}, {}
+    Ut = 0.25
+    for l in range(L + 1):
+        dl = o.dims(l)
+        if l == L:
+            coef = T.fine_coefficients(dl, o.h(l), K, G, d, fl, 1e-5)
+        else:
+            coef = T.coarse_coefficients(dl, o.field(O.KE, l), o.field(O.GE, l))
+        mats[l] = T.elastic_matrix(dl, o.h(l), coef)
+        vals, fixed, free = bc_info(o, l, Ut)
+        bcs[l] = (vals, free)
+    f = {L: np.zeros(3 * int(np.prod(dims)))}
+    u = {L: vflat(u0).copy()}
+    o.set(O.FU, L, np.zeros((3,) + dims[::-1]))
+    r_tb = textbook_cycle_u(o, Ut, (50, 4, 2), mats, bcs, u, f, L)
+    o.cycle_u(L, Ut, t=1, f_norm=1.0)
+    scale = np.max(np.abs(u[L]))
+    assert np.max(np.abs(vflat(o.vec(O.U, L)) - u[L])) <= 1e-10 * scale
+    assert np.max(np.abs(vflat(o.vec(O.RU, L)) - r_tb)) <= 1e-9 * np.max(np.abs(r_tb))
+
+
+def test_u_entire_converges_to_direct_solve():
+    o, dims, K, G, d, fl = fine_mech(31, maxlevel=2, enb=(2, 2, 2), tol_u=1e-26,
+                                     mg_u=(50, 4, 2, 1, 40))
+    L = 2
+    o.set(O.U, L, np.zeros((3,) + dims[::-1]))
+    fnorm, cycles = o.u_entire(1.0, 0, 0.0)
+    assert 1 <= cycles <= 40
+    # at t=0 from u=0 every split flag is 0
+    assert np.all(o.field(O.FLAG, L)[:-1, :-1, :-1] == 0)
+    A = T.elastic_matrix(dims, o.h(L), T.fine_coefficients(dims, o.h(L), K, G, d, np.zeros(dims[::-1]), 1e-5))
+    vals, fixed, free = bc_info(o, L, 1.0)
+    n = A.shape[0]
+    uc = np.zeros(n)
+    for q, v in vals.items():
+        uc[q] = v
+    rhs = -(A @ uc)[free]
+    got = vflat(o.vec(O.U, L))
+    # The reference's pins (u_x at (x_mid, y_a), u_y at (x_b, y_mid)) leave
+    # one zero-energy mode: the rotation about the vertical axis through the
+    # corner (x_b, y_a).  The direct solve fixes it with a rank-one penalty
+    # and solutions are compared modulo that mode.
+    m = rotation_mode(o, L).reshape(-1)
+    assert np.linalg.norm(A @ m) <= 1e-12 * np.linalg.norm(A.diagonal()) * np.linalg.norm(m)
+    Aff = A[free][:, free].toarray()
+    mf = m[free]
+    Aff += np.mean(np.diag(Aff)) * np.outer(mf, mf) / (mf @ mf)
+    x = uc.copy()
+    x[free] = np.linalg.solve(Aff, rhs)
+    e = got - x
+    e -= (e @ m) / (m @ m) * m
+    # the FAS cycle stagnates near a squared relative residual of 1e-19 here,
+    # so the error bound follows from the true residual of the iterate
+    res = np.linalg.norm((A @ got)[free])
+    assert res <= 2e-9 * np.sqrt(fnorm)
+    assert np.linalg.norm(e) <= 1e-7 * np.linalg.norm(x)
+    # f_norm is the squared reaction norm on the Dirichlet DOFs
+    assert fnorm == pytest.approx(np.sum((A @ x)[fixed] ** 2), rel=1e-8)
+    # externe_force_nodal stores A u on the Dirichlet DOFs
+    assert np.allclose(vflat(o.vec(O.FU, L))[fixed], (A @ got)[fixed], rtol=1e-10)
+    hist = o.history("u")
+    assert hist[-1] / fnorm <= 1e-17
+
+
+def test_phase_field_converges_to_direct_solve():
+    o = make(2, (2, 2, 2), DOM, lc=1.5, mg_d=(10, 2, 1, 1, 40), tol_d_rel=1e-24, tol_d_abs=1e-30)
+    L = 2
+    dims = o.dims(L)
+    rng = np.random.default_rng(41)
+    Hf = rng.uniform(0, 3, dims[::-1])
+    gcs = [rng.uniform(0.5, 2, o.dims(l)[::-1]) for l in range(L + 1)]
+    o.set(O.H, L, Hf)
+    o.voigt_all_level(O.H)
+    o.set(O.GC, L, gcs[L])
+    o.coarse_vr_nodal(O.GC)
+    o.fd_h(L)
+    fdn, cycles = o.phase_field(0, 0.0)
+    A = T.phase_matrix(dims, o.h(L), Hf, gcs[L], 1.5)
+    x = spla.spsolve(A.tocsc(), vflat(o.field(O.FD, L)))
+    assert np.linalg.norm(vflat(o.field(O.D, L)) - x) <= 1e-10 * np.linalg.norm(x)
+    assert fdn == pytest.approx(np.sum(o.field(O.FD, L) ** 2), rel=1e-14)
+    assert 1 <= cycles < 40
