@@ -1,0 +1,136 @@
+// Internal declarations shared by the kernels (hpcr_kernels.hip) and the host
+// orchestration (hpcr_solver.hip).  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace hpcr {
+
+// Element-tile geometry of the operator kernels: 16x16 elements per layer,
+// one element per thread, marching in z; a block owns the 15x15 interior
+// node columns of its tile (neighbouring tiles overlap by one element).
+constexpr int TILE = 16;
+constexpr int OWN = TILE - 1;
+constexpr int TPB = TILE * TILE;        // 256 threads
+constexpr int PLANE = (TILE + 1) * (TILE + 1);  // 289 nodes per LDS plane
+constexpr int ZCHUNK = 16;              // node planes per block
+
+// Device scalar slots (one set per context, reused by every level: the
+// smoothers of different levels never overlap in time).
+enum Scal {
+  S_RZ = 0,    // r.z of the current CG iteration
+  S_PW = 1,    // p.Ap
+  S_ALPHA = 2,
+  S_BETA = 3,
+  S_RR = 4,    // r.r after the last update
+  S_FF = 5,    // f.f (f_norm at t == 0)
+  S_RED = 6,   // generic reduction result (min / max / sum)
+  S_ZERO = 7,  // count of zero diagonal entries
+  S_COUNT = 8
+};
+
+struct LevelGeom {
+  int nx, ny, nz;
+  double hx, hy, hz;
+  const unsigned char* fx;  // Dirichlet axis flags: bit0 x == x_mid, bit1 x == x_b
+  const unsigned char* fy;  // bit0 y == y_a,   bit1 y == y_mid
+  const unsigned char* fz;  // bit0 z == z_a,   bit1 z == z_b
+};
+
+struct Reduce {
+  double* partials;     // >= blocks * 4 doubles
+  unsigned int* counter; // zero between kernels
+  double* scal;          // S_COUNT doubles
+};
+
+// Mechanical coefficients of one level.
+struct MechCoef {
+  int fine;                   // 1: nodal K, G, d + split flag; 0: element K_e, G_e
+  const double* K;
+  const double* G;
+  const double* d;
+  const signed char* flag;
+  const double* Ke;
+  const double* Ge;
+  double small_k;
+};
+
+// Phase-field coefficients of one level.
+struct PfCoef {
+  const double* H;   // element-indexed
+  const double* gc;  // nodal
+  double lc;
+};
+
+// ---- launchers (hpcr_kernels.hip); all asynchronous on `s` -------------
+// y = A x  (3 comps, stride N)
+void launch_mech_apply(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, double* y);
+// r = f - A u, masked; at t0fine also f := A u on Dirichlet DOFs; reductions
+// r.z (with z = r*invD), r.r and f.f into scal.
+void launch_mech_resid(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* u, double* f,
+                       double* r, const double* invD, int t0fine, const Reduce& R);
+// p_new = r*invD + beta p_old (first: p_new = r*invD); w = A p_new; p.w -> alpha
+void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* r,
+                       const double* invD, const double* p_old, double* p_new, double* w, int first,
+                       const Reduce& R);
+// u += alpha p; r -= alpha w; mask r; r.z, r.r -> beta, rz, rr
+void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, const double* p,
+                       const double* w, const double* invD, const Reduce& R);
+// invD = 1 / diag(A); zero entries counted into scal[S_ZERO]; diag optionally kept
+void launch_mech_diag(hipStream_t s, const LevelGeom& L, const MechCoef& C, double* diag_or_null,
+                      double* invD, const Reduce& R);
+
+void launch_pf_apply(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* x, double* y);
+// R = fd - A d; r.z, r.r, fd.fd
+void launch_pf_resid(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* d, const double* fd,
+                     double* r, const double* invD, const Reduce& R);
+void launch_pf_pcg_a(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* r, const double* invD,
+                     const double* p_old, double* p_new, double* w, int first, const Reduce& R);
+void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, const double* p, const double* w,
+                     const double* invD, const Reduce& R);
+void launch_pf_diag(hipStream_t s, const LevelGeom& L, const PfCoef& C, double* diag_or_null, double* invD,
+                    const Reduce& R);
+
+// Dirichlet_BD on planes with z flags; `kplanes` lists the flagged k indices.
+void launch_dirichlet_bd(hipStream_t s, const LevelGeom& L, double* u, double Ut, const int* kplanes_host,
+                         int nplanes);
+// f := Fi on Dirichlet DOFs
+void launch_dirichlet_copy(hipStream_t s, const LevelGeom& L, double* f, const double* Fi,
+                           const int* kplanes_host, int nplanes);
+// injection u_c = u_f(2i), optional Dirichlet values, copy into uold
+void launch_inject(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* uf,
+                   double* uc, double* ucold, int apply_bc, double Ut);
+// f_c = Fic + full weighting of r_f
+void launch_restrict(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* rf,
+                     const double* Fic, double* fc);
+// u_f += P(u_c - u_c,old), optional Dirichlet values
+void launch_prolong(hipStream_t s, const LevelGeom& Lf, const LevelGeom& Lc, int ncomp, double* uf,
+                    const double* uc, const double* ucold, int apply_bc, double Ut);
+
+// split flag + element means on the finest level
+void launch_flag_means(hipStream_t s, const LevelGeom& L, const double* u, const double* K, const double* G,
+                       const double* d, signed char* flag, double* Ke, double* Ge, double small_k,
+                       int compute_flag);
+// element VR (or Voigt only) coarsening from level f to level c (whole grid)
+void launch_elem_vr(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, const double* Hf, double* Hc,
+                    int voigt_only);
+// nodal 27-point VR coarsening
+void launch_nodal_vr(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, const double* Mf, double* Mc);
+void launch_fd_h(hipStream_t s, const LevelGeom& L, const double* H, double* fd);
+void launch_strain_history(hipStream_t s, const LevelGeom& L, const double* u, const double* K,
+                           const double* G, const double* gc, double* H, double lc, double u_prop);
+void launch_elastic_limit(hipStream_t s, const LevelGeom& L, const double* u, const double* K, const double* G,
+                          const double* gc, double lc, const Reduce& R);
+// reductions into scal[S_RED]
+void launch_top_sum(hipStream_t s, const LevelGeom& L, const double* fz, const Reduce& R);
+void launch_max_diff(hipStream_t s, size_t n, const double* a, const double* b_or_null, const Reduce& R);
+
+// squared norm into scal[S_RR]
+void launch_norm2(hipStream_t s, size_t n, const double* a, const Reduce& R);
+// flag conversions
+void launch_flag_to_double(hipStream_t s, size_t n, const signed char* f, double* out);
+void launch_double_to_flag(hipStream_t s, size_t n, const double* in, signed char* f);
+void launch_fill(hipStream_t s, size_t n, double* p, double v);
+
+}  // namespace hpcr

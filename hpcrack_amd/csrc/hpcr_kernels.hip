@@ -1,0 +1,1480 @@
+// HIP kernels for gfx950 (MI355X): matrix-free trilinear-hex operators of
+// HPCrack's degraded elasticity and AT2 phase field, the fused Jacobi-PCG
+// passes, multigrid transfers, coarse coefficients and Gauss-point updates.
+//
+// Operator formulation.  On a uniform voxel grid the gradient of a trilinear
+// field at the 2x2x2 Gauss points is sum-factorised: d/dx at Gauss point
+// (gx,gy,gz) does not depend on gx and equals the x-edge differences of the
+// element interpolated bilinearly in y and z.  The transposed contraction
+// B^T sigma is factorised the same way.  This needs ~1.2 kflop per element
+// instead of ~8 kflop for the reference's per-Gauss-point loops
+// (mechanical_pcg.cpp:114-172), and gives the same operator up to rounding.
+//
+// Tiling.  A block of 256 threads owns a 16x16 column of elements (one per
+// thread) and marches up a chunk of z-layers.  Each step loads one node plane
+// (u and, on the finest level, K, G, d) into LDS, evaluates one element layer,
+// and exchanges the 8 corner contributions of every element through LDS so
+// that each node sums its 4 in-plane neighbours in a fixed order.  The node
+// plane below the layer is then complete and handed to the fused epilogue
+// (residual, PCG update, diagonal ...); the plane above is carried in
+// registers.  No atomics: results are deterministic.  Tiles overlap by one
+// element so that the 15x15 interior node columns are owned by one block.
+//
+// Reductions.  Every block reduces its partial sums with wavefront shuffles
+// in a fixed tree, writes them to a partials array, and the last block to
+// finish (ticket counter) reduces the partials in fixed order and updates the
+// device scalars (alpha, beta, r.z ...).  Deterministic, one launch per pass.
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+
+#include "hpcr_internal.h"
+
+namespace hpcr {
+
+// L_i(g): 1-D linear shape function of node i at Gauss point g (xi = -+1/sqrt3)
+#define GA 0.78867513459481288225  // (1 + 1/sqrt(3)) / 2
+#define GB 0.21132486540518711775  // (1 - 1/sqrt(3)) / 2
+
+__device__ __forceinline__ double Lw(int i, int g) { return (i == g) ? GA : GB; }
+
+__device__ __forceinline__ size_t nidx(const LevelGeom& L, int i, int j, int k) {
+  return (size_t)i + (size_t)L.nx * ((size_t)j + (size_t)L.ny * (size_t)k);
+}
+__device__ __forceinline__ size_t Nof(const LevelGeom& L) { return (size_t)L.nx * L.ny * L.nz; }
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+enum RedOp { RED_SUM = 0, RED_MIN = 1, RED_MAX = 2 };
+
+template <int OP>
+__device__ __forceinline__ double red2(double a, double b) {
+  if (OP == RED_SUM) return a + b;
+  if (OP == RED_MIN) return (b < a) ? b : a;
+  return (b > a) ? b : a;
+}
+
+template <int OP>
+__device__ __forceinline__ double wave_red(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = red2<OP>(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// block of blockDim.x (multiple of 64, <= 1024) threads -> result in thread 0
+template <int OP, int NV>
+__device__ __forceinline__ void block_red(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_red<OP>(v[k]);
+  if (lane == 0)
+    for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; k++) {
+      double a = sh[k * 16];
+      for (int w = 1; w < nw; w++) a = red2<OP>(a, sh[k * 16 + w]);
+      v[k] = a;
+    }
+}
+
+__device__ __forceinline__ unsigned lin_block() {
+  return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+__device__ __forceinline__ unsigned num_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
+
+// Block partials -> last block -> fin(result[NV]) run by thread 0 of the last block.
+template <int OP, int NV, class Fin>
+__device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R, double init, Fin fin) {
+  __shared__ double sh[NV * 16];
+  __shared__ int is_last;
+  block_red<OP, NV>(v, sh);
+  const unsigned nb = num_blocks();
+  if (threadIdx.x == 0) {
+    const unsigned b = lin_block();
+    for (int k = 0; k < NV; k++)
+      __hip_atomic_store(&R.partials[(size_t)b * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned t = atomicAdd(R.counter, 1u);
+    is_last = (t == nb - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double acc[NV];
+  for (int k = 0; k < NV; k++) acc[k] = init;
+  for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
+    for (int k = 0; k < NV; k++)
+      acc[k] = red2<OP>(acc[k], __hip_atomic_load(&R.partials[(size_t)i * NV + k], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT));
+  block_red<OP, NV>(acc, sh);
+  if (threadIdx.x == 0) {
+    fin(acc);
+    __hip_atomic_store(R.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element algebra (sum factorisation)
+// ---------------------------------------------------------------------------
+// trilinear interpolation of 8 nodal values v[iz][iy][ix] to Gauss points o[gz][gy][gx]
+__device__ __forceinline__ void interp8(const double (&v)[2][2][2], double (&o)[2][2][2]) {
+  double t[2][2][2], s[2][2][2];
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int iy = 0; iy < 2; iy++) {
+      t[iz][iy][0] = GA * v[iz][iy][0] + GB * v[iz][iy][1];
+      t[iz][iy][1] = GB * v[iz][iy][0] + GA * v[iz][iy][1];
+    }
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) {
+      s[iz][0][gx] = GA * t[iz][0][gx] + GB * t[iz][1][gx];
+      s[iz][1][gx] = GB * t[iz][0][gx] + GA * t[iz][1][gx];
+    }
+#pragma unroll
+  for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) {
+      o[0][gy][gx] = GA * s[0][gy][gx] + GB * s[1][gy][gx];
+      o[1][gy][gx] = GB * s[0][gy][gx] + GA * s[1][gy][gx];
+    }
+}
+// transpose of interp8: nodal f[iz][iy][ix] += sum_g L L L q[gz][gy][gx]
+__device__ __forceinline__ void interp8_t_add(const double (&q)[2][2][2], double (&f)[2][2][2]) {
+  double s[2][2][2], t[2][2][2];
+#pragma unroll
+  for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) {
+      s[0][gy][gx] = GA * q[0][gy][gx] + GB * q[1][gy][gx];
+      s[1][gy][gx] = GB * q[0][gy][gx] + GA * q[1][gy][gx];
+    }
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) {
+      t[iz][0][gx] = GA * s[iz][0][gx] + GB * s[iz][1][gx];
+      t[iz][1][gx] = GB * s[iz][0][gx] + GA * s[iz][1][gx];
+    }
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int iy = 0; iy < 2; iy++) {
+      f[iz][iy][0] += GA * t[iz][iy][0] + GB * t[iz][iy][1];
+      f[iz][iy][1] += GB * t[iz][iy][0] + GA * t[iz][iy][1];
+    }
+}
+// bilinear interpolation of 4 edge values e[a][b] (a: outer axis, b: inner
+// axis) to Gauss pairs o[ga][gb], with the inner stage scaled by s
+__device__ __forceinline__ void interp4(const double (&e)[2][2], double s, double (&o)[2][2]) {
+  double t0 = s * (GA * e[0][0] + GB * e[0][1]);
+  double t1 = s * (GB * e[0][0] + GA * e[0][1]);
+  double u0 = s * (GA * e[1][0] + GB * e[1][1]);
+  double u1 = s * (GB * e[1][0] + GA * e[1][1]);
+  o[0][0] = GA * t0 + GB * u0;
+  o[0][1] = GA * t1 + GB * u1;
+  o[1][0] = GB * t0 + GA * u0;
+  o[1][1] = GB * t1 + GA * u1;
+}
+// transpose of interp4
+__device__ __forceinline__ void interp4_t(const double (&S)[2][2], double s, double (&T)[2][2]) {
+  double q00 = GA * S[0][0] + GB * S[1][0];  // a=0, gb=0
+  double q01 = GA * S[0][1] + GB * S[1][1];
+  double q10 = GB * S[0][0] + GA * S[1][0];  // a=1
+  double q11 = GB * S[0][1] + GA * S[1][1];
+  T[0][0] = s * (GA * q00 + GB * q01);
+  T[0][1] = s * (GB * q00 + GA * q01);
+  T[1][0] = s * (GA * q10 + GB * q11);
+  T[1][1] = s * (GB * q10 + GA * q11);
+}
+
+// Gradient of a nodal scalar v[iz][iy][ix] at the Gauss points:
+// gxv[gz][gy] (d/dx), gyv[gz][gx] (d/dy), gzv[gy][gx] (d/dz)
+__device__ __forceinline__ void grad8(const double (&v)[2][2][2], double sx, double sy, double sz,
+                                      double (&gxv)[2][2], double (&gyv)[2][2], double (&gzv)[2][2]) {
+  double ex[2][2], ey[2][2], ez[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      ex[a][b] = v[a][b][1] - v[a][b][0];  // [iz][iy]
+      ey[a][b] = v[a][1][b] - v[a][0][b];  // [iz][ix]
+      ez[a][b] = v[1][a][b] - v[0][a][b];  // [iy][ix]
+    }
+  interp4(ex, sx, gxv);
+  interp4(ey, sy, gyv);
+  interp4(ez, sz, gzv);
+}
+// Transposed gradient: f[iz][iy][ix] += sum_g dN/dx_j * S_j
+__device__ __forceinline__ void grad8_t_add(const double (&Sx)[2][2], const double (&Sy)[2][2],
+                                            const double (&Sz)[2][2], double sx, double sy, double sz,
+                                            double (&f)[2][2][2]) {
+  double T[2][2];
+  interp4_t(Sx, sx, T);  // T[iz][iy]
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      f[a][b][1] += T[a][b];
+      f[a][b][0] -= T[a][b];
+    }
+  interp4_t(Sy, sy, T);  // T[iz][ix]
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      f[a][1][b] += T[a][b];
+      f[a][0][b] -= T[a][b];
+    }
+  interp4_t(Sz, sz, T);  // T[iy][ix]
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      f[1][a][b] += T[a][b];
+      f[0][a][b] -= T[a][b];
+    }
+}
+
+// gauss_point_MP (material_gauss_point.cpp:18-41): zero if any nodal value is 0
+__device__ __forceinline__ void gp_mp(const double (&v)[2][2][2], double (&o)[2][2][2]) {
+  bool z = false;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) z |= (v[a][b][c] == 0.0);
+  interp8(v, o);
+  if (z) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) o[a][b][c] = 0.0;
+  }
+}
+// (1-k)(1-d)^2 + k with d clamped to [0,1] (material_gauss_point.cpp:50-56)
+__device__ __forceinline__ double degr(double dg, double k) {
+  dg = dg < 0.0 ? 0.0 : (dg > 1.0 ? 1.0 : dg);
+  return (1.0 - k) * (1.0 - dg) * (1.0 - dg) + k;
+}
+
+// Gauss-point Lame coefficients (lambda = K_d - 2/3 G_d, mu = G_d) of a fine
+// element (MP_all_GAUSS_Point, mechanical_pcg.cpp:26-61)
+__device__ __forceinline__ void fine_lame(const double (&Kn)[2][2][2], const double (&Gn)[2][2][2],
+                                          const double (&dn)[2][2][2], int flag, double small_k,
+                                          double (&lam)[2][2][2], double (&mu)[2][2][2]) {
+  double Kg[2][2][2], Gg[2][2][2], dg[2][2][2];
+  gp_mp(Kn, Kg);
+  gp_mp(Gn, Gg);
+  gp_mp(dn, dg);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const double g = degr(dg[a][b][c], small_k);
+        const double Kd = (flag == 0) ? g * Kg[a][b][c] : Kg[a][b][c];
+        const double Gd = (flag == 2) ? Gg[a][b][c] : g * Gg[a][b][c];
+        lam[a][b][c] = Kd - (2.0 / 3.0) * Gd;
+        mu[a][b][c] = Gd;
+      }
+}
+
+// y_e = K_e u_e for one element: u[c][iz][iy][ix] -> f[c][iz][iy][ix]
+// (Internal_Force_element, mechanical_pcg.cpp:114-172)
+template <bool CONST_COEF>
+__device__ __forceinline__ void mech_element(const double (&u)[3][2][2][2], const double (&lam)[2][2][2],
+                                             const double (&mu)[2][2][2], double lam0, double mu0,
+                                             double sx, double sy, double sz, double tx, double ty, double tz,
+                                             double (&f)[3][2][2][2]) {
+  double dx[3][2][2], dy[3][2][2], dz[3][2][2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) grad8(u[c], sx, sy, sz, dx[c], dy[c], dz[c]);
+  double Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) Sx[c][a][b] = Sy[c][a][b] = Sz[c][a][b] = 0.0;
+#pragma unroll
+  for (int gz = 0; gz < 2; gz++)
+#pragma unroll
+    for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+      for (int gx = 0; gx < 2; gx++) {
+        const double exx = dx[0][gz][gy], eyy = dy[1][gz][gx], ezz = dz[2][gy][gx];
+        const double gxy = dx[1][gz][gy] + dy[0][gz][gx];
+        const double gxz = dx[2][gz][gy] + dz[0][gy][gx];
+        const double gyz = dy[2][gz][gx] + dz[1][gy][gx];
+        const double tr = exx + eyy + ezz;
+        const double l = CONST_COEF ? lam0 : lam[gz][gy][gx];
+        const double m = CONST_COEF ? mu0 : mu[gz][gy][gx];
+        const double lt = l * tr, m2 = 2.0 * m;
+        const double sxx = lt + m2 * exx, syy = lt + m2 * eyy, szz = lt + m2 * ezz;
+        const double sxy = m * gxy, sxz = m * gxz, syz = m * gyz;
+        Sx[0][gz][gy] += sxx; Sx[1][gz][gy] += sxy; Sx[2][gz][gy] += sxz;
+        Sy[0][gz][gx] += sxy; Sy[1][gz][gx] += syy; Sy[2][gz][gx] += syz;
+        Sz[0][gy][gx] += sxz; Sz[1][gy][gx] += syz; Sz[2][gy][gx] += szz;
+      }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = 0.0;
+    grad8_t_add(Sx[c], Sy[c], Sz[c], tx, ty, tz, f[c]);
+  }
+}
+
+// Jacobi diagonal of one element (PreM_u_element, mechanical_pcg.cpp:63-100)
+__device__ __forceinline__ void mech_element_diag(const double (&lam)[2][2][2], const double (&mu)[2][2][2],
+                                                  double ihx2, double ihy2, double ihz2, double wg,
+                                                  double (&f)[3][2][2][2]) {
+#pragma unroll
+  for (int ka = 0; ka < 2; ka++)
+#pragma unroll
+    for (int ja = 0; ja < 2; ja++)
+#pragma unroll
+      for (int ia = 0; ia < 2; ia++) {
+        double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int gz = 0; gz < 2; gz++)
+#pragma unroll
+          for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+            for (int gx = 0; gx < 2; gx++) {
+              const double lx = Lw(ia, gx), ly = Lw(ja, gy), lz = Lw(ka, gz);
+              const double X = ihx2 * ly * ly * lz * lz;
+              const double Y = ihy2 * lx * lx * lz * lz;
+              const double Z = ihz2 * lx * lx * ly * ly;
+              const double l = lam[gz][gy][gx], m = mu[gz][gy][gx];
+              const double p = l + 2.0 * m;
+              s0 += p * X + m * (Y + Z);
+              s1 += p * Y + m * (X + Z);
+              s2 += p * Z + m * (X + Y);
+            }
+        f[0][ka][ja][ia] = wg * s0;
+        f[1][ka][ja][ia] = wg * s1;
+        f[2][ka][ja][ia] = wg * s2;
+      }
+}
+
+// Phase-field element (Fld_element, phase_field_pcg.cpp:61-91)
+__device__ __forceinline__ void pf_coefs(const double (&gcn)[2][2][2], double He, double lc, double wg,
+                                         double (&m)[2][2][2], double (&k)[2][2][2]) {
+  double gg[2][2][2];
+  gp_mp(gcn, gg);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        m[a][b][c] = wg * (2.0 * He + gg[a][b][c] / lc);
+        k[a][b][c] = wg * gg[a][b][c] * lc;
+      }
+}
+__device__ __forceinline__ void pf_element(const double (&v)[2][2][2], const double (&m)[2][2][2],
+                                           const double (&k)[2][2][2], double sx, double sy, double sz,
+                                           double (&f)[2][2][2]) {
+  double dv[2][2][2];
+  interp8(v, dv);
+  double q[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        q[a][b][c] = m[a][b][c] * dv[a][b][c];
+        f[a][b][c] = 0.0;
+      }
+  interp8_t_add(q, f);
+  double gxv[2][2], gyv[2][2], gzv[2][2];
+  grad8(v, sx, sy, sz, gxv, gyv, gzv);
+  double Sx[2][2], Sy[2][2], Sz[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      Sx[a][b] = gxv[a][b] * (k[a][b][0] + k[a][b][1]);  // [gz][gy], sum over gx
+      Sy[a][b] = gyv[a][b] * (k[a][0][b] + k[a][1][b]);  // [gz][gx], sum over gy
+      Sz[a][b] = gzv[a][b] * (k[0][a][b] + k[1][a][b]);  // [gy][gx], sum over gz
+    }
+  grad8_t_add(Sx, Sy, Sz, sx, sy, sz, f);
+}
+// Md_element (phase_field_pcg.cpp:20-47)
+__device__ __forceinline__ void pf_element_diag(const double (&m)[2][2][2], const double (&k)[2][2][2],
+                                                double ihx2, double ihy2, double ihz2, double (&f)[2][2][2]) {
+#pragma unroll
+  for (int ka = 0; ka < 2; ka++)
+#pragma unroll
+    for (int ja = 0; ja < 2; ja++)
+#pragma unroll
+      for (int ia = 0; ia < 2; ia++) {
+        double s = 0;
+#pragma unroll
+        for (int gz = 0; gz < 2; gz++)
+#pragma unroll
+          for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+            for (int gx = 0; gx < 2; gx++) {
+              const double lx = Lw(ia, gx), ly = Lw(ja, gy), lz = Lw(ka, gz);
+              const double N2 = lx * lx * ly * ly * lz * lz;
+              const double G2 = ihx2 * ly * ly * lz * lz + ihy2 * lx * lx * lz * lz + ihz2 * lx * lx * ly * ly;
+              s += m[gz][gy][gx] * N2 + k[gz][gy][gx] * G2;
+            }
+        f[ka][ja][ia] = s;
+      }
+}
+
+// Dirichlet DOF test of the reference's boundary conditions
+// (boundary_conditions.cpp:28-66): returns a mask, bit c set when component
+// c is prescribed; bit 3 marks u_z = Ut (top face) rather than 0.
+__device__ __forceinline__ int dir_mask(const LevelGeom& L, int i, int j, int k) {
+  const unsigned fz = L.fz[k];
+  if (fz == 0) return 0;
+  int m = 0;
+  const unsigned fx = L.fx[i], fy = L.fy[j];
+  if ((fz & 1) && (fx & 1) && (fy & 1)) m |= 1;
+  if ((fz & 1) && (fx & 2) && (fy & 2)) m |= 2;
+  if (fz & 1) m |= 4;
+  if (fz & 2) m |= 4 | 8;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernel framework
+// ---------------------------------------------------------------------------
+template <class Op>
+__global__ __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op op) {
+  constexpr int NC = Op::NC;
+  constexpr int NV = Op::NV;
+  __shared__ double sN[2][NV][PLANE];
+  __shared__ double sOut[4 * NC][TPB];
+  const int tid = threadIdx.x;
+  const int tx = tid % TILE, ty = tid / TILE;
+  const int bx0 = blockIdx.x * OWN, by0 = blockIdx.y * OWN;
+  const int z0 = blockIdx.z * ZCHUNK;
+  const int zend = min(z0 + ZCHUNK, L.nz);
+  const int ex = bx0 - 1 + tx, ey = by0 - 1 + ty;
+  const bool owner = (tx >= 1) && (ty >= 1) && (ex < L.nx) && (ey < L.ny);
+  const bool exy = (ex >= 0) && (ey >= 0) && (ex <= L.nx - 2) && (ey <= L.ny - 2);
+
+  auto load_plane = [&](int pz, int slot) {
+    for (int q = tid; q < PLANE; q += TPB) {
+      const int lx = q % (TILE + 1), ly = q / (TILE + 1);
+      const int gx = bx0 - 1 + lx, gy = by0 - 1 + ly;
+      const bool ok = gx >= 0 && gy >= 0 && gx < L.nx && gy < L.ny && pz >= 0 && pz < L.nz;
+      double v[NV];
+      op.load(L, gx, gy, pz, ok, v);
+#pragma unroll
+      for (int k = 0; k < NV; k++) sN[slot][k][q] = v[k];
+    }
+  };
+
+  typename Op::Acc acc;
+  double carry[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) carry[c] = 0.0;
+
+  load_plane(z0 - 1, (z0 - 1) & 1);
+  for (int kz = z0 - 1; kz < zend; ++kz) {
+    load_plane(kz + 1, (kz + 1) & 1);
+    __syncthreads();
+    double f[NC][2][2][2];
+    if (exy && kz >= 0 && kz <= L.nz - 2) {
+      op.element(L, sN, kz & 1, tx, ty, ex, ey, kz, f);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+        for (int ix = 0; ix < 2; ix++) sOut[c * 4 + iy * 2 + ix][tid] = f[c][0][iy][ix];
+    __syncthreads();
+    if (owner && kz >= z0) {
+      double v[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        v[c] = carry[c] + sOut[c * 4 + 0][tid] + sOut[c * 4 + 1][tid - 1] + sOut[c * 4 + 2][tid - TILE] +
+               sOut[c * 4 + 3][tid - TILE - 1];
+      op.epilogue(L, sN, kz & 1, tx, ty, ex, ey, kz, v, acc);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+        for (int ix = 0; ix < 2; ix++) sOut[c * 4 + iy * 2 + ix][tid] = f[c][1][iy][ix];
+    __syncthreads();
+    if (owner) {
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        carry[c] = sOut[c * 4 + 0][tid] + sOut[c * 4 + 1][tid - 1] + sOut[c * 4 + 2][tid - TILE] +
+                   sOut[c * 4 + 3][tid - TILE - 1];
+    }
+  }
+  op.finish(acc);
+}
+
+// gather the 8 nodes of value index `k0..` from the two LDS planes
+template <int NV>
+__device__ __forceinline__ void gather8(const double (&sN)[2][NV][PLANE], int s0, int tx, int ty, int k,
+                                        double (&v)[2][2][2]) {
+  const int q = tx + (TILE + 1) * ty;
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++) {
+    const int s = s0 ^ iz;
+    v[iz][0][0] = sN[s][k][q];
+    v[iz][0][1] = sN[s][k][q + 1];
+    v[iz][1][0] = sN[s][k][q + TILE + 1];
+    v[iz][1][1] = sN[s][k][q + TILE + 2];
+  }
+}
+
+struct NoAcc {};
+
+// ---- mechanical tile ops --------------------------------------------------
+enum MechMode { M_APPLY = 0, M_RESID = 1, M_PCGA = 2, M_DIAG = 3 };
+
+template <int MODE, bool FINE>
+struct MechOp {
+  static constexpr int NC = 3;
+  static constexpr int NU = (MODE == M_DIAG) ? 0 : 3;       // vector values per node
+  static constexpr int NV = NU + (FINE ? 3 : 0) > 0 ? NU + (FINE ? 3 : 0) : 1;
+  struct Acc {
+    double a[3];
+    __device__ Acc() { a[0] = a[1] = a[2] = 0.0; }
+  };
+  MechCoef C;
+  const double* x;      // APPLY/RESID: input vector; PCGA: r
+  const double* invD;   // RESID/PCGA
+  const double* p_old;  // PCGA
+  double* out;          // APPLY: y; RESID: r; PCGA: w; DIAG: invD
+  double* out2;         // RESID: f; PCGA: p_new; DIAG: diag (or null)
+  int flag;             // RESID: t0fine; PCGA: first
+  double sx, sy, sz, tx_, ty_, tz_, wg, ihx2, ihy2, ihz2;
+  Reduce R;
+  size_t N;
+
+  __device__ __forceinline__ void load(const LevelGeom& L, int gx, int gy, int gz, bool ok, double (&v)[NV]) const {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = 0.0;
+    if (!ok) return;
+    const size_t q = nidx(L, gx, gy, gz);
+    if (MODE == M_APPLY || MODE == M_RESID) {
+#pragma unroll
+      for (int c = 0; c < NU; c++) v[c] = x[c * N + q];
+    } else if (MODE == M_PCGA) {
+      double beta = 0.0;
+      if (!flag) beta = R.scal[S_BETA];
+#pragma unroll
+      for (int c = 0; c < NU; c++) {
+        const double z = x[c * N + q] * invD[c * N + q];
+        v[c] = flag ? z : z + beta * p_old[c * N + q];
+      }
+    }
+    if (FINE) {
+      v[NU + 0] = C.K[q];
+      v[NU + 1] = C.G[q];
+      v[NU + 2] = C.d[q];
+    }
+  }
+
+  __device__ __forceinline__ void lame(const double (&sN)[2][NV][PLANE], int s0, int tx, int ty, int ex, int ey,
+                                       int ez, const LevelGeom& L, double (&lam)[2][2][2], double (&mu)[2][2][2],
+                                       double& lam0, double& mu0) const {
+    if (FINE) {
+      double Kn[2][2][2], Gn[2][2][2], dn[2][2][2];
+      gather8<NV>(sN, s0, tx, ty, NU + 0, Kn);
+      gather8<NV>(sN, s0, tx, ty, NU + 1, Gn);
+      gather8<NV>(sN, s0, tx, ty, NU + 2, dn);
+      const int fl = C.flag[nidx(L, ex, ey, ez)];
+      fine_lame(Kn, Gn, dn, fl, C.small_k, lam, mu);
+      lam0 = mu0 = 0.0;
+    } else {
+      const size_t e = nidx(L, ex, ey, ez);
+      const double Ke = C.Ke[e], Ge = C.Ge[e];
+      lam0 = Ke - (2.0 / 3.0) * Ge;
+      mu0 = Ge;
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+          for (int c = 0; c < 2; c++) {
+            lam[a][b][c] = lam0;
+            mu[a][b][c] = mu0;
+          }
+    }
+  }
+
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, double (&f)[3][2][2][2]) const {
+    double lam[2][2][2], mu[2][2][2], lam0, mu0;
+    lame(sN, s0, tx, ty, ex, ey, ez, L, lam, mu, lam0, mu0);
+    if (MODE == M_DIAG) {
+      mech_element_diag(lam, mu, ihx2, ihy2, ihz2, wg, f);
+    } else {
+      double u[3][2][2][2];
+#pragma unroll
+      for (int c = 0; c < 3; c++) gather8<NV>(sN, s0, tx, ty, c, u[c]);
+      if (FINE)
+        mech_element<false>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
+      else
+        mech_element<true>(u, lam, mu, lam0, mu0, sx, sy, sz, tx_, ty_, tz_, f);
+    }
+  }
+
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
+                                           int ty, int i, int j, int k, const double (&v)[3], Acc& acc) const {
+    const size_t q = nidx(L, i, j, k);
+    if (MODE == M_APPLY) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[c * N + q] = v[c];
+    } else if (MODE == M_RESID) {
+      const int dm = dir_mask(L, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double fv = out2[c * N + q];
+        const bool dir = (dm >> c) & 1;
+        if (flag && dir) {
+          fv = v[c];
+          out2[c * N + q] = fv;
+        }
+        const double r = dir ? 0.0 : fv - v[c];
+        out[c * N + q] = r;
+        acc.a[0] += r * (r * invD[c * N + q]);
+        acc.a[1] += r * r;
+        acc.a[2] += fv * fv;
+      }
+    } else if (MODE == M_PCGA) {
+      const int qs = tx + (TILE + 1) * ty;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double p = sN[s0][c][qs];
+        out2[c * N + q] = p;
+        out[c * N + q] = v[c];
+        acc.a[0] += p * v[c];
+      }
+    } else {  // M_DIAG
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if (out2) out2[c * N + q] = v[c];
+        out[c * N + q] = 1.0 / v[c];
+        acc.a[0] += (v[c] == 0.0) ? 1.0 : 0.0;
+      }
+    }
+  }
+
+  __device__ __forceinline__ void finish(Acc& acc) const {
+    if (MODE == M_APPLY) return;
+    double* scal = R.scal;
+    if (MODE == M_RESID) {
+      const int t0 = flag;
+      reduce_finalize<RED_SUM, 3>(acc.a, R, 0.0, [=](double (&s)[3]) {
+        scal[S_RZ] = s[0];
+        scal[S_RR] = s[1];
+        if (t0) scal[S_FF] = s[2];
+      });
+    } else if (MODE == M_PCGA) {
+      double v1[1] = {acc.a[0]};
+      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) {
+        scal[S_PW] = s[0];
+        scal[S_ALPHA] = scal[S_RZ] / s[0];
+      });
+    } else {
+      double v1[1] = {acc.a[0]};
+      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) { scal[S_ZERO] = s[0]; });
+    }
+  }
+};
+
+// ---- phase-field tile ops -------------------------------------------------
+template <int MODE>
+struct PfOp {
+  static constexpr int NC = 1;
+  static constexpr int NU = (MODE == M_DIAG) ? 0 : 1;
+  static constexpr int NV = NU + 1;  // + gc
+  struct Acc {
+    double a[3];
+    __device__ Acc() { a[0] = a[1] = a[2] = 0.0; }
+  };
+  PfCoef C;
+  const double* x;      // APPLY/RESID: d; PCGA: R
+  const double* invD;
+  const double* p_old;
+  double* out;          // APPLY: y; RESID: R; PCGA: w; DIAG: invD
+  double* out2;         // RESID: fd (read); PCGA: p_new; DIAG: diag or null
+  int flag;             // PCGA: first
+  double sx, sy, sz, wg, ihx2, ihy2, ihz2;
+  Reduce R;
+
+  __device__ __forceinline__ void load(const LevelGeom& L, int gx, int gy, int gz, bool ok, double (&v)[NV]) const {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = 0.0;
+    if (!ok) return;
+    const size_t q = nidx(L, gx, gy, gz);
+    if (MODE == M_APPLY || MODE == M_RESID) {
+      v[0] = x[q];
+    } else if (MODE == M_PCGA) {
+      const double z = x[q] * invD[q];
+      v[0] = flag ? z : z + R.scal[S_BETA] * p_old[q];
+    }
+    v[NU] = C.gc[q];
+  }
+
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, double (&f)[1][2][2][2]) const {
+    double gcn[2][2][2];
+    gather8<NV>(sN, s0, tx, ty, NU, gcn);
+    double m[2][2][2], k[2][2][2];
+    pf_coefs(gcn, C.H[nidx(L, ex, ey, ez)], C.lc, wg, m, k);
+    if (MODE == M_DIAG) {
+      pf_element_diag(m, k, ihx2, ihy2, ihz2, f[0]);
+    } else {
+      double v[2][2][2];
+      gather8<NV>(sN, s0, tx, ty, 0, v);
+      pf_element(v, m, k, sx, sy, sz, f[0]);
+    }
+  }
+
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
+                                           int ty, int i, int j, int k, const double (&v)[1], Acc& acc) const {
+    const size_t q = nidx(L, i, j, k);
+    if (MODE == M_APPLY) {
+      out[q] = v[0];
+    } else if (MODE == M_RESID) {
+      const double fv = out2[q];
+      const double r = fv - v[0];
+      out[q] = r;
+      acc.a[0] += r * (r * invD[q]);
+      acc.a[1] += r * r;
+      acc.a[2] += fv * fv;
+    } else if (MODE == M_PCGA) {
+      const double p = sN[s0][0][tx + (TILE + 1) * ty];
+      out2[q] = p;
+      out[q] = v[0];
+      acc.a[0] += p * v[0];
+    } else {
+      if (out2) out2[q] = v[0];
+      out[q] = 1.0 / v[0];
+      acc.a[0] += (v[0] == 0.0) ? 1.0 : 0.0;
+    }
+  }
+
+  __device__ __forceinline__ void finish(Acc& acc) const {
+    if (MODE == M_APPLY) return;
+    double* scal = R.scal;
+    if (MODE == M_RESID) {
+      reduce_finalize<RED_SUM, 3>(acc.a, R, 0.0, [=](double (&s)[3]) {
+        scal[S_RZ] = s[0];
+        scal[S_RR] = s[1];
+        scal[S_FF] = s[2];
+      });
+    } else if (MODE == M_PCGA) {
+      double v1[1] = {acc.a[0]};
+      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) {
+        scal[S_PW] = s[0];
+        scal[S_ALPHA] = scal[S_RZ] / s[0];
+      });
+    } else {
+      double v1[1] = {acc.a[0]};
+      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) { scal[S_ZERO] = s[0]; });
+    }
+  }
+};
+
+static dim3 tile_grid(const LevelGeom& L) {
+  return dim3((L.nx + OWN - 1) / OWN, (L.ny + OWN - 1) / OWN, (L.nz + ZCHUNK - 1) / ZCHUNK);
+}
+
+template <int MODE, bool FINE>
+static MechOp<MODE, FINE> mech_op(const LevelGeom& L, const MechCoef& C) {
+  MechOp<MODE, FINE> op{};
+  op.C = C;
+  op.N = (size_t)L.nx * L.ny * L.nz;
+  op.sx = 1.0 / L.hx;
+  op.sy = 1.0 / L.hy;
+  op.sz = 1.0 / L.hz;
+  op.wg = 0.125 * L.hx * L.hy * L.hz;
+  op.tx_ = op.wg / L.hx;
+  op.ty_ = op.wg / L.hy;
+  op.tz_ = op.wg / L.hz;
+  op.ihx2 = 1.0 / (L.hx * L.hx);
+  op.ihy2 = 1.0 / (L.hy * L.hy);
+  op.ihz2 = 1.0 / (L.hz * L.hz);
+  return op;
+}
+
+template <int MODE>
+static void run_mech(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, const double* invD,
+                     const double* p_old, double* out, double* out2, int flag, const Reduce& R) {
+  if (C.fine) {
+    auto op = mech_op<MODE, true>(L, C);
+    op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.flag = flag; op.R = R;
+    tile_kernel<MechOp<MODE, true>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
+  } else {
+    auto op = mech_op<MODE, false>(L, C);
+    op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.flag = flag; op.R = R;
+    tile_kernel<MechOp<MODE, false>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
+  }
+}
+
+void launch_mech_apply(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, double* y) {
+  Reduce R{};
+  run_mech<M_APPLY>(s, L, C, x, nullptr, nullptr, y, nullptr, 0, R);
+}
+void launch_mech_resid(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* u, double* f,
+                       double* r, const double* invD, int t0fine, const Reduce& R) {
+  run_mech<M_RESID>(s, L, C, u, invD, nullptr, r, f, t0fine, R);
+}
+void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* r,
+                       const double* invD, const double* p_old, double* p_new, double* w, int first,
+                       const Reduce& R) {
+  run_mech<M_PCGA>(s, L, C, r, invD, p_old, w, p_new, first, R);
+}
+void launch_mech_diag(hipStream_t s, const LevelGeom& L, const MechCoef& C, double* diag_or_null,
+                      double* invD, const Reduce& R) {
+  run_mech<M_DIAG>(s, L, C, nullptr, nullptr, nullptr, invD, diag_or_null, 0, R);
+}
+
+template <int MODE>
+static void run_pf(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* x, const double* invD,
+                   const double* p_old, double* out, double* out2, int flag, const Reduce& R) {
+  PfOp<MODE> op{};
+  op.C = C;
+  op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.flag = flag; op.R = R;
+  op.sx = 1.0 / L.hx;
+  op.sy = 1.0 / L.hy;
+  op.sz = 1.0 / L.hz;
+  op.wg = 0.125 * L.hx * L.hy * L.hz;
+  op.ihx2 = 1.0 / (L.hx * L.hx);
+  op.ihy2 = 1.0 / (L.hy * L.hy);
+  op.ihz2 = 1.0 / (L.hz * L.hz);
+  tile_kernel<PfOp<MODE>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
+}
+
+void launch_pf_apply(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* x, double* y) {
+  Reduce R{};
+  run_pf<M_APPLY>(s, L, C, x, nullptr, nullptr, y, nullptr, 0, R);
+}
+void launch_pf_resid(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* d, const double* fd,
+                     double* r, const double* invD, const Reduce& R) {
+  run_pf<M_RESID>(s, L, C, d, invD, nullptr, r, const_cast<double*>(fd), 0, R);
+}
+void launch_pf_pcg_a(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* r, const double* invD,
+                     const double* p_old, double* p_new, double* w, int first, const Reduce& R) {
+  run_pf<M_PCGA>(s, L, C, r, invD, p_old, w, p_new, first, R);
+}
+void launch_pf_diag(hipStream_t s, const LevelGeom& L, const PfCoef& C, double* diag_or_null, double* invD,
+                    const Reduce& R) {
+  run_pf<M_DIAG>(s, L, C, nullptr, nullptr, nullptr, invD, diag_or_null, 0, R);
+}
+
+// ---------------------------------------------------------------------------
+// Pointwise PCG pass B
+// ---------------------------------------------------------------------------
+constexpr int PW_TPB = 256;
+static int pw_blocks(size_t n) {
+  size_t b = (n + PW_TPB - 1) / PW_TPB;
+  return (int)(b < 2048 ? (b == 0 ? 1 : b) : 2048);
+}
+
+__global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, double* __restrict__ u,
+                                                            double* __restrict__ r, const double* __restrict__ p,
+                                                            const double* __restrict__ w,
+                                                            const double* __restrict__ invD, const Reduce R) {
+  const size_t N = Nof(L);
+  const size_t plane = (size_t)L.nx * L.ny;
+  const double alpha = R.scal[S_ALPHA];
+  double acc[2] = {0.0, 0.0};
+  for (size_t q = (size_t)blockIdx.x * PW_TPB + threadIdx.x; q < N; q += (size_t)gridDim.x * PW_TPB) {
+    const int k = (int)(q / plane);
+    int dm = 0;
+    if (L.fz[k]) {
+      const size_t rem = q - (size_t)k * plane;
+      const int j = (int)(rem / L.nx), i = (int)(rem - (size_t)j * L.nx);
+      dm = dir_mask(L, i, j, k);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t o = c * N + q;
+      u[o] = u[o] + alpha * p[o];
+      double rv = r[o] - alpha * w[o];
+      if ((dm >> c) & 1) rv = 0.0;
+      r[o] = rv;
+      acc[0] += rv * (rv * invD[o]);
+      acc[1] += rv * rv;
+    }
+  }
+  double* scal = R.scal;
+  reduce_finalize<RED_SUM, 2>(acc, R, 0.0, [=](double (&s)[2]) {
+    scal[S_BETA] = s[0] / scal[S_RZ];
+    scal[S_RZ] = s[0];
+    scal[S_RR] = s[1];
+  });
+}
+
+__global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __restrict__ d, double* __restrict__ r,
+                                                          const double* __restrict__ p,
+                                                          const double* __restrict__ w,
+                                                          const double* __restrict__ invD, const Reduce R) {
+  const double alpha = R.scal[S_ALPHA];
+  double acc[2] = {0.0, 0.0};
+  for (size_t q = (size_t)blockIdx.x * PW_TPB + threadIdx.x; q < N; q += (size_t)gridDim.x * PW_TPB) {
+    d[q] = d[q] + alpha * p[q];
+    const double rv = r[q] - alpha * w[q];
+    r[q] = rv;
+    acc[0] += rv * (rv * invD[q]);
+    acc[1] += rv * rv;
+  }
+  double* scal = R.scal;
+  reduce_finalize<RED_SUM, 2>(acc, R, 0.0, [=](double (&s)[2]) {
+    scal[S_BETA] = s[0] / scal[S_RZ];
+    scal[S_RZ] = s[0];
+    scal[S_RR] = s[1];
+  });
+}
+
+void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, const double* p,
+                       const double* w, const double* invD, const Reduce& R) {
+  const size_t N = (size_t)L.nx * L.ny * L.nz;
+  hipLaunchKernelGGL(mech_pcg_b_kernel, dim3(pw_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, p, w, invD, R);
+}
+void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, const double* p, const double* w,
+                     const double* invD, const Reduce& R) {
+  const size_t N = (size_t)L.nx * L.ny * L.nz;
+  hipLaunchKernelGGL(pf_pcg_b_kernel, dim3(pw_blocks(N)), dim3(PW_TPB), 0, s, N, d, r, p, w, invD, R);
+}
+
+// ---------------------------------------------------------------------------
+// Dirichlet boundary conditions (boundary_conditions.cpp:28-111)
+// ---------------------------------------------------------------------------
+struct Planes {
+  int k[4];
+  int n;
+};
+
+__global__ void dirichlet_bd_kernel(const LevelGeom L, double* u, double Ut, Planes P) {
+  const size_t N = Nof(L);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= L.nx) return;
+  for (int t = 0; t < P.n; t++) {
+    const int k = P.k[t];
+    const int dm = dir_mask(L, i, j, k);
+    const size_t q = nidx(L, i, j, k);
+    if (dm & 1) u[q] = 0.0;
+    if (dm & 2) u[N + q] = 0.0;
+    if (dm & 4) u[2 * N + q] = (dm & 8) ? Ut : 0.0;
+  }
+}
+__global__ void dirichlet_copy_kernel(const LevelGeom L, double* f, const double* Fi, Planes P) {
+  const size_t N = Nof(L);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= L.nx) return;
+  for (int t = 0; t < P.n; t++) {
+    const int k = P.k[t];
+    const int dm = dir_mask(L, i, j, k);
+    const size_t q = nidx(L, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      if ((dm >> c) & 1) f[c * N + q] = Fi[c * N + q];
+  }
+}
+static Planes planes_of(const int* kp, int n) {
+  Planes P{};
+  P.n = n > 4 ? 4 : n;
+  for (int t = 0; t < P.n; t++) P.k[t] = kp[t];
+  return P;
+}
+void launch_dirichlet_bd(hipStream_t s, const LevelGeom& L, double* u, double Ut, const int* kp, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(dirichlet_bd_kernel, dim3((L.nx + 63) / 64, L.ny), dim3(64), 0, s, L, u, Ut, planes_of(kp, n));
+}
+void launch_dirichlet_copy(hipStream_t s, const LevelGeom& L, double* f, const double* Fi, const int* kp, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(dirichlet_copy_kernel, dim3((L.nx + 63) / 64, L.ny), dim3(64), 0, s, L, f, Fi,
+                     planes_of(kp, n));
+}
+
+// ---------------------------------------------------------------------------
+// Transfers (multigrid_intergrid.cpp)
+// ---------------------------------------------------------------------------
+// solution_injection + Dirichlet_BD + copy to uold (multigrid_intergrid.cpp:25-121)
+__global__ void inject_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp, const double* __restrict__ uf,
+                              double* __restrict__ uc, double* __restrict__ ucold, int apply_bc, double Ut) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= Lc.nx) return;
+  const size_t Nc = Nof(Lc), Nf = Nof(Lf);
+  const size_t qc = nidx(Lc, i, j, k), qf = nidx(Lf, 2 * i, 2 * j, 2 * k);
+  const int dm = apply_bc ? dir_mask(Lc, i, j, k) : 0;
+  for (int c = 0; c < ncomp; c++) {
+    double v = uf[c * Nf + qf];
+    if ((dm >> c) & 1) v = (c == 2 && (dm & 8)) ? Ut : 0.0;
+    uc[c * Nc + qc] = v;
+    if (ucold) ucold[c * Nc + qc] = v;
+  }
+}
+void launch_inject(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* uf,
+                   double* uc, double* ucold, int apply_bc, double Ut) {
+  hipLaunchKernelGGL(inject_kernel, dim3((Lc.nx + 63) / 64, Lc.ny, Lc.nz), dim3(64), 0, s, Lc, Lf, ncomp, uf, uc,
+                     ucold, apply_bc, Ut);
+}
+
+// coarsen_f_standard_routine (multigrid_intergrid.cpp:124-342)
+__global__ void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp, const double* __restrict__ rf,
+                                const double* __restrict__ Fic, double* __restrict__ fc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= Lc.nx) return;
+  const size_t Nc = Nof(Lc), Nf = Nof(Lf);
+  const int x = 2 * i, y = 2 * j, z = 2 * k;
+  for (int c = 0; c < ncomp; c++) {
+    const double* r = rf + c * Nf;
+    double s8 = 0.0, s4 = 0.0, s2 = 0.0, s1 = 0.0;
+    for (int dz = -1; dz <= 1; dz++) {
+      const int zz = z + dz;
+      if (zz < 0 || zz > Lf.nz - 1) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        const int yy = y + dy;
+        if (yy < 0 || yy > Lf.ny - 1) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          const int xx = x + dx;
+          if (xx < 0 || xx > Lf.nx - 1) continue;
+          const double v = r[nidx(Lf, xx, yy, zz)];
+          const int o = (dx != 0) + (dy != 0) + (dz != 0);
+          if (o == 0) s8 += v;
+          else if (o == 1) s4 += v;
+          else if (o == 2) s2 += v;
+          else s1 += v;
+        }
+      }
+    }
+    const size_t qc = nidx(Lc, i, j, k);
+    fc[c * Nc + qc] = Fic[c * Nc + qc] + 0.125 * (s1 + 2.0 * s2 + 4.0 * s4 + 8.0 * s8);
+  }
+}
+void launch_restrict(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* rf,
+                     const double* Fic, double* fc) {
+  hipLaunchKernelGGL(restrict_kernel, dim3((Lc.nx + 63) / 64, Lc.ny, Lc.nz), dim3(64), 0, s, Lc, Lf, ncomp, rf, Fic,
+                     fc);
+}
+
+// refine_standard_routine + Dirichlet_BD (multigrid_intergrid.cpp:416-549)
+__global__ void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp, double* __restrict__ uf,
+                               const double* __restrict__ uc, const double* __restrict__ ucold, int apply_bc,
+                               double Ut) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= Lf.nx) return;
+  const size_t Nc = Nof(Lc), Nf = Nof(Lf);
+  const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
+  const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
+  const size_t qf = nidx(Lf, x, y, z);
+  const int dm = apply_bc ? dir_mask(Lf, x, y, z) : 0;
+  for (int c = 0; c < ncomp; c++) {
+    const double* a = uc + c * Nc;
+    const double* b = ucold + c * Nc;
+    auto E = [&](int ii, int jj, int kk) {
+      const size_t q = nidx(Lc, ii, jj, kk);
+      return a[q] - b[q];
+    };
+    double corr;
+    const int odd = (x & 1) + (y & 1) + (z & 1);
+    if (odd == 0) {
+      corr = E(i0, j0, k0);
+    } else if (odd == 1) {
+      corr = 0.5 * (E(i1, j1, k1) + E(i0, j0, k0));
+    } else if (odd == 2) {
+      if (!(z & 1))
+        corr = 0.25 * (E(i1, j1, k0) + E(i1, j0, k0) + E(i0, j1, k0) + E(i0, j0, k0));
+      else if (!(y & 1))
+        corr = 0.25 * (E(i1, j0, k1) + E(i0, j0, k0) + E(i1, j0, k0) + E(i0, j0, k1));
+      else
+        corr = 0.25 * (E(i0, j1, k1) + E(i0, j0, k0) + E(i0, j1, k0) + E(i0, j0, k1));
+    } else {
+      corr = 0.125 * (E(i1, j1, k1) + E(i1, j0, k1) + E(i0, j1, k1) + E(i0, j0, k1) + E(i1, j1, k0) +
+                      E(i0, j1, k0) + E(i1, j0, k0) + E(i0, j0, k0));
+    }
+    double v = uf[c * Nf + qf] + corr;
+    if ((dm >> c) & 1) v = (c == 2 && (dm & 8)) ? Ut : 0.0;
+    uf[c * Nf + qf] = v;
+  }
+}
+void launch_prolong(hipStream_t s, const LevelGeom& Lf, const LevelGeom& Lc, int ncomp, double* uf,
+                    const double* uc, const double* ucold, int apply_bc, double Ut) {
+  hipLaunchKernelGGL(prolong_kernel, dim3((Lf.nx + 63) / 64, Lf.ny, Lf.nz), dim3(64), 0, s, Lf, Lc, ncomp, uf, uc,
+                     ucold, apply_bc, Ut);
+}
+
+// ---------------------------------------------------------------------------
+// Per-element Gauss-point kernels on the finest level
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load8(const LevelGeom& L, const double* f, int i, int j, int k,
+                                      double (&v)[2][2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) v[a][b][c] = f[nidx(L, i + c, j + b, k + a)];
+}
+
+struct Strain {
+  double xx, yy, zz, xy, xz, yz;  // tensor components (xy = gamma_xy / 2)
+};
+__device__ __forceinline__ Strain strain_at(const double (&dx)[3][2][2], const double (&dy)[3][2][2],
+                                            const double (&dz)[3][2][2], int gx, int gy, int gz) {
+  Strain e;
+  e.xx = dx[0][gz][gy];
+  e.yy = dy[1][gz][gx];
+  e.zz = dz[2][gy][gx];
+  e.xy = 0.5 * (dx[1][gz][gy] + dy[0][gz][gx]);
+  e.xz = 0.5 * (dx[2][gz][gy] + dz[0][gy][gx]);
+  e.yz = 0.5 * (dy[2][gz][gx] + dz[1][gy][gx]);
+  return e;
+}
+// epsilon_dev (mechanical_strain.cpp:47-61) and Phi_positive/negative
+// (strain_history.cpp:134-159)
+__device__ __forceinline__ double dev2(const Strain& e) {
+  const double t = (1.0 / 3.0) * (e.xx + e.yy + e.zz);
+  const double a = e.xx - t, b = e.yy - t, c = e.zz - t;
+  return a * a + b * b + c * c + 2.0 * (e.xy * e.xy + e.xz * e.xz + e.yz * e.yz);
+}
+__device__ __forceinline__ double phi_pos(double K, double G, const Strain& e) {
+  const double tr = e.xx + e.yy + e.zz;
+  const double d2 = dev2(e);
+  return (tr >= 0) ? 0.5 * K * tr * tr + G * d2 : G * d2;
+}
+__device__ __forceinline__ double phi_neg(double K, const Strain& e) {
+  const double tr = e.xx + e.yy + e.zz;
+  return (tr >= 0) ? 0.0 : 0.5 * K * tr * tr;
+}
+
+struct ElemArgs {
+  const double* u;
+  const double* K;
+  const double* G;
+  const double* d;
+  const double* gc;
+};
+
+__device__ __forceinline__ void elem_grads(const LevelGeom& L, const double* u, int i, int j, int k,
+                                           double (&dx)[3][2][2], double (&dy)[3][2][2], double (&dz)[3][2][2]) {
+  const size_t N = Nof(L);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double v[2][2][2];
+    load8(L, u + c * N, i, j, k, v);
+    grad8(v, 1.0 / L.hx, 1.0 / L.hy, 1.0 / L.hz, dx[c], dy[c], dz[c]);
+  }
+}
+
+// MP_choice_element + MP_element (material_selection.cpp:19-45, 67-81)
+__global__ void flag_means_kernel(const LevelGeom L, const double* __restrict__ u, const double* __restrict__ K,
+                                  const double* __restrict__ G, const double* __restrict__ d,
+                                  signed char* __restrict__ flag, double* __restrict__ Ke, double* __restrict__ Ge,
+                                  double small_k, int compute_flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i > L.nx - 2) return;
+  double Kn[2][2][2], Gn[2][2][2], dn[2][2][2], Kg[2][2][2], Gg[2][2][2], dg[2][2][2];
+  load8(L, K, i, j, k, Kn);
+  load8(L, G, i, j, k, Gn);
+  gp_mp(Kn, Kg);
+  gp_mp(Gn, Gg);
+  const size_t e = nidx(L, i, j, k);
+  int fl;
+  if (compute_flag) {
+    double dx[3][2][2], dy[3][2][2], dz[3][2][2];
+    elem_grads(L, u, i, j, k, dx, dy, dz);
+    double tr = 0.0, pp = 0.0, pn = 0.0;
+    for (int gz = 0; gz < 2; gz++)
+      for (int gy = 0; gy < 2; gy++)
+        for (int gx = 0; gx < 2; gx++) {
+          const Strain s = strain_at(dx, dy, dz, gx, gy, gz);
+          tr += s.xx + s.yy + s.zz;
+          pp += phi_pos(Kg[gz][gy][gx], Gg[gz][gy][gx], s);
+          pn += phi_neg(Kg[gz][gy][gx], s);
+        }
+    fl = 0;
+    if (tr >= 0.0) fl = 0;
+    if (tr < 0.0) fl = 1;
+    if (pp < pn) fl = 2;
+    flag[e] = (signed char)fl;
+  } else {
+    fl = flag[e];
+  }
+  load8(L, d, i, j, k, dn);
+  gp_mp(dn, dg);
+  double sK = 0.0, sG = 0.0;
+  for (int gz = 0; gz < 2; gz++)
+    for (int gy = 0; gy < 2; gy++)
+      for (int gx = 0; gx < 2; gx++) {
+        const double g = degr(dg[gz][gy][gx], small_k);
+        sK += (fl == 0) ? g * Kg[gz][gy][gx] : Kg[gz][gy][gx];
+        sG += (fl == 2) ? Gg[gz][gy][gx] : g * Gg[gz][gy][gx];
+      }
+  Ke[e] = 0.125 * sK;
+  Ge[e] = 0.125 * sG;
+}
+void launch_flag_means(hipStream_t s, const LevelGeom& L, const double* u, const double* K, const double* G,
+                       const double* d, signed char* flag, double* Ke, double* Ge, double small_k,
+                       int compute_flag) {
+  hipLaunchKernelGGL(flag_means_kernel, dim3((L.nx - 1 + 63) / 64, L.ny - 1, L.nz - 1), dim3(64), 0, s, L, u, K, G,
+                     d, flag, Ke, Ge, small_k, compute_flag);
+}
+
+// strain_history (strain_history.cpp:162-210)
+__global__ void strain_history_kernel(const LevelGeom L, const double* __restrict__ u, const double* __restrict__ K,
+                                      const double* __restrict__ G, const double* __restrict__ gc,
+                                      double* __restrict__ H, double lc, double u_prop) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i > L.nx - 2) return;
+  double Kn[2][2][2], Gn[2][2][2], cn[2][2][2], Kg[2][2][2], Gg[2][2][2], cg[2][2][2];
+  load8(L, K, i, j, k, Kn);
+  load8(L, G, i, j, k, Gn);
+  load8(L, gc, i, j, k, cn);
+  gp_mp(Kn, Kg);
+  gp_mp(Gn, Gg);
+  gp_mp(cn, cg);
+  double dx[3][2][2], dy[3][2][2], dz[3][2][2];
+  elem_grads(L, u, i, j, k, dx, dy, dz);
+  double tmp = 0.0;
+  for (int gz = 0; gz < 2; gz++)
+    for (int gy = 0; gy < 2; gy++)
+      for (int gx = 0; gx < 2; gx++) {
+        Strain s = strain_at(dx, dy, dz, gx, gy, gz);
+        s.xx *= u_prop; s.yy *= u_prop; s.zz *= u_prop;
+        s.xy *= u_prop; s.xz *= u_prop; s.yz *= u_prop;
+        tmp += phi_pos(Kg[gz][gy][gx], Gg[gz][gy][gx], s) - 0.5 * cg[gz][gy][gx] / lc;
+      }
+  const double Ht = 0.125 * tmp;
+  const size_t e = nidx(L, i, j, k);
+  if (H[e] < Ht) H[e] = Ht;
+}
+void launch_strain_history(hipStream_t s, const LevelGeom& L, const double* u, const double* K, const double* G,
+                           const double* gc, double* H, double lc, double u_prop) {
+  hipLaunchKernelGGL(strain_history_kernel, dim3((L.nx - 1 + 63) / 64, L.ny - 1, L.nz - 1), dim3(64), 0, s, L, u,
+                     K, G, gc, H, lc, u_prop);
+}
+
+// U_e (strain_history.cpp:86-131): min over Gauss points of sqrt(phi_c / W)
+__global__ __launch_bounds__(256) void elastic_limit_kernel(const LevelGeom L, const double* __restrict__ u,
+                                                            const double* __restrict__ K,
+                                                            const double* __restrict__ G,
+                                                            const double* __restrict__ gc, double lc,
+                                                            const Reduce R) {
+  const size_t ne = (size_t)(L.nx - 1) * (L.ny - 1) * (L.nz - 1);
+  double m[1] = {1e10};
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % (L.nx - 1));
+    const size_t r = q / (L.nx - 1);
+    const int j = (int)(r % (L.ny - 1));
+    const int k = (int)(r / (L.ny - 1));
+    double Kn[2][2][2], Gn[2][2][2], cn[2][2][2], Kg[2][2][2], Gg[2][2][2], cg[2][2][2];
+    load8(L, K, i, j, k, Kn);
+    load8(L, G, i, j, k, Gn);
+    load8(L, gc, i, j, k, cn);
+    gp_mp(Kn, Kg);
+    gp_mp(Gn, Gg);
+    gp_mp(cn, cg);
+    double dx[3][2][2], dy[3][2][2], dz[3][2][2];
+    elem_grads(L, u, i, j, k, dx, dy, dz);
+    for (int gz = 0; gz < 2; gz++)
+      for (int gy = 0; gy < 2; gy++)
+        for (int gx = 0; gx < 2; gx++) {
+          const Strain e = strain_at(dx, dy, dz, gx, gy, gz);
+          const double Kv = Kg[gz][gy][gx], Gv = Gg[gz][gy][gx];
+          const double tr = e.xx + e.yy + e.zz;
+          const double sxx = Kv * tr + 2.0 * Gv * (e.xx - 1.0 / 3.0 * tr);
+          const double syy = Kv * tr + 2.0 * Gv * (e.yy - 1.0 / 3.0 * tr);
+          const double szz = Kv * tr + 2.0 * Gv * (e.zz - 1.0 / 3.0 * tr);
+          const double W = 0.5 * (sxx * e.xx + syy * e.yy + szz * e.zz +
+                                  2.0 * (2.0 * Gv) * (e.xy * e.xy + e.xz * e.xz + e.yz * e.yz));
+          const double phic = 0.5 * cg[gz][gy][gx] / lc;
+          const double ut = sqrt(phic / W);
+          if (ut < m[0]) m[0] = ut;
+        }
+  }
+  double* scal = R.scal;
+  reduce_finalize<RED_MIN, 1>(m, R, 1e10, [=](double (&s)[1]) { scal[S_RED] = s[0]; });
+}
+void launch_elastic_limit(hipStream_t s, const LevelGeom& L, const double* u, const double* K, const double* G,
+                          const double* gc, double lc, const Reduce& R) {
+  const size_t ne = (size_t)(L.nx - 1) * (L.ny - 1) * (L.nz - 1);
+  hipLaunchKernelGGL(elastic_limit_kernel, dim3(pw_blocks(ne)), dim3(256), 0, s, L, u, K, G, gc, lc, R);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse coefficients (material_coarse_element.cpp, image_input.cpp)
+// ---------------------------------------------------------------------------
+__global__ void elem_vr_kernel(const LevelGeom Lc, const LevelGeom Lf, const double* __restrict__ Hf,
+                               double* __restrict__ Hc, int voigt_only) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= Lc.nx) return;
+  const size_t qc = nidx(Lc, i, j, k);
+  const bool inside = i <= Lc.nx - 2 && j <= Lc.ny - 2 && k <= Lc.nz - 2;
+  double V = 0.0, Rm = 0.0;
+  if (inside) {
+    const int x = 2 * i, y = 2 * j, z = 2 * k;
+    const double h0 = Hf[nidx(Lf, x, y, z)], h1 = Hf[nidx(Lf, x + 1, y, z)];
+    const double h2 = Hf[nidx(Lf, x, y + 1, z)], h3 = Hf[nidx(Lf, x + 1, y + 1, z)];
+    const double h4 = Hf[nidx(Lf, x, y, z + 1)], h5 = Hf[nidx(Lf, x + 1, y, z + 1)];
+    const double h6 = Hf[nidx(Lf, x, y + 1, z + 1)], h7 = Hf[nidx(Lf, x + 1, y + 1, z + 1)];
+    V = 0.125 * (h0 + h1 + h2 + h3 + h4 + h5 + h6 + h7);
+    if (!voigt_only)
+      Rm = 8. / (1. / h0 + 1. / h1 + 1. / h2 + 1. / h3 + 1. / h4 + 1. / h5 + 1. / h6 + 1. / h7);
+  }
+  if (voigt_only) {
+    if (inside) Hc[qc] = V;
+  } else {
+    Hc[qc] = 0.5 * (0.5 * (V + Rm) + sqrt(V * Rm));
+  }
+}
+void launch_elem_vr(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, const double* Hf, double* Hc,
+                    int voigt_only) {
+  hipLaunchKernelGGL(elem_vr_kernel, dim3((Lc.nx + 63) / 64, Lc.ny, Lc.nz), dim3(64), 0, s, Lc, Lf, Hf, Hc,
+                     voigt_only);
+}
+
+// voight / reuss / coarse_VR (image_input.cpp:144-273)
+__global__ void nodal_vr_kernel(const LevelGeom Lc, const LevelGeom Lf, const double* __restrict__ M,
+                                double* __restrict__ Mc) {
+  const int ic = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jc = blockIdx.y, kc = blockIdx.z;
+  if (ic >= Lc.nx) return;
+  const int x = 2 * ic, y = 2 * jc, z = 2 * kc;
+  const int xm = max(x - 1, 0), xp = min(x + 1, Lf.nx - 1);
+  const int ym = max(y - 1, 0), yp = min(y + 1, Lf.ny - 1);
+  const int zm = max(z - 1, 0), zp = min(z + 1, Lf.nz - 1);
+  const int P[27][3] = {{xm, ym, zm}, {xm, ym, zp}, {xm, yp, zm}, {xm, yp, zp}, {xp, ym, zm}, {xp, ym, zp},
+                        {xp, yp, zm}, {xp, yp, zp}, {xm, ym, z},  {xm, yp, z},  {xp, ym, z},  {xp, yp, z},
+                        {xm, y, zm},  {xm, y, zp},  {xp, y, zm},  {xp, y, zp},  {x, ym, zm},  {x, ym, zp},
+                        {x, yp, zm},  {x, yp, zp},  {x, y, zm},   {x, y, zp},   {x, ym, z},   {x, yp, z},
+                        {xm, y, z},   {xp, y, z},   {x, y, z}};
+  double sv = 0.0, sr = 0.0;
+  for (int t = 0; t < 27; t++) {
+    const double v = M[nidx(Lf, P[t][0], P[t][1], P[t][2])];
+    sv += v;
+    sr += 1. / v;
+  }
+  const double V = 1.0 / 27.0 * sv;
+  const double R = 27.0 / sr;
+  Mc[nidx(Lc, ic, jc, kc)] = 0.5 * (0.5 * (V + R) + sqrt(V * R));
+}
+void launch_nodal_vr(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, const double* Mf, double* Mc) {
+  hipLaunchKernelGGL(nodal_vr_kernel, dim3((Lc.nx + 63) / 64, Lc.ny, Lc.nz), dim3(64), 0, s, Lc, Lf, Mf, Mc);
+}
+
+// fd_h (phase_field_right_hand_side.cpp:14-47) as a node gather
+__global__ void fd_h_kernel(const LevelGeom L, const double* __restrict__ H, double* __restrict__ fd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= L.nx) return;
+  const double wg = 0.125 * L.hx * L.hy * L.hz;
+  const double S = (GA + GB) * (GA + GB) * (GA + GB);  // sum_g N_a(g)
+  double acc = 0.0;
+  for (int dz = 1; dz >= 0; dz--)
+    for (int dy = 1; dy >= 0; dy--)
+      for (int dx = 1; dx >= 0; dx--) {
+        const int ei = i - dx, ej = j - dy, ek = k - dz;
+        if (ei < 0 || ej < 0 || ek < 0 || ei > L.nx - 2 || ej > L.ny - 2 || ek > L.nz - 2) continue;
+        acc += wg * 2.0 * H[nidx(L, ei, ej, ek)] * S;
+      }
+  fd[nidx(L, i, j, k)] = acc;
+}
+void launch_fd_h(hipStream_t s, const LevelGeom& L, const double* H, double* fd) {
+  hipLaunchKernelGGL(fd_h_kernel, dim3((L.nx + 63) / 64, L.ny, L.nz), dim3(64), 0, s, L, H, fd);
+}
+
+// externe_force_sum (mechanical_right_hnd.cpp:40-59)
+__global__ __launch_bounds__(256) void top_sum_kernel(const LevelGeom L, const double* __restrict__ fz,
+                                                      const Reduce R) {
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t base = (size_t)(L.nz - 1) * plane;
+  double a[1] = {0.0};
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane; q += (size_t)gridDim.x * blockDim.x)
+    a[0] += fz[base + q];
+  double* scal = R.scal;
+  reduce_finalize<RED_SUM, 1>(a, R, 0.0, [=](double (&s)[1]) { scal[S_RED] = s[0]; });
+}
+void launch_top_sum(hipStream_t s, const LevelGeom& L, const double* fz, const Reduce& R) {
+  const size_t plane = (size_t)L.nx * L.ny;
+  hipLaunchKernelGGL(top_sum_kernel, dim3(pw_blocks(plane)), dim3(256), 0, s, L, fz, R);
+}
+
+// delta_d_max / d_max (phase_field_max_search.cpp:17-53): max(0, max(a - b))
+__global__ __launch_bounds__(256) void max_diff_kernel(size_t n, const double* __restrict__ a,
+                                                       const double* __restrict__ b, const Reduce R) {
+  double m[1] = {0.0};
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const double v = b ? a[q] - b[q] : a[q];
+    if (m[0] < v) m[0] = v;
+  }
+  double* scal = R.scal;
+  reduce_finalize<RED_MAX, 1>(m, R, 0.0, [=](double (&s)[1]) { scal[S_RED] = s[0]; });
+}
+void launch_max_diff(hipStream_t s, size_t n, const double* a, const double* b_or_null, const Reduce& R) {
+  hipLaunchKernelGGL(max_diff_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, a, b_or_null, R);
+}
+
+// squared norm of ncomp*N values into scal[S_RR] (grid_norm, pcg_utilities.cpp:35-58)
+__global__ __launch_bounds__(256) void norm2_kernel(size_t n, const double* __restrict__ a, const Reduce R) {
+  double m[1] = {0.0};
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    m[0] += a[q] * a[q];
+  double* scal = R.scal;
+  reduce_finalize<RED_SUM, 1>(m, R, 0.0, [=](double (&s)[1]) { scal[S_RR] = s[0]; });
+}
+void launch_norm2(hipStream_t s, size_t n, const double* a, const Reduce& R) {
+  hipLaunchKernelGGL(norm2_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, a, R);
+}
+
+__global__ void flag_to_double_kernel(size_t n, const signed char* f, double* o) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    o[q] = (double)f[q];
+}
+__global__ void double_to_flag_kernel(size_t n, const double* in, signed char* f) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    f[q] = (signed char)in[q];
+}
+__global__ void fill_kernel(size_t n, double* p, double v) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    p[q] = v;
+}
+void launch_flag_to_double(hipStream_t s, size_t n, const signed char* f, double* out) {
+  hipLaunchKernelGGL(flag_to_double_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, f, out);
+}
+void launch_double_to_flag(hipStream_t s, size_t n, const double* in, signed char* f) {
+  hipLaunchKernelGGL(double_to_flag_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, in, f);
+}
+void launch_fill(hipStream_t s, size_t n, double* p, double v) {
+  hipLaunchKernelGGL(fill_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, p, v);
+}
+
+}  // namespace hpcr

@@ -1,0 +1,1000 @@
+// Host orchestration and C-ABI (include/hpcrack.h): one context per problem,
+// the whole level hierarchy resident in device memory, every kernel on one
+// HIP stream of default priority.  The control flow restates the reference's
+// solve entry points (u_entire, phase_field), cycle drivers (cycle_u,
+// cycle_d) and smoothers (pcg_u, pcg_d); the vector algebra of each smoother
+// iteration runs in two fused passes whose scalars (alpha, beta) never leave
+// the device.  The host synchronises once per multigrid cycle to read the
+// residual norm for the stopping test.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hpcrack.h"
+#include "hpcr_internal.h"
+
+using namespace hpcr;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Lev {
+  LevelGeom g{};
+  size_t N = 0;
+  // mechanics (3 components each)
+  double *u = nullptr, *fu = nullptr, *ru = nullptr, *uold = nullptr;
+  double *p = nullptr, *p2 = nullptr, *w = nullptr, *invDu = nullptr;
+  // phase field
+  double *d = nullptr, *fd = nullptr, *rd = nullptr, *dold = nullptr;
+  double *pd = nullptr, *pd2 = nullptr, *wd = nullptr, *invDd = nullptr;
+  // coefficients
+  double *H = nullptr, *K = nullptr, *G = nullptr, *gc = nullptr, *Ke = nullptr, *Ge = nullptr;
+  signed char* flag = nullptr;
+  double* dprev = nullptr;
+  unsigned char *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  std::vector<int> kplanes;
+};
+
+struct TimingRec {
+  int kernel, level;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct hpcr_ctx {
+  int device = 0;
+  hpcr_geometry geo{};
+  hpcr_params par{};
+  int L = 0;  // maxlevel
+  std::vector<Lev> lev;
+  hipStream_t stream = nullptr;
+  double* scal = nullptr;
+  double* partials = nullptr;
+  unsigned int* counter = nullptr;
+  double* host_scal = nullptr;  // pinned
+  size_t partial_cap = 0;
+  long long bytes = 0;
+  std::vector<void*> allocs;
+  std::string err;
+  bool timing = false;
+  std::vector<TimingRec> trec;
+  long long smoother_iters = 0;
+  bool diag_u_valid = false, diag_d_valid = false;
+  int last_rr_level = -1;  // level whose residual norm scal[S_RR] holds
+};
+
+namespace {
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                      \
+      return HPCR_ERR_DEVICE;                                                               \
+    }                                                                                       \
+  } while (0)
+
+int set_err(hpcr_ctx* c, int code, const std::string& m) {
+  c->err = m;
+  return code;
+}
+
+// coordinate of index i (get_coordinate, boundary_conditions.cpp:13-17, one
+// rank per axis); the product is rounded on its own, as in the reference.
+double coord(double origin, double h, int i) {
+  volatile double p = (double)i * h;
+  return origin + p;
+}
+
+int dev_alloc(hpcr_ctx* c, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    c->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+    *p = nullptr;
+    return HPCR_ERR_MEMORY;
+  }
+  c->allocs.push_back(*p);
+  c->bytes += (long long)bytes;
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) {
+    c->err = std::string("hipMemset failed: ") + hipGetErrorString(e);
+    return HPCR_ERR_DEVICE;
+  }
+  return HPCR_OK;
+}
+
+template <class T>
+int alloc_n(hpcr_ctx* c, T** p, size_t n) {
+  return dev_alloc(c, (void**)p, n * sizeof(T));
+}
+
+Reduce red(hpcr_ctx* c) { return Reduce{c->partials, c->counter, c->scal}; }
+
+MechCoef mcoef(hpcr_ctx* c, int l) {
+  Lev& v = c->lev[l];
+  MechCoef m{};
+  m.fine = (l == c->L) ? 1 : 0;
+  m.K = v.K;
+  m.G = v.G;
+  m.d = v.d;
+  m.flag = v.flag;
+  m.Ke = v.Ke;
+  m.Ge = v.Ge;
+  m.small_k = c->par.small_k;
+  return m;
+}
+PfCoef pcoef(hpcr_ctx* c, int l) { return PfCoef{c->lev[l].H, c->lev[l].gc, c->par.lc}; }
+
+int read_scal(hpcr_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->host_scal, c->scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HPCR_OK;
+}
+
+// optional HIP-event bracket around one launch
+struct TBracket {
+  hpcr_ctx* c;
+  int kernel, level;
+  hipEvent_t a = nullptr, b = nullptr;
+  TBracket(hpcr_ctx* c_, int k, int l) : c(c_), kernel(k), level(l) {
+    if (!c->timing) return;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, c->stream);
+  }
+  ~TBracket() {
+    if (!c->timing) return;
+    (void)hipEventRecord(b, c->stream);
+    c->trec.push_back(TimingRec{kernel, level, a, b});
+  }
+};
+
+int check_launch(hpcr_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(c, HPCR_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  return HPCR_OK;
+}
+
+// ---- mechanics ------------------------------------------------------------
+int mech_diag(hpcr_ctx* c, int l, double* diag_out) {
+  launch_mech_diag(c->stream, c->lev[l].g, mcoef(c, l), diag_out, c->lev[l].invDu, red(c));
+  return check_launch(c, "mech_diag");
+}
+int mech_diag_all(hpcr_ctx* c) {
+  for (int l = 0; l <= c->L; l++) {
+    int rc = mech_diag(c, l, nullptr);
+    if (rc) return rc;
+    if ((rc = read_scal(c))) return rc;
+    if (c->host_scal[S_ZERO] != 0.0)
+      return set_err(c, HPCR_ERR_ZERO_DIAGONAL, "Division by zero in grid /= operation (mechanical diagonal)");
+  }
+  c->diag_u_valid = true;
+  return HPCR_OK;
+}
+
+// pcg_u (mechanical_pcg.cpp:223-330)
+int pcg_u(hpcr_ctx* c, int l, int NB, double Ut, int t) {
+  Lev& v = c->lev[l];
+  int rc;
+  if (!c->diag_u_valid) {
+    if ((rc = mech_diag(c, l, nullptr))) return rc;
+    if ((rc = read_scal(c))) return rc;
+    if (c->host_scal[S_ZERO] != 0.0)
+      return set_err(c, HPCR_ERR_ZERO_DIAGONAL, "Division by zero in grid /= operation (mechanical diagonal)");
+  }
+  launch_dirichlet_bd(c->stream, v.g, v.u, Ut, v.kplanes.data(), (int)v.kplanes.size());
+  const int t0fine = (t == 0 && l == c->L) ? 1 : 0;
+  {
+    TBracket tb(c, HPCR_K_MECH_RESID, l);
+    launch_mech_resid(c->stream, v.g, mcoef(c, l), v.u, v.fu, v.ru, v.invDu, t0fine, red(c));
+  }
+  double* pb[2] = {v.p, v.p2};
+  for (int i = 0; i < NB; i++) {
+    double* p_old = pb[i & 1];
+    double* p_new = pb[(i + 1) & 1];
+    {
+      TBracket tb(c, HPCR_K_MECH_PCG_A, l);
+      launch_mech_pcg_a(c->stream, v.g, mcoef(c, l), v.ru, v.invDu, p_old, p_new, v.w, i == 0, red(c));
+    }
+    {
+      TBracket tb(c, HPCR_K_MECH_PCG_B, l);
+      launch_mech_pcg_b(c->stream, v.g, v.u, v.ru, p_new, v.w, v.invDu, red(c));
+    }
+    c->smoother_iters++;
+  }
+  c->last_rr_level = l;
+  return check_launch(c, "pcg_u");
+}
+
+// coarsen_u + coarsen_fu (multigrid_intergrid.cpp:106-121, 374-413)
+int coarsen_u(hpcr_ctx* c, int lf, double Ut) {
+  Lev& f = c->lev[lf];
+  Lev& k = c->lev[lf - 1];
+  launch_inject(c->stream, k.g, f.g, 3, f.u, k.u, k.uold, 1, Ut);
+  launch_mech_apply(c->stream, k.g, mcoef(c, lf - 1), k.u, k.w);
+  launch_restrict(c->stream, k.g, f.g, 3, f.ru, k.w, k.fu);
+  return check_launch(c, "coarsen_u");
+}
+
+// cycle_u (mechanical_multigrid.cpp:18-54)
+int cycle_u(hpcr_ctx* c, int l, double Ut, int t) {
+  int rc;
+  if (l == 0) return pcg_u(c, 0, c->par.mg_u.nu0, Ut, t);
+  if ((rc = pcg_u(c, l, c->par.mg_u.nu1, Ut, t))) return rc;
+  if ((rc = coarsen_u(c, l, Ut))) return rc;
+  for (int j = 1; j <= c->par.mg_u.gamma; j++)
+    if ((rc = cycle_u(c, l - 1, Ut, t))) return rc;
+  Lev& f = c->lev[l];
+  Lev& k = c->lev[l - 1];
+  launch_prolong(c->stream, f.g, k.g, 3, f.u, k.u, k.uold, 1, Ut);
+  return pcg_u(c, l, c->par.mg_u.nu2, Ut, t);
+}
+
+int mech_setup(hpcr_ctx* c) {
+  Lev& F = c->lev[c->L];
+  launch_flag_means(c->stream, F.g, F.u, F.K, F.G, F.d, F.flag, F.Ke, F.Ge, c->par.small_k, 1);
+  for (int l = c->L; l > 0; l--) {
+    launch_elem_vr(c->stream, c->lev[l - 1].g, c->lev[l].g, c->lev[l].Ke, c->lev[l - 1].Ke, 0);
+    launch_elem_vr(c->stream, c->lev[l - 1].g, c->lev[l].g, c->lev[l].Ge, c->lev[l - 1].Ge, 0);
+  }
+  c->diag_u_valid = false;
+  return check_launch(c, "mech_setup");
+}
+
+// finest-level squared residual norm after a cycle
+int finest_rr(hpcr_ctx* c, bool mech, double* out) {
+  int rc;
+  if (c->last_rr_level != c->L) {
+    Lev& F = c->lev[c->L];
+    launch_norm2(c->stream, (mech ? 3 : 1) * F.N, mech ? F.ru : F.rd, red(c));
+  }
+  if ((rc = read_scal(c))) return rc;
+  *out = c->host_scal[S_RR];
+  return HPCR_OK;
+}
+
+// u_entire (mechanical_displacement.cpp:25-115)
+int u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {
+  int rc;
+  const int L = c->L;
+  Lev& F = c->lev[L];
+  c->smoother_iters = 0;
+  if ((rc = mech_setup(c))) return rc;
+  if ((rc = mech_diag_all(c))) return rc;
+  // Initial_residual_u (mechanical_pcg.cpp:193-220)
+  launch_mech_resid(c->stream, F.g, mcoef(c, L), F.u, F.fu, F.ru, F.invDu, 0, red(c));
+  if ((rc = check_launch(c, "initial residual"))) return rc;
+  if ((rc = read_scal(c))) return rc;
+  double r_norm = c->host_scal[S_RR];
+  if (st) {
+    st->r_norm_initial = r_norm;
+    st->cycles = 0;
+  }
+  double rel = 1.0;
+  if (t >= 1 && *f_norm != 0) rel = r_norm / *f_norm;
+  int cycles = 0;
+  for (int j = 1; j <= c->par.mg_u.ncy; j++) {
+    if (rel > c->par.tol_u) {
+      if ((rc = cycle_u(c, L, Ut, t))) return rc;
+      cycles++;
+      if ((rc = finest_rr(c, true, &r_norm))) return rc;
+      if (t == 0) *f_norm = c->host_scal[S_FF];
+      rel = r_norm / *f_norm;
+      if (st && cycles <= HPCR_HIST_MAX) st->history[cycles - 1] = r_norm;
+    }
+    if (rel <= c->par.tol_u) break;
+  }
+  // externe_force_nodal (mechanical_right_hnd.cpp:18-38)
+  launch_dirichlet_bd(c->stream, F.g, F.u, Ut, F.kplanes.data(), (int)F.kplanes.size());
+  launch_mech_apply(c->stream, F.g, mcoef(c, L), F.u, F.w);
+  launch_dirichlet_copy(c->stream, F.g, F.fu, F.w, F.kplanes.data(), (int)F.kplanes.size());
+  if ((rc = check_launch(c, "externe_force_nodal"))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->diag_u_valid = false;
+  if (st) {
+    st->cycles = cycles;
+    st->r_norm_final = r_norm;
+    st->smoother_iters = c->smoother_iters;
+  }
+  return HPCR_OK;
+}
+
+// ---- phase field ------------------------------------------------------------
+int pf_diag(hpcr_ctx* c, int l, double* diag_out) {
+  launch_pf_diag(c->stream, c->lev[l].g, pcoef(c, l), diag_out, c->lev[l].invDd, red(c));
+  return check_launch(c, "pf_diag");
+}
+
+// pcg_d (phase_field_pcg.cpp:133-214)
+int pcg_d(hpcr_ctx* c, int l, int NB) {
+  Lev& v = c->lev[l];
+  int rc;
+  if (!c->diag_d_valid) {
+    if ((rc = pf_diag(c, l, nullptr))) return rc;
+    if ((rc = read_scal(c))) return rc;
+    if (c->host_scal[S_ZERO] != 0.0)
+      return set_err(c, HPCR_ERR_ZERO_DIAGONAL, "Division by zero in grid /= operation (phase-field diagonal)");
+  }
+  launch_pf_resid(c->stream, v.g, pcoef(c, l), v.d, v.fd, v.rd, v.invDd, red(c));
+  double* pb[2] = {v.pd, v.pd2};
+  for (int i = 0; i < NB; i++) {
+    double* p_old = pb[i & 1];
+    double* p_new = pb[(i + 1) & 1];
+    {
+      TBracket tb(c, HPCR_K_PF_PCG_A, l);
+      launch_pf_pcg_a(c->stream, v.g, pcoef(c, l), v.rd, v.invDd, p_old, p_new, v.wd, i == 0, red(c));
+    }
+    {
+      TBracket tb(c, HPCR_K_PF_PCG_B, l);
+      launch_pf_pcg_b(c->stream, v.g, v.d, v.rd, p_new, v.wd, v.invDd, red(c));
+    }
+    c->smoother_iters++;
+  }
+  c->last_rr_level = l;
+  return check_launch(c, "pcg_d");
+}
+
+// cycle_d (phase_field_multigrid.cpp:18-44)
+int cycle_d(hpcr_ctx* c, int l) {
+  int rc;
+  if (l == 0) return pcg_d(c, 0, c->par.mg_d.nu0);
+  if ((rc = pcg_d(c, l, c->par.mg_d.nu1))) return rc;
+  Lev& f = c->lev[l];
+  Lev& k = c->lev[l - 1];
+  // coarsen_d + coarsen_f_d (multigrid_intergrid.cpp:92-103, 345-371)
+  launch_inject(c->stream, k.g, f.g, 1, f.d, k.d, k.dold, 0, 0.0);
+  launch_pf_apply(c->stream, k.g, pcoef(c, l - 1), k.d, k.wd);
+  launch_restrict(c->stream, k.g, f.g, 1, f.rd, k.wd, k.fd);
+  for (int j = 1; j <= c->par.mg_d.gamma; j++)
+    if ((rc = cycle_d(c, l - 1))) return rc;
+  // refine_d (multigrid_intergrid.cpp:523-533)
+  launch_prolong(c->stream, f.g, k.g, 1, f.d, k.d, k.dold, 0, 0.0);
+  for (int i = 1; i <= c->par.mg_d.nu2; i++)
+    if ((rc = pcg_d(c, l, c->par.mg_d.nu2))) return rc;
+  return check_launch(c, "cycle_d");
+}
+
+int pf_diag_all(hpcr_ctx* c) {
+  for (int l = 0; l <= c->L; l++) {
+    int rc = pf_diag(c, l, nullptr);
+    if (rc) return rc;
+    if ((rc = read_scal(c))) return rc;
+    if (c->host_scal[S_ZERO] != 0.0)
+      return set_err(c, HPCR_ERR_ZERO_DIAGONAL, "Division by zero in grid /= operation (phase-field diagonal)");
+  }
+  c->diag_d_valid = true;
+  return HPCR_OK;
+}
+
+// phase_field (phase_field.cpp:15-57)
+int phase_field(hpcr_ctx* c, int t, double* fd_norm, hpcr_stats* st) {
+  (void)t;
+  int rc;
+  const int L = c->L;
+  Lev& F = c->lev[L];
+  c->smoother_iters = 0;
+  if ((rc = pf_diag_all(c))) return rc;
+  launch_pf_resid(c->stream, F.g, pcoef(c, L), F.d, F.fd, F.rd, F.invDd, red(c));
+  if ((rc = check_launch(c, "initial residual"))) return rc;
+  if ((rc = read_scal(c))) return rc;
+  double r_norm = c->host_scal[S_RR];
+  if (st) st->r_norm_initial = r_norm;
+  double sres = 1.0;
+  int cycles = 0;
+  for (int j = 1; j <= c->par.mg_d.ncy; j++) {
+    if (r_norm > c->par.tol_d_abs) {
+      if (sres > c->par.tol_d_rel) {
+        if ((rc = cycle_d(c, L))) return rc;
+        cycles++;
+      }
+      if ((rc = finest_rr(c, false, &r_norm))) return rc;
+      *fd_norm = c->host_scal[S_FF];
+      sres = r_norm / *fd_norm;
+      if (st && cycles >= 1 && cycles <= HPCR_HIST_MAX) st->history[cycles - 1] = r_norm;
+      if (sres <= c->par.tol_d_rel) break;
+    } else {
+      break;
+    }
+  }
+  c->diag_d_valid = false;
+  if (st) {
+    st->cycles = cycles;
+    st->r_norm_final = r_norm;
+    st->smoother_iters = c->smoother_iters;
+  }
+  return HPCR_OK;
+}
+
+// ---- field access ----------------------------------------------------------
+bool field_vec(int f) { return f == HPCR_U || f == HPCR_FU || f == HPCR_RU || f == HPCR_UOLD || f == HPCR_P || f == HPCR_W; }
+
+double* field_ptr(hpcr_ctx* c, int f, int l) {
+  Lev& v = c->lev[l];
+  switch (f) {
+    case HPCR_U: return v.u;
+    case HPCR_FU: return v.fu;
+    case HPCR_RU: return v.ru;
+    case HPCR_UOLD: return v.uold;
+    case HPCR_P: return v.p;
+    case HPCR_W: return v.w;
+    case HPCR_D: return v.d;
+    case HPCR_FD: return v.fd;
+    case HPCR_RD: return v.rd;
+    case HPCR_DOLD: return v.dold;
+    case HPCR_H: return v.H;
+    case HPCR_K: return v.K;
+    case HPCR_G: return v.G;
+    case HPCR_GC: return v.gc;
+    case HPCR_KE: return v.Ke;
+    case HPCR_GE: return v.Ge;
+    case HPCR_DPREV: return v.dprev;
+    case HPCR_PD: return v.pd;
+    case HPCR_WD: return v.wd;
+    default: return nullptr;
+  }
+}
+
+int check_fl(hpcr_ctx* c, int f, int l, int comp) {
+  if (!c) return HPCR_ERR_ARG;
+  if (l < 0 || l > c->L) return set_err(c, HPCR_ERR_ARG, "level out of range");
+  if (f < 0 || f >= HPCR_FIELD_COUNT) return set_err(c, HPCR_ERR_ARG, "unknown field");
+  if (comp < 0 || comp >= (field_vec(f) ? 3 : 1)) return set_err(c, HPCR_ERR_ARG, "component out of range");
+  if (f != HPCR_FLAG && !field_ptr(c, f, l)) return set_err(c, HPCR_ERR_ARG, "field not held on this level");
+  if (f == HPCR_FLAG && !c->lev[l].flag) return set_err(c, HPCR_ERR_ARG, "field not held on this level");
+  return HPCR_OK;
+}
+
+int build_levels(hpcr_ctx* c) {
+  const hpcr_geometry& G = c->geo;
+  const int L = G.maxlevel;
+  c->L = L;
+  c->lev.resize(L + 1);
+  // initialize (initialization.cpp:19-134), one rank per axis
+  double hx = (G.xb - G.xa) / G.element_nb[0];
+  double hy = (G.yb - G.ya) / G.element_nb[1];
+  double hz = (G.zb - G.za) / G.element_nb[2];
+  long ii = G.element_nb[0], jj = G.element_nb[1], kk = G.element_nb[2];
+  size_t max_tile_blocks = 1;
+  for (int l = 0; l <= L; l++) {
+    Lev& v = c->lev[l];
+    v.g.nx = (int)ii + 1;
+    v.g.ny = (int)jj + 1;
+    v.g.nz = (int)kk + 1;
+    v.g.hx = hx;
+    v.g.hy = hy;
+    v.g.hz = hz;
+    v.N = (size_t)v.g.nx * v.g.ny * v.g.nz;
+    hx *= 0.5; hy *= 0.5; hz *= 0.5;
+    ii *= 2; jj *= 2; kk *= 2;
+    size_t tb = (size_t)((v.g.nx + OWN - 1) / OWN) * ((v.g.ny + OWN - 1) / OWN) * ((v.g.nz + ZCHUNK - 1) / ZCHUNK);
+    if (tb > max_tile_blocks) max_tile_blocks = tb;
+  }
+  int rc;
+  for (int l = 0; l <= L; l++) {
+    Lev& v = c->lev[l];
+    const size_t N = v.N;
+    double** vec3[] = {&v.u, &v.fu, &v.ru, &v.p, &v.p2, &v.w, &v.invDu};
+    for (double** pp : vec3)
+      if ((rc = alloc_n(c, pp, 3 * N))) return rc;
+    if (l < L && (rc = alloc_n(c, &v.uold, 3 * N))) return rc;
+    double** sc[] = {&v.d, &v.fd, &v.rd, &v.pd, &v.pd2, &v.wd, &v.invDd, &v.H, &v.K, &v.G, &v.gc, &v.Ke, &v.Ge};
+    for (double** pp : sc)
+      if ((rc = alloc_n(c, pp, N))) return rc;
+    if (l < L && (rc = alloc_n(c, &v.dold, N))) return rc;
+    if (l == L) {
+      if ((rc = alloc_n(c, &v.flag, N))) return rc;
+      if ((rc = alloc_n(c, &v.dprev, N))) return rc;
+    }
+    // Dirichlet axis flags by coordinate equality (boundary_conditions.cpp:28-66)
+    std::vector<unsigned char> fx(v.g.nx), fy(v.g.ny), fz(v.g.nz);
+    const double xmid = 0.5 * (G.xa + G.xb), ymid = 0.5 * (G.ya + G.yb);
+    for (int i = 0; i < v.g.nx; i++) {
+      const double x = coord(G.xa, v.g.hx, i);
+      fx[i] = (x == xmid ? 1 : 0) | (x == G.xb ? 2 : 0);
+    }
+    for (int j = 0; j < v.g.ny; j++) {
+      const double y = coord(G.ya, v.g.hy, j);
+      fy[j] = (y == G.ya ? 1 : 0) | (y == ymid ? 2 : 0);
+    }
+    for (int k = 0; k < v.g.nz; k++) {
+      const double z = coord(G.za, v.g.hz, k);
+      fz[k] = (z == G.za ? 1 : 0) | (z == G.zb ? 2 : 0);
+      if (fz[k]) v.kplanes.push_back(k);
+    }
+    if (v.kplanes.size() > 4) return set_err(c, HPCR_ERR_ARG, "more than 4 Dirichlet z-planes on one level");
+    if ((rc = alloc_n(c, &v.fx, v.g.nx)) || (rc = alloc_n(c, &v.fy, v.g.ny)) || (rc = alloc_n(c, &v.fz, v.g.nz)))
+      return rc;
+    HIPCHK(c, hipMemcpy(v.fx, fx.data(), fx.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(v.fy, fy.data(), fy.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(v.fz, fz.data(), fz.size(), hipMemcpyHostToDevice));
+    v.g.fx = v.fx;
+    v.g.fy = v.fy;
+    v.g.fz = v.fz;
+  }
+  c->partial_cap = (max_tile_blocks > 4096 ? max_tile_blocks : 4096) * 4;
+  if ((rc = alloc_n(c, &c->partials, c->partial_cap))) return rc;
+  if ((rc = alloc_n(c, &c->counter, 1))) return rc;
+  if ((rc = alloc_n(c, &c->scal, S_COUNT))) return rc;
+  HIPCHK(c, hipHostMalloc((void**)&c->host_scal, S_COUNT * sizeof(double), hipHostMallocDefault));
+  return HPCR_OK;
+}
+
+void destroy(hpcr_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->trec) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->host_scal) (void)hipHostFree(c->host_scal);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // namespace
+
+// =============================================================================
+// C-ABI
+// =============================================================================
+extern "C" {
+
+int hpcr_device_count(int* n) {
+  if (!n) return HPCR_ERR_ARG;
+  *n = 0;
+  hipError_t e = hipGetDeviceCount(n);
+  if (e != hipSuccess) {
+    *n = 0;
+    return HPCR_ERR_DEVICE;
+  }
+  return HPCR_OK;
+}
+
+const char* hpcr_last_error(const hpcr_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int hpcr_ctx_create(const hpcr_geometry* geo, const hpcr_params* par, int device, hpcr_ctx** out) {
+  g_create_error.clear();
+  if (!geo || !par || !out) {
+    g_create_error = "null argument";
+    return HPCR_ERR_ARG;
+  }
+  *out = nullptr;
+  if (geo->maxlevel < 0 || geo->maxlevel > 12 || geo->element_nb[0] < 1 || geo->element_nb[1] < 1 ||
+      geo->element_nb[2] < 1) {
+    g_create_error = "bad geometry";
+    return HPCR_ERR_ARG;
+  }
+  if (par->mg_u.ncy < 0 || par->mg_d.ncy < 0 || par->mg_u.gamma < 1 || par->mg_d.gamma < 1) {
+    g_create_error = "bad multigrid parameters";
+    return HPCR_ERR_ARG;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    g_create_error = "no HIP device";
+    return HPCR_ERR_DEVICE;
+  }
+  if (device < 0 || device >= n) {
+    g_create_error = "device index out of range";
+    return HPCR_ERR_ARG;
+  }
+  hpcr_ctx* c = new hpcr_ctx();
+  c->device = device;
+  c->geo = *geo;
+  c->par = *par;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    g_create_error = "cannot create stream";
+    destroy(c);
+    return HPCR_ERR_DEVICE;
+  }
+  int rc = build_levels(c);
+  if (rc) {
+    g_create_error = c->err;
+    destroy(c);
+    return rc;
+  }
+  *out = c;
+  return HPCR_OK;
+}
+
+int hpcr_ctx_destroy(hpcr_ctx* ctx) {
+  destroy(ctx);
+  return HPCR_OK;
+}
+
+int hpcr_level_dims(const hpcr_ctx* c, int l, int dims[3]) {
+  if (!c || !dims || l < 0 || l > c->L) return HPCR_ERR_ARG;
+  dims[0] = c->lev[l].g.nx;
+  dims[1] = c->lev[l].g.ny;
+  dims[2] = c->lev[l].g.nz;
+  return HPCR_OK;
+}
+int hpcr_level_h(const hpcr_ctx* c, int l, double h[3]) {
+  if (!c || !h || l < 0 || l > c->L) return HPCR_ERR_ARG;
+  h[0] = c->lev[l].g.hx;
+  h[1] = c->lev[l].g.hy;
+  h[2] = c->lev[l].g.hz;
+  return HPCR_OK;
+}
+long long hpcr_device_bytes(const hpcr_ctx* c) { return c ? c->bytes : 0; }
+
+int hpcr_synchronize(hpcr_ctx* c) {
+  if (!c) return HPCR_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HPCR_OK;
+}
+
+int hpcr_set_field(hpcr_ctx* c, int f, int l, int comp, const double* host) {
+  int rc = check_fl(c, f, l, comp);
+  if (rc) return rc;
+  if (!host) return set_err(c, HPCR_ERR_ARG, "null host pointer");
+  (void)hipSetDevice(c->device);
+  Lev& v = c->lev[l];
+  if (f == HPCR_FLAG) {
+    HIPCHK(c, hipMemcpyAsync(v.wd, host, v.N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_double_to_flag(c->stream, v.N, v.wd, v.flag);
+  } else {
+    double* p = field_ptr(c, f, l) + (size_t)comp * v.N;
+    HIPCHK(c, hipMemcpyAsync(p, host, v.N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  if (f == HPCR_K || f == HPCR_G || f == HPCR_D || f == HPCR_FLAG || f == HPCR_KE || f == HPCR_GE)
+    c->diag_u_valid = false;
+  if (f == HPCR_H || f == HPCR_GC) c->diag_d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HPCR_OK;
+}
+
+int hpcr_get_field(hpcr_ctx* c, int f, int l, int comp, double* host) {
+  int rc = check_fl(c, f, l, comp);
+  if (rc) return rc;
+  if (!host) return set_err(c, HPCR_ERR_ARG, "null host pointer");
+  (void)hipSetDevice(c->device);
+  Lev& v = c->lev[l];
+  if (f == HPCR_FLAG) {
+    // borrow the scalar work field of the phase field as staging
+    launch_flag_to_double(c->stream, v.N, v.flag, v.wd);
+    HIPCHK(c, hipMemcpyAsync(host, v.wd, v.N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    const double* p = field_ptr(c, f, l) + (size_t)comp * v.N;
+    HIPCHK(c, hipMemcpyAsync(host, p, v.N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HPCR_OK;
+}
+
+int hpcr_fill_field(hpcr_ctx* c, int f, int l, int comp, double value) {
+  int rc = check_fl(c, f, l, comp);
+  if (rc) return rc;
+  if (f == HPCR_FLAG) return set_err(c, HPCR_ERR_ARG, "fill of the flag field");
+  (void)hipSetDevice(c->device);
+  Lev& v = c->lev[l];
+  launch_fill(c->stream, v.N, field_ptr(c, f, l) + (size_t)comp * v.N, value);
+  c->diag_u_valid = c->diag_d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "fill");
+}
+
+int hpcr_coarse_vr(hpcr_ctx* c, int f) {
+  if (!c) return HPCR_ERR_ARG;
+  if (f != HPCR_K && f != HPCR_G && f != HPCR_GC) return set_err(c, HPCR_ERR_ARG, "coarse_vr takes K, G or GC");
+  (void)hipSetDevice(c->device);
+  for (int l = c->L; l > 0; l--)
+    launch_nodal_vr(c->stream, c->lev[l - 1].g, c->lev[l].g, field_ptr(c, f, l), field_ptr(c, f, l - 1));
+  c->diag_u_valid = c->diag_d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "coarse_vr");
+}
+
+int hpcr_u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {
+  if (!c || !f_norm) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (st) memset(st, 0, sizeof(*st));
+  return u_entire(c, Ut, t, f_norm, st);
+}
+
+int hpcr_phase_field(hpcr_ctx* c, int t, double* fd_norm, hpcr_stats* st) {
+  if (!c || !fd_norm) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (st) memset(st, 0, sizeof(*st));
+  return phase_field(c, t, fd_norm, st);
+}
+
+int hpcr_strain_history(hpcr_ctx* c, double u_prop) {
+  if (!c) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  launch_strain_history(c->stream, F.g, F.u, F.K, F.G, F.gc, F.H, c->par.lc, u_prop);
+  c->diag_d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "strain_history");
+}
+
+int hpcr_voigt_all_level(hpcr_ctx* c, int f) {
+  if (!c) return HPCR_ERR_ARG;
+  if (f != HPCR_H && f != HPCR_KE && f != HPCR_GE) return set_err(c, HPCR_ERR_ARG, "element field expected");
+  (void)hipSetDevice(c->device);
+  for (int l = c->L; l > 0; l--)
+    launch_elem_vr(c->stream, c->lev[l - 1].g, c->lev[l].g, field_ptr(c, f, l), field_ptr(c, f, l - 1), 1);
+  c->diag_u_valid = c->diag_d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "voigt_all_level");
+}
+
+int hpcr_vr_all_level(hpcr_ctx* c, int f) {
+  if (!c) return HPCR_ERR_ARG;
+  if (f != HPCR_H && f != HPCR_KE && f != HPCR_GE) return set_err(c, HPCR_ERR_ARG, "element field expected");
+  (void)hipSetDevice(c->device);
+  for (int l = c->L; l > 0; l--)
+    launch_elem_vr(c->stream, c->lev[l - 1].g, c->lev[l].g, field_ptr(c, f, l), field_ptr(c, f, l - 1), 0);
+  c->diag_u_valid = c->diag_d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "vr_all_level");
+}
+
+int hpcr_fd_h(hpcr_ctx* c, int l) {
+  if (!c || l < 0 || l > c->L) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  launch_fd_h(c->stream, c->lev[l].g, c->lev[l].H, c->lev[l].fd);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "fd_h");
+}
+
+int hpcr_U_e(hpcr_ctx* c, double* Ue) {
+  if (!c || !Ue) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  launch_elastic_limit(c->stream, F.g, F.u, F.K, F.G, F.gc, c->par.lc, red(c));
+  int rc = read_scal(c);
+  if (rc) return rc;
+  *Ue = c->host_scal[S_RED];
+  return check_launch(c, "U_e");
+}
+
+int hpcr_externe_force_sum(hpcr_ctx* c, double* s) {
+  if (!c || !s) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  launch_top_sum(c->stream, F.g, F.fu + 2 * F.N, red(c));
+  int rc = read_scal(c);
+  if (rc) return rc;
+  *s = c->host_scal[S_RED];
+  return check_launch(c, "externe_force_sum");
+}
+
+int hpcr_delta_d_max(hpcr_ctx* c, double* out) {
+  if (!c || !out) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  launch_max_diff(c->stream, F.N, F.d, F.dprev, red(c));
+  int rc = read_scal(c);
+  if (rc) return rc;
+  *out = c->host_scal[S_RED];
+  return check_launch(c, "delta_d_max");
+}
+
+int hpcr_d_max(hpcr_ctx* c, double* out) {
+  if (!c || !out) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  launch_max_diff(c->stream, F.N, F.d, nullptr, red(c));
+  int rc = read_scal(c);
+  if (rc) return rc;
+  *out = c->host_scal[S_RED];
+  return check_launch(c, "d_max");
+}
+
+int hpcr_save_d_old(hpcr_ctx* c) {
+  if (!c) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  HIPCHK(c, hipMemcpyAsync(F.dprev, F.d, F.N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HPCR_OK;
+}
+
+int hpcr_internal_force(hpcr_ctx* c, int l, int src, int dst) {
+  if (!c || l < 0 || l > c->L || !field_vec(src) || !field_vec(dst) || src == dst) return HPCR_ERR_ARG;
+  if (!field_ptr(c, src, l) || !field_ptr(c, dst, l)) return set_err(c, HPCR_ERR_ARG, "field not held on level");
+  (void)hipSetDevice(c->device);
+  launch_mech_apply(c->stream, c->lev[l].g, mcoef(c, l), field_ptr(c, src, l), field_ptr(c, dst, l));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "internal_force");
+}
+
+int hpcr_prem_u(hpcr_ctx* c, int l, int dst) {
+  if (!c || l < 0 || l > c->L || !field_vec(dst) || !field_ptr(c, dst, l)) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  int rc = mech_diag(c, l, field_ptr(c, dst, l));
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HPCR_OK;
+}
+
+int hpcr_fld(hpcr_ctx* c, int l, int src, int dst) {
+  if (!c || l < 0 || l > c->L || field_vec(src) || field_vec(dst) || src == dst) return HPCR_ERR_ARG;
+  if (!field_ptr(c, src, l) || !field_ptr(c, dst, l)) return set_err(c, HPCR_ERR_ARG, "field not held on level");
+  (void)hipSetDevice(c->device);
+  launch_pf_apply(c->stream, c->lev[l].g, pcoef(c, l), field_ptr(c, src, l), field_ptr(c, dst, l));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "fld");
+}
+
+int hpcr_precondition_m(hpcr_ctx* c, int l, int dst) {
+  if (!c || l < 0 || l > c->L || field_vec(dst) || !field_ptr(c, dst, l)) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  int rc = pf_diag(c, l, field_ptr(c, dst, l));
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HPCR_OK;
+}
+
+int hpcr_mp_choice(hpcr_ctx* c) {
+  if (!c) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  // the means are recomputed as a by-product; only the flag is the result
+  launch_flag_means(c->stream, F.g, F.u, F.K, F.G, F.d, F.flag, F.Ke, F.Ge, c->par.small_k, 1);
+  c->diag_u_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "mp_choice");
+}
+
+int hpcr_mp_all_element(hpcr_ctx* c) {
+  if (!c) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  launch_flag_means(c->stream, F.g, F.u, F.K, F.G, F.d, F.flag, F.Ke, F.Ge, c->par.small_k, 0);
+  c->diag_u_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "mp_all_element");
+}
+
+int hpcr_mech_setup(hpcr_ctx* c) {
+  if (!c) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  int rc = mech_setup(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HPCR_OK;
+}
+
+int hpcr_pcg_u(hpcr_ctx* c, int l, int nb, double Ut, int t, double* f_norm) {
+  if (!c || l < 0 || l > c->L || nb < 0 || !f_norm) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  c->diag_u_valid = false;
+  int rc = pcg_u(c, l, nb, Ut, t);
+  if (rc) return rc;
+  if ((rc = read_scal(c))) return rc;
+  if (t == 0 && l == c->L) *f_norm = c->host_scal[S_FF];
+  return HPCR_OK;
+}
+
+int hpcr_pcg_d(hpcr_ctx* c, int l, int nb, double* fd_norm) {
+  if (!c || l < 0 || l > c->L || nb < 0 || !fd_norm) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  c->diag_d_valid = false;
+  int rc = pcg_d(c, l, nb);
+  if (rc) return rc;
+  if ((rc = read_scal(c))) return rc;
+  *fd_norm = c->host_scal[S_FF];
+  return HPCR_OK;
+}
+
+int hpcr_cycle_u(hpcr_ctx* c, int l, double Ut, int t, double* f_norm) {
+  if (!c || l < 0 || l > c->L || !f_norm) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  int rc = mech_diag_all(c);
+  if (rc) return rc;
+  c->last_rr_level = -1;
+  rc = cycle_u(c, l, Ut, t);
+  c->diag_u_valid = false;
+  if (rc) return rc;
+  if ((rc = read_scal(c))) return rc;
+  if (t == 0 && l == c->L) *f_norm = c->host_scal[S_FF];
+  return HPCR_OK;
+}
+
+int hpcr_cycle_d(hpcr_ctx* c, int l, double* fd_norm) {
+  if (!c || l < 0 || l > c->L || !fd_norm) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  int rc = pf_diag_all(c);
+  if (rc) return rc;
+  rc = cycle_d(c, l);
+  c->diag_d_valid = false;
+  if (rc) return rc;
+  if ((rc = read_scal(c))) return rc;
+  *fd_norm = c->host_scal[S_FF];
+  return HPCR_OK;
+}
+
+int hpcr_injection(hpcr_ctx* c, int lc, int f, int comp) {
+  if (!c || lc < 0 || lc >= c->L) return HPCR_ERR_ARG;
+  int rc = check_fl(c, f, lc, comp);
+  if (rc) return rc;
+  if (!field_ptr(c, f, lc + 1)) return set_err(c, HPCR_ERR_ARG, "field not held on the fine level");
+  (void)hipSetDevice(c->device);
+  Lev& k = c->lev[lc];
+  Lev& fl = c->lev[lc + 1];
+  launch_inject(c->stream, k.g, fl.g, 1, field_ptr(c, f, lc + 1) + (size_t)comp * fl.N,
+                field_ptr(c, f, lc) + (size_t)comp * k.N, nullptr, 0, 0.0);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "injection");
+}
+
+int hpcr_restrict(hpcr_ctx* c, int lf, int rf, int fic, int dst, int comp) {
+  if (!c || lf < 1 || lf > c->L) return HPCR_ERR_ARG;
+  int rc;
+  if ((rc = check_fl(c, rf, lf, comp)) || (rc = check_fl(c, fic, lf - 1, comp)) || (rc = check_fl(c, dst, lf - 1, comp)))
+    return rc;
+  (void)hipSetDevice(c->device);
+  Lev& k = c->lev[lf - 1];
+  Lev& f = c->lev[lf];
+  launch_restrict(c->stream, k.g, f.g, 1, field_ptr(c, rf, lf) + (size_t)comp * f.N,
+                  field_ptr(c, fic, lf - 1) + (size_t)comp * k.N, field_ptr(c, dst, lf - 1) + (size_t)comp * k.N);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "restrict");
+}
+
+int hpcr_prolong(hpcr_ctx* c, int lf, int uf, int uc, int ucold, int comp) {
+  if (!c || lf < 1 || lf > c->L) return HPCR_ERR_ARG;
+  int rc;
+  if ((rc = check_fl(c, uf, lf, comp)) || (rc = check_fl(c, uc, lf - 1, comp)) || (rc = check_fl(c, ucold, lf - 1, comp)))
+    return rc;
+  (void)hipSetDevice(c->device);
+  Lev& k = c->lev[lf - 1];
+  Lev& f = c->lev[lf];
+  launch_prolong(c->stream, f.g, k.g, 1, field_ptr(c, uf, lf) + (size_t)comp * f.N,
+                 field_ptr(c, uc, lf - 1) + (size_t)comp * k.N, field_ptr(c, ucold, lf - 1) + (size_t)comp * k.N, 0,
+                 0.0);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "prolong");
+}
+
+int hpcr_dirichlet_bd(hpcr_ctx* c, int l, int f, double Ut) {
+  if (!c || l < 0 || l > c->L || !field_vec(f) || !field_ptr(c, f, l)) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  Lev& v = c->lev[l];
+  launch_dirichlet_bd(c->stream, v.g, field_ptr(c, f, l), Ut, v.kplanes.data(), (int)v.kplanes.size());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "dirichlet_bd");
+}
+
+int hpcr_timing_enable(hpcr_ctx* c, int on) {
+  if (!c) return HPCR_ERR_ARG;
+  c->timing = on != 0;
+  return HPCR_OK;
+}
+int hpcr_timing_reset(hpcr_ctx* c) {
+  if (!c) return HPCR_ERR_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->trec) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  c->trec.clear();
+  return HPCR_OK;
+}
+int hpcr_timing_query(hpcr_ctx* c, int kernel, int level, double* total_ms, long long* count) {
+  if (!c || !total_ms || !count) return HPCR_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  long long n = 0;
+  for (auto& r : c->trec) {
+    if (r.kernel != kernel || r.level != level) continue;
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, r.a, r.b));
+    tot += ms;
+    n++;
+  }
+  *total_ms = tot;
+  *count = n;
+  return HPCR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+"""GPU parity of the solve entry points and of the staggered load-step loop
+(crack_propagation.cpp:65-121) against the oracle on seeded two-phase images.
+
+Bar: u and d within 1e-9 relative L2 (FP64 path), MG cycle counts identical
+(the acceptance bar allows +-1), crack set d > 0.95 identical."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_common import make_pair, rel_l2, require_gpu, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def HA():
+    return require_gpu()
+
+
+def staggered(obj, steps, voxel_size, lc, schedule="reference", du=None, log=None):
+    """crack_propagation (crack_propagation.cpp:28-123) over an Oracle or a
+    Solver (same method names); returns per-step records."""
+    L = obj.maxlevel
+    f_norm, fd_norm = 0.0, 0.0
+    Ut, delta_U, u_prop = 0.0, 0.0, 1.0
+    recs = []
+    for t in range(steps):
+        f_norm, cyc_u = obj.u_entire(1.0, t, f_norm)
+        if t == 0:
+            if schedule == "reference":
+                Ue = obj.U_e()
+                Ut = Ue
+                delta_U = 1.5e-3 * Ue
+            else:
+                Ut, delta_U = du, du
+            u_prop = Ut
+        obj.strain_history(u_prop)
+        obj.voigt_all_level(O.H)
+        for l in range(L, -1, -1):
+            obj.fd_h(l)
+        fd_norm, cyc_d = obj.phase_field(t, fd_norm)
+        ddmax = obj.delta_d_max()
+        obj.save_d_old()
+        FeW = voxel_size * obj.externe_force_sum()
+        Ut = Ut + delta_U
+        u_prop = Ut
+        recs.append(dict(t=t, cyc_u=cyc_u, cyc_d=cyc_d, f_norm=f_norm, fd_norm=fd_norm, Ut=Ut,
+                         Fw=u_prop * FeW, ddmax=ddmax))
+        if log:
+            log(f"  t={t} cyc_u={cyc_u} cyc_d={cyc_d} Fw={u_prop * FeW:.6e}")
+    return recs
+
+
+@pytest.mark.parametrize("ml,tol_u", [(2, 1e-6), (3, 1e-6), (3, 1e-16)], ids=["17c", "33c", "33c-tight"])
+def test_u_entire_and_phase_field(HA, ml, tol_u):
+    dom = (0.0, 4.0 * 2 ** ml, 0.0, 4.0 * 2 ** ml, 0.0, 4.0 * 2 ** ml)
+    say(f"[gpu] u_entire ml={ml} tol={tol_u}")
+    kw = dict(lc=2.0, small_k=1e-6, tol_u=tol_u, mg_u=(50, 4, 2, 1, 30), mg_d=(10, 2, 1, 1, 30),
+              tol_d_rel=1e-16, tol_d_abs=1e-30)
+    o, s, _ = make_pair(HA, ml, (4, 4, 4), dom, seed=20, lab_incl=4, **kw)
+    L = ml
+    fo, co = o.u_entire(1.0, 0, 0.0)
+    fs, cs = s.u_entire(1.0, 0, 0.0)
+    say(f"  cycles oracle={co} gpu={cs} f_norm {fo:.6e} {fs:.6e}")
+    assert cs == co
+    assert fs == pytest.approx(fo, rel=1e-9)
+    assert rel_l2(s.get(O.U, L), o.vec(O.U, L)) < 1e-9
+    assert rel_l2(s.get(O.FU, L), o.vec(O.FU, L)) < 1e-9
+    hs, ho = s.history(), o.history("u")
+    assert len(hs) == len(ho)
+    assert np.allclose(hs, ho, rtol=1e-6, atol=0)
+    # phase field driven by the strain history of this solution
+    Ue = o.U_e()
+    assert s.U_e() == pytest.approx(Ue, rel=1e-9)
+    for obj in (o, s):
+        obj.strain_history(Ue * 3.0)
+        obj.voigt_all_level(O.H)
+        for l in range(L, -1, -1):
+            obj.fd_h(l)
+    assert rel_l2(s.get(O.H, L), o.field(O.H, L)) < 1e-9
+    fdo, cdo = o.phase_field(0, 0.0)
+    fds, cds = s.phase_field(0, 0.0)
+    say(f"  pf cycles oracle={cdo} gpu={cds}")
+    assert cds == cdo
+    assert fds == pytest.approx(fdo, rel=1e-12)
+    assert rel_l2(s.get(O.D, L), o.field(O.D, L)) < 1e-9
+
+
+def test_staggered_steps_match_oracle(HA):
+    ml = 2
+    dom = (0.0, 16.0, 0.0, 16.0, 0.0, 16.0)
+    say("[gpu] staggered 17^3, 6 steps")
+    kw = dict(lc=2.0, small_k=1e-6, tol_u=1e-16, mg_u=(50, 4, 2, 1, 30), mg_d=(10, 2, 1, 1, 30),
+              tol_d_rel=1e-16, tol_d_abs=1e-30)
+    o, s, _ = make_pair(HA, ml, (4, 4, 4), dom, seed=21, lab_incl=3, **kw)
+    ro = staggered(o, 6, 1e-6, 2.0, schedule="fixed", du=0.05)
+    rs = staggered(s, 6, 1e-6, 2.0, schedule="fixed", du=0.05)
+    for a, b in zip(ro, rs):
+        say(f"  t={a['t']} cyc_u {a['cyc_u']}/{b['cyc_u']} cyc_d {a['cyc_d']}/{b['cyc_d']} Fw {a['Fw']:.6e}/{b['Fw']:.6e}")
+        assert abs(a["cyc_u"] - b["cyc_u"]) <= 1 and abs(a["cyc_d"] - b["cyc_d"]) <= 1
+        assert b["Fw"] == pytest.approx(a["Fw"], rel=1e-8)
+        assert b["ddmax"] == pytest.approx(a["ddmax"], rel=1e-8, abs=1e-14)
+    L = ml
+    assert rel_l2(s.get(O.U, L), o.vec(O.U, L)) < 1e-9
+    assert rel_l2(s.get(O.D, L), o.field(O.D, L)) < 1e-9
+    assert rel_l2(s.get(O.H, L), o.field(O.H, L)) < 1e-9
+    do, ds = o.field(O.D, L), s.get(O.D, L)
+    near = np.abs(do - 0.95) < 1e-6
+    assert np.array_equal((do > 0.95)[~near], (ds > 0.95)[~near])
