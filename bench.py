@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain FP64 MG-PCG mechanical solve on one MI355X.
+
+Workload (default): the 257^3-node three-phase image (200 stiff inclusions
+r ~ U(3,10), pores r ~ U(2,5) to 5 %, seed 1002), element_nb 4 (7 levels),
+multigrid u = (nu0 50, nu1 4, nu2 2, V-cycle), k = 1e-6, relative residual
+1e-8 (1e-16 in the solver's squared convention), at most 30 cycles.
+
+A step is one u_entire solve from u = 0 (t = 0): split flag, element moduli,
+coarse homogenisation, diagonals, initial residual, V-cycles to the tolerance
+or the cap, and the Dirichlet reactions.  All inputs are in HBM before the
+timed region.  value = 3 N_fine * (V-cycles in the timed steps) / time.
+
+The JSON line also carries the roofline of the dominant kernel (the fused
+p-update + degraded-elasticity apply + p.Ap pass on the finest level, timed
+with HIP events on the solver's stream) and a CPU baseline: the oracle run on
+the host cores over a bounded sample (see cpu_baseline.sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# canonical algorithmic bytes per fine node of the fused pass A:
+# read r, 1/D, p_old (3 x 24), K, G, d (24), flag (1); write p, Ap (2 x 24)
+PASS_A_BYTES_PER_NODE = 145.0
+METRIC = "MDOF·iter/s + solves/s, FP32-MG vs FP64 @1e-8 residual; p50/p99 serve latency"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(path):
+    """'bytes_per_launch: X' line of a committed rocprofv3 PMC summary."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.strip().startswith("bytes_per_launch:"):
+                try:
+                    return float(line.split(":", 1)[1].strip().split()[0])
+                except ValueError:
+                    return None
+    return None
+
+
+def cpu_baseline(workload, n_sample, cycles, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    import hpcrack_amd.workloads as W
+
+    w = W.scaled(workload, n_sample)
+    w.mg_u = (w.mg_u[0], w.mg_u[1], w.mg_u[2], w.mg_u[3], cycles)
+    img, frac, K, G, gc = w.materials()
+    O.set_threads(threads)
+    o = O.Oracle(w.maxlevel, w.element_nb, w.domain, **w.solver_kwargs())
+    L = w.maxlevel
+    for f, a in ((O.K, K), (O.G, G), (O.GC, gc)):
+        o.set(f, L, a)
+    for f in (O.K, O.G, O.GC):
+        o.coarse_vr_nodal(f)
+    t0 = time.perf_counter()
+    _, cyc = o.u_entire(1.0, 0, 0.0)
+    dt = time.perf_counter() - t0
+    dof = 3 * int(np.prod(w.dims))
+    return {
+        "value": dof * cyc / dt / 1e6,
+        "unit": "MDOF·iter/s",
+        "cores": O.lib().ora_get_threads(),
+        "kind": "port",
+        "sample": f"oracle u_entire on the same generator scaled to {n_sample}^3 nodes "
+                  f"({w.n_incl} inclusions, {100 * frac[1]:.1f}% pores), t=0 from u=0, "
+                  f"{cyc} V-cycles in {dt:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="C2")
+    ap.add_argument("--ncy", type=int, default=30)
+    ap.add_argument("--cpu-sample", type=int, default=129, help="nodes per axis of the CPU sample (0: skip)")
+    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_mech_pcg_a_C2_257.txt"))
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is a design, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import hpcrack_amd as HA
+    import hpcrack_amd.workloads as W
+    from hpcrack_amd.capi import FU, K_MECH_PCG_A, K_MECH_PCG_B, K_MECH_RESID, U
+
+    w = W.WORKLOADS[args.workload]
+    w.mg_u = (w.mg_u[0], w.mg_u[1], w.mg_u[2], w.mg_u[3], args.ncy)
+    t = time.perf_counter()
+    img, frac, K, G, gc = w.materials()
+    log(f"[bench] {w.name}: dims {w.dims}, levels {w.maxlevel + 1}, inclusions {frac[0]:.4f}, "
+        f"pores {frac[1]:.4f} ({time.perf_counter() - t:.1f} s)")
+    s = HA.Solver(w.maxlevel, w.element_nb, w.domain, **w.solver_kwargs())
+    W.upload(s, w, K, G, gc)
+    L = w.maxlevel
+    N = int(np.prod(w.dims))
+    log(f"[bench] device memory {s.device_bytes() / 2**30:.2f} GiB")
+
+    def step():
+        s.fill(U, L, 0.0, 0), s.fill(U, L, 0.0, 1), s.fill(U, L, 0.0, 2)
+        s.fill(FU, L, 0.0, 0), s.fill(FU, L, 0.0, 1), s.fill(FU, L, 0.0, 2)
+        fn, cyc = s.u_entire(1.0, 0, 0.0)
+        return fn, cyc
+
+    for i in range(args.warmup):
+        fn, cyc = step()
+        log(f"[bench] warmup {i}: {cyc} cycles, rel. residual^2 {s.last_stats.r_norm_final / fn:.3e}")
+
+    s.timing(True)
+    s.timing_reset()
+    s.synchronize()
+    t0 = time.perf_counter()
+    cycles = 0
+    rels = []
+    for i in range(args.steps):
+        fn, cyc = step()
+        cycles += cyc
+        rels.append(s.last_stats.r_norm_final / fn)
+    s.synchronize()
+    t1 = time.perf_counter()
+    s.timing(False)
+    elapsed = t1 - t0
+    log(f"[bench] timed {args.steps} solves, {cycles} cycles in {elapsed:.3f} s")
+
+    kern = {}
+    for name, kid in (("mech_pcg_a", K_MECH_PCG_A), ("mech_pcg_b", K_MECH_PCG_B), ("mech_resid", K_MECH_RESID)):
+        ms, n = s.timing_query(kid, L)
+        kern[name] = {"total_ms": ms, "launches": n, "avg_ms": ms / n if n else None}
+    a = kern["mech_pcg_a"]
+    avg_s = a["avg_ms"] / 1e3
+    achieved = PASS_A_BYTES_PER_NODE * N / avg_s / 1e9
+    traffic = read_traffic(args.traffic_file)
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "kernel": "mech_pcg_a (finest level)",
+        "avg_launch_ms": round(a["avg_ms"], 4),
+        "algorithmic_bytes_per_launch": PASS_A_BYTES_PER_NODE * N,
+        "fine_level_share": round(sum(k["total_ms"] for k in kern.values()) / (elapsed * 1e3), 3),
+    }
+    value = 3 * N * cycles / elapsed / 1e6
+
+    cpu = None
+    if args.cpu_sample > 0:
+        log(f"[bench] cpu baseline: oracle at {args.cpu_sample}^3, {args.cpu_cycles} cycles, "
+            f"{args.cpu_threads} threads")
+        cpu = cpu_baseline(w, args.cpu_sample, args.cpu_cycles, args.cpu_threads)
+        log(f"[bench] cpu baseline {cpu['value']:.3f} MDOF-iter/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "MDOF·iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: plain FP64 u_entire, {w.dims[0]}x{w.dims[1]}x{w.dims[2]} nodes, "
+                        f"{w.maxlevel + 1} levels, nu_u=({w.mg_u[0]},{w.mg_u[1]},{w.mg_u[2]}) V-cycle",
+            "dof": 3 * N,
+            "cycles_per_solve": cycles / args.steps,
+            "solves_per_s": round(args.steps / elapsed, 3),
+            "tol_u_squared": w.tol_u,
+            "ncy": args.ncy,
+            "final_rel_residual_squared": max(rels),
+            "inclusion_fraction": round(frac[0], 4),
+            "pore_fraction": round(frac[1], 4),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_finest_level": kern,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

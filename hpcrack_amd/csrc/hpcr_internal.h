@@ -14,7 +14,7 @@ constexpr int TILE = 16;
 constexpr int OWN = TILE - 1;
 constexpr int TPB = TILE * TILE;        // 256 threads
 constexpr int PLANE = (TILE + 1) * (TILE + 1);  // 289 nodes per LDS plane
-constexpr int ZCHUNK = 16;              // node planes per block
+constexpr int ZCHUNK_MAX = 16;          // node planes per block (fine levels)
 
 // Device scalar slots (one set per context, reused by every level: the
 // smoothers of different levels never overlap in time).
@@ -32,6 +32,7 @@ enum Scal {
 
 struct LevelGeom {
   int nx, ny, nz;
+  int zc;  // node planes per block of the tile kernels (small levels use short chunks)
   double hx, hy, hz;
   const unsigned char* fx;  // Dirichlet axis flags: bit0 x == x_mid, bit1 x == x_b
   const unsigned char* fy;  // bit0 y == y_a,   bit1 y == y_mid
@@ -62,6 +63,18 @@ struct PfCoef {
   const double* gc;  // nodal
   double lc;
 };
+
+// z-chunk of the tile kernels: the longest chunk that still gives >= 1024
+// blocks, so that small levels are not serialised along z
+inline int choose_zchunk(int nx, int ny, int nz) {
+  const long xy = (long)((nx + OWN - 1) / OWN) * ((ny + OWN - 1) / OWN);
+  for (int zc = ZCHUNK_MAX; zc > 2; zc /= 2)
+    if (xy * ((nz + zc - 1) / zc) >= 1024) return zc;
+  return 2;
+}
+inline long tile_blocks(int nx, int ny, int nz, int zc) {
+  return (long)((nx + OWN - 1) / OWN) * ((ny + OWN - 1) / OWN) * ((nz + zc - 1) / zc);
+}
 
 // ---- launchers (hpcr_kernels.hip); all asynchronous on `s` -------------
 // y = A x  (3 comps, stride N)
@@ -104,9 +117,10 @@ void launch_inject(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int 
 // f_c = Fic + full weighting of r_f
 void launch_restrict(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* rf,
                      const double* Fic, double* fc);
-// u_f += P(u_c - u_c,old), optional Dirichlet values
+// u_f += P(u_c - u_c,old), optional Dirichlet values; `scratch` holds the
+// coarse difference (ncomp * N_c doubles, may alias neither input)
 void launch_prolong(hipStream_t s, const LevelGeom& Lf, const LevelGeom& Lc, int ncomp, double* uf,
-                    const double* uc, const double* ucold, int apply_bc, double Ut);
+                    const double* uc, const double* ucold, int apply_bc, double Ut, double* scratch);
 
 // split flag + element means on the finest level
 void launch_flag_means(hipStream_t s, const LevelGeom& L, const double* u, const double* K, const double* G,

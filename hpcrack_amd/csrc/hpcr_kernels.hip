@@ -41,7 +41,7 @@ __device__ __forceinline__ double Lw(int i, int g) { return (i == g) ? GA : GB; 
 __device__ __forceinline__ size_t nidx(const LevelGeom& L, int i, int j, int k) {
   return (size_t)i + (size_t)L.nx * ((size_t)j + (size_t)L.ny * (size_t)k);
 }
-__device__ __forceinline__ size_t Nof(const LevelGeom& L) { return (size_t)L.nx * L.ny * L.nz; }
+__host__ __device__ __forceinline__ size_t Nof(const LevelGeom& L) { return (size_t)L.nx * L.ny * L.nz; }
 
 // ---------------------------------------------------------------------------
 // Reductions
@@ -464,8 +464,8 @@ __global__ __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op o
   const int tid = threadIdx.x;
   const int tx = tid % TILE, ty = tid / TILE;
   const int bx0 = blockIdx.x * OWN, by0 = blockIdx.y * OWN;
-  const int z0 = blockIdx.z * ZCHUNK;
-  const int zend = min(z0 + ZCHUNK, L.nz);
+  const int z0 = blockIdx.z * L.zc;
+  const int zend = min(z0 + L.zc, L.nz);
   const int ex = bx0 - 1 + tx, ey = by0 - 1 + ty;
   const bool owner = (tx >= 1) && (ty >= 1) && (ex < L.nx) && (ey < L.ny);
   const bool exy = (ex >= 0) && (ey >= 0) && (ex <= L.nx - 2) && (ey <= L.ny - 2);
@@ -580,10 +580,10 @@ struct MechOp {
     for (int k = 0; k < NV; k++) v[k] = 0.0;
     if (!ok) return;
     const size_t q = nidx(L, gx, gy, gz);
-    if (MODE == M_APPLY || MODE == M_RESID) {
+    if constexpr (MODE == M_APPLY || MODE == M_RESID) {
 #pragma unroll
       for (int c = 0; c < NU; c++) v[c] = x[c * N + q];
-    } else if (MODE == M_PCGA) {
+    } else if constexpr (MODE == M_PCGA) {
       double beta = 0.0;
       if (!flag) beta = R.scal[S_BETA];
 #pragma unroll
@@ -592,7 +592,7 @@ struct MechOp {
         v[c] = flag ? z : z + beta * p_old[c * N + q];
       }
     }
-    if (FINE) {
+    if constexpr (FINE) {
       v[NU + 0] = C.K[q];
       v[NU + 1] = C.G[q];
       v[NU + 2] = C.d[q];
@@ -602,7 +602,7 @@ struct MechOp {
   __device__ __forceinline__ void lame(const double (&sN)[2][NV][PLANE], int s0, int tx, int ty, int ex, int ey,
                                        int ez, const LevelGeom& L, double (&lam)[2][2][2], double (&mu)[2][2][2],
                                        double& lam0, double& mu0) const {
-    if (FINE) {
+    if constexpr (FINE) {
       double Kn[2][2][2], Gn[2][2][2], dn[2][2][2];
       gather8<NV>(sN, s0, tx, ty, NU + 0, Kn);
       gather8<NV>(sN, s0, tx, ty, NU + 1, Gn);
@@ -631,13 +631,13 @@ struct MechOp {
                                           int ty, int ex, int ey, int ez, double (&f)[3][2][2][2]) const {
     double lam[2][2][2], mu[2][2][2], lam0, mu0;
     lame(sN, s0, tx, ty, ex, ey, ez, L, lam, mu, lam0, mu0);
-    if (MODE == M_DIAG) {
+    if constexpr (MODE == M_DIAG) {
       mech_element_diag(lam, mu, ihx2, ihy2, ihz2, wg, f);
     } else {
       double u[3][2][2][2];
 #pragma unroll
       for (int c = 0; c < 3; c++) gather8<NV>(sN, s0, tx, ty, c, u[c]);
-      if (FINE)
+      if constexpr (FINE)
         mech_element<false>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
       else
         mech_element<true>(u, lam, mu, lam0, mu0, sx, sy, sz, tx_, ty_, tz_, f);
@@ -647,10 +647,10 @@ struct MechOp {
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
                                            int ty, int i, int j, int k, const double (&v)[3], Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
-    if (MODE == M_APPLY) {
+    if constexpr (MODE == M_APPLY) {
 #pragma unroll
       for (int c = 0; c < 3; c++) out[c * N + q] = v[c];
-    } else if (MODE == M_RESID) {
+    } else if constexpr (MODE == M_RESID) {
       const int dm = dir_mask(L, i, j, k);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -666,7 +666,7 @@ struct MechOp {
         acc.a[1] += r * r;
         acc.a[2] += fv * fv;
       }
-    } else if (MODE == M_PCGA) {
+    } else if constexpr (MODE == M_PCGA) {
       const int qs = tx + (TILE + 1) * ty;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -804,7 +804,7 @@ struct PfOp {
 };
 
 static dim3 tile_grid(const LevelGeom& L) {
-  return dim3((L.nx + OWN - 1) / OWN, (L.ny + OWN - 1) / OWN, (L.nz + ZCHUNK - 1) / ZCHUNK);
+  return dim3((L.nx + OWN - 1) / OWN, (L.ny + OWN - 1) / OWN, (L.nz + L.zc - 1) / L.zc);
 }
 
 template <int MODE, bool FINE>
@@ -1022,11 +1022,12 @@ void launch_dirichlet_copy(hipStream_t s, const LevelGeom& L, double* f, const d
 // Transfers (multigrid_intergrid.cpp)
 // ---------------------------------------------------------------------------
 // solution_injection + Dirichlet_BD + copy to uold (multigrid_intergrid.cpp:25-121)
-__global__ void inject_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp, const double* __restrict__ uf,
-                              double* __restrict__ uc, double* __restrict__ ucold, int apply_bc, double Ut) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= Lc.nx) return;
+__global__ __launch_bounds__(256) void inject_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
+                                                     const double* __restrict__ uf, double* __restrict__ uc,
+                                                     double* __restrict__ ucold, int apply_bc, double Ut) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= Lc.nx || j >= Lc.ny) return;
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const size_t qc = nidx(Lc, i, j, k), qf = nidx(Lf, 2 * i, 2 * j, 2 * k);
   const int dm = apply_bc ? dir_mask(Lc, i, j, k) : 0;
@@ -1039,84 +1040,97 @@ __global__ void inject_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
 }
 void launch_inject(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* uf,
                    double* uc, double* ucold, int apply_bc, double Ut) {
-  hipLaunchKernelGGL(inject_kernel, dim3((Lc.nx + 63) / 64, Lc.ny, Lc.nz), dim3(64), 0, s, Lc, Lf, ncomp, uf, uc,
-                     ucold, apply_bc, Ut);
+  hipLaunchKernelGGL(inject_kernel, dim3((Lc.nx + 63) / 64, (Lc.ny + 3) / 4, Lc.nz), dim3(64, 4), 0, s, Lc, Lf,
+                     ncomp, uf, uc, ucold, apply_bc, Ut);
 }
 
-// coarsen_f_standard_routine (multigrid_intergrid.cpp:124-342)
-__global__ void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp, const double* __restrict__ rf,
-                                const double* __restrict__ Fic, double* __restrict__ fc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= Lc.nx) return;
+// 2-D blocks of 64 x 4 threads over (x, y), one z-plane per blockIdx.z
+constexpr int XB = 64, YB = 4;
+static dim3 node_grid(const LevelGeom& L) { return dim3((L.nx + XB - 1) / XB, (L.ny + YB - 1) / YB, L.nz); }
+
+// coarsen_f_standard_routine (multigrid_intergrid.cpp:124-342): full weighting
+// with out-of-range neighbours counted as 0, written in its separable form
+// (1-D weights 1/2, 1, 1/2 per axis)
+__global__ __launch_bounds__(XB * YB) void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
+                                                           const double* __restrict__ rf,
+                                                           const double* __restrict__ Fic,
+                                                           double* __restrict__ fc) {
+  const int i = blockIdx.x * XB + threadIdx.x;
+  const int j = blockIdx.y * YB + threadIdx.y, k = blockIdx.z;
+  if (i >= Lc.nx || j >= Lc.ny) return;
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const int x = 2 * i, y = 2 * j, z = 2 * k;
+  double wx[3], wy[3], wz[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    wx[d] = (x + d - 1 < 0 || x + d - 1 > Lf.nx - 1) ? 0.0 : (d == 1 ? 1.0 : 0.5);
+    wy[d] = (y + d - 1 < 0 || y + d - 1 > Lf.ny - 1) ? 0.0 : (d == 1 ? 1.0 : 0.5);
+    wz[d] = (z + d - 1 < 0 || z + d - 1 > Lf.nz - 1) ? 0.0 : (d == 1 ? 1.0 : 0.5);
+  }
+  const size_t qc = nidx(Lc, i, j, k);
   for (int c = 0; c < ncomp; c++) {
     const double* r = rf + c * Nf;
-    double s8 = 0.0, s4 = 0.0, s2 = 0.0, s1 = 0.0;
-    for (int dz = -1; dz <= 1; dz++) {
-      const int zz = z + dz;
-      if (zz < 0 || zz > Lf.nz - 1) continue;
-      for (int dy = -1; dy <= 1; dy++) {
-        const int yy = y + dy;
-        if (yy < 0 || yy > Lf.ny - 1) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-          const int xx = x + dx;
-          if (xx < 0 || xx > Lf.nx - 1) continue;
-          const double v = r[nidx(Lf, xx, yy, zz)];
-          const int o = (dx != 0) + (dy != 0) + (dz != 0);
-          if (o == 0) s8 += v;
-          else if (o == 1) s4 += v;
-          else if (o == 2) s2 += v;
-          else s1 += v;
-        }
+    double acc = 0.0;
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++) {
+      if (wz[dz] == 0.0) continue;
+      double accy = 0.0;
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) {
+        if (wy[dy] == 0.0) continue;
+        const size_t row = nidx(Lf, 0, y + dy - 1, z + dz - 1);
+        double accx = 0.0;
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+          if (wx[dx] != 0.0) accx += wx[dx] * r[row + x + dx - 1];
+        accy += wy[dy] * accx;
       }
+      acc += wz[dz] * accy;
     }
-    const size_t qc = nidx(Lc, i, j, k);
-    fc[c * Nc + qc] = Fic[c * Nc + qc] + 0.125 * (s1 + 2.0 * s2 + 4.0 * s4 + 8.0 * s8);
+    fc[c * Nc + qc] = Fic[c * Nc + qc] + acc;
   }
 }
 void launch_restrict(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* rf,
                      const double* Fic, double* fc) {
-  hipLaunchKernelGGL(restrict_kernel, dim3((Lc.nx + 63) / 64, Lc.ny, Lc.nz), dim3(64), 0, s, Lc, Lf, ncomp, rf, Fic,
-                     fc);
+  hipLaunchKernelGGL(restrict_kernel, node_grid(Lc), dim3(XB, YB), 0, s, Lc, Lf, ncomp, rf, Fic, fc);
 }
 
-// refine_standard_routine + Dirichlet_BD (multigrid_intergrid.cpp:416-549)
-__global__ void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp, double* __restrict__ uf,
-                               const double* __restrict__ uc, const double* __restrict__ ucold, int apply_bc,
-                               double Ut) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  if (x >= Lf.nx) return;
+// e_c = u_c - u_c,old (the coarse-grid correction, multigrid_intergrid.cpp:484-518)
+__global__ void diff_kernel(size_t n, const double* __restrict__ a, const double* __restrict__ b,
+                            double* __restrict__ e) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    e[q] = a[q] - b[q];
+}
+
+// refine_standard_routine + Dirichlet_BD (multigrid_intergrid.cpp:416-549):
+// trilinear interpolation of the correction e_c, one writer per fine node
+__global__ __launch_bounds__(XB * YB) void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp,
+                                                          double* __restrict__ uf, const double* __restrict__ e,
+                                                          int apply_bc, double Ut) {
+  const int x = blockIdx.x * XB + threadIdx.x;
+  const int y = blockIdx.y * YB + threadIdx.y, z = blockIdx.z;
+  if (x >= Lf.nx || y >= Lf.ny) return;
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
   const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
+  const double wx = (x & 1) ? 0.5 : 1.0, wy = (y & 1) ? 0.5 : 1.0, wz = (z & 1) ? 0.5 : 1.0;
   const size_t qf = nidx(Lf, x, y, z);
   const int dm = apply_bc ? dir_mask(Lf, x, y, z) : 0;
+  const size_t q00 = nidx(Lc, 0, j0, k0), q10 = nidx(Lc, 0, j1, k0), q01 = nidx(Lc, 0, j0, k1),
+               q11 = nidx(Lc, 0, j1, k1);
   for (int c = 0; c < ncomp; c++) {
-    const double* a = uc + c * Nc;
-    const double* b = ucold + c * Nc;
-    auto E = [&](int ii, int jj, int kk) {
-      const size_t q = nidx(Lc, ii, jj, kk);
-      return a[q] - b[q];
-    };
+    const double* ec = e + c * Nc;
     double corr;
-    const int odd = (x & 1) + (y & 1) + (z & 1);
-    if (odd == 0) {
-      corr = E(i0, j0, k0);
-    } else if (odd == 1) {
-      corr = 0.5 * (E(i1, j1, k1) + E(i0, j0, k0));
-    } else if (odd == 2) {
-      if (!(z & 1))
-        corr = 0.25 * (E(i1, j1, k0) + E(i1, j0, k0) + E(i0, j1, k0) + E(i0, j0, k0));
-      else if (!(y & 1))
-        corr = 0.25 * (E(i1, j0, k1) + E(i0, j0, k0) + E(i1, j0, k0) + E(i0, j0, k1));
-      else
-        corr = 0.25 * (E(i0, j1, k1) + E(i0, j0, k0) + E(i0, j1, k0) + E(i0, j0, k1));
+    if (!(x & 1) && !(y & 1) && !(z & 1)) {
+      corr = ec[q00 + i0];
     } else {
-      corr = 0.125 * (E(i1, j1, k1) + E(i1, j0, k1) + E(i0, j1, k1) + E(i0, j0, k1) + E(i1, j1, k0) +
-                      E(i0, j1, k0) + E(i1, j0, k0) + E(i0, j0, k0));
+      // separable trilinear weights (0.5 on each odd axis); identical to the
+      // reference's averages of 2, 4 or 8 corrections up to rounding order
+      const double a00 = ec[q00 + i0] + ((x & 1) ? ec[q00 + i1] : 0.0);
+      const double a10 = (y & 1) ? ec[q10 + i0] + ((x & 1) ? ec[q10 + i1] : 0.0) : 0.0;
+      const double a01 = (z & 1) ? ec[q01 + i0] + ((x & 1) ? ec[q01 + i1] : 0.0) : 0.0;
+      const double a11 = ((y & 1) && (z & 1)) ? ec[q11 + i0] + ((x & 1) ? ec[q11 + i1] : 0.0) : 0.0;
+      corr = wx * wy * wz * (a00 + a10 + a01 + a11);
     }
     double v = uf[c * Nf + qf] + corr;
     if ((dm >> c) & 1) v = (c == 2 && (dm & 8)) ? Ut : 0.0;
@@ -1124,9 +1138,10 @@ __global__ void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp
   }
 }
 void launch_prolong(hipStream_t s, const LevelGeom& Lf, const LevelGeom& Lc, int ncomp, double* uf,
-                    const double* uc, const double* ucold, int apply_bc, double Ut) {
-  hipLaunchKernelGGL(prolong_kernel, dim3((Lf.nx + 63) / 64, Lf.ny, Lf.nz), dim3(64), 0, s, Lf, Lc, ncomp, uf, uc,
-                     ucold, apply_bc, Ut);
+                    const double* uc, const double* ucold, int apply_bc, double Ut, double* scratch) {
+  const size_t n = (size_t)ncomp * Nof(Lc);
+  hipLaunchKernelGGL(diff_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, uc, ucold, scratch);
+  hipLaunchKernelGGL(prolong_kernel, node_grid(Lf), dim3(XB, YB), 0, s, Lf, Lc, ncomp, uf, scratch, apply_bc, Ut);
 }
 
 // ---------------------------------------------------------------------------

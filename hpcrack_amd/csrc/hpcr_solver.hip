@@ -143,13 +143,13 @@ struct TBracket {
   int kernel, level;
   hipEvent_t a = nullptr, b = nullptr;
   TBracket(hpcr_ctx* c_, int k, int l) : c(c_), kernel(k), level(l) {
-    if (!c->timing) return;
+    if (!c->timing || l != c->L) return;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     (void)hipEventRecord(a, c->stream);
   }
   ~TBracket() {
-    if (!c->timing) return;
+    if (!c->timing || level != c->L) return;
     (void)hipEventRecord(b, c->stream);
     c->trec.push_back(TimingRec{kernel, level, a, b});
   }
@@ -232,7 +232,7 @@ int cycle_u(hpcr_ctx* c, int l, double Ut, int t) {
     if ((rc = cycle_u(c, l - 1, Ut, t))) return rc;
   Lev& f = c->lev[l];
   Lev& k = c->lev[l - 1];
-  launch_prolong(c->stream, f.g, k.g, 3, f.u, k.u, k.uold, 1, Ut);
+  launch_prolong(c->stream, f.g, k.g, 3, f.u, k.u, k.uold, 1, Ut, k.w);
   return pcg_u(c, l, c->par.mg_u.nu2, Ut, t);
 }
 
@@ -354,7 +354,7 @@ int cycle_d(hpcr_ctx* c, int l) {
   for (int j = 1; j <= c->par.mg_d.gamma; j++)
     if ((rc = cycle_d(c, l - 1))) return rc;
   // refine_d (multigrid_intergrid.cpp:523-533)
-  launch_prolong(c->stream, f.g, k.g, 1, f.d, k.d, k.dold, 0, 0.0);
+  launch_prolong(c->stream, f.g, k.g, 1, f.d, k.d, k.dold, 0, 0.0, k.wd);
   for (int i = 1; i <= c->par.mg_d.nu2; i++)
     if ((rc = pcg_d(c, l, c->par.mg_d.nu2))) return rc;
   return check_launch(c, "cycle_d");
@@ -470,9 +470,10 @@ int build_levels(hpcr_ctx* c) {
     v.g.hy = hy;
     v.g.hz = hz;
     v.N = (size_t)v.g.nx * v.g.ny * v.g.nz;
+    v.g.zc = choose_zchunk(v.g.nx, v.g.ny, v.g.nz);
     hx *= 0.5; hy *= 0.5; hz *= 0.5;
     ii *= 2; jj *= 2; kk *= 2;
-    size_t tb = (size_t)((v.g.nx + OWN - 1) / OWN) * ((v.g.ny + OWN - 1) / OWN) * ((v.g.nz + ZCHUNK - 1) / ZCHUNK);
+    size_t tb = (size_t)tile_blocks(v.g.nx, v.g.ny, v.g.nz, v.g.zc);
     if (tb > max_tile_blocks) max_tile_blocks = tb;
   }
   int rc;
@@ -949,9 +950,13 @@ int hpcr_prolong(hpcr_ctx* c, int lf, int uf, int uc, int ucold, int comp) {
   (void)hipSetDevice(c->device);
   Lev& k = c->lev[lf - 1];
   Lev& f = c->lev[lf];
+  double* tmp = nullptr;
+  if (hipMallocAsync((void**)&tmp, k.N * sizeof(double), c->stream) != hipSuccess)
+    return set_err(c, HPCR_ERR_MEMORY, "prolong scratch");
   launch_prolong(c->stream, f.g, k.g, 1, field_ptr(c, uf, lf) + (size_t)comp * f.N,
                  field_ptr(c, uc, lf - 1) + (size_t)comp * k.N, field_ptr(c, ucold, lf - 1) + (size_t)comp * k.N, 0,
-                 0.0);
+                 0.0, tmp);
+  (void)hipFreeAsync(tmp, c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c, "prolong");
 }

@@ -1,0 +1,110 @@
+"""Seeded synthetic workloads (voxel images = nodes, as in HPCrack where an
+N^3-voxel image gives N^3 nodes).
+
+Phases (gray level = label, so the stepwise CSV interpolation of image_MP is
+exact): matrix 0 (E 3 GPa, nu 0.3, gc 0.1 N/mm = 100 N/m), stiff inclusion 1
+(E 70 GPa, nu 0.2, gc 1 N/mm = 1000 N/m), pore 2 (E 1e-6 E_matrix, nu 0.3,
+gc 1e-3 N/mm = 1 N/m).  Voxel 1 um, element_nb 4 on the coarsest level,
+coordinates in voxel units (h_fine = 1), lc = 2 voxels, k = 1e-6.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import io as hio
+
+GRAY = (0, 1, 2)
+E_PA = (3e9, 70e9, 3e9 * 1e-6)
+NU = (0.3, 0.2, 0.3)
+GC = (100.0, 1000.0, 1.0)
+
+
+@dataclass
+class Workload:
+    name: str
+    n: int                       # nodes per axis
+    seed: int
+    n_incl: int
+    incl_r: tuple
+    pore_fraction: float = 0.0
+    pore_r: tuple = (2.0, 5.0)
+    element_nb: tuple = (4, 4, 4)
+    voxel_size: float = 1e-6
+    lc_voxels: float = 2.0
+    small_k: float = 1e-6
+    # squared relative residual: the reference tests squared norms
+    # (pcg_utilities.cpp:35-58), so 1e-8 unsquared is 1e-16
+    tol_u: float = 1e-16
+    tol_d_rel: float = 1e-16
+    tol_d_abs: float = 1e-30
+    mg_u: tuple = (50, 4, 2, 1, 30)
+    mg_d: tuple = (10, 2, 1, 1, 30)
+    steps: int = 5
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def maxlevel(self):
+        m, c = 0, self.element_nb[0]
+        while 2 * c <= self.n - 1:
+            c *= 2
+            m += 1
+        return m
+
+    @property
+    def domain(self):
+        ext = [e * 2 ** self.maxlevel for e in self.element_nb]
+        return (0.0, float(ext[0]), 0.0, float(ext[1]), 0.0, float(ext[2]))
+
+    @property
+    def dims(self):
+        return tuple(e * 2 ** self.maxlevel + 1 for e in self.element_nb)
+
+    def image(self):
+        return hio.synth_image(self.dims, self.seed, n_incl=self.n_incl, incl_r=self.incl_r, incl_label=1,
+                               pore_fraction=self.pore_fraction, pore_r=self.pore_r, pore_label=2)
+
+    def table(self):
+        return hio.material_table(GRAY, E_PA, NU, GC)
+
+    def materials(self):
+        img, frac = self.image()
+        K, G, gc = hio.image_mp(self.table(), img, self.voxel_size)
+        return img, frac, K, G, gc
+
+    def solver_kwargs(self):
+        return dict(small_k=self.small_k, lc=self.lc_voxels, mg_u=self.mg_u, mg_d=self.mg_d, tol_u=self.tol_u,
+                    tol_d_rel=self.tol_d_rel, tol_d_abs=self.tol_d_abs)
+
+    def load_increment(self):
+        """top u_z += 1e-4 * height per step"""
+        return 1e-4 * (self.dims[2] - 1)
+
+
+C1 = Workload("C1-129-two-phase", 129, 1001, n_incl=30, incl_r=(4.0, 12.0), steps=5)
+C2 = Workload("C2-257-three-phase", 257, 1002, n_incl=200, incl_r=(3.0, 10.0), pore_fraction=0.05,
+              pore_r=(2.0, 5.0), steps=20)
+WORKLOADS = {"C1": C1, "C2": C2}
+
+
+def scaled(w, n, name=None):
+    """Same generator at another size, inclusion count scaled by volume."""
+    f = ((n - 1) / (w.n - 1)) ** 3
+    return Workload(name or f"{w.name}@{n}", n, w.seed, n_incl=max(1, int(round(w.n_incl * f))),
+                    incl_r=w.incl_r, pore_fraction=w.pore_fraction, pore_r=w.pore_r, element_nb=w.element_nb,
+                    voxel_size=w.voxel_size, lc_voxels=w.lc_voxels, small_k=w.small_k, tol_u=w.tol_u,
+                    tol_d_rel=w.tol_d_rel, tol_d_abs=w.tol_d_abs, mg_u=w.mg_u, mg_d=w.mg_d, steps=w.steps)
+
+
+def upload(solver, w, K, G, gc):
+    """Finest-level materials, then coarse_VR on the device."""
+    L = w.maxlevel
+    from .capi import GC as F_GC, G as F_G, K as F_K
+    solver.set(F_K, L, K)
+    solver.set(F_G, L, G)
+    solver.set(F_GC, L, gc)
+    for f in (F_K, F_G, F_GC):
+        solver.coarse_vr(f)
+
+
+def label_counts(img):
+    return {int(v): int(c) for v, c in zip(*np.unique(img, return_counts=True))}

@@ -181,8 +181,9 @@ int hpcr_prolong(hpcr_ctx* ctx, int fine_level, int uf_field, int uc_field, int 
 int hpcr_dirichlet_bd(hpcr_ctx* ctx, int level, int field, double Ut);
 
 /* ---- measurement ------------------------------------------------------- */
-/* HIP-event timing of the kernels in enum hpcr_kernel, on the context's
- * stream.  total_ms sums the recorded launches since the last reset. */
+/* HIP-event timing of the kernels in enum hpcr_kernel launched on the
+ * finest level, on the context's stream.  total_ms sums the recorded
+ * launches since the last reset. */
 int hpcr_timing_enable(hpcr_ctx* ctx, int on);
 int hpcr_timing_reset(hpcr_ctx* ctx);
 int hpcr_timing_query(hpcr_ctx* ctx, int kernel, int level, double* total_ms, long long* count);

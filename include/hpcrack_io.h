@@ -1,0 +1,99 @@
+/*
+ * hpcrack_io.h — host-side input formats of HPCrack, kept byte-compatible:
+ * config.txt (key=value), the gray-level material CSV and the uint8 .raw
+ * image, plus the level geometry they imply and a seeded synthetic image
+ * generator for benchmarks.  No device is touched by these functions.
+ */
+#ifndef HPCRACK_IO_H
+#define HPCRACK_IO_H
+
+#include <stddef.h>
+#include "hpcrack.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Parameters (include/configuration.h:22-44).  `seen` has one bit per key in
+ * the order of enum hpcr_config_key; unset keys keep the zero value. */
+typedef struct {
+  int D;
+  int np[3];
+  double X_start[3];
+  char IMGname[1024];
+  int voxel_nb[3];    /* voxel_nb_X/Y/Z */
+  int roi_start[3];   /* ROI_start_X/Y/Z */
+  double voxel_size;
+  int element_nb[3];
+  hpcr_mg mg_u, mg_d;
+  double lc;          /* as parsed (an integer in the reference, configuration.cpp:72-73) */
+  int outK, outG, outgc, outdesplacement, outH, outTdata, Tdatafrequence;
+  unsigned long long seen;
+} hpcr_config;
+
+enum hpcr_config_key {
+  HPCR_KEY_D = 0, HPCR_KEY_NP, HPCR_KEY_X_START, HPCR_KEY_IMGNAME, HPCR_KEY_VOXEL_NB_X, HPCR_KEY_VOXEL_NB_Y,
+  HPCR_KEY_VOXEL_NB_Z, HPCR_KEY_ROI_X, HPCR_KEY_ROI_Y, HPCR_KEY_ROI_Z, HPCR_KEY_VOXEL_SIZE, HPCR_KEY_ELEMENT_NB,
+  HPCR_KEY_NCY_U, HPCR_KEY_NU0_U, HPCR_KEY_NU1_U, HPCR_KEY_NU2_U, HPCR_KEY_GAMMA_U, HPCR_KEY_NCY_D,
+  HPCR_KEY_NU0_D, HPCR_KEY_NU1_D, HPCR_KEY_NU2_D, HPCR_KEY_GAMMA_D, HPCR_KEY_LC, HPCR_KEY_OUTK, HPCR_KEY_OUTG,
+  HPCR_KEY_OUTGC, HPCR_KEY_OUTDESPLACEMENT, HPCR_KEY_OUTH, HPCR_KEY_OUTTDATA, HPCR_KEY_TDATAFREQUENCE,
+  HPCR_KEY_COUNT
+};
+
+/* Parameters::readFromFile (src/configuration.cpp:14-97).  strict != 0 keeps
+ * the reference's parsing: the key is the raw text before '=', so lines
+ * written "lc = 3.;" are ignored, and lc goes through stoi.  strict == 0
+ * also trims blanks around the key and the value. */
+int hpcr_read_config(const char* path, int strict, hpcr_config* out);
+
+/* maxlevel (src/main.cpp:52-68), input_setting (src/image_input.cpp:21-29)
+ * and initialize (src/initialization.cpp:19-47): the level stack a config
+ * implies, and lc multiplied by the finest h (src/main.cpp:92-94). */
+int hpcr_config_geometry(const hpcr_config* cfg, hpcr_geometry* geo, double* lc_scaled);
+
+/* material_property (src/csv_reader.cpp:16-71): gray levels and E, nu, gc */
+#define HPCR_MAX_PHASES 256
+typedef struct {
+  int n;
+  int gray[HPCR_MAX_PHASES];
+  double E[HPCR_MAX_PHASES], nu[HPCR_MAX_PHASES], gc[HPCR_MAX_PHASES];
+} hpcr_material_table;
+int hpcr_read_material_csv(const char* path, hpcr_material_table* out);
+
+/* input / GL_MPI (src/image_input.cpp:33-118): the (nx,ny,nz) sub-block that
+ * starts at roi_start of an x-fastest uint8 image of vx*vy*vz voxels. */
+int hpcr_read_raw(const char* path, int vx, int vy, int vz, const int roi_start[3], int nx, int ny, int nz,
+                  unsigned char* out);
+int hpcr_write_raw(const char* path, const unsigned char* img, size_t n);
+
+/* image_MP (src/image_input.cpp:120-142): gray level -> K, G, gc with the
+ * reference's integer-division interpolation; E is scaled by voxel_size.
+ * Returns HPCR_ERR_ARG for a gray level outside the table. */
+int hpcr_image_mp(const hpcr_material_table* tab, const unsigned char* img, size_t n, double voxel_size,
+                  double* K, double* G, double* gc);
+
+/* Seeded synthetic voxel image: non-overlapping spheres on integer node
+ * centres placed by rejection (xoshiro256** seeded through splitmix64).
+ * Inclusions first (count n_incl, radii U(incl_rmin, incl_rmax)), then pores
+ * (radii U(pore_rmin, pore_rmax)) until their voxel fraction reaches
+ * pore_fraction.  `gap` voxels separate any two spheres.  realised[0] and
+ * realised[1] receive the inclusion and pore volume fractions. */
+typedef struct {
+  int dims[3];
+  unsigned long long seed;
+  int matrix_label;
+  int n_incl;
+  double incl_rmin, incl_rmax;
+  int incl_label;
+  double pore_fraction;
+  double pore_rmin, pore_rmax;
+  int pore_label;
+  int gap;
+  int max_tries;
+} hpcr_synth_spec;
+int hpcr_synth_image(const hpcr_synth_spec* spec, unsigned char* out, double realised[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPCRACK_IO_H */

@@ -318,3 +318,37 @@ def test_phase_field_converges_to_direct_solve():
     assert np.linalg.norm(vflat(o.field(O.D, L)) - x) <= 1e-10 * np.linalg.norm(x)
     assert fdn == pytest.approx(np.sum(o.field(O.FD, L) ** 2), rel=1e-14)
     assert 1 <= cycles < 40
+
+
+def test_cycle_from_rough_start_is_rounding_sensitive():
+    """Documents a property of the reference algorithm that bounds what any
+    re-implementation can match: its boundary conditions leave a rigid
+    rotation mode (see test_u_entire_converges_to_direct_solve) and the
+    coarsest smoother runs a fixed 50 CG iterations, far past convergence on a
+    5^3 grid.  On a two-phase voxel material with damage (zeros included),
+    from a rough random start, a 1e-14 perturbation of the input changes the
+    cycle's result at the 1e-5 level."""
+    from gpu_common import material_from_labels, spheres_labels
+
+    def run(eps):
+        o = make(1, (4, 4, 4), (0.0, 8.0, 0.0, 8.0, 0.0, 8.0), lc=2.0)
+        dims = o.dims(1)
+        K, G, gc = material_from_labels(spheres_labels(dims, 3, 1.5, 2.0, 13))
+        for f, a in ((O.K, K), (O.G, G), (O.GC, gc)):
+            o.set(f, 1, a)
+        for f in (O.K, O.G, O.GC):
+            o.coarse_vr_nodal(f)
+        rng = np.random.default_rng(14)
+        d = rng.uniform(-0.05, 0.9, dims[::-1])
+        d[rng.random(dims[::-1]) < 0.1] = 0.0
+        o.set(O.D, 1, d)
+        u0 = np.random.default_rng(15).standard_normal((3,) + dims[::-1]) * 0.01 * (1 + eps)
+        o.set(O.U, 1, u0)
+        o.set(O.FU, 1, np.zeros((3,) + dims[::-1]))
+        o.mech_setup()
+        fn = o.cycle_u(1, 1.0, 0, 0.0)
+        return fn, vflat(o.vec(O.U, 1)).copy()
+    fa, ua = run(0.0)
+    fb, ub = run(1e-14)
+    assert np.linalg.norm(ua - ub) / np.linalg.norm(ua) > 1e-8
+    assert abs(fa - fb) / fa > 1e-6
