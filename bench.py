@@ -26,9 +26,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# canonical algorithmic bytes per fine node of the fused pass A:
-# read r, 1/D, p_old (3 x 24), K, G, d (24), flag (1); write p, Ap (2 x 24)
-PASS_A_BYTES_PER_NODE = 145.0
+# algorithmic bytes per fine node of the fused pass A (p = z + beta p_old;
+# w = A p; p.w): read z, p_old (2 x 24), K, G, d (24), flag (1); write p, w
+# (2 x 24).  Pass B (u, r update, z = r/D) moves 192 B per node; one smoother
+# iteration moves 313 B per node in total.
+PASS_A_BYTES_PER_NODE = 121.0
+PASS_B_BYTES_PER_NODE = 192.0
+# smoother residual r = f - A u, z = r/D: read u, f, 1/D (3 x 24), K, G, d, flag
+# (25); write r, z (2 x 24)
+RESID_BYTES_PER_NODE = 145.0
 METRIC = "MDOF·iter/s + solves/s, FP32-MG vs FP64 @1e-8 residual; p50/p99 serve latency"
 
 
@@ -48,6 +54,22 @@ def read_traffic(path):
                 except ValueError:
                     return None
     return None
+
+
+def copy_calibration(mib):
+    """One plain device-to-device copy of `mib` MiB (HIP's blit kernel), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for rc in (hip.hipMalloc(ctypes.byref(a), n), hip.hipMalloc(ctypes.byref(b), n),
+               hip.hipMemset(a, 1, n), hip.hipDeviceSynchronize(),
+               hip.hipMemcpy(b, a, n, 3), hip.hipDeviceSynchronize(),  # 3: device to device
+               hip.hipFree(a), hip.hipFree(b)):
+        if rc != 0:
+            raise RuntimeError(f"copy calibration: HIP error {rc}")
+    log(f"[bench] copy calibration: {mib} MiB device-to-device copy")
 
 
 def cpu_baseline(workload, n_sample, cycles, threads):
@@ -92,6 +114,8 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_mech_pcg_a_C2_257.txt"))
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, one device copy of this size (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -146,6 +170,10 @@ def main():
     for name, kid in (("mech_pcg_a", K_MECH_PCG_A), ("mech_pcg_b", K_MECH_PCG_B), ("mech_resid", K_MECH_RESID)):
         ms, n = s.timing_query(kid, L)
         kern[name] = {"total_ms": ms, "launches": n, "avg_ms": ms / n if n else None}
+    for name, b in (("mech_pcg_a", PASS_A_BYTES_PER_NODE), ("mech_pcg_b", PASS_B_BYTES_PER_NODE),
+                    ("mech_resid", RESID_BYTES_PER_NODE)):
+        k = kern[name]
+        k["algorithmic_GBs"] = round(b * N / (k["avg_ms"] / 1e3) / 1e9, 1) if k["avg_ms"] else None
     a = kern["mech_pcg_a"]
     avg_s = a["avg_ms"] / 1e3
     achieved = PASS_A_BYTES_PER_NODE * N / avg_s / 1e9
@@ -201,6 +229,8 @@ def main():
         "kernels_finest_level": kern,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
 
 
 if __name__ == "__main__":

@@ -7,7 +7,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhpcrack_amd.so")
+# HPCRACK_AMD_LIB selects another build of the same library (e.g. one
+# compiled with different tuning macros)
+LIB_PATH = os.environ.get("HPCRACK_AMD_LIB") or os.path.join(HERE, "libhpcrack_amd.so")
 
 OK = 0
 ERR_ARG, ERR_DEVICE, ERR_MEMORY, ERR_IO, ERR_ZERO_DIAGONAL, ERR_STATE = -1, -2, -3, -4, -5, -6

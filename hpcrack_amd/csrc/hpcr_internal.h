@@ -7,14 +7,18 @@
 
 namespace hpcr {
 
-// Element-tile geometry of the operator kernels: 16x16 elements per layer,
-// one element per thread, marching in z; a block owns the 15x15 interior
-// node columns of its tile (neighbouring tiles overlap by one element).
-constexpr int TILE = 16;
-constexpr int OWN = TILE - 1;
-constexpr int TPB = TILE * TILE;        // 256 threads
-constexpr int PLANE = (TILE + 1) * (TILE + 1);  // 289 nodes per LDS plane
-constexpr int ZCHUNK_MAX = 16;          // node planes per block (fine levels)
+// Element-tile geometry of the operator kernels: 32x8 elements per layer,
+// one element per thread, marching in z; a block owns the 31x7 interior node
+// columns of its tile (neighbouring tiles overlap by one element).  32-wide
+// rows keep the tile's global rows long (33 doubles) for cache-line use.
+constexpr int TX = 32, TY = 8;
+constexpr int OWNX = TX - 1, OWNY = TY - 1;
+constexpr int TPB = TX * TY;                  // 256 threads
+constexpr int PX = TX + 1;                    // node row length in LDS
+constexpr int PLANE = (TX + 1) * (TY + 1);    // 297 nodes per LDS plane
+constexpr int ZCHUNK_MAX = 32;                // node planes per block (fine levels)
+// levels small enough for the one-workgroup smoother (small_pcg_kernel)
+constexpr int SMALL_NODES = 512, SMALL_ELEMS = 256;
 
 // Device scalar slots (one set per context, reused by every level: the
 // smoothers of different levels never overlap in time).
@@ -67,43 +71,54 @@ struct PfCoef {
 // z-chunk of the tile kernels: the longest chunk that still gives >= 1024
 // blocks, so that small levels are not serialised along z
 inline int choose_zchunk(int nx, int ny, int nz) {
-  const long xy = (long)((nx + OWN - 1) / OWN) * ((ny + OWN - 1) / OWN);
+  const long xy = (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY);
   for (int zc = ZCHUNK_MAX; zc > 2; zc /= 2)
     if (xy * ((nz + zc - 1) / zc) >= 1024) return zc;
   return 2;
 }
 inline long tile_blocks(int nx, int ny, int nz, int zc) {
-  return (long)((nx + OWN - 1) / OWN) * ((ny + OWN - 1) / OWN) * ((nz + zc - 1) / zc);
+  return (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY) * ((nz + zc - 1) / zc);
 }
 
 // ---- launchers (hpcr_kernels.hip); all asynchronous on `s` -------------
 // y = A x  (3 comps, stride N)
 void launch_mech_apply(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, double* y);
-// r = f - A u, masked; at t0fine also f := A u on Dirichlet DOFs; reductions
-// r.z (with z = r*invD), r.r and f.f into scal.
+// r = f - A u, masked, and z = r*invD; at t0fine also f := A u on Dirichlet
+// DOFs; reductions r.z, r.r and f.f into scal.
 void launch_mech_resid(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* u, double* f,
-                       double* r, const double* invD, int t0fine, const Reduce& R);
-// p_new = r*invD + beta p_old (first: p_new = r*invD); w = A p_new; p.w -> alpha
-void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* r,
-                       const double* invD, const double* p_old, double* p_new, double* w, int first,
-                       const Reduce& R);
-// u += alpha p; r -= alpha w; mask r; r.z, r.r -> beta, rz, rr
-void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, const double* p,
+                       double* r, double* z, const double* invD, int t0fine, const Reduce& R);
+// blocks of the pass-B launches over n nodes (sizes the reduction partials)
+int passb_blocks(size_t n);
+// p_new = z + beta p_old (first: p_new = z); w = A p_new; p.w -> alpha
+void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* z,
+                       const double* p_old, double* p_new, double* w, int first, const Reduce& R);
+// u += alpha p; r -= alpha w; mask r; z = r*invD; r.z, r.r -> beta, rz, rr
+void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const double* p,
                        const double* w, const double* invD, const Reduce& R);
 // invD = 1 / diag(A); zero entries counted into scal[S_ZERO]; diag optionally kept
 void launch_mech_diag(hipStream_t s, const LevelGeom& L, const MechCoef& C, double* diag_or_null,
                       double* invD, const Reduce& R);
 
 void launch_pf_apply(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* x, double* y);
-// R = fd - A d; r.z, r.r, fd.fd
+// R = fd - A d, z = R*invD; r.z, r.r, fd.fd
 void launch_pf_resid(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* d, const double* fd,
-                     double* r, const double* invD, const Reduce& R);
-void launch_pf_pcg_a(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* r, const double* invD,
-                     const double* p_old, double* p_new, double* w, int first, const Reduce& R);
-void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, const double* p, const double* w,
-                     const double* invD, const Reduce& R);
+                     double* r, double* z, const double* invD, const Reduce& R);
+void launch_pf_pcg_a(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* z, const double* p_old,
+                     double* p_new, double* w, int first, const Reduce& R);
+void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, double* z, const double* p,
+                     const double* w, const double* invD, const Reduce& R);
 void launch_pf_diag(hipStream_t s, const LevelGeom& L, const PfCoef& C, double* diag_or_null, double* invD,
                     const Reduce& R);
+
+// whole smoother call (Dirichlet values, residual, NB iterations) of a small
+// coarse level in one workgroup; small_level() tells whether L fits
+bool small_level(const LevelGeom& L);
+void launch_small_pcg_u(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, double* u,
+                        const double* f, const double* invD, double* r, double* z, double* p, double* w, double Ut,
+                        int NB, double* scal);
+void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, const double* gc, double lc, double* d,
+                        const double* fd, const double* invD, double* r, double* z, double* p, double* w, int NB,
+                        double* scal);
 
 // Dirichlet_BD on planes with z flags; `kplanes` lists the flagged k indices.
 void launch_dirichlet_bd(hipStream_t s, const LevelGeom& L, double* u, double Ut, const int* kplanes_host,

@@ -264,6 +264,68 @@ int hpcr_write_raw(const char* path, const unsigned char* img, size_t n) {
   return w == n ? HPCR_OK : HPCR_ERR_IO;
 }
 
+namespace {
+std::string history_path(const char* dir, const char* name, int t, int proc) {
+  if (std::strstr(name, ".dat")) return name;
+  return std::string(dir ? dir : "Time_step_data") + "/" + name + "_t_" + std::to_string(t) + "_Proc_" +
+         std::to_string(proc) + ".dat";
+}
+}  // namespace
+
+int hpcr_write_vector_3d(const char* dir, const char* name, int t, int proc, const double* v, size_t n) {
+  if (!name || !v) return HPCR_ERR_ARG;
+  const std::string path = history_path(dir, name, t, proc);
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return HPCR_ERR_IO;
+  const size_t w = fwrite(v, sizeof(double), n, f);
+  const int cl = fclose(f);
+  return (w == n && cl == 0) ? HPCR_OK : HPCR_ERR_IO;
+}
+
+int hpcr_read_vector_3d(const char* dir, const char* name, int t, int proc, double* v, size_t n) {
+  if (!name || !v) return HPCR_ERR_ARG;
+  const std::string path = history_path(dir, name, t, proc);
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return HPCR_ERR_IO;
+  const size_t r = fread(v, sizeof(double), n, f);
+  const bool at_end = (r == n) && (fgetc(f) == EOF);
+  fclose(f);
+  return at_end ? HPCR_OK : HPCR_ERR_IO;
+}
+
+int hpcr_write_step_state(const char* dir, const hpcr_step_state* s) {
+  if (!s) return HPCR_ERR_ARG;
+  const std::string path = std::string(dir ? dir : "Time_step_data") + "/state_t_" + std::to_string(s->t) + ".txt";
+  FILE* f = fopen(path.c_str(), "w");
+  if (!f) return HPCR_ERR_IO;
+  fprintf(f, "t %d\nUt %.17g\ndelta_U %.17g\nu_prop %.17g\nf_norm %.17g\nUe %.17g\n", s->t, s->Ut, s->delta_U,
+          s->u_prop, s->f_norm, s->Ue);
+  return fclose(f) == 0 ? HPCR_OK : HPCR_ERR_IO;
+}
+
+int hpcr_read_step_state(const char* dir, int t, hpcr_step_state* s) {
+  if (!s) return HPCR_ERR_ARG;
+  const std::string path = std::string(dir ? dir : "Time_step_data") + "/state_t_" + std::to_string(t) + ".txt";
+  std::ifstream in(path);
+  if (!in) return HPCR_ERR_IO;
+  hpcr_step_state v{};
+  int got = 0;
+  std::string key;
+  while (in >> key) {
+    if (key == "t") { in >> v.t; got |= 1; }
+    else if (key == "Ut") { in >> v.Ut; got |= 2; }
+    else if (key == "delta_U") { in >> v.delta_U; got |= 4; }
+    else if (key == "u_prop") { in >> v.u_prop; got |= 8; }
+    else if (key == "f_norm") { in >> v.f_norm; got |= 16; }
+    else if (key == "Ue") { in >> v.Ue; got |= 32; }
+    else return HPCR_ERR_IO;
+    if (in.fail()) return HPCR_ERR_IO;
+  }
+  if (got != 63 || v.t != t) return HPCR_ERR_IO;
+  *s = v;
+  return HPCR_OK;
+}
+
 int hpcr_image_mp(const hpcr_material_table* tab, const unsigned char* img, size_t n, double voxel_size,
                   double* K, double* G, double* gc) {
   if (!tab || !img || !K || !G || !gc || tab->n < 2) return HPCR_ERR_ARG;

@@ -10,15 +10,22 @@
 // instead of ~8 kflop for the reference's per-Gauss-point loops
 // (mechanical_pcg.cpp:114-172), and gives the same operator up to rounding.
 //
-// Tiling.  A block of 256 threads owns a 16x16 column of elements (one per
-// thread) and marches up a chunk of z-layers.  Each step loads one node plane
-// (u and, on the finest level, K, G, d) into LDS, evaluates one element layer,
-// and exchanges the 8 corner contributions of every element through LDS so
-// that each node sums its 4 in-plane neighbours in a fixed order.  The node
+// Tiling.  A block of 256 threads owns a 32x8 column of elements (one per
+// thread) and marches up a chunk of z-layers.  Each step evaluates one
+// element layer from two node planes held in LDS (u and, on the finest level,
+// K, G, d) while the global loads of the plane after next are in flight.  The
+// 8 corner contributions of every element are exchanged so that each node
+// sums its 4 in-plane neighbours in a fixed order: the x-neighbour is the
+// adjacent lane (DPP row shift), the y-neighbour goes through LDS.  The node
 // plane below the layer is then complete and handed to the fused epilogue
 // (residual, PCG update, diagonal ...); the plane above is carried in
 // registers.  No atomics: results are deterministic.  Tiles overlap by one
-// element so that the 15x15 interior node columns are owned by one block.
+// element so that the 31x7 interior node columns are owned by one block.
+//
+// Small levels.  A level of at most SMALL_NODES nodes and SMALL_ELEMS
+// elements runs a whole Jacobi-PCG smoother call in one workgroup
+// (small_pcg_kernel): vectors in registers, p and the element contributions
+// in LDS, block-wide reductions, no launches between iterations.
 //
 // Reductions.  Every block reduces its partial sums with wavefront shuffles
 // in a fixed tree, writes them to a partials array, and the last block to
@@ -455,30 +462,72 @@ __device__ __forceinline__ int dir_mask(const LevelGeom& L, int i, int j, int k)
 // ---------------------------------------------------------------------------
 // Tile kernel framework
 // ---------------------------------------------------------------------------
+// lane i <- lane i-1 across the wavefront (DPP wave_shr:1); lane 0 gets 0.
+// Element (tx, ty) and its x-neighbour (tx-1, ty) are adjacent lanes because
+// TX divides the wavefront into whole rows; lanes with tx == 0 own no node.
+__device__ __forceinline__ double wave_shr1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so that tiles sharing halo rows share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb / 8, rem = nb % 8;
+  const int xcd = b % 8, j = b / 8;
+  return (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + j;
+}
+
+// optional occupancy target of the tile kernels (waves per SIMD), a tuning macro
+#ifdef HPCR_TILE_WAVES_PER_EU
+#define HPCR_TILE_ATTR __attribute__((amdgpu_waves_per_eu(HPCR_TILE_WAVES_PER_EU)))
+#else
+#define HPCR_TILE_ATTR
+#endif
+
 template <class Op>
-__global__ __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op op) {
-  constexpr int NC = Op::NC;
-  constexpr int NV = Op::NV;
+__global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op op) {
+  constexpr int NC = Op::NC;  // output components
+  constexpr int NV = Op::NV;  // values per node in LDS
+  constexpr int NR = Op::NR;  // raw values per node fetched from HBM
   __shared__ double sN[2][NV][PLANE];
-  __shared__ double sOut[4 * NC][TPB];
+  __shared__ double sX[2 * NC][TPB];
   const int tid = threadIdx.x;
-  const int tx = tid % TILE, ty = tid / TILE;
-  const int bx0 = blockIdx.x * OWN, by0 = blockIdx.y * OWN;
-  const int z0 = blockIdx.z * L.zc;
+  const int tx = tid % TX, ty = tid / TX;
+  const int ntx = (L.nx + OWNX - 1) / OWNX, nty = (L.ny + OWNY - 1) / OWNY;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx0 = (t % ntx) * OWNX, by0 = ((t / ntx) % nty) * OWNY;
+  const int z0 = (t / (ntx * nty)) * L.zc;
   const int zend = min(z0 + L.zc, L.nz);
   const int ex = bx0 - 1 + tx, ey = by0 - 1 + ty;
   const bool owner = (tx >= 1) && (ty >= 1) && (ex < L.nx) && (ey < L.ny);
   const bool exy = (ex >= 0) && (ey >= 0) && (ex <= L.nx - 2) && (ey <= L.ny - 2);
+  const typename Op::State st = op.state();
 
-  auto load_plane = [&](int pz, int slot) {
-    for (int q = tid; q < PLANE; q += TPB) {
-      const int lx = q % (TILE + 1), ly = q / (TILE + 1);
-      const int gx = bx0 - 1 + lx, gy = by0 - 1 + ly;
-      const bool ok = gx >= 0 && gy >= 0 && gx < L.nx && gy < L.ny && pz >= 0 && pz < L.nz;
-      double v[NV];
-      op.load(L, gx, gy, pz, ok, v);
+  // node slots of this thread in a plane: tid and tid + TPB
+  const bool has2 = tid + TPB < PLANE;
+  const int q1 = tid, q2 = tid + TPB;
+  const int gx1 = bx0 - 1 + q1 % PX, gy1 = by0 - 1 + q1 / PX;
+  const int gx2 = bx0 - 1 + q2 % PX, gy2 = by0 - 1 + q2 / PX;
+  const bool in1 = gx1 >= 0 && gy1 >= 0 && gx1 < L.nx && gy1 < L.ny;
+  const bool in2 = has2 && gx2 >= 0 && gy2 >= 0 && gx2 < L.nx && gy2 < L.ny;
+  double r1[NR], r2[NR];
+  auto fetch = [&](int pz) {
+    const bool zok = pz >= 0 && pz < L.nz;
+    op.fetch(L, gx1, gy1, pz, in1 && zok, st, r1);
+    op.fetch(L, gx2, gy2, pz, in2 && zok, st, r2);
+  };
+  auto commit = [&](int slot) {
+    double v[NV];
+    op.commit(r1, st, v);
 #pragma unroll
-      for (int k = 0; k < NV; k++) sN[slot][k][q] = v[k];
+    for (int k = 0; k < NV; k++) sN[slot][k][q1] = v[k];
+    if (has2) {
+      op.commit(r2, st, v);
+#pragma unroll
+      for (int k = 0; k < NV; k++) sN[slot][k][q2] = v[k];
     }
   };
 
@@ -487,13 +536,17 @@ __global__ __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op o
 #pragma unroll
   for (int c = 0; c < NC; c++) carry[c] = 0.0;
 
-  load_plane(z0 - 1, (z0 - 1) & 1);
+  // planes z0-1 and z0 into LDS, plane z0+1 in flight
+  fetch(z0 - 1);
+  commit((z0 - 1) & 1);
+  fetch(z0);
+  commit(z0 & 1);
+  fetch(z0 + 1);
+  __syncthreads();
   for (int kz = z0 - 1; kz < zend; ++kz) {
-    load_plane(kz + 1, (kz + 1) & 1);
-    __syncthreads();
     double f[NC][2][2][2];
     if (exy && kz >= 0 && kz <= L.nz - 2) {
-      op.element(L, sN, kz & 1, tx, ty, ex, ey, kz, f);
+      op.element(L, sN, kz & 1, tx, ty, ex, ey, kz, st, f);
     } else {
 #pragma unroll
       for (int c = 0; c < NC; c++)
@@ -502,55 +555,58 @@ __global__ __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op o
 #pragma unroll
           for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = 0.0;
     }
+    // x-direction: node (tx, .) takes corner 0 of its own element and corner 1
+    // of element tx-1 (adjacent lane); g[c][iz][iy]
+    double g[NC][2][2];
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll
-      for (int iy = 0; iy < 2; iy++)
+      for (int iz = 0; iz < 2; iz++)
 #pragma unroll
-        for (int ix = 0; ix < 2; ix++) sOut[c * 4 + iy * 2 + ix][tid] = f[c][0][iy][ix];
-    __syncthreads();
-    if (owner && kz >= z0) {
-      double v[NC];
+        for (int iy = 0; iy < 2; iy++) g[c][iz][iy] = f[c][iz][iy][0] + wave_shr1(f[c][iz][iy][1]);
+    // y-direction through LDS: the iy = 1 halves go to the row above
 #pragma unroll
-      for (int c = 0; c < NC; c++)
-        v[c] = carry[c] + sOut[c * 4 + 0][tid] + sOut[c * 4 + 1][tid - 1] + sOut[c * 4 + 2][tid - TILE] +
-               sOut[c * 4 + 3][tid - TILE - 1];
-      op.epilogue(L, sN, kz & 1, tx, ty, ex, ey, kz, v, acc);
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int iz = 0; iz < 2; iz++) sX[c * 2 + iz][tid] = g[c][iz][1];
+    // values of this thread's own node of plane kz, read before the slot is reused
+    double own[NC];
+    if constexpr (Op::NEEDS_OWN) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) own[c] = sN[kz & 1][c][tx + PX * ty];
     }
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < NC; c++)
-#pragma unroll
-      for (int iy = 0; iy < 2; iy++)
-#pragma unroll
-        for (int ix = 0; ix < 2; ix++) sOut[c * 4 + iy * 2 + ix][tid] = f[c][1][iy][ix];
-    __syncthreads();
+    // plane kz is no longer read: it receives plane kz+2; plane kz+3 goes in flight
+    commit(kz & 1);
+    fetch(kz + 3);
     if (owner) {
+      double vb[NC];
 #pragma unroll
-      for (int c = 0; c < NC; c++)
-        carry[c] = sOut[c * 4 + 0][tid] + sOut[c * 4 + 1][tid - 1] + sOut[c * 4 + 2][tid - TILE] +
-                   sOut[c * 4 + 3][tid - TILE - 1];
+      for (int c = 0; c < NC; c++) {
+        vb[c] = carry[c] + g[c][0][0] + sX[c * 2 + 0][tid - TX];
+        carry[c] = g[c][1][0] + sX[c * 2 + 1][tid - TX];
+      }
+      if (kz >= z0) op.epilogue(L, own, ex, ey, kz, vb, st, acc);
     }
+    __syncthreads();
   }
   op.finish(acc);
 }
 
-// gather the 8 nodes of value index `k0..` from the two LDS planes
+// the 8 corner values of element (tx, ty) from the two LDS planes
 template <int NV>
 __device__ __forceinline__ void gather8(const double (&sN)[2][NV][PLANE], int s0, int tx, int ty, int k,
                                         double (&v)[2][2][2]) {
-  const int q = tx + (TILE + 1) * ty;
+  const int q = tx + PX * ty;
 #pragma unroll
   for (int iz = 0; iz < 2; iz++) {
     const int s = s0 ^ iz;
     v[iz][0][0] = sN[s][k][q];
     v[iz][0][1] = sN[s][k][q + 1];
-    v[iz][1][0] = sN[s][k][q + TILE + 1];
-    v[iz][1][1] = sN[s][k][q + TILE + 2];
+    v[iz][1][0] = sN[s][k][q + PX];
+    v[iz][1][1] = sN[s][k][q + PX + 1];
   }
 }
-
-struct NoAcc {};
 
 // ---- mechanical tile ops --------------------------------------------------
 enum MechMode { M_APPLY = 0, M_RESID = 1, M_PCGA = 2, M_DIAG = 3 };
@@ -558,44 +614,69 @@ enum MechMode { M_APPLY = 0, M_RESID = 1, M_PCGA = 2, M_DIAG = 3 };
 template <int MODE, bool FINE>
 struct MechOp {
   static constexpr int NC = 3;
-  static constexpr int NU = (MODE == M_DIAG) ? 0 : 3;       // vector values per node
-  static constexpr int NV = NU + (FINE ? 3 : 0) > 0 ? NU + (FINE ? 3 : 0) : 1;
+  static constexpr int NU = (MODE == M_DIAG) ? 0 : 3;  // vector values per node in LDS
+  static constexpr int NK = FINE ? 3 : 0;              // K, G, d
+  static constexpr int NV = (NU + NK) > 0 ? NU + NK : 1;
+  static constexpr int NR = (MODE == M_PCGA) ? 6 + NK : NV;  // PCGA fetches z and p_old
   struct Acc {
     double a[3];
     __device__ Acc() { a[0] = a[1] = a[2] = 0.0; }
   };
+  struct State {
+    double beta;
+  };
   MechCoef C;
-  const double* x;      // APPLY/RESID: input vector; PCGA: r
-  const double* invD;   // RESID/PCGA
+  const double* x;      // APPLY/RESID: input vector; PCGA: z
+  const double* invD;   // RESID
   const double* p_old;  // PCGA
   double* out;          // APPLY: y; RESID: r; PCGA: w; DIAG: invD
   double* out2;         // RESID: f; PCGA: p_new; DIAG: diag (or null)
+  double* out3;         // RESID: z
   int flag;             // RESID: t0fine; PCGA: first
   double sx, sy, sz, tx_, ty_, tz_, wg, ihx2, ihy2, ihz2;
   Reduce R;
   size_t N;
 
-  __device__ __forceinline__ void load(const LevelGeom& L, int gx, int gy, int gz, bool ok, double (&v)[NV]) const {
+  __device__ __forceinline__ State state() const {
+    State s;
+    s.beta = (MODE == M_PCGA && !flag) ? R.scal[S_BETA] : 0.0;
+    return s;
+  }
+
+  __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State& st,
+                                        double (&r)[NR]) const {
 #pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = 0.0;
+    for (int k = 0; k < NR; k++) r[k] = 0.0;
     if (!ok) return;
     const size_t q = nidx(L, gx, gy, gz);
     if constexpr (MODE == M_APPLY || MODE == M_RESID) {
 #pragma unroll
-      for (int c = 0; c < NU; c++) v[c] = x[c * N + q];
+      for (int c = 0; c < 3; c++) r[c] = x[c * N + q];
     } else if constexpr (MODE == M_PCGA) {
-      double beta = 0.0;
-      if (!flag) beta = R.scal[S_BETA];
 #pragma unroll
-      for (int c = 0; c < NU; c++) {
-        const double z = x[c * N + q] * invD[c * N + q];
-        v[c] = flag ? z : z + beta * p_old[c * N + q];
+      for (int c = 0; c < 3; c++) r[c] = x[c * N + q];
+      if (!flag) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[3 + c] = p_old[c * N + q];
       }
     }
     if constexpr (FINE) {
-      v[NU + 0] = C.K[q];
-      v[NU + 1] = C.G[q];
-      v[NU + 2] = C.d[q];
+      constexpr int o = NR - 3;
+      r[o + 0] = C.K[q];
+      r[o + 1] = C.G[q];
+      r[o + 2] = C.d[q];
+    }
+  }
+
+  __device__ __forceinline__ void commit(const double (&r)[NR], const State& st, double (&v)[NV]) const {
+    if constexpr (MODE == M_PCGA) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c] = r[c] + st.beta * r[3 + c];
+#pragma unroll
+      for (int k = 0; k < NK; k++) v[3 + k] = r[6 + k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NV; k++) v[k] = r[k];
     }
   }
 
@@ -628,7 +709,8 @@ struct MechOp {
   }
 
   __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, double (&f)[3][2][2][2]) const {
+                                          int ty, int ex, int ey, int ez, const State& st,
+                                          double (&f)[3][2][2][2]) const {
     double lam[2][2][2], mu[2][2][2], lam0, mu0;
     lame(sN, s0, tx, ty, ex, ey, ez, L, lam, mu, lam0, mu0);
     if constexpr (MODE == M_DIAG) {
@@ -644,8 +726,10 @@ struct MechOp {
     }
   }
 
-  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                           int ty, int i, int j, int k, const double (&v)[3], Acc& acc) const {
+  static constexpr bool NEEDS_OWN = (MODE == M_PCGA);
+  // v: assembled values of node (i, j, k); own: the node's LDS values (p)
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k,
+                                           const double (&v)[3], const State& st, Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
     if constexpr (MODE == M_APPLY) {
 #pragma unroll
@@ -661,16 +745,17 @@ struct MechOp {
           out2[c * N + q] = fv;
         }
         const double r = dir ? 0.0 : fv - v[c];
+        const double z = r * invD[c * N + q];
         out[c * N + q] = r;
-        acc.a[0] += r * (r * invD[c * N + q]);
+        out3[c * N + q] = z;
+        acc.a[0] += r * z;
         acc.a[1] += r * r;
         acc.a[2] += fv * fv;
       }
     } else if constexpr (MODE == M_PCGA) {
-      const int qs = tx + (TILE + 1) * ty;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const double p = sN[s0][c][qs];
+        const double p = own[c];
         out2[c * N + q] = p;
         out[c * N + q] = v[c];
         acc.a[0] += p * v[c];
@@ -686,16 +771,16 @@ struct MechOp {
   }
 
   __device__ __forceinline__ void finish(Acc& acc) const {
-    if (MODE == M_APPLY) return;
+    if constexpr (MODE == M_APPLY) return;
     double* scal = R.scal;
-    if (MODE == M_RESID) {
+    if constexpr (MODE == M_RESID) {
       const int t0 = flag;
       reduce_finalize<RED_SUM, 3>(acc.a, R, 0.0, [=](double (&s)[3]) {
         scal[S_RZ] = s[0];
         scal[S_RR] = s[1];
         if (t0) scal[S_FF] = s[2];
       });
-    } else if (MODE == M_PCGA) {
+    } else if constexpr (MODE == M_PCGA) {
       double v1[1] = {acc.a[0]};
       reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) {
         scal[S_PW] = s[0];
@@ -714,41 +799,64 @@ struct PfOp {
   static constexpr int NC = 1;
   static constexpr int NU = (MODE == M_DIAG) ? 0 : 1;
   static constexpr int NV = NU + 1;  // + gc
+  static constexpr int NR = (MODE == M_PCGA) ? 3 : NV;
   struct Acc {
     double a[3];
     __device__ Acc() { a[0] = a[1] = a[2] = 0.0; }
   };
+  struct State {
+    double beta;
+  };
   PfCoef C;
-  const double* x;      // APPLY/RESID: d; PCGA: R
-  const double* invD;
-  const double* p_old;
+  const double* x;      // APPLY/RESID: d; PCGA: z
+  const double* invD;   // RESID
+  const double* p_old;  // PCGA
   double* out;          // APPLY: y; RESID: R; PCGA: w; DIAG: invD
   double* out2;         // RESID: fd (read); PCGA: p_new; DIAG: diag or null
+  double* out3;         // RESID: z
   int flag;             // PCGA: first
   double sx, sy, sz, wg, ihx2, ihy2, ihz2;
   Reduce R;
 
-  __device__ __forceinline__ void load(const LevelGeom& L, int gx, int gy, int gz, bool ok, double (&v)[NV]) const {
+  __device__ __forceinline__ State state() const {
+    State s;
+    s.beta = (MODE == M_PCGA && !flag) ? R.scal[S_BETA] : 0.0;
+    return s;
+  }
+
+  __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State& st,
+                                        double (&r)[NR]) const {
 #pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = 0.0;
+    for (int k = 0; k < NR; k++) r[k] = 0.0;
     if (!ok) return;
     const size_t q = nidx(L, gx, gy, gz);
-    if (MODE == M_APPLY || MODE == M_RESID) {
-      v[0] = x[q];
-    } else if (MODE == M_PCGA) {
-      const double z = x[q] * invD[q];
-      v[0] = flag ? z : z + R.scal[S_BETA] * p_old[q];
+    if constexpr (MODE == M_APPLY || MODE == M_RESID) {
+      r[0] = x[q];
+    } else if constexpr (MODE == M_PCGA) {
+      r[0] = x[q];
+      if (!flag) r[1] = p_old[q];
     }
-    v[NU] = C.gc[q];
+    r[NR - 1] = C.gc[q];
+  }
+
+  __device__ __forceinline__ void commit(const double (&r)[NR], const State& st, double (&v)[NV]) const {
+    if constexpr (MODE == M_PCGA) {
+      v[0] = r[0] + st.beta * r[1];
+      v[1] = r[2];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NV; k++) v[k] = r[k];
+    }
   }
 
   __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, double (&f)[1][2][2][2]) const {
+                                          int ty, int ex, int ey, int ez, const State& st,
+                                          double (&f)[1][2][2][2]) const {
     double gcn[2][2][2];
     gather8<NV>(sN, s0, tx, ty, NU, gcn);
     double m[2][2][2], k[2][2][2];
     pf_coefs(gcn, C.H[nidx(L, ex, ey, ez)], C.lc, wg, m, k);
-    if (MODE == M_DIAG) {
+    if constexpr (MODE == M_DIAG) {
       pf_element_diag(m, k, ihx2, ihy2, ihz2, f[0]);
     } else {
       double v[2][2][2];
@@ -757,20 +865,23 @@ struct PfOp {
     }
   }
 
-  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                           int ty, int i, int j, int k, const double (&v)[1], Acc& acc) const {
+  static constexpr bool NEEDS_OWN = (MODE == M_PCGA);
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[1], int i, int j, int k,
+                                           const double (&v)[1], const State& st, Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
-    if (MODE == M_APPLY) {
+    if constexpr (MODE == M_APPLY) {
       out[q] = v[0];
-    } else if (MODE == M_RESID) {
+    } else if constexpr (MODE == M_RESID) {
       const double fv = out2[q];
       const double r = fv - v[0];
+      const double z = r * invD[q];
       out[q] = r;
-      acc.a[0] += r * (r * invD[q]);
+      out3[q] = z;
+      acc.a[0] += r * z;
       acc.a[1] += r * r;
       acc.a[2] += fv * fv;
-    } else if (MODE == M_PCGA) {
-      const double p = sN[s0][0][tx + (TILE + 1) * ty];
+    } else if constexpr (MODE == M_PCGA) {
+      const double p = own[0];
       out2[q] = p;
       out[q] = v[0];
       acc.a[0] += p * v[0];
@@ -782,15 +893,15 @@ struct PfOp {
   }
 
   __device__ __forceinline__ void finish(Acc& acc) const {
-    if (MODE == M_APPLY) return;
+    if constexpr (MODE == M_APPLY) return;
     double* scal = R.scal;
-    if (MODE == M_RESID) {
+    if constexpr (MODE == M_RESID) {
       reduce_finalize<RED_SUM, 3>(acc.a, R, 0.0, [=](double (&s)[3]) {
         scal[S_RZ] = s[0];
         scal[S_RR] = s[1];
         scal[S_FF] = s[2];
       });
-    } else if (MODE == M_PCGA) {
+    } else if constexpr (MODE == M_PCGA) {
       double v1[1] = {acc.a[0]};
       reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) {
         scal[S_PW] = s[0];
@@ -803,9 +914,7 @@ struct PfOp {
   }
 };
 
-static dim3 tile_grid(const LevelGeom& L) {
-  return dim3((L.nx + OWN - 1) / OWN, (L.ny + OWN - 1) / OWN, (L.nz + L.zc - 1) / L.zc);
-}
+static dim3 tile_grid(const LevelGeom& L) { return dim3((unsigned)tile_blocks(L.nx, L.ny, L.nz, L.zc)); }
 
 template <int MODE, bool FINE>
 static MechOp<MODE, FINE> mech_op(const LevelGeom& L, const MechCoef& C) {
@@ -827,42 +936,44 @@ static MechOp<MODE, FINE> mech_op(const LevelGeom& L, const MechCoef& C) {
 
 template <int MODE>
 static void run_mech(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, const double* invD,
-                     const double* p_old, double* out, double* out2, int flag, const Reduce& R) {
+                     const double* p_old, double* out, double* out2, double* out3, int flag, const Reduce& R) {
   if (C.fine) {
     auto op = mech_op<MODE, true>(L, C);
-    op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.flag = flag; op.R = R;
+    op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
+    op.flag = flag; op.R = R;
     tile_kernel<MechOp<MODE, true>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
   } else {
     auto op = mech_op<MODE, false>(L, C);
-    op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.flag = flag; op.R = R;
+    op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
+    op.flag = flag; op.R = R;
     tile_kernel<MechOp<MODE, false>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
   }
 }
 
 void launch_mech_apply(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, double* y) {
   Reduce R{};
-  run_mech<M_APPLY>(s, L, C, x, nullptr, nullptr, y, nullptr, 0, R);
+  run_mech<M_APPLY>(s, L, C, x, nullptr, nullptr, y, nullptr, nullptr, 0, R);
 }
 void launch_mech_resid(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* u, double* f,
-                       double* r, const double* invD, int t0fine, const Reduce& R) {
-  run_mech<M_RESID>(s, L, C, u, invD, nullptr, r, f, t0fine, R);
+                       double* r, double* z, const double* invD, int t0fine, const Reduce& R) {
+  run_mech<M_RESID>(s, L, C, u, invD, nullptr, r, f, z, t0fine, R);
 }
-void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* r,
-                       const double* invD, const double* p_old, double* p_new, double* w, int first,
-                       const Reduce& R) {
-  run_mech<M_PCGA>(s, L, C, r, invD, p_old, w, p_new, first, R);
+void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* z,
+                       const double* p_old, double* p_new, double* w, int first, const Reduce& R) {
+  run_mech<M_PCGA>(s, L, C, z, nullptr, p_old, w, p_new, nullptr, first, R);
 }
 void launch_mech_diag(hipStream_t s, const LevelGeom& L, const MechCoef& C, double* diag_or_null,
                       double* invD, const Reduce& R) {
-  run_mech<M_DIAG>(s, L, C, nullptr, nullptr, nullptr, invD, diag_or_null, 0, R);
+  run_mech<M_DIAG>(s, L, C, nullptr, nullptr, nullptr, invD, diag_or_null, nullptr, 0, R);
 }
 
 template <int MODE>
 static void run_pf(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* x, const double* invD,
-                   const double* p_old, double* out, double* out2, int flag, const Reduce& R) {
+                   const double* p_old, double* out, double* out2, double* out3, int flag, const Reduce& R) {
   PfOp<MODE> op{};
   op.C = C;
-  op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.flag = flag; op.R = R;
+  op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
+  op.flag = flag; op.R = R;
   op.sx = 1.0 / L.hx;
   op.sy = 1.0 / L.hy;
   op.sz = 1.0 / L.hz;
@@ -875,19 +986,19 @@ static void run_pf(hipStream_t s, const LevelGeom& L, const PfCoef& C, const dou
 
 void launch_pf_apply(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* x, double* y) {
   Reduce R{};
-  run_pf<M_APPLY>(s, L, C, x, nullptr, nullptr, y, nullptr, 0, R);
+  run_pf<M_APPLY>(s, L, C, x, nullptr, nullptr, y, nullptr, nullptr, 0, R);
 }
 void launch_pf_resid(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* d, const double* fd,
-                     double* r, const double* invD, const Reduce& R) {
-  run_pf<M_RESID>(s, L, C, d, invD, nullptr, r, const_cast<double*>(fd), 0, R);
+                     double* r, double* z, const double* invD, const Reduce& R) {
+  run_pf<M_RESID>(s, L, C, d, invD, nullptr, r, const_cast<double*>(fd), z, 0, R);
 }
-void launch_pf_pcg_a(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* r, const double* invD,
-                     const double* p_old, double* p_new, double* w, int first, const Reduce& R) {
-  run_pf<M_PCGA>(s, L, C, r, invD, p_old, w, p_new, first, R);
+void launch_pf_pcg_a(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* z, const double* p_old,
+                     double* p_new, double* w, int first, const Reduce& R) {
+  run_pf<M_PCGA>(s, L, C, z, nullptr, p_old, w, p_new, nullptr, first, R);
 }
 void launch_pf_diag(hipStream_t s, const LevelGeom& L, const PfCoef& C, double* diag_or_null, double* invD,
                     const Reduce& R) {
-  run_pf<M_DIAG>(s, L, C, nullptr, nullptr, nullptr, invD, diag_or_null, 0, R);
+  run_pf<M_DIAG>(s, L, C, nullptr, nullptr, nullptr, invD, diag_or_null, nullptr, 0, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -898,31 +1009,45 @@ static int pw_blocks(size_t n) {
   size_t b = (n + PW_TPB - 1) / PW_TPB;
   return (int)(b < 2048 ? (b == 0 ? 1 : b) : 2048);
 }
+// grid of the PCG pass-B kernels (at most HPCR_PASSB_MAX_BLOCKS blocks,
+// grid-stride beyond)
+#ifndef HPCR_PASSB_MAX_BLOCKS
+#define HPCR_PASSB_MAX_BLOCKS 2048
+#endif
+int passb_blocks(size_t n) {
+  const size_t b = (n + PW_TPB - 1) / PW_TPB;
+  const size_t cap = HPCR_PASSB_MAX_BLOCKS;
+  return (int)(b == 0 ? 1 : (b < cap ? b : cap));
+}
 
 __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, double* __restrict__ u,
-                                                            double* __restrict__ r, const double* __restrict__ p,
+                                                            double* __restrict__ r, double* __restrict__ z,
+                                                            const double* __restrict__ p,
                                                             const double* __restrict__ w,
                                                             const double* __restrict__ invD, const Reduce R) {
-  const size_t N = Nof(L);
-  const size_t plane = (size_t)L.nx * L.ny;
+  // node counts stay below 2^32 (a 1625^3 grid would not fit in HBM)
+  const unsigned N = (unsigned)Nof(L);
+  const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
   const double alpha = R.scal[S_ALPHA];
   double acc[2] = {0.0, 0.0};
-  for (size_t q = (size_t)blockIdx.x * PW_TPB + threadIdx.x; q < N; q += (size_t)gridDim.x * PW_TPB) {
-    const int k = (int)(q / plane);
+  for (unsigned q = blockIdx.x * PW_TPB + threadIdx.x; q < N; q += gridDim.x * PW_TPB) {
+    const unsigned k = q / plane;
     int dm = 0;
     if (L.fz[k]) {
-      const size_t rem = q - (size_t)k * plane;
-      const int j = (int)(rem / L.nx), i = (int)(rem - (size_t)j * L.nx);
-      dm = dir_mask(L, i, j, k);
+      const unsigned rem = q - k * plane;
+      const unsigned j = rem / (unsigned)L.nx, i = rem - j * (unsigned)L.nx;
+      dm = dir_mask(L, (int)i, (int)j, (int)k);
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const size_t o = c * N + q;
+      const size_t o = (size_t)c * N + q;
       u[o] = u[o] + alpha * p[o];
       double rv = r[o] - alpha * w[o];
       if ((dm >> c) & 1) rv = 0.0;
       r[o] = rv;
-      acc[0] += rv * (rv * invD[o]);
+      const double zv = rv * invD[o];
+      z[o] = zv;
+      acc[0] += rv * zv;
       acc[1] += rv * rv;
     }
   }
@@ -935,7 +1060,7 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
 }
 
 __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __restrict__ d, double* __restrict__ r,
-                                                          const double* __restrict__ p,
+                                                          double* __restrict__ z, const double* __restrict__ p,
                                                           const double* __restrict__ w,
                                                           const double* __restrict__ invD, const Reduce R) {
   const double alpha = R.scal[S_ALPHA];
@@ -944,7 +1069,9 @@ __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __re
     d[q] = d[q] + alpha * p[q];
     const double rv = r[q] - alpha * w[q];
     r[q] = rv;
-    acc[0] += rv * (rv * invD[q]);
+    const double zv = rv * invD[q];
+    z[q] = zv;
+    acc[0] += rv * zv;
     acc[1] += rv * rv;
   }
   double* scal = R.scal;
@@ -955,15 +1082,15 @@ __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __re
   });
 }
 
-void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, const double* p,
+void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const double* p,
                        const double* w, const double* invD, const Reduce& R) {
   const size_t N = (size_t)L.nx * L.ny * L.nz;
-  hipLaunchKernelGGL(mech_pcg_b_kernel, dim3(pw_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, p, w, invD, R);
+  hipLaunchKernelGGL(mech_pcg_b_kernel, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, z, p, w, invD, R);
 }
-void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, const double* p, const double* w,
-                     const double* invD, const Reduce& R) {
+void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, double* z, const double* p,
+                     const double* w, const double* invD, const Reduce& R) {
   const size_t N = (size_t)L.nx * L.ny * L.nz;
-  hipLaunchKernelGGL(pf_pcg_b_kernel, dim3(pw_blocks(N)), dim3(PW_TPB), 0, s, N, d, r, p, w, invD, R);
+  hipLaunchKernelGGL(pf_pcg_b_kernel, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, N, d, r, z, p, w, invD, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,6 +1143,318 @@ void launch_dirichlet_copy(hipStream_t s, const LevelGeom& L, double* f, const d
   if (n <= 0) return;
   hipLaunchKernelGGL(dirichlet_copy_kernel, dim3((L.nx + 63) / 64, L.ny), dim3(64), 0, s, L, f, Fi,
                      planes_of(kp, n));
+}
+
+// ---------------------------------------------------------------------------
+// Whole smoother call of a small coarse level in one workgroup
+// (pcg_u, mechanical_pcg.cpp:223-330; pcg_d, phase_field_pcg.cpp:133-214)
+// ---------------------------------------------------------------------------
+// Thread t owns nodes t and t + SMALL_TPB (u, f, r, z, 1/D and p in
+// registers) and element t.  Per operator apply: p -> LDS, barrier, every
+// element writes its 8 corner contributions to LDS, barrier, every node sums
+// the contributions of its (up to) 8 elements in a fixed order.  Reductions
+// are block-wide with a fixed shuffle tree.  The recurrence is that of the
+// two-pass path: p = z + beta p, alpha = r.z / p.Ap, beta = r.z_new / r.z.
+constexpr int SMALL_TPB = 256;
+constexpr int SMALL_NPT = SMALL_NODES / SMALL_TPB;
+static_assert(SMALL_NODES % SMALL_TPB == 0 && SMALL_ELEMS == SMALL_TPB, "small-level layout");
+
+template <int NV>
+__device__ __forceinline__ void block_sum_all(double (&v)[NV], double* sh) {
+  block_red<RED_SUM, NV>(v, sh);
+  __shared__ double bc[NV];
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; k++) bc[k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = bc[k];
+  __syncthreads();
+}
+
+struct SmallArgs {
+  LevelGeom L;
+  // mechanics: element K_e, G_e; phase field: element H, nodal gc
+  const double* Ke;
+  const double* Ge;
+  const double* H;
+  const double* gc;
+  double lc;
+  double* x;           // u (3 comps) or d
+  const double* f;     // right-hand side
+  const double* invD;  // inverse Jacobi diagonal
+  double* r;           // final residual
+  double* z;           // final r / D
+  double* p;           // final search direction
+  double* w;           // final A p
+  double Ut;
+  int NB;
+  double* scal;
+};
+
+template <int NC>
+__global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgs a) {
+  const LevelGeom& L = a.L;
+  __shared__ double sP[NC][SMALL_NODES];
+  __shared__ double sE[NC * 8][SMALL_ELEMS];
+  __shared__ double sh[4 * 16];
+  const int tid = threadIdx.x;
+  const int N = L.nx * L.ny * L.nz;
+  const int ex_n = L.nx - 1, ey_n = L.ny - 1;
+  const int E = ex_n * ey_n * (L.nz - 1);
+  const double sx = 1.0 / L.hx, sy = 1.0 / L.hy, sz = 1.0 / L.hz;
+  const double wg = 0.125 * L.hx * L.hy * L.hz;
+
+  // element of this thread and its constant coefficients
+  const bool has_e = tid < E;
+  int ei = 0, ej = 0, ek = 0;
+  if (has_e) {
+    ei = tid % ex_n;
+    ej = (tid / ex_n) % ey_n;
+    ek = tid / (ex_n * ey_n);
+  }
+  double lam0 = 0.0, mu0 = 0.0;
+  double pm[2][2][2], pk[2][2][2];
+  if (has_e) {
+    const size_t e = nidx(L, ei, ej, ek);
+    if constexpr (NC == 3) {
+      lam0 = a.Ke[e] - (2.0 / 3.0) * a.Ge[e];
+      mu0 = a.Ge[e];
+    } else {
+      double gcn[2][2][2];
+#pragma unroll
+      for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+        for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+          for (int ix = 0; ix < 2; ix++) gcn[iz][iy][ix] = a.gc[nidx(L, ei + ix, ej + iy, ek + iz)];
+      pf_coefs(gcn, a.H[e], a.lc, wg, pm, pk);
+    }
+  }
+
+  // nodes of this thread
+  int qn[SMALL_NPT], dm[SMALL_NPT];
+  double xv[SMALL_NPT][NC], fv[SMALL_NPT][NC], iD[SMALL_NPT][NC];
+#pragma unroll
+  for (int s = 0; s < SMALL_NPT; s++) {
+    const int q = tid + s * SMALL_TPB;
+    qn[s] = q < N ? q : -1;
+    dm[s] = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) xv[s][c] = fv[s][c] = iD[s][c] = 0.0;
+    if (q < N) {
+      if constexpr (NC == 3) {
+        const int i = q % L.nx, j = (q / L.nx) % L.ny, k = q / (L.nx * L.ny);
+        dm[s] = dir_mask(L, i, j, k);
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        xv[s][c] = a.x[(size_t)c * N + q];
+        fv[s][c] = a.f[(size_t)c * N + q];
+        iD[s][c] = a.invD[(size_t)c * N + q];
+      }
+      // Dirichlet_BD (boundary_conditions.cpp:28-66)
+      if constexpr (NC == 3) {
+        if (dm[s] & 1) xv[s][0] = 0.0;
+        if (dm[s] & 2) xv[s][1] = 0.0;
+        if (dm[s] & 4) xv[s][2] = (dm[s] & 8) ? a.Ut : 0.0;
+      }
+    }
+  }
+
+  // y = A v for v in sP: element contributions, then node sums
+  auto apply = [&](double (&y)[SMALL_NPT][NC]) {
+    __syncthreads();
+    if (has_e) {
+      double f[NC][2][2][2];
+      if constexpr (NC == 3) {
+        double u[3][2][2][2];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+              for (int ix = 0; ix < 2; ix++)
+                u[c][iz][iy][ix] = sP[c][(ei + ix) + L.nx * ((ej + iy) + L.ny * (ek + iz))];
+        double dl[2][2][2], dmu[2][2][2];
+        mech_element<true>(u, dl, dmu, lam0, mu0, sx, sy, sz, wg / L.hx, wg / L.hy, wg / L.hz, f);
+      } else {
+        double v[2][2][2];
+#pragma unroll
+        for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+          for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+            for (int ix = 0; ix < 2; ix++) v[iz][iy][ix] = sP[0][(ei + ix) + L.nx * ((ej + iy) + L.ny * (ek + iz))];
+        pf_element(v, pm, pk, sx, sy, sz, f[0]);
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+          for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+            for (int ix = 0; ix < 2; ix++) sE[c * 8 + iz * 4 + iy * 2 + ix][tid] = f[c][iz][iy][ix];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SMALL_NPT; s++) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) y[s][c] = 0.0;
+      if (qn[s] < 0) continue;
+      const int q = qn[s];
+      const int i = q % L.nx, j = (q / L.nx) % L.ny, k = q / (L.nx * L.ny);
+#pragma unroll
+      for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+        for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+          for (int ix = 0; ix < 2; ix++) {
+            const int e_i = i - ix, e_j = j - iy, e_k = k - iz;
+            if (e_i < 0 || e_j < 0 || e_k < 0 || e_i >= ex_n || e_j >= ey_n || e_k >= L.nz - 1) continue;
+            const int e = e_i + ex_n * (e_j + ey_n * e_k);
+#pragma unroll
+            for (int c = 0; c < NC; c++) y[s][c] += sE[c * 8 + iz * 4 + iy * 2 + ix][e];
+          }
+    }
+  };
+
+  // initial residual r = f - A x (masked), z = r / D
+  double r[SMALL_NPT][NC], z[SMALL_NPT][NC], p[SMALL_NPT][NC], w[SMALL_NPT][NC];
+#pragma unroll
+  for (int s = 0; s < SMALL_NPT; s++)
+    if (qn[s] >= 0)
+#pragma unroll
+      for (int c = 0; c < NC; c++) sP[c][qn[s]] = xv[s][c];
+  apply(w);
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < SMALL_NPT; s++)
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const bool dir = (dm[s] >> c) & 1;
+      r[s][c] = (qn[s] < 0 || dir) ? 0.0 : fv[s][c] - w[s][c];
+      z[s][c] = r[s][c] * iD[s][c];
+      p[s][c] = 0.0;
+      acc[0] += r[s][c] * z[s][c];
+      acc[1] += r[s][c] * r[s][c];
+      acc[2] += fv[s][c] * fv[s][c];
+    }
+  block_sum_all<3>(acc, sh);
+  // the phase-field smoother resets fd_norm on every call (phase_field_pcg.cpp:163)
+  if (NC == 1 && tid == 0) a.scal[S_FF] = acc[2];
+  double rz = acc[0], rr = acc[1], beta = 0.0, pw = 0.0, alpha = 0.0;
+
+  for (int it = 0; it < a.NB; it++) {
+    // pass A: p = z + beta p; w = A p; p.w
+#pragma unroll
+    for (int s = 0; s < SMALL_NPT; s++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        p[s][c] = (it == 0) ? z[s][c] : z[s][c] + beta * p[s][c];
+        if (qn[s] >= 0) sP[c][qn[s]] = p[s][c];
+      }
+    apply(w);
+    double pa[1] = {0.0};
+#pragma unroll
+    for (int s = 0; s < SMALL_NPT; s++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) pa[0] += p[s][c] * w[s][c];
+    block_sum_all<1>(pa, sh);
+    pw = pa[0];
+    alpha = rz / pw;
+    // pass B: x += alpha p; r -= alpha w; mask; z = r / D; r.z, r.r
+    double ab[2] = {0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < SMALL_NPT; s++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        xv[s][c] = xv[s][c] + alpha * p[s][c];
+        double rv = r[s][c] - alpha * w[s][c];
+        if ((dm[s] >> c) & 1) rv = 0.0;
+        r[s][c] = rv;
+        z[s][c] = rv * iD[s][c];
+        ab[0] += rv * z[s][c];
+        ab[1] += rv * rv;
+      }
+    block_sum_all<2>(ab, sh);
+    beta = ab[0] / rz;
+    rz = ab[0];
+    rr = ab[1];
+  }
+
+#pragma unroll
+  for (int s = 0; s < SMALL_NPT; s++) {
+    if (qn[s] < 0) continue;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const size_t o = (size_t)c * N + qn[s];
+      a.x[o] = xv[s][c];
+      a.r[o] = r[s][c];
+      a.z[o] = z[s][c];
+      if (a.NB > 0) {
+        a.p[o] = p[s][c];
+        a.w[o] = w[s][c];
+      }
+    }
+  }
+  if (tid == 0) {
+    a.scal[S_RZ] = rz;
+    a.scal[S_RR] = rr;
+    if (a.NB > 0) {
+      a.scal[S_BETA] = beta;
+      a.scal[S_PW] = pw;
+      a.scal[S_ALPHA] = alpha;
+    }
+  }
+}
+
+bool small_level(const LevelGeom& L) {
+  const long n = (long)L.nx * L.ny * L.nz;
+  const long e = (long)(L.nx - 1) * (L.ny - 1) * (L.nz - 1);
+  return n <= SMALL_NODES && e <= SMALL_ELEMS && L.nx >= 2 && L.ny >= 2 && L.nz >= 2;
+}
+
+void launch_small_pcg_u(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, double* u,
+                        const double* f, const double* invD, double* r, double* z, double* p, double* w, double Ut,
+                        int NB, double* scal) {
+  SmallArgs a{};
+  a.L = L;
+  a.Ke = Ke;
+  a.Ge = Ge;
+  a.x = u;
+  a.f = f;
+  a.invD = invD;
+  a.r = r;
+  a.z = z;
+  a.p = p;
+  a.w = w;
+  a.Ut = Ut;
+  a.NB = NB;
+  a.scal = scal;
+  small_pcg_kernel<3><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
+}
+
+void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, const double* gc, double lc, double* d,
+                        const double* fd, const double* invD, double* r, double* z, double* p, double* w, int NB,
+                        double* scal) {
+  SmallArgs a{};
+  a.L = L;
+  a.H = H;
+  a.gc = gc;
+  a.lc = lc;
+  a.x = d;
+  a.f = fd;
+  a.invD = invD;
+  a.r = r;
+  a.z = z;
+  a.p = p;
+  a.w = w;
+  a.NB = NB;
+  a.scal = scal;
+  small_pcg_kernel<1><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
 }
 
 // ---------------------------------------------------------------------------

@@ -28,10 +28,10 @@ struct Lev {
   size_t N = 0;
   // mechanics (3 components each)
   double *u = nullptr, *fu = nullptr, *ru = nullptr, *uold = nullptr;
-  double *p = nullptr, *p2 = nullptr, *w = nullptr, *invDu = nullptr;
+  double *p = nullptr, *p2 = nullptr, *w = nullptr, *invDu = nullptr, *zu = nullptr;
   // phase field
   double *d = nullptr, *fd = nullptr, *rd = nullptr, *dold = nullptr;
-  double *pd = nullptr, *pd2 = nullptr, *wd = nullptr, *invDd = nullptr;
+  double *pd = nullptr, *pd2 = nullptr, *wd = nullptr, *invDd = nullptr, *zd = nullptr;
   // coefficients
   double *H = nullptr, *K = nullptr, *G = nullptr, *gc = nullptr, *Ke = nullptr, *Ge = nullptr;
   signed char* flag = nullptr;
@@ -188,23 +188,29 @@ int pcg_u(hpcr_ctx* c, int l, int NB, double Ut, int t) {
     if (c->host_scal[S_ZERO] != 0.0)
       return set_err(c, HPCR_ERR_ZERO_DIAGONAL, "Division by zero in grid /= operation (mechanical diagonal)");
   }
+  double* pb[2] = {v.p, v.p2};
+  if (l < c->L && small_level(v.g)) {
+    launch_small_pcg_u(c->stream, v.g, v.Ke, v.Ge, v.u, v.fu, v.invDu, v.ru, v.zu, pb[NB & 1], v.w, Ut, NB, c->scal);
+    c->smoother_iters += NB;
+    c->last_rr_level = l;
+    return check_launch(c, "pcg_u (small level)");
+  }
   launch_dirichlet_bd(c->stream, v.g, v.u, Ut, v.kplanes.data(), (int)v.kplanes.size());
   const int t0fine = (t == 0 && l == c->L) ? 1 : 0;
   {
     TBracket tb(c, HPCR_K_MECH_RESID, l);
-    launch_mech_resid(c->stream, v.g, mcoef(c, l), v.u, v.fu, v.ru, v.invDu, t0fine, red(c));
+    launch_mech_resid(c->stream, v.g, mcoef(c, l), v.u, v.fu, v.ru, v.zu, v.invDu, t0fine, red(c));
   }
-  double* pb[2] = {v.p, v.p2};
   for (int i = 0; i < NB; i++) {
     double* p_old = pb[i & 1];
     double* p_new = pb[(i + 1) & 1];
     {
       TBracket tb(c, HPCR_K_MECH_PCG_A, l);
-      launch_mech_pcg_a(c->stream, v.g, mcoef(c, l), v.ru, v.invDu, p_old, p_new, v.w, i == 0, red(c));
+      launch_mech_pcg_a(c->stream, v.g, mcoef(c, l), v.zu, p_old, p_new, v.w, i == 0, red(c));
     }
     {
       TBracket tb(c, HPCR_K_MECH_PCG_B, l);
-      launch_mech_pcg_b(c->stream, v.g, v.u, v.ru, p_new, v.w, v.invDu, red(c));
+      launch_mech_pcg_b(c->stream, v.g, v.u, v.ru, v.zu, p_new, v.w, v.invDu, red(c));
     }
     c->smoother_iters++;
   }
@@ -268,7 +274,7 @@ int u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {
   if ((rc = mech_setup(c))) return rc;
   if ((rc = mech_diag_all(c))) return rc;
   // Initial_residual_u (mechanical_pcg.cpp:193-220)
-  launch_mech_resid(c->stream, F.g, mcoef(c, L), F.u, F.fu, F.ru, F.invDu, 0, red(c));
+  launch_mech_resid(c->stream, F.g, mcoef(c, L), F.u, F.fu, F.ru, F.zu, F.invDu, 0, red(c));
   if ((rc = check_launch(c, "initial residual"))) return rc;
   if ((rc = read_scal(c))) return rc;
   double r_norm = c->host_scal[S_RR];
@@ -321,18 +327,25 @@ int pcg_d(hpcr_ctx* c, int l, int NB) {
     if (c->host_scal[S_ZERO] != 0.0)
       return set_err(c, HPCR_ERR_ZERO_DIAGONAL, "Division by zero in grid /= operation (phase-field diagonal)");
   }
-  launch_pf_resid(c->stream, v.g, pcoef(c, l), v.d, v.fd, v.rd, v.invDd, red(c));
   double* pb[2] = {v.pd, v.pd2};
+  if (l < c->L && small_level(v.g)) {
+    launch_small_pcg_d(c->stream, v.g, v.H, v.gc, c->par.lc, v.d, v.fd, v.invDd, v.rd, v.zd, pb[NB & 1], v.wd, NB,
+                       c->scal);
+    c->smoother_iters += NB;
+    c->last_rr_level = l;
+    return check_launch(c, "pcg_d (small level)");
+  }
+  launch_pf_resid(c->stream, v.g, pcoef(c, l), v.d, v.fd, v.rd, v.zd, v.invDd, red(c));
   for (int i = 0; i < NB; i++) {
     double* p_old = pb[i & 1];
     double* p_new = pb[(i + 1) & 1];
     {
       TBracket tb(c, HPCR_K_PF_PCG_A, l);
-      launch_pf_pcg_a(c->stream, v.g, pcoef(c, l), v.rd, v.invDd, p_old, p_new, v.wd, i == 0, red(c));
+      launch_pf_pcg_a(c->stream, v.g, pcoef(c, l), v.zd, p_old, p_new, v.wd, i == 0, red(c));
     }
     {
       TBracket tb(c, HPCR_K_PF_PCG_B, l);
-      launch_pf_pcg_b(c->stream, v.g, v.d, v.rd, p_new, v.wd, v.invDd, red(c));
+      launch_pf_pcg_b(c->stream, v.g, v.d, v.rd, v.zd, p_new, v.wd, v.invDd, red(c));
     }
     c->smoother_iters++;
   }
@@ -380,7 +393,7 @@ int phase_field(hpcr_ctx* c, int t, double* fd_norm, hpcr_stats* st) {
   Lev& F = c->lev[L];
   c->smoother_iters = 0;
   if ((rc = pf_diag_all(c))) return rc;
-  launch_pf_resid(c->stream, F.g, pcoef(c, L), F.d, F.fd, F.rd, F.invDd, red(c));
+  launch_pf_resid(c->stream, F.g, pcoef(c, L), F.d, F.fd, F.rd, F.zd, F.invDd, red(c));
   if ((rc = check_launch(c, "initial residual"))) return rc;
   if ((rc = read_scal(c))) return rc;
   double r_norm = c->host_scal[S_RR];
@@ -480,11 +493,11 @@ int build_levels(hpcr_ctx* c) {
   for (int l = 0; l <= L; l++) {
     Lev& v = c->lev[l];
     const size_t N = v.N;
-    double** vec3[] = {&v.u, &v.fu, &v.ru, &v.p, &v.p2, &v.w, &v.invDu};
+    double** vec3[] = {&v.u, &v.fu, &v.ru, &v.p, &v.p2, &v.w, &v.invDu, &v.zu};
     for (double** pp : vec3)
       if ((rc = alloc_n(c, pp, 3 * N))) return rc;
     if (l < L && (rc = alloc_n(c, &v.uold, 3 * N))) return rc;
-    double** sc[] = {&v.d, &v.fd, &v.rd, &v.pd, &v.pd2, &v.wd, &v.invDd, &v.H, &v.K, &v.G, &v.gc, &v.Ke, &v.Ge};
+    double** sc[] = {&v.d, &v.fd, &v.rd, &v.pd, &v.pd2, &v.wd, &v.invDd, &v.zd, &v.H, &v.K, &v.G, &v.gc, &v.Ke, &v.Ge};
     for (double** pp : sc)
       if ((rc = alloc_n(c, pp, N))) return rc;
     if (l < L && (rc = alloc_n(c, &v.dold, N))) return rc;
@@ -518,7 +531,13 @@ int build_levels(hpcr_ctx* c) {
     v.g.fy = v.fy;
     v.g.fz = v.fz;
   }
-  c->partial_cap = (max_tile_blocks > 4096 ? max_tile_blocks : 4096) * 4;
+  {
+    long pb = 4096;
+    if (max_tile_blocks > pb) pb = max_tile_blocks;
+    for (const Lev& v : c->lev)
+      if (passb_blocks(v.N) > pb) pb = passb_blocks(v.N);
+    c->partial_cap = (size_t)pb * 4;
+  }
   if ((rc = alloc_n(c, &c->partials, c->partial_cap))) return rc;
   if ((rc = alloc_n(c, &c->counter, 1))) return rc;
   if ((rc = alloc_n(c, &c->scal, S_COUNT))) return rc;

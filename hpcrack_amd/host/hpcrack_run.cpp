@@ -17,11 +17,21 @@
 //   --stop-force F      stop when the reaction falls below F (default 0.5)
 //   --out PATH          F_U file (default F_U.dat)
 //   --device N          HIP device (default 0)
+//   --write-history     write u, v, w, H, d every --history-every steps
+//                       (default: when the config sets outTdata=1, every
+//                       Tdatafrequence steps), as write_vector_3D does
+//                       (src/crack_propagation.cpp:101-108), plus the loop
+//                       state of that step
+//   --history-every N   history interval in load steps
+//   --history-dir DIR   history folder (default Time_step_data)
+//   --resume T          start from the history of step T and continue at
+//                       step T+1, appending to the F_U file
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -48,6 +58,8 @@ int main(int argc, char** argv) {
   int steps = 5000, device = 0;
   bool strict = true;
   double small_k = 1e-5, tol_u = 1e-6, tol_d = 1e-12, du = 0.0, stop_force = 0.5;
+  int write_history = -1, history_every = 0, resume = -1;
+  std::string history_dir = "Time_step_data";
   for (int a = 1; a < argc; a++) {
     std::string s = argv[a];
     auto next = [&](const char* name) -> const char* {
@@ -67,6 +79,10 @@ int main(int argc, char** argv) {
     else if (s == "--stop-force") stop_force = std::atof(next("--stop-force"));
     else if (s == "--out") out_path = next("--out");
     else if (s == "--device") device = std::atoi(next("--device"));
+    else if (s == "--write-history") write_history = 1;
+    else if (s == "--history-every") history_every = std::atoi(next("--history-every"));
+    else if (s == "--history-dir") history_dir = next("--history-dir");
+    else if (s == "--resume") resume = std::atoi(next("--resume"));
     else if (s.size() > 2 && s[0] == '-' && s[1] == '-') {
       std::cerr << "unknown option " << s << std::endl;
       return 2;
@@ -123,15 +139,47 @@ int main(int argc, char** argv) {
   for (int f : {HPCR_K, HPCR_G, HPCR_GC})
     if ((rc = hpcr_coarse_vr(ctx, f))) return fail("coarse_VR", rc, ctx);
 
-  std::ofstream F_U(out_path);
+  // history output (outTdata / Tdatafrequence, src/crack_propagation.cpp:64,101)
+  if (write_history < 0) write_history = (((cfg.seen >> HPCR_KEY_OUTTDATA) & 1ULL) && cfg.outTdata == 1) ? 1 : 0;
+  if (history_every <= 0)
+    history_every = (((cfg.seen >> HPCR_KEY_TDATAFREQUENCE) & 1ULL) && cfg.Tdatafrequence > 0) ? cfg.Tdatafrequence : 1;
+  if (write_history) std::filesystem::create_directories(history_dir);
+
+  const double U_unit = 1.0;
+  double f_norm = 0.0, fd_norm = 0.0, Ut = 0.0, delta_U = 0.0, u_prop = 1.0, Ue = 0.0;
+  int t_begin = 0;
+  std::vector<double> buf(N);
+  static const char* const vec_names[3] = {"u", "v", "w"};
+  if (resume >= 0) {
+    // the fields of step `resume` and the loop state at its end
+    hpcr_step_state ss;
+    if ((rc = hpcr_read_step_state(history_dir.c_str(), resume, &ss))) return fail("reading the step state", rc, ctx);
+    for (int c = 0; c < 3; c++) {
+      if ((rc = hpcr_read_vector_3d(history_dir.c_str(), vec_names[c], resume, 0, buf.data(), N)) ||
+          (rc = hpcr_set_field(ctx, HPCR_U, L, c, buf.data())))
+        return fail("reading u", rc, ctx);
+    }
+    if ((rc = hpcr_read_vector_3d(history_dir.c_str(), "H", resume, 0, buf.data(), N)) ||
+        (rc = hpcr_set_field(ctx, HPCR_H, L, 0, buf.data())))
+      return fail("reading H", rc, ctx);
+    if ((rc = hpcr_read_vector_3d(history_dir.c_str(), "d", resume, 0, buf.data(), N)) ||
+        (rc = hpcr_set_field(ctx, HPCR_D, L, 0, buf.data())) || (rc = hpcr_save_d_old(ctx)))
+      return fail("reading d", rc, ctx);
+    Ut = ss.Ut;
+    delta_U = ss.delta_U;
+    u_prop = ss.u_prop;
+    f_norm = ss.f_norm;
+    Ue = ss.Ue;
+    t_begin = resume + 1;
+    std::cout << "resuming after step " << resume << ", Ut=" << Ut << std::endl;
+  }
+  std::ofstream F_U(out_path, resume >= 0 ? std::ios::app : std::ios::trunc);
   if (!F_U.is_open()) {
     std::cout << "Can not open file!" << std::endl;
     return 1;
   }
   const auto t1 = std::chrono::steady_clock::now();
-  const double U_unit = 1.0;
-  double f_norm = 0.0, fd_norm = 0.0, Ut = 0.0, delta_U = 0.0, u_prop = 1.0;
-  for (int t = 0; t < steps; t++) {
+  for (int t = t_begin; t < steps; t++) {
     std::cout << "***** At Time step " << t << std::endl;
     hpcr_stats su, sd;
     if ((rc = hpcr_u_entire(ctx, U_unit, t, &f_norm, &su))) return fail("u_entire", rc, ctx);
@@ -140,7 +188,6 @@ int main(int argc, char** argv) {
         Ut = du;
         delta_U = du;
       } else {
-        double Ue = 0.0;
         if ((rc = hpcr_U_e(ctx, &Ue))) return fail("U_e", rc, ctx);
         Ut = Ue;
         delta_U = 1.5e-3 * Ue;
@@ -158,8 +205,23 @@ int main(int argc, char** argv) {
         (rc = hpcr_externe_force_sum(ctx, &FeW)) || (rc = hpcr_d_max(ctx, &dmax)))
       return fail("step reductions", rc, ctx);
     FeW = cfg.voxel_size * FeW;
+    if (write_history && t % history_every == 0) {
+      for (int c = 0; c < 3; c++)
+        if ((rc = hpcr_get_field(ctx, HPCR_U, L, c, buf.data())) ||
+            (rc = hpcr_write_vector_3d(history_dir.c_str(), vec_names[c], t, 0, buf.data(), N)))
+          return fail("writing u", rc, ctx);
+      if ((rc = hpcr_get_field(ctx, HPCR_H, L, 0, buf.data())) ||
+          (rc = hpcr_write_vector_3d(history_dir.c_str(), "H", t, 0, buf.data(), N)) ||
+          (rc = hpcr_get_field(ctx, HPCR_D, L, 0, buf.data())) ||
+          (rc = hpcr_write_vector_3d(history_dir.c_str(), "d", t, 0, buf.data(), N)))
+        return fail("writing H, d", rc, ctx);
+    }
     Ut = Ut + delta_U;
     u_prop = Ut / U_unit;
+    if (write_history && t % history_every == 0) {
+      const hpcr_step_state ss{t, Ut, delta_U, u_prop, f_norm, Ue};
+      if ((rc = hpcr_write_step_state(history_dir.c_str(), &ss))) return fail("writing the step state", rc, ctx);
+    }
     F_U << "T= " << t << ",U= " << std::setprecision(10) << Ut << " ,Fw= " << std::setprecision(10) << u_prop * FeW
         << " ,deltaU= " << delta_U << " ,delatad_max= " << ddmax << std::endl;
     std::cout << " u cycles " << su.cycles << ", d cycles " << sd.cycles << ", dmax " << dmax << std::endl;

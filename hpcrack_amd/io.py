@@ -1,5 +1,6 @@
 """Host-side input formats over the C-ABI of include/hpcrack_io.h:
-config.txt, material CSV, uint8 .raw images, geometry, synthetic images."""
+config.txt, material CSV, uint8 .raw images, geometry, synthetic images,
+load-step history files and the load-loop state."""
 import ctypes
 
 import numpy as np
@@ -38,6 +39,11 @@ class SynthSpec(ctypes.Structure):
                 ("max_tries", ctypes.c_int)]
 
 
+class StepState(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_int), ("Ut", ctypes.c_double), ("delta_U", ctypes.c_double),
+                ("u_prop", ctypes.c_double), ("f_norm", ctypes.c_double), ("Ue", ctypes.c_double)]
+
+
 _proto_done = False
 
 
@@ -61,6 +67,13 @@ def _lib():
         L.hpcr_image_mp.restype = ctypes.c_int
         L.hpcr_image_mp.argtypes = [ctypes.POINTER(MaterialTable), ub, ctypes.c_size_t, ctypes.c_double, dp, dp,
                                     dp]
+        for fn in (L.hpcr_write_vector_3d, L.hpcr_read_vector_3d):
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, dp, ctypes.c_size_t]
+        L.hpcr_write_step_state.restype = ctypes.c_int
+        L.hpcr_write_step_state.argtypes = [ctypes.c_char_p, ctypes.POINTER(StepState)]
+        L.hpcr_read_step_state.restype = ctypes.c_int
+        L.hpcr_read_step_state.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(StepState)]
         L.hpcr_synth_image.restype = ctypes.c_int
         L.hpcr_synth_image.argtypes = [ctypes.POINTER(SynthSpec), ub, dp]
         _proto_done = True
@@ -160,3 +173,36 @@ def synth_image(dims, seed, n_incl=0, incl_r=(4, 12), incl_label=1, pore_fractio
     if rc != 0:
         raise IoError("synthetic image failed")
     return img, (frac[0], frac[1])
+
+
+def write_vector_3d(dir, name, t, proc, v):
+    """write_vector_3D: raw FP64, x-fastest (v shaped (nz, ny, nx))"""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    rc = _lib().hpcr_write_vector_3d(dir.encode(), name.encode(), t, proc,
+                                     v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), v.size)
+    if rc != 0:
+        raise IoError(f"cannot write history {name} t={t} ({rc})")
+
+
+def read_vector_3d(dir, name, t, proc, shape):
+    out = np.empty(shape, dtype=np.float64)
+    rc = _lib().hpcr_read_vector_3d(dir.encode(), name.encode(), t, proc,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size)
+    if rc != 0:
+        raise IoError(f"cannot read history {name} t={t} ({rc})")
+    return out
+
+
+def write_step_state(dir, **kw):
+    s = StepState(**kw)
+    rc = _lib().hpcr_write_step_state(dir.encode(), ctypes.byref(s))
+    if rc != 0:
+        raise IoError(f"cannot write step state ({rc})")
+
+
+def read_step_state(dir, t):
+    s = StepState()
+    rc = _lib().hpcr_read_step_state(dir.encode(), t, ctypes.byref(s))
+    if rc != 0:
+        raise IoError(f"cannot read step state t={t} ({rc})")
+    return {k: getattr(s, k) for k, _ in StepState._fields_}

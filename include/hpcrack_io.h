@@ -66,6 +66,27 @@ int hpcr_read_raw(const char* path, int vx, int vy, int vz, const int roi_start[
                   unsigned char* out);
 int hpcr_write_raw(const char* path, const unsigned char* img, size_t n);
 
+/* Load-step history files.
+ * write_vector_3D / read_vector_3D (src/write_read_history.cpp:18-86,
+ * include/write_read_history.h:17-20): one FP64 grid as raw doubles,
+ * x-fastest.  A `name` containing ".dat" is the file path itself; otherwise
+ * the file is <dir>/<name>_t_<t>_Proc_<proc>.dat (the reference's dir is
+ * "Time_step_data").  Reading requires the file to hold exactly n doubles. */
+int hpcr_write_vector_3d(const char* dir, const char* name, int t, int proc, const double* v, size_t n);
+int hpcr_read_vector_3d(const char* dir, const char* name, int t, int proc, double* v, size_t n);
+
+/* Scalar state of the load loop (crack_propagation, src/crack_propagation.cpp:
+ * 28-123) at the end of step t, which the reference's history files do not
+ * keep: the next load Ut, its increment, u_prop = Ut / U_unit, the frozen
+ * f_norm of step 0 and Ue.  Text file <dir>/state_t_<t>.txt, one "key value"
+ * per line, values printed with 17 significant digits. */
+typedef struct hpcr_step_state {
+  int t;
+  double Ut, delta_U, u_prop, f_norm, Ue;
+} hpcr_step_state;
+int hpcr_write_step_state(const char* dir, const hpcr_step_state* s);
+int hpcr_read_step_state(const char* dir, int t, hpcr_step_state* s);
+
 /* image_MP (src/image_input.cpp:120-142): gray level -> K, G, gc with the
  * reference's integer-division interpolation; E is scaled by voxel_size.
  * Returns HPCR_ERR_ARG for a gray level outside the table. */

@@ -24,12 +24,9 @@ def parse_fu(path):
     return rows
 
 
-@pytest.mark.parametrize("schedule", ["fixed", "reference"])
-def test_driver_matches_oracle_staggered_loop(tmp_path, schedule):
-    require_gpu()
+def write_case(tmp_path, n=17, seed=31):
     from hpcrack_amd import io as hio
-    n = 17
-    img, _ = hio.synth_image((n, n, n), 31, n_incl=3, incl_r=(2.0, 4.0))
+    img, _ = hio.synth_image((n, n, n), seed, n_incl=3, incl_r=(2.0, 4.0))
     hio.write_raw(str(tmp_path / "img.raw"), img)
     (tmp_path / "mat.csv").write_text("Gray level,0,1\nE/Pa,3000000000,70000000000\nnu,0.3,0.2\ngc/(N/m),100,1000\n")
     (tmp_path / "config.txt").write_text(
@@ -37,10 +34,21 @@ def test_driver_matches_oracle_staggered_loop(tmp_path, schedule):
         "ROI_start_X=0\nROI_start_Y=0\nROI_start_Z=0\nvoxel_size=1e-6\nelement_nb=4,4,4\n"
         "ncy_u=30\nnu0_u=50\nnu1_u=4\nnu2_u=2\ngamma_u=1\nncy_d=30\nnu0_d=10\nnu1_d=2\nnu2_d=1\ngamma_d=1\n"
         "lc=2\n" % (tmp_path / "img.raw", n, n, n))
-    steps = 4
-    args = [DRIVER, str(tmp_path / "config.txt"), "--csv", str(tmp_path / "mat.csv"), "--steps", str(steps),
-            "--small-k", "1e-6", "--tol-u", "1e-16", "--tol-d", "1e-16", "--out", str(tmp_path / "F_U.dat"),
+    return img
+
+
+def driver_args(tmp_path, steps, out="F_U.dat"):
+    return [DRIVER, str(tmp_path / "config.txt"), "--csv", str(tmp_path / "mat.csv"), "--steps", str(steps),
+            "--small-k", "1e-6", "--tol-u", "1e-16", "--tol-d", "1e-16", "--out", str(tmp_path / out),
             "--stop-force", "-1e300"]
+
+
+@pytest.mark.parametrize("schedule", ["fixed", "reference"])
+def test_driver_matches_oracle_staggered_loop(tmp_path, schedule):
+    require_gpu()
+    img = write_case(tmp_path)
+    steps = 4
+    args = driver_args(tmp_path, steps)
     if schedule == "fixed":
         args += ["--du", "0.05"]
     say(f"[gpu] driver {schedule}")
@@ -60,3 +68,36 @@ def test_driver_matches_oracle_staggered_loop(tmp_path, schedule):
         assert g["U"] == pytest.approx(e["Ut"], rel=1e-9)
         assert g["Fw"] == pytest.approx(e["Fw"], rel=1e-7)
         assert g["delatad_max"] == pytest.approx(e["ddmax"], rel=1e-7, abs=1e-12)
+
+
+def test_driver_resume_reproduces_uninterrupted_run(tmp_path):
+    """History of step 1 (u, v, w, H, d in the reference's write_vector_3D
+    layout, plus the loop state), then a second process resumes from it: the
+    F_U lines of steps 2 and 3 are identical to those of the run that never
+    stopped (the device path is deterministic)."""
+    require_gpu()
+    from hpcrack_amd import io as hio
+    write_case(tmp_path)
+    hist = tmp_path / "hist"
+    say("[gpu] driver history + resume")
+    # a fixed load increment large enough that H and d grow from step 0
+    r = subprocess.run(driver_args(tmp_path, 4, "full.dat") + ["--du", "0.05", "--write-history",
+                                                                "--history-every", "1", "--history-dir", str(hist)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for t in range(4):
+        for name in ("u", "v", "w", "H", "d"):
+            assert os.path.getsize(hist / f"{name}_t_{t}_Proc_0.dat") == 17 ** 3 * 8
+    d1 = hio.read_vector_3d(str(hist), "d", 1, 0, (17, 17, 17))
+    H1 = hio.read_vector_3d(str(hist), "H", 1, 0, (17, 17, 17))
+    assert H1.max() > 0 and np.isfinite(d1).all()
+    st = hio.read_step_state(str(hist), 1)
+    assert st["t"] == 1 and st["f_norm"] > 0
+    full = open(tmp_path / "full.dat").read().splitlines()
+    (tmp_path / "resumed.dat").write_text("\n".join(full[:2]) + "\n")
+    r = subprocess.run(driver_args(tmp_path, 4, "resumed.dat") + ["--du", "0.05", "--resume", "1",
+                                                                   "--history-dir", str(hist)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    resumed = open(tmp_path / "resumed.dat").read().splitlines()
+    assert resumed == full

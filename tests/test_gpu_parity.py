@@ -197,6 +197,48 @@ def test_pcg_smoothers(HA, case, nb):
     assert rel_l2(s.get(O.RD, L), o.field(O.RD, L)) < 1e-10
 
 
+# Coarse levels of at most 512 nodes and 256 elements run a whole smoother
+# call in one workgroup; level 0 of every case here takes that path (5^3,
+# 3x5x4 ... nodes), level 1 of the 9^3 case does not (512 elements).
+SMALL_CASES = [CASES[0], CASES[2], (2, (2, 4, 3), (-1.0, 3.0, 0.0, 8.0, 0.5, 6.5))]
+
+
+@pytest.mark.parametrize("case", SMALL_CASES, ids=["9c", "9x13x9", "9x17x13"])
+@pytest.mark.parametrize("nb", [0, 1, 4, 12])
+def test_small_level_smoothers(HA, case, nb):
+    ml, enb, dom = case
+    say(f"[gpu] small-level pcg nb={nb} {case}")
+    o, s, _ = make_pair(HA, ml, enb, dom, seed=20, lc=2.0)
+    rng = np.random.default_rng(21)
+    for l in (0, 1):
+        dims = o.dims(l)
+        Ke = rng.uniform(1e3, 5e4, dims[::-1])
+        Ge = rng.uniform(1e3, 3e4, dims[::-1])
+        u0 = rng.standard_normal((3,) + dims[::-1]) * 0.1
+        f0 = rng.standard_normal((3,) + dims[::-1]) * 1e3
+        Hn = rng.uniform(0, 20, dims[::-1])
+        d0 = rng.uniform(0, 0.5, dims[::-1])
+        fd = rng.standard_normal(dims[::-1])
+        for obj in (o, s):
+            for f, a in ((O.KE, Ke), (O.GE, Ge), (O.U, u0), (O.FU, f0), (O.H, Hn), (O.D, d0), (O.FD, fd)):
+                obj.set(f, l, a)
+        fo = o.pcg_u(l, nb, 0.3, 0, 0.0)
+        fs = s.pcg_u(l, nb, 0.3, 0, 0.0)
+        assert fs == fo
+        # random right-hand sides are not orthogonal to the zero-energy rotation
+        # mode, so long runs lose digits; 12 iterations stay far from that
+        tol = 1e-11 if nb <= 4 else 1e-10
+        assert rel_l2(s.get(O.U, l), o.vec(O.U, l)) < tol, f"u level {l}"
+        if nb <= 4:
+            assert rel_l2(s.get(O.RU, l), o.vec(O.RU, l)) < 1e-10, f"r level {l}"
+        fo = o.pcg_d(l, nb)
+        fs = s.pcg_d(l, nb)
+        assert fs == pytest.approx(fo, rel=1e-13)
+        assert rel_l2(s.get(O.D, l), o.field(O.D, l)) < tol, f"d level {l}"
+        if nb <= 4:
+            assert rel_l2(s.get(O.RD, l), o.field(O.RD, l)) < 1e-10, f"R level {l}"
+
+
 # The x and y pins of the reference's boundary conditions exist only where
 # the element counts in x and y are even on every level; the 9x13x9 case has
 # 3 elements in y on level 0 and is replaced by one with valid pins.

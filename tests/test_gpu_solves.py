@@ -82,7 +82,12 @@ def test_u_entire_and_phase_field(HA, ml, tol_u):
     fds, cds = s.phase_field(0, 0.0)
     say(f"  pf cycles oracle={cdo} gpu={cds}")
     assert cds == cdo
-    assert fds == pytest.approx(fdo, rel=1e-12)
+    # fd_norm is |fd|^2 of the finest level (the last smoother call); fd = fd_h(H)
+    # inherits H's 1e-9 agreement
+    if cdo > 0:
+        assert fds == pytest.approx(float(np.sum(s.get(O.FD, L) ** 2)), rel=1e-13)
+        assert fdo == pytest.approx(float(np.sum(o.field(O.FD, L) ** 2)), rel=1e-13)
+    assert fds == pytest.approx(fdo, rel=2e-9)
     assert rel_l2(s.get(O.D, L), o.field(O.D, L)) < 1e-9
 
 

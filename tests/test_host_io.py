@@ -139,3 +139,38 @@ def test_synthetic_generator_is_deterministic_and_non_overlapping(hio):
     two = np.argwhere(a == 2)
     d2 = ((one[:, None, :] - two[None, :, :]) ** 2).sum(-1)
     assert d2.min() >= 2
+
+
+def test_history_files_match_write_vector_3d_layout(hio, tmp_path):
+    # write_vector_3D (write_read_history.cpp:18-52): raw float64, x-fastest,
+    # <dir>/<name>_t_<t>_Proc_<id>.dat; a name holding ".dat" is the path itself
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal((3, 4, 5))  # (nz, ny, nx)
+    hio.write_vector_3d(str(tmp_path), "H", 15, 0, v)
+    path = tmp_path / "H_t_15_Proc_0.dat"
+    raw = path.read_bytes()
+    assert len(raw) == v.size * 8
+    assert np.array_equal(np.frombuffer(raw, dtype="<f8").reshape(v.shape), v)
+    assert np.array_equal(hio.read_vector_3d(str(tmp_path), "H", 15, 0, v.shape), v)
+    explicit = str(tmp_path / "u_field.dat")
+    hio.write_vector_3d("unused", explicit, 0, 0, v)
+    assert open(explicit, "rb").read() == raw
+    # a file of the wrong length is refused rather than read partially
+    with pytest.raises(hio.IoError):
+        hio.read_vector_3d(str(tmp_path), "H", 15, 0, (3, 4, 4))
+    with pytest.raises(hio.IoError):
+        hio.read_vector_3d(str(tmp_path), "H", 15, 0, (3, 4, 6))
+    with pytest.raises(hio.IoError):
+        hio.read_vector_3d(str(tmp_path), "d", 15, 0, v.shape)
+
+
+def test_step_state_round_trip_is_exact(hio, tmp_path):
+    st = dict(t=10, Ut=1.2345678901234567e-3, delta_U=1.5e-3 * 0.1234567890123, u_prop=1.2345678901234567e-3,
+              f_norm=3.0000000000000004e17, Ue=0.1234567890123)
+    hio.write_step_state(str(tmp_path), **st)
+    assert hio.read_step_state(str(tmp_path), 10) == st
+    with pytest.raises(hio.IoError):
+        hio.read_step_state(str(tmp_path), 11)
+    (tmp_path / "state_t_12.txt").write_text("t 12\nUt 1\n")
+    with pytest.raises(hio.IoError):
+        hio.read_step_state(str(tmp_path), 12)
