@@ -114,6 +114,10 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_mech_pcg_a_C2_257.txt"))
+    ap.add_argument("--mixed", type=int, default=1,
+                    help="also time the mixed-precision solve (FP64 flexible CG + FP32 V-cycle) to the same "
+                         "tolerance (0: skip)")
+    ap.add_argument("--mixed-cheb", type=int, default=2, help="Chebyshev degree of the FP32 smoother")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -140,9 +144,12 @@ def main():
     N = int(np.prod(w.dims))
     log(f"[bench] device memory {s.device_bytes() / 2**30:.2f} GiB")
 
-    def step():
+    def zero_state():
         s.fill(U, L, 0.0, 0), s.fill(U, L, 0.0, 1), s.fill(U, L, 0.0, 2)
         s.fill(FU, L, 0.0, 0), s.fill(FU, L, 0.0, 1), s.fill(FU, L, 0.0, 2)
+
+    def step():
+        zero_state()
         fn, cyc = s.u_entire(1.0, 0, 0.0)
         return fn, cyc
 
@@ -192,6 +199,42 @@ def main():
     }
     value = 3 * N * cycles / elapsed / 1e6
 
+    mixed = None
+    if args.mixed:
+        u_plain = s.get(U, L)
+        for i in range(args.warmup):
+            zero_state()
+            fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, cheb_degree=args.mixed_cheb)
+            log(f"[bench] mixed warmup {i}: {it} iterations, rel. residual^2 {s.last_stats.r_norm_final / fm:.3e}")
+        s.synchronize()
+        t0 = time.perf_counter()
+        its, mrels = 0, []
+        for i in range(args.steps):
+            zero_state()
+            fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, cheb_degree=args.mixed_cheb)
+            its += it
+            mrels.append(s.last_stats.r_norm_final / fm)
+        s.synchronize()
+        mel = time.perf_counter() - t0
+        u_mixed = s.get(U, L)
+        mode = W.rotation_mode(w, L)
+        du = (u_mixed - u_plain).ravel()
+        du -= (du @ mode) / (mode @ mode) * mode
+        mixed = {
+            "solves_per_s": round(args.steps / mel, 3),
+            "ms_per_solve": round(mel / args.steps * 1e3, 2),
+            "fcg_iterations_per_solve": its / args.steps,
+            "mdof_iter_s": round(3 * N * its / mel / 1e6, 2),
+            "final_rel_residual_squared": max(mrels),
+            "reached_tol": bool(max(mrels) <= w.tol_u),
+            "speedup_solves_vs_plain": round((args.steps / mel) / (args.steps / elapsed), 3),
+            "u_rel_l2_vs_plain_fp64": float(np.linalg.norm(du) / np.linalg.norm(u_plain)),
+            "plain_fp64_final_rel_residual_squared": max(rels),
+            "cheb_degree": args.mixed_cheb,
+            "dtype": "f32 V-cycle (Chebyshev-Jacobi, transfers, operators), f64 outer flexible CG, f64 coarsest",
+        }
+        log(f"[bench] mixed: {its} iterations in {mel:.3f} s, {mixed['solves_per_s']} solves/s")
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"[bench] cpu baseline: oracle at {args.cpu_sample}^3, {args.cpu_cycles} cycles, "
@@ -226,6 +269,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": mixed,
         "kernels_finest_level": kern,
     }
     print(json.dumps(out), flush=True)

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libhpcrack_amd.so")
 DRIVER = os.path.join(HERE, "hpcrack_run")
 
-HIP_SOURCES = ["csrc/hpcr_kernels.hip", "csrc/hpcr_solver.hip"]
+HIP_SOURCES = ["csrc/hpcr_kernels.hip", "csrc/hpcr_mixed.hip", "csrc/hpcr_solver.hip"]
 HOST_SOURCES = ["csrc/hpcr_io.cpp"]
 DRIVER_SOURCES = ["host/hpcrack_run.cpp"]
 

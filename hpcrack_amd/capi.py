@@ -51,6 +51,12 @@ _i = ctypes.c_int
 _d = ctypes.c_double
 
 # name -> (restype, argtypes)
+class MixedParams(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int), ("tol", ctypes.c_double), ("cheb_degree", ctypes.c_int),
+                ("power_iters", ctypes.c_int), ("cheb_ratio", ctypes.c_double), ("cheb_upper", ctypes.c_double),
+                ("replace_every", ctypes.c_int)]
+
+
 PROTOTYPES = {
     "hpcr_device_count": (_i, [_ip]),
     "hpcr_ctx_create": (_i, [ctypes.POINTER(Geometry), ctypes.POINTER(Params), _i, ctypes.POINTER(_vp)]),
@@ -66,6 +72,7 @@ PROTOTYPES = {
     "hpcr_coarse_vr": (_i, [_vp, _i]),
     "hpcr_u_entire": (_i, [_vp, _d, _i, _dp, ctypes.POINTER(Stats)]),
     "hpcr_phase_field": (_i, [_vp, _i, _dp, ctypes.POINTER(Stats)]),
+    "hpcr_u_entire_mixed": (_i, [_vp, _d, _i, _dp, ctypes.POINTER(MixedParams), ctypes.POINTER(Stats)]),
     "hpcr_strain_history": (_i, [_vp, _d]),
     "hpcr_voigt_all_level": (_i, [_vp, _i]),
     "hpcr_vr_all_level": (_i, [_vp, _i]),

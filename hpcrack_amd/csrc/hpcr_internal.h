@@ -31,7 +31,14 @@ enum Scal {
   S_FF = 5,    // f.f (f_norm at t == 0)
   S_RED = 6,   // generic reduction result (min / max / sum)
   S_ZERO = 7,  // count of zero diagonal entries
-  S_COUNT = 8
+  // flexible CG of the mixed-precision solve
+  S_ZQ = 8,      // z.q_prev (preconditioned residual against the previous A p)
+  S_PQ = 9,      // p.q
+  S_PR = 10,     // p.r
+  S_FALPHA = 11, // p.r / p.q
+  S_LAM = 12,    // Rayleigh quotient of the power iteration
+  S_SCRATCH = 16,  // 8 slots written by the coarsest solve of the preconditioner
+  S_COUNT = 24
 };
 
 struct LevelGeom {
@@ -59,6 +66,19 @@ struct MechCoef {
   const double* Ke;
   const double* Ge;
   double small_k;
+};
+
+// FP32 copies of the mechanical coefficients of one level (preconditioner of
+// the mixed-precision solve).
+struct MixCoef {
+  int fine;
+  const float* K;
+  const float* G;
+  const float* d;
+  const signed char* flag;
+  const float* Ke;
+  const float* Ge;
+  float small_k;
 };
 
 // Phase-field coefficients of one level.
@@ -119,6 +139,33 @@ void launch_small_pcg_u(hipStream_t s, const LevelGeom& L, const double* Ke, con
 void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, const double* gc, double lc, double* d,
                         const double* fd, const double* invD, double* r, double* z, double* p, double* w, int NB,
                         double* scal);
+
+// ---- mixed-precision solve (hpcr_mixed.hip) ----
+// Chebyshev step: r -= A s (masked), d = cd s + cr D^-1 r, x += d; with qprev,
+// also x.qprev -> scal[S_ZQ]
+void launch_mix_cheb(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout, float* r,
+                     float* x, const float* invD, float cd, float cr, const double* qprev, const Reduce& R);
+// r -= A s (masked)
+void launch_mix_res(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* r);
+// w = D^-1 A v (masked); Rayleigh quotient (v.Av)/(v.Dv) -> scal[S_LAM]
+void launch_mix_power(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* v, float* w,
+                      const float* invD, const Reduce& R);
+void launch_power_init(hipStream_t st, const LevelGeom& L, float* v);
+// flexible CG pass A (FP64): p = z + beta p_old, q = A p, p.q, p.r -> alpha
+void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
+                  double* p_new, double* q, const double* r, int first, const Reduce& R);
+// flexible CG pass B (FP64) + start of the next preconditioner application
+void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, const double* p, const double* q,
+                  float* r32, float* d, float* x, const float* invD, float cr0, const Reduce& R);
+void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, float* r32, float* d, float* x,
+                      const float* invD, float cr0);
+void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,
+                         float* dc, float* xc, const float* invD, float cr0, int start);
+void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, float* xf, float* cf,
+                        const float* ec);
+void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b);
+void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, float* x,
+                            const float* b, const double* invD, int NB, double* scal);
 
 // Dirichlet_BD on planes with z flags; `kplanes` lists the flagged k indices.
 void launch_dirichlet_bd(hipStream_t s, const LevelGeom& L, double* u, double Ut, const int* kplanes_host,

@@ -36,583 +36,17 @@
 #include <cmath>
 
 #include "hpcr_internal.h"
+#include "hpcr_tile.h"
 
 namespace hpcr {
-
-// L_i(g): 1-D linear shape function of node i at Gauss point g (xi = -+1/sqrt3)
-#define GA 0.78867513459481288225  // (1 + 1/sqrt(3)) / 2
-#define GB 0.21132486540518711775  // (1 - 1/sqrt(3)) / 2
-
-__device__ __forceinline__ double Lw(int i, int g) { return (i == g) ? GA : GB; }
-
-__device__ __forceinline__ size_t nidx(const LevelGeom& L, int i, int j, int k) {
-  return (size_t)i + (size_t)L.nx * ((size_t)j + (size_t)L.ny * (size_t)k);
-}
-__host__ __device__ __forceinline__ size_t Nof(const LevelGeom& L) { return (size_t)L.nx * L.ny * L.nz; }
-
-// ---------------------------------------------------------------------------
-// Reductions
-// ---------------------------------------------------------------------------
-enum RedOp { RED_SUM = 0, RED_MIN = 1, RED_MAX = 2 };
-
-template <int OP>
-__device__ __forceinline__ double red2(double a, double b) {
-  if (OP == RED_SUM) return a + b;
-  if (OP == RED_MIN) return (b < a) ? b : a;
-  return (b > a) ? b : a;
-}
-
-template <int OP>
-__device__ __forceinline__ double wave_red(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = red2<OP>(v, __shfl_down(v, off, 64));
-  return v;
-}
-
-// block of blockDim.x (multiple of 64, <= 1024) threads -> result in thread 0
-template <int OP, int NV>
-__device__ __forceinline__ void block_red(double (&v)[NV], double* sh) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; k++) v[k] = wave_red<OP>(v[k]);
-  if (lane == 0)
-    for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int k = 0; k < NV; k++) {
-      double a = sh[k * 16];
-      for (int w = 1; w < nw; w++) a = red2<OP>(a, sh[k * 16 + w]);
-      v[k] = a;
-    }
-}
-
-__device__ __forceinline__ unsigned lin_block() {
-  return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-}
-__device__ __forceinline__ unsigned num_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
-
-// Block partials -> last block -> fin(result[NV]) run by thread 0 of the last block.
-template <int OP, int NV, class Fin>
-__device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R, double init, Fin fin) {
-  __shared__ double sh[NV * 16];
-  __shared__ int is_last;
-  block_red<OP, NV>(v, sh);
-  const unsigned nb = num_blocks();
-  if (threadIdx.x == 0) {
-    const unsigned b = lin_block();
-    for (int k = 0; k < NV; k++)
-      __hip_atomic_store(&R.partials[(size_t)b * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned t = atomicAdd(R.counter, 1u);
-    is_last = (t == nb - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double acc[NV];
-  for (int k = 0; k < NV; k++) acc[k] = init;
-  for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
-    for (int k = 0; k < NV; k++)
-      acc[k] = red2<OP>(acc[k], __hip_atomic_load(&R.partials[(size_t)i * NV + k], __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT));
-  block_red<OP, NV>(acc, sh);
-  if (threadIdx.x == 0) {
-    fin(acc);
-    __hip_atomic_store(R.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Element algebra (sum factorisation)
-// ---------------------------------------------------------------------------
-// trilinear interpolation of 8 nodal values v[iz][iy][ix] to Gauss points o[gz][gy][gx]
-__device__ __forceinline__ void interp8(const double (&v)[2][2][2], double (&o)[2][2][2]) {
-  double t[2][2][2], s[2][2][2];
-#pragma unroll
-  for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-    for (int iy = 0; iy < 2; iy++) {
-      t[iz][iy][0] = GA * v[iz][iy][0] + GB * v[iz][iy][1];
-      t[iz][iy][1] = GB * v[iz][iy][0] + GA * v[iz][iy][1];
-    }
-#pragma unroll
-  for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-    for (int gx = 0; gx < 2; gx++) {
-      s[iz][0][gx] = GA * t[iz][0][gx] + GB * t[iz][1][gx];
-      s[iz][1][gx] = GB * t[iz][0][gx] + GA * t[iz][1][gx];
-    }
-#pragma unroll
-  for (int gy = 0; gy < 2; gy++)
-#pragma unroll
-    for (int gx = 0; gx < 2; gx++) {
-      o[0][gy][gx] = GA * s[0][gy][gx] + GB * s[1][gy][gx];
-      o[1][gy][gx] = GB * s[0][gy][gx] + GA * s[1][gy][gx];
-    }
-}
-// transpose of interp8: nodal f[iz][iy][ix] += sum_g L L L q[gz][gy][gx]
-__device__ __forceinline__ void interp8_t_add(const double (&q)[2][2][2], double (&f)[2][2][2]) {
-  double s[2][2][2], t[2][2][2];
-#pragma unroll
-  for (int gy = 0; gy < 2; gy++)
-#pragma unroll
-    for (int gx = 0; gx < 2; gx++) {
-      s[0][gy][gx] = GA * q[0][gy][gx] + GB * q[1][gy][gx];
-      s[1][gy][gx] = GB * q[0][gy][gx] + GA * q[1][gy][gx];
-    }
-#pragma unroll
-  for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-    for (int gx = 0; gx < 2; gx++) {
-      t[iz][0][gx] = GA * s[iz][0][gx] + GB * s[iz][1][gx];
-      t[iz][1][gx] = GB * s[iz][0][gx] + GA * s[iz][1][gx];
-    }
-#pragma unroll
-  for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-    for (int iy = 0; iy < 2; iy++) {
-      f[iz][iy][0] += GA * t[iz][iy][0] + GB * t[iz][iy][1];
-      f[iz][iy][1] += GB * t[iz][iy][0] + GA * t[iz][iy][1];
-    }
-}
-// bilinear interpolation of 4 edge values e[a][b] (a: outer axis, b: inner
-// axis) to Gauss pairs o[ga][gb], with the inner stage scaled by s
-__device__ __forceinline__ void interp4(const double (&e)[2][2], double s, double (&o)[2][2]) {
-  double t0 = s * (GA * e[0][0] + GB * e[0][1]);
-  double t1 = s * (GB * e[0][0] + GA * e[0][1]);
-  double u0 = s * (GA * e[1][0] + GB * e[1][1]);
-  double u1 = s * (GB * e[1][0] + GA * e[1][1]);
-  o[0][0] = GA * t0 + GB * u0;
-  o[0][1] = GA * t1 + GB * u1;
-  o[1][0] = GB * t0 + GA * u0;
-  o[1][1] = GB * t1 + GA * u1;
-}
-// transpose of interp4
-__device__ __forceinline__ void interp4_t(const double (&S)[2][2], double s, double (&T)[2][2]) {
-  double q00 = GA * S[0][0] + GB * S[1][0];  // a=0, gb=0
-  double q01 = GA * S[0][1] + GB * S[1][1];
-  double q10 = GB * S[0][0] + GA * S[1][0];  // a=1
-  double q11 = GB * S[0][1] + GA * S[1][1];
-  T[0][0] = s * (GA * q00 + GB * q01);
-  T[0][1] = s * (GB * q00 + GA * q01);
-  T[1][0] = s * (GA * q10 + GB * q11);
-  T[1][1] = s * (GB * q10 + GA * q11);
-}
-
-// Gradient of a nodal scalar v[iz][iy][ix] at the Gauss points:
-// gxv[gz][gy] (d/dx), gyv[gz][gx] (d/dy), gzv[gy][gx] (d/dz)
-__device__ __forceinline__ void grad8(const double (&v)[2][2][2], double sx, double sy, double sz,
-                                      double (&gxv)[2][2], double (&gyv)[2][2], double (&gzv)[2][2]) {
-  double ex[2][2], ey[2][2], ez[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      ex[a][b] = v[a][b][1] - v[a][b][0];  // [iz][iy]
-      ey[a][b] = v[a][1][b] - v[a][0][b];  // [iz][ix]
-      ez[a][b] = v[1][a][b] - v[0][a][b];  // [iy][ix]
-    }
-  interp4(ex, sx, gxv);
-  interp4(ey, sy, gyv);
-  interp4(ez, sz, gzv);
-}
-// Transposed gradient: f[iz][iy][ix] += sum_g dN/dx_j * S_j
-__device__ __forceinline__ void grad8_t_add(const double (&Sx)[2][2], const double (&Sy)[2][2],
-                                            const double (&Sz)[2][2], double sx, double sy, double sz,
-                                            double (&f)[2][2][2]) {
-  double T[2][2];
-  interp4_t(Sx, sx, T);  // T[iz][iy]
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      f[a][b][1] += T[a][b];
-      f[a][b][0] -= T[a][b];
-    }
-  interp4_t(Sy, sy, T);  // T[iz][ix]
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      f[a][1][b] += T[a][b];
-      f[a][0][b] -= T[a][b];
-    }
-  interp4_t(Sz, sz, T);  // T[iy][ix]
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      f[1][a][b] += T[a][b];
-      f[0][a][b] -= T[a][b];
-    }
-}
-
-// gauss_point_MP (material_gauss_point.cpp:18-41): zero if any nodal value is 0
-__device__ __forceinline__ void gp_mp(const double (&v)[2][2][2], double (&o)[2][2][2]) {
-  bool z = false;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int c = 0; c < 2; c++) z |= (v[a][b][c] == 0.0);
-  interp8(v, o);
-  if (z) {
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++)
-#pragma unroll
-        for (int c = 0; c < 2; c++) o[a][b][c] = 0.0;
-  }
-}
-// (1-k)(1-d)^2 + k with d clamped to [0,1] (material_gauss_point.cpp:50-56)
-__device__ __forceinline__ double degr(double dg, double k) {
-  dg = dg < 0.0 ? 0.0 : (dg > 1.0 ? 1.0 : dg);
-  return (1.0 - k) * (1.0 - dg) * (1.0 - dg) + k;
-}
-
-// Gauss-point Lame coefficients (lambda = K_d - 2/3 G_d, mu = G_d) of a fine
-// element (MP_all_GAUSS_Point, mechanical_pcg.cpp:26-61)
-__device__ __forceinline__ void fine_lame(const double (&Kn)[2][2][2], const double (&Gn)[2][2][2],
-                                          const double (&dn)[2][2][2], int flag, double small_k,
-                                          double (&lam)[2][2][2], double (&mu)[2][2][2]) {
-  double Kg[2][2][2], Gg[2][2][2], dg[2][2][2];
-  gp_mp(Kn, Kg);
-  gp_mp(Gn, Gg);
-  gp_mp(dn, dg);
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int c = 0; c < 2; c++) {
-        const double g = degr(dg[a][b][c], small_k);
-        const double Kd = (flag == 0) ? g * Kg[a][b][c] : Kg[a][b][c];
-        const double Gd = (flag == 2) ? Gg[a][b][c] : g * Gg[a][b][c];
-        lam[a][b][c] = Kd - (2.0 / 3.0) * Gd;
-        mu[a][b][c] = Gd;
-      }
-}
-
-// y_e = K_e u_e for one element: u[c][iz][iy][ix] -> f[c][iz][iy][ix]
-// (Internal_Force_element, mechanical_pcg.cpp:114-172)
-template <bool CONST_COEF>
-__device__ __forceinline__ void mech_element(const double (&u)[3][2][2][2], const double (&lam)[2][2][2],
-                                             const double (&mu)[2][2][2], double lam0, double mu0,
-                                             double sx, double sy, double sz, double tx, double ty, double tz,
-                                             double (&f)[3][2][2][2]) {
-  double dx[3][2][2], dy[3][2][2], dz[3][2][2];
-#pragma unroll
-  for (int c = 0; c < 3; c++) grad8(u[c], sx, sy, sz, dx[c], dy[c], dz[c]);
-  double Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++) Sx[c][a][b] = Sy[c][a][b] = Sz[c][a][b] = 0.0;
-#pragma unroll
-  for (int gz = 0; gz < 2; gz++)
-#pragma unroll
-    for (int gy = 0; gy < 2; gy++)
-#pragma unroll
-      for (int gx = 0; gx < 2; gx++) {
-        const double exx = dx[0][gz][gy], eyy = dy[1][gz][gx], ezz = dz[2][gy][gx];
-        const double gxy = dx[1][gz][gy] + dy[0][gz][gx];
-        const double gxz = dx[2][gz][gy] + dz[0][gy][gx];
-        const double gyz = dy[2][gz][gx] + dz[1][gy][gx];
-        const double tr = exx + eyy + ezz;
-        const double l = CONST_COEF ? lam0 : lam[gz][gy][gx];
-        const double m = CONST_COEF ? mu0 : mu[gz][gy][gx];
-        const double lt = l * tr, m2 = 2.0 * m;
-        const double sxx = lt + m2 * exx, syy = lt + m2 * eyy, szz = lt + m2 * ezz;
-        const double sxy = m * gxy, sxz = m * gxz, syz = m * gyz;
-        Sx[0][gz][gy] += sxx; Sx[1][gz][gy] += sxy; Sx[2][gz][gy] += sxz;
-        Sy[0][gz][gx] += sxy; Sy[1][gz][gx] += syy; Sy[2][gz][gx] += syz;
-        Sz[0][gy][gx] += sxz; Sz[1][gy][gx] += syz; Sz[2][gy][gx] += szz;
-      }
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = 0.0;
-    grad8_t_add(Sx[c], Sy[c], Sz[c], tx, ty, tz, f[c]);
-  }
-}
-
-// Jacobi diagonal of one element (PreM_u_element, mechanical_pcg.cpp:63-100)
-__device__ __forceinline__ void mech_element_diag(const double (&lam)[2][2][2], const double (&mu)[2][2][2],
-                                                  double ihx2, double ihy2, double ihz2, double wg,
-                                                  double (&f)[3][2][2][2]) {
-#pragma unroll
-  for (int ka = 0; ka < 2; ka++)
-#pragma unroll
-    for (int ja = 0; ja < 2; ja++)
-#pragma unroll
-      for (int ia = 0; ia < 2; ia++) {
-        double s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-        for (int gz = 0; gz < 2; gz++)
-#pragma unroll
-          for (int gy = 0; gy < 2; gy++)
-#pragma unroll
-            for (int gx = 0; gx < 2; gx++) {
-              const double lx = Lw(ia, gx), ly = Lw(ja, gy), lz = Lw(ka, gz);
-              const double X = ihx2 * ly * ly * lz * lz;
-              const double Y = ihy2 * lx * lx * lz * lz;
-              const double Z = ihz2 * lx * lx * ly * ly;
-              const double l = lam[gz][gy][gx], m = mu[gz][gy][gx];
-              const double p = l + 2.0 * m;
-              s0 += p * X + m * (Y + Z);
-              s1 += p * Y + m * (X + Z);
-              s2 += p * Z + m * (X + Y);
-            }
-        f[0][ka][ja][ia] = wg * s0;
-        f[1][ka][ja][ia] = wg * s1;
-        f[2][ka][ja][ia] = wg * s2;
-      }
-}
-
-// Phase-field element (Fld_element, phase_field_pcg.cpp:61-91)
-__device__ __forceinline__ void pf_coefs(const double (&gcn)[2][2][2], double He, double lc, double wg,
-                                         double (&m)[2][2][2], double (&k)[2][2][2]) {
-  double gg[2][2][2];
-  gp_mp(gcn, gg);
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int c = 0; c < 2; c++) {
-        m[a][b][c] = wg * (2.0 * He + gg[a][b][c] / lc);
-        k[a][b][c] = wg * gg[a][b][c] * lc;
-      }
-}
-__device__ __forceinline__ void pf_element(const double (&v)[2][2][2], const double (&m)[2][2][2],
-                                           const double (&k)[2][2][2], double sx, double sy, double sz,
-                                           double (&f)[2][2][2]) {
-  double dv[2][2][2];
-  interp8(v, dv);
-  double q[2][2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int c = 0; c < 2; c++) {
-        q[a][b][c] = m[a][b][c] * dv[a][b][c];
-        f[a][b][c] = 0.0;
-      }
-  interp8_t_add(q, f);
-  double gxv[2][2], gyv[2][2], gzv[2][2];
-  grad8(v, sx, sy, sz, gxv, gyv, gzv);
-  double Sx[2][2], Sy[2][2], Sz[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      Sx[a][b] = gxv[a][b] * (k[a][b][0] + k[a][b][1]);  // [gz][gy], sum over gx
-      Sy[a][b] = gyv[a][b] * (k[a][0][b] + k[a][1][b]);  // [gz][gx], sum over gy
-      Sz[a][b] = gzv[a][b] * (k[0][a][b] + k[1][a][b]);  // [gy][gx], sum over gz
-    }
-  grad8_t_add(Sx, Sy, Sz, sx, sy, sz, f);
-}
-// Md_element (phase_field_pcg.cpp:20-47)
-__device__ __forceinline__ void pf_element_diag(const double (&m)[2][2][2], const double (&k)[2][2][2],
-                                                double ihx2, double ihy2, double ihz2, double (&f)[2][2][2]) {
-#pragma unroll
-  for (int ka = 0; ka < 2; ka++)
-#pragma unroll
-    for (int ja = 0; ja < 2; ja++)
-#pragma unroll
-      for (int ia = 0; ia < 2; ia++) {
-        double s = 0;
-#pragma unroll
-        for (int gz = 0; gz < 2; gz++)
-#pragma unroll
-          for (int gy = 0; gy < 2; gy++)
-#pragma unroll
-            for (int gx = 0; gx < 2; gx++) {
-              const double lx = Lw(ia, gx), ly = Lw(ja, gy), lz = Lw(ka, gz);
-              const double N2 = lx * lx * ly * ly * lz * lz;
-              const double G2 = ihx2 * ly * ly * lz * lz + ihy2 * lx * lx * lz * lz + ihz2 * lx * lx * ly * ly;
-              s += m[gz][gy][gx] * N2 + k[gz][gy][gx] * G2;
-            }
-        f[ka][ja][ia] = s;
-      }
-}
-
-// Dirichlet DOF test of the reference's boundary conditions
-// (boundary_conditions.cpp:28-66): returns a mask, bit c set when component
-// c is prescribed; bit 3 marks u_z = Ut (top face) rather than 0.
-__device__ __forceinline__ int dir_mask(const LevelGeom& L, int i, int j, int k) {
-  const unsigned fz = L.fz[k];
-  if (fz == 0) return 0;
-  int m = 0;
-  const unsigned fx = L.fx[i], fy = L.fy[j];
-  if ((fz & 1) && (fx & 1) && (fy & 1)) m |= 1;
-  if ((fz & 1) && (fx & 2) && (fy & 2)) m |= 2;
-  if (fz & 1) m |= 4;
-  if (fz & 2) m |= 4 | 8;
-  return m;
-}
-
-// ---------------------------------------------------------------------------
-// Tile kernel framework
-// ---------------------------------------------------------------------------
-// lane i <- lane i-1 across the wavefront (DPP wave_shr:1); lane 0 gets 0.
-// Element (tx, ty) and its x-neighbour (tx-1, ty) are adjacent lanes because
-// TX divides the wavefront into whole rows; lanes with tx == 0 own no node.
-__device__ __forceinline__ double wave_shr1(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of tiles so that tiles sharing halo rows share an L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  const int q = nb / 8, rem = nb % 8;
-  const int xcd = b % 8, j = b / 8;
-  return (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + j;
-}
-
-// optional occupancy target of the tile kernels (waves per SIMD), a tuning macro
-#ifdef HPCR_TILE_WAVES_PER_EU
-#define HPCR_TILE_ATTR __attribute__((amdgpu_waves_per_eu(HPCR_TILE_WAVES_PER_EU)))
-#else
-#define HPCR_TILE_ATTR
-#endif
-
-template <class Op>
-__global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op op) {
-  constexpr int NC = Op::NC;  // output components
-  constexpr int NV = Op::NV;  // values per node in LDS
-  constexpr int NR = Op::NR;  // raw values per node fetched from HBM
-  __shared__ double sN[2][NV][PLANE];
-  __shared__ double sX[2 * NC][TPB];
-  const int tid = threadIdx.x;
-  const int tx = tid % TX, ty = tid / TX;
-  const int ntx = (L.nx + OWNX - 1) / OWNX, nty = (L.ny + OWNY - 1) / OWNY;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx0 = (t % ntx) * OWNX, by0 = ((t / ntx) % nty) * OWNY;
-  const int z0 = (t / (ntx * nty)) * L.zc;
-  const int zend = min(z0 + L.zc, L.nz);
-  const int ex = bx0 - 1 + tx, ey = by0 - 1 + ty;
-  const bool owner = (tx >= 1) && (ty >= 1) && (ex < L.nx) && (ey < L.ny);
-  const bool exy = (ex >= 0) && (ey >= 0) && (ex <= L.nx - 2) && (ey <= L.ny - 2);
-  const typename Op::State st = op.state();
-
-  // node slots of this thread in a plane: tid and tid + TPB
-  const bool has2 = tid + TPB < PLANE;
-  const int q1 = tid, q2 = tid + TPB;
-  const int gx1 = bx0 - 1 + q1 % PX, gy1 = by0 - 1 + q1 / PX;
-  const int gx2 = bx0 - 1 + q2 % PX, gy2 = by0 - 1 + q2 / PX;
-  const bool in1 = gx1 >= 0 && gy1 >= 0 && gx1 < L.nx && gy1 < L.ny;
-  const bool in2 = has2 && gx2 >= 0 && gy2 >= 0 && gx2 < L.nx && gy2 < L.ny;
-  double r1[NR], r2[NR];
-  auto fetch = [&](int pz) {
-    const bool zok = pz >= 0 && pz < L.nz;
-    op.fetch(L, gx1, gy1, pz, in1 && zok, st, r1);
-    op.fetch(L, gx2, gy2, pz, in2 && zok, st, r2);
-  };
-  auto commit = [&](int slot) {
-    double v[NV];
-    op.commit(r1, st, v);
-#pragma unroll
-    for (int k = 0; k < NV; k++) sN[slot][k][q1] = v[k];
-    if (has2) {
-      op.commit(r2, st, v);
-#pragma unroll
-      for (int k = 0; k < NV; k++) sN[slot][k][q2] = v[k];
-    }
-  };
-
-  typename Op::Acc acc;
-  double carry[NC];
-#pragma unroll
-  for (int c = 0; c < NC; c++) carry[c] = 0.0;
-
-  // planes z0-1 and z0 into LDS, plane z0+1 in flight
-  fetch(z0 - 1);
-  commit((z0 - 1) & 1);
-  fetch(z0);
-  commit(z0 & 1);
-  fetch(z0 + 1);
-  __syncthreads();
-  for (int kz = z0 - 1; kz < zend; ++kz) {
-    double f[NC][2][2][2];
-    if (exy && kz >= 0 && kz <= L.nz - 2) {
-      op.element(L, sN, kz & 1, tx, ty, ex, ey, kz, st, f);
-    } else {
-#pragma unroll
-      for (int c = 0; c < NC; c++)
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-          for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = 0.0;
-    }
-    // x-direction: node (tx, .) takes corner 0 of its own element and corner 1
-    // of element tx-1 (adjacent lane); g[c][iz][iy]
-    double g[NC][2][2];
-#pragma unroll
-    for (int c = 0; c < NC; c++)
-#pragma unroll
-      for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-        for (int iy = 0; iy < 2; iy++) g[c][iz][iy] = f[c][iz][iy][0] + wave_shr1(f[c][iz][iy][1]);
-    // y-direction through LDS: the iy = 1 halves go to the row above
-#pragma unroll
-    for (int c = 0; c < NC; c++)
-#pragma unroll
-      for (int iz = 0; iz < 2; iz++) sX[c * 2 + iz][tid] = g[c][iz][1];
-    // values of this thread's own node of plane kz, read before the slot is reused
-    double own[NC];
-    if constexpr (Op::NEEDS_OWN) {
-#pragma unroll
-      for (int c = 0; c < NC; c++) own[c] = sN[kz & 1][c][tx + PX * ty];
-    }
-    __syncthreads();
-    // plane kz is no longer read: it receives plane kz+2; plane kz+3 goes in flight
-    commit(kz & 1);
-    fetch(kz + 3);
-    if (owner) {
-      double vb[NC];
-#pragma unroll
-      for (int c = 0; c < NC; c++) {
-        vb[c] = carry[c] + g[c][0][0] + sX[c * 2 + 0][tid - TX];
-        carry[c] = g[c][1][0] + sX[c * 2 + 1][tid - TX];
-      }
-      if (kz >= z0) op.epilogue(L, own, ex, ey, kz, vb, st, acc);
-    }
-    __syncthreads();
-  }
-  op.finish(acc);
-}
-
-// the 8 corner values of element (tx, ty) from the two LDS planes
-template <int NV>
-__device__ __forceinline__ void gather8(const double (&sN)[2][NV][PLANE], int s0, int tx, int ty, int k,
-                                        double (&v)[2][2][2]) {
-  const int q = tx + PX * ty;
-#pragma unroll
-  for (int iz = 0; iz < 2; iz++) {
-    const int s = s0 ^ iz;
-    v[iz][0][0] = sN[s][k][q];
-    v[iz][0][1] = sN[s][k][q + 1];
-    v[iz][1][0] = sN[s][k][q + PX];
-    v[iz][1][1] = sN[s][k][q + PX + 1];
-  }
-}
 
 // ---- mechanical tile ops --------------------------------------------------
 enum MechMode { M_APPLY = 0, M_RESID = 1, M_PCGA = 2, M_DIAG = 3 };
 
 template <int MODE, bool FINE>
 struct MechOp {
+  using T = double;
+  using TR = double;
   static constexpr int NC = 3;
   static constexpr int NU = (MODE == M_DIAG) ? 0 : 3;  // vector values per node in LDS
   static constexpr int NK = FINE ? 3 : 0;              // K, G, d
@@ -685,9 +119,9 @@ struct MechOp {
                                        double& lam0, double& mu0) const {
     if constexpr (FINE) {
       double Kn[2][2][2], Gn[2][2][2], dn[2][2][2];
-      gather8<NV>(sN, s0, tx, ty, NU + 0, Kn);
-      gather8<NV>(sN, s0, tx, ty, NU + 1, Gn);
-      gather8<NV>(sN, s0, tx, ty, NU + 2, dn);
+      gather8(sN, s0, tx, ty, NU + 0, Kn);
+      gather8(sN, s0, tx, ty, NU + 1, Gn);
+      gather8(sN, s0, tx, ty, NU + 2, dn);
       const int fl = C.flag[nidx(L, ex, ey, ez)];
       fine_lame(Kn, Gn, dn, fl, C.small_k, lam, mu);
       lam0 = mu0 = 0.0;
@@ -718,7 +152,7 @@ struct MechOp {
     } else {
       double u[3][2][2][2];
 #pragma unroll
-      for (int c = 0; c < 3; c++) gather8<NV>(sN, s0, tx, ty, c, u[c]);
+      for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
       if constexpr (FINE)
         mech_element<false>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
       else
@@ -796,6 +230,8 @@ struct MechOp {
 // ---- phase-field tile ops -------------------------------------------------
 template <int MODE>
 struct PfOp {
+  using T = double;
+  using TR = double;
   static constexpr int NC = 1;
   static constexpr int NU = (MODE == M_DIAG) ? 0 : 1;
   static constexpr int NV = NU + 1;  // + gc
@@ -853,14 +289,14 @@ struct PfOp {
                                           int ty, int ex, int ey, int ez, const State& st,
                                           double (&f)[1][2][2][2]) const {
     double gcn[2][2][2];
-    gather8<NV>(sN, s0, tx, ty, NU, gcn);
+    gather8(sN, s0, tx, ty, NU, gcn);
     double m[2][2][2], k[2][2][2];
     pf_coefs(gcn, C.H[nidx(L, ex, ey, ez)], C.lc, wg, m, k);
     if constexpr (MODE == M_DIAG) {
       pf_element_diag(m, k, ihx2, ihy2, ihz2, f[0]);
     } else {
       double v[2][2][2];
-      gather8<NV>(sN, s0, tx, ty, 0, v);
+      gather8(sN, s0, tx, ty, 0, v);
       pf_element(v, m, k, sx, sy, sz, f[0]);
     }
   }
@@ -1171,7 +607,8 @@ __device__ __forceinline__ void block_sum_all(double (&v)[NV], double* sh) {
   __syncthreads();
 }
 
-struct SmallArgs {
+template <class TIO>
+struct SmallArgsT {
   LevelGeom L;
   // mechanics: element K_e, G_e; phase field: element H, nodal gc
   const double* Ke;
@@ -1179,20 +616,22 @@ struct SmallArgs {
   const double* H;
   const double* gc;
   double lc;
-  double* x;           // u (3 comps) or d
-  const double* f;     // right-hand side
+  TIO* x;              // u (3 comps) or d
+  const TIO* f;        // right-hand side
   const double* invD;  // inverse Jacobi diagonal
-  double* r;           // final residual
-  double* z;           // final r / D
-  double* p;           // final search direction
-  double* w;           // final A p
+  TIO* r;              // final residual (optional)
+  TIO* z;              // final r / D (optional)
+  TIO* p;              // final search direction (optional)
+  TIO* w;              // final A p (optional)
   double Ut;
   int NB;
   double* scal;
 };
+using SmallArgs = SmallArgsT<double>;
 
-template <int NC>
-__global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgs a) {
+// TIO: type of the vectors in HBM; the iteration itself is FP64
+template <int NC, class TIO>
+__global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgsT<TIO> a) {
   const LevelGeom& L = a.L;
   __shared__ double sP[NC][SMALL_NODES];
   __shared__ double sE[NC * 8][SMALL_ELEMS];
@@ -1248,8 +687,8 @@ __global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgs a)
       }
 #pragma unroll
       for (int c = 0; c < NC; c++) {
-        xv[s][c] = a.x[(size_t)c * N + q];
-        fv[s][c] = a.f[(size_t)c * N + q];
+        xv[s][c] = (double)a.x[(size_t)c * N + q];
+        fv[s][c] = (double)a.f[(size_t)c * N + q];
         iD[s][c] = a.invD[(size_t)c * N + q];
       }
       // Dirichlet_BD (boundary_conditions.cpp:28-66)
@@ -1391,12 +830,12 @@ __global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgs a)
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       const size_t o = (size_t)c * N + qn[s];
-      a.x[o] = xv[s][c];
-      a.r[o] = r[s][c];
-      a.z[o] = z[s][c];
-      if (a.NB > 0) {
-        a.p[o] = p[s][c];
-        a.w[o] = w[s][c];
+      a.x[o] = (TIO)xv[s][c];
+      if (a.r) a.r[o] = (TIO)r[s][c];
+      if (a.z) a.z[o] = (TIO)z[s][c];
+      if (a.NB > 0 && a.p) {
+        a.p[o] = (TIO)p[s][c];
+        a.w[o] = (TIO)w[s][c];
       }
     }
   }
@@ -1409,6 +848,23 @@ __global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgs a)
       a.scal[S_ALPHA] = alpha;
     }
   }
+}
+
+// coarsest solve of the FP32 preconditioner: zero start, homogeneous
+// Dirichlet values, FP64 iteration on float right-hand side and result
+void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, float* x,
+                            const float* b, const double* invD, int NB, double* scal) {
+  SmallArgsT<float> a{};
+  a.L = L;
+  a.Ke = Ke;
+  a.Ge = Ge;
+  a.x = x;
+  a.f = b;
+  a.invD = invD;
+  a.Ut = 0.0;
+  a.NB = NB;
+  a.scal = scal;
+  small_pcg_kernel<3, float><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
 }
 
 bool small_level(const LevelGeom& L) {
@@ -1434,7 +890,7 @@ void launch_small_pcg_u(hipStream_t s, const LevelGeom& L, const double* Ke, con
   a.Ut = Ut;
   a.NB = NB;
   a.scal = scal;
-  small_pcg_kernel<3><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
+  small_pcg_kernel<3, double><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
 }
 
 void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, const double* gc, double lc, double* d,
@@ -1454,7 +910,7 @@ void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, cons
   a.w = w;
   a.NB = NB;
   a.scal = scal;
-  small_pcg_kernel<1><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
+  small_pcg_kernel<1, double><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
 }
 
 // ---------------------------------------------------------------------------

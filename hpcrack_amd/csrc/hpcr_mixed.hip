@@ -1,0 +1,517 @@
+// Mixed-precision form of the mechanical solve: an FP64 flexible conjugate
+// gradient (FCG(1), one-term orthogonalisation) on the finest level whose
+// preconditioner is one FP32 multigrid V-cycle.
+//
+// Preconditioner.  Correction scheme on the same level hierarchy and
+// rediscretised operators as the FAS solve (finest: nodal K, G, d with the
+// split flag; coarse: element K_e, G_e), all vectors and coefficients in
+// FP32, Chebyshev-Jacobi smoothing of a fixed degree before and after the
+// coarse correction (symmetric, so the V-cycle is a fixed SPD operator up to
+// the inexact coarsest solve), the coarsest level solved by FP64 Jacobi-PCG
+// in one workgroup.  Chebyshev bounds come from a power iteration on D^-1 A
+// per level, once per solve.
+//
+// Every smoother step is one tile pass: the stencil of the previous
+// direction s gives A s, and the epilogue updates r -= A s, the new direction
+// d = c_d s + c_r D^-1 r and the iterate x += d.  Restriction starts the
+// coarse level's smoother (d_0 = D^-1 r / theta, x = d_0); prolongation
+// writes the coarse correction c = P e into the direction buffer, so that the
+// first post-smoothing step is the same pass with c_d = 0.
+//
+// Outer iteration (FP64): beta = -(z.q_prev)/(p_prev.q_prev) with z the
+// preconditioner output, p = z + beta p_prev, q = A p, alpha = (p.r)/(p.q),
+// u += alpha p, r -= alpha q; the residual is recomputed from u every few
+// iterations and before convergence is accepted.
+#include <hip/hip_runtime.h>
+
+#include "hpcr_internal.h"
+#include "hpcr_tile.h"
+
+namespace hpcr {
+
+enum FMode { F_CHEB = 0, F_RES = 1, F_POWER = 2 };
+
+template <int MODE, bool FINE>
+struct MechOpF {
+  using T = float;
+  using TR = float;
+  static constexpr int NC = 3;
+  static constexpr int NU = 3;
+  static constexpr int NK = FINE ? 3 : 0;
+  static constexpr int NV = NU + NK;
+  static constexpr int NR = NV;
+  static constexpr bool NEEDS_OWN = (MODE != F_RES);
+  struct Acc {
+    double a[2];
+    __device__ Acc() { a[0] = a[1] = 0.0; }
+  };
+  struct State {};
+  MixCoef C;
+  const float* s;        // stencil vector
+  float* dout;           // CHEB: new direction; POWER: D^-1 A s
+  float* r;              // CHEB, RES: residual, updated in place
+  float* x;              // CHEB: iterate, updated in place
+  const float* invD;
+  float cd, cr;          // CHEB: d = cd s + cr D^-1 r
+  const double* qprev;   // CHEB (optional): accumulate x.qprev into S_ZQ
+  float sx, sy, sz, tx_, ty_, tz_;
+  Reduce R;
+  size_t N;
+
+  __device__ __forceinline__ State state() const { return State{}; }
+
+  __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State&,
+                                        float (&v)[NR]) const {
+#pragma unroll
+    for (int k = 0; k < NR; k++) v[k] = 0.0f;
+    if (!ok) return;
+    const size_t q = nidx(L, gx, gy, gz);
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = s[c * N + q];
+    if constexpr (FINE) {
+      v[3] = C.K[q];
+      v[4] = C.G[q];
+      v[5] = C.d[q];
+    }
+  }
+  __device__ __forceinline__ void commit(const float (&v)[NR], const State&, float (&o)[NV]) const {
+#pragma unroll
+    for (int k = 0; k < NV; k++) o[k] = v[k];
+  }
+
+  __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[2][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, const State&,
+                                          float (&f)[3][2][2][2]) const {
+    float u[3][2][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
+    if constexpr (FINE) {
+      float Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
+      gather8(sN, s0, tx, ty, 3, Kn);
+      gather8(sN, s0, tx, ty, 4, Gn);
+      gather8(sN, s0, tx, ty, 5, dn);
+      fine_lame(Kn, Gn, dn, (int)C.flag[nidx(L, ex, ey, ez)], C.small_k, lam, mu);
+      mech_element<false>(u, lam, mu, 0.0f, 0.0f, sx, sy, sz, tx_, ty_, tz_, f);
+    } else {
+      const size_t e = nidx(L, ex, ey, ez);
+      const float Ke = C.Ke[e], Ge = C.Ge[e];
+      const float lam0 = Ke - (2.0f / 3.0f) * Ge;
+      float dl[2][2][2], dm[2][2][2];
+      mech_element<true>(u, dl, dm, lam0, Ge, sx, sy, sz, tx_, ty_, tz_, f);
+    }
+  }
+
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const float (&own)[3], int i, int j, int k,
+                                           const float (&v)[3], const State&, Acc& acc) const {
+    const size_t q = nidx(L, i, j, k);
+    const int dm = dir_mask(L, i, j, k);
+    if constexpr (MODE == F_CHEB) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const size_t o = c * N + q;
+        const float rn = ((dm >> c) & 1) ? 0.0f : r[o] - v[c];
+        const float dn = cd * own[c] + cr * invD[o] * rn;
+        const float xn = x[o] + dn;
+        r[o] = rn;
+        dout[o] = dn;
+        x[o] = xn;
+        if (qprev) acc.a[0] += (double)xn * qprev[o];
+      }
+    } else if constexpr (MODE == F_RES) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const size_t o = c * N + q;
+        r[o] = ((dm >> c) & 1) ? 0.0f : r[o] - v[c];
+      }
+    } else {  // F_POWER
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const size_t o = c * N + q;
+        const bool dir = (dm >> c) & 1;
+        const float di = invD[o];
+        dout[o] = dir ? 0.0f : di * v[c];
+        if (!dir) {
+          acc.a[0] += (double)own[c] * (double)v[c];
+          acc.a[1] += (double)own[c] * (double)own[c] / (double)di;
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void finish(Acc& acc) const {
+    double* scal = R.scal;
+    if constexpr (MODE == F_CHEB) {
+      if (!qprev) return;
+      double v1[1] = {acc.a[0]};
+      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&t)[1]) { scal[S_ZQ] = t[0]; });
+    } else if constexpr (MODE == F_POWER) {
+      reduce_finalize<RED_SUM, 2>(acc.a, R, 0.0, [=](double (&t)[2]) { scal[S_LAM] = t[0] / t[1]; });
+    }
+  }
+};
+
+// FCG pass A (FP64, finest level): p = z + beta p_prev with z the FP32
+// preconditioner output; q = A p; p.q, p.r -> alpha
+struct FcgOpA {
+  using T = double;
+  using TR = double;
+  static constexpr int NC = 3;
+  static constexpr int NU = 3;
+  static constexpr int NV = 6;
+  static constexpr int NR = 9;
+  static constexpr bool NEEDS_OWN = true;
+  struct Acc {
+    double a[2];
+    __device__ Acc() { a[0] = a[1] = 0.0; }
+  };
+  struct State {
+    double beta;
+  };
+  MechCoef C;
+  const float* z;
+  const double* p_old;
+  double* p_new;
+  double* q;
+  const double* r;
+  int first;
+  double sx, sy, sz, tx_, ty_, tz_;
+  Reduce R;
+  size_t N;
+
+  __device__ __forceinline__ State state() const {
+    State st;
+    st.beta = first ? 0.0 : -R.scal[S_ZQ] / R.scal[S_PQ];
+    return st;
+  }
+  __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State&,
+                                        double (&v)[NR]) const {
+#pragma unroll
+    for (int k = 0; k < NR; k++) v[k] = 0.0;
+    if (!ok) return;
+    const size_t o = nidx(L, gx, gy, gz);
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = (double)z[c * N + o];
+    if (!first) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[3 + c] = p_old[c * N + o];
+    }
+    v[6] = C.K[o];
+    v[7] = C.G[o];
+    v[8] = C.d[o];
+  }
+  __device__ __forceinline__ void commit(const double (&v)[NR], const State& st, double (&w)[NV]) const {
+#pragma unroll
+    for (int c = 0; c < 3; c++) w[c] = v[c] + st.beta * v[3 + c];
+    w[3] = v[6];
+    w[4] = v[7];
+    w[5] = v[8];
+  }
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, const State&,
+                                          double (&f)[3][2][2][2]) const {
+    double u[3][2][2][2], Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
+    gather8(sN, s0, tx, ty, 3, Kn);
+    gather8(sN, s0, tx, ty, 4, Gn);
+    gather8(sN, s0, tx, ty, 5, dn);
+    fine_lame(Kn, Gn, dn, (int)C.flag[nidx(L, ex, ey, ez)], C.small_k, lam, mu);
+    mech_element<false>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
+  }
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k,
+                                           const double (&v)[3], const State&, Acc& acc) const {
+    const size_t o = nidx(L, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      p_new[c * N + o] = own[c];
+      q[c * N + o] = v[c];
+      acc.a[0] += own[c] * v[c];
+      acc.a[1] += own[c] * r[c * N + o];
+    }
+  }
+  __device__ __forceinline__ void finish(Acc& acc) const {
+    double* scal = R.scal;
+    reduce_finalize<RED_SUM, 2>(acc.a, R, 0.0, [=](double (&t)[2]) {
+      scal[S_PQ] = t[0];
+      scal[S_PR] = t[1];
+      scal[S_FALPHA] = t[1] / t[0];
+    });
+  }
+};
+
+static dim3 mix_tile_grid(const LevelGeom& L) { return dim3((unsigned)tile_blocks(L.nx, L.ny, L.nz, L.zc)); }
+
+template <int MODE, bool FINE>
+static void run_f32(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout, float* r,
+                    float* x, const float* invD, float cd, float cr, const double* qprev, const Reduce& R) {
+  MechOpF<MODE, FINE> op{};
+  op.C = C;
+  op.s = s;
+  op.dout = dout;
+  op.r = r;
+  op.x = x;
+  op.invD = invD;
+  op.cd = cd;
+  op.cr = cr;
+  op.qprev = qprev;
+  const double wg = 0.125 * L.hx * L.hy * L.hz;
+  op.sx = (float)(1.0 / L.hx);
+  op.sy = (float)(1.0 / L.hy);
+  op.sz = (float)(1.0 / L.hz);
+  op.tx_ = (float)(wg / L.hx);
+  op.ty_ = (float)(wg / L.hy);
+  op.tz_ = (float)(wg / L.hz);
+  op.R = R;
+  op.N = (size_t)L.nx * L.ny * L.nz;
+  tile_kernel<MechOpF<MODE, FINE>><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
+}
+
+template <int MODE>
+static void run_f32_level(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout,
+                          float* r, float* x, const float* invD, float cd, float cr, const double* qprev,
+                          const Reduce& R) {
+  if (C.fine)
+    run_f32<MODE, true>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+  else
+    run_f32<MODE, false>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+}
+
+void launch_mix_cheb(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout, float* r,
+                     float* x, const float* invD, float cd, float cr, const double* qprev, const Reduce& R) {
+  run_f32_level<F_CHEB>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+}
+void launch_mix_res(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* r) {
+  Reduce R{};
+  run_f32_level<F_RES>(st, L, C, s, nullptr, r, nullptr, nullptr, 0.0f, 0.0f, nullptr, R);
+}
+void launch_mix_power(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* v, float* w,
+                      const float* invD, const Reduce& R) {
+  run_f32_level<F_POWER>(st, L, C, v, w, nullptr, nullptr, invD, 0.0f, 0.0f, nullptr, R);
+}
+
+void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
+                  double* p_new, double* q, const double* r, int first, const Reduce& R) {
+  FcgOpA op{};
+  op.C = C;
+  op.z = z;
+  op.p_old = p_old;
+  op.p_new = p_new;
+  op.q = q;
+  op.r = r;
+  op.first = first;
+  const double wg = 0.125 * L.hx * L.hy * L.hz;
+  op.sx = 1.0 / L.hx;
+  op.sy = 1.0 / L.hy;
+  op.sz = 1.0 / L.hz;
+  op.tx_ = wg / L.hx;
+  op.ty_ = wg / L.hy;
+  op.tz_ = wg / L.hz;
+  op.R = R;
+  op.N = (size_t)L.nx * L.ny * L.nz;
+  tile_kernel<FcgOpA><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
+}
+
+// ---------------------------------------------------------------------------
+// pointwise kernels
+// ---------------------------------------------------------------------------
+constexpr int MX_TPB = 256;
+static int mx_blocks(size_t n, size_t cap) {
+  const size_t b = (n + MX_TPB - 1) / MX_TPB;
+  return (int)(b == 0 ? 1 : (b < cap ? b : cap));
+}
+
+__device__ __forceinline__ int node_mask(const LevelGeom& L, unsigned q) {
+  const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
+  const unsigned k = q / plane;
+  if (!L.fz[k]) return 0;
+  const unsigned rem = q - k * plane;
+  const unsigned j = rem / (unsigned)L.nx, i = rem - j * (unsigned)L.nx;
+  return dir_mask(L, (int)i, (int)j, (int)k);
+}
+
+// start of the finest-level smoother from r (FP64): r32 = r, d = cr0 D^-1 r, x = d
+__global__ __launch_bounds__(MX_TPB) void mix_start_kernel(unsigned N, const double* __restrict__ r64,
+                                                           float* __restrict__ r32, float* __restrict__ d,
+                                                           float* __restrict__ x, const float* __restrict__ invD,
+                                                           float cr0) {
+  for (unsigned q = blockIdx.x * MX_TPB + threadIdx.x; q < N; q += gridDim.x * MX_TPB) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t o = (size_t)c * N + q;
+      const float rv = (float)r64[o];
+      const float dv = cr0 * invD[o] * rv;
+      r32[o] = rv;
+      d[o] = dv;
+      x[o] = dv;
+    }
+  }
+}
+
+// FCG pass B (FP64): u += alpha p; r -= alpha q (masked); r.r; then the start
+// of the next preconditioner application from the new r
+__global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double* __restrict__ u,
+                                                       double* __restrict__ r, const double* __restrict__ p,
+                                                       const double* __restrict__ q, float* __restrict__ r32,
+                                                       float* __restrict__ d, float* __restrict__ x,
+                                                       const float* __restrict__ invD, float cr0, const Reduce R) {
+  const unsigned N = (unsigned)Nof(L);
+  const double alpha = R.scal[S_FALPHA];
+  double acc[1] = {0.0};
+  for (unsigned n = blockIdx.x * MX_TPB + threadIdx.x; n < N; n += gridDim.x * MX_TPB) {
+    const int dm = node_mask(L, n);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t o = (size_t)c * N + n;
+      u[o] = u[o] + alpha * p[o];
+      double rv = r[o] - alpha * q[o];
+      if ((dm >> c) & 1) rv = 0.0;
+      r[o] = rv;
+      acc[0] += rv * rv;
+      const float rf = (float)rv;
+      const float dv = cr0 * invD[o] * rf;
+      r32[o] = rf;
+      d[o] = dv;
+      x[o] = dv;
+    }
+  }
+  double* scal = R.scal;
+  reduce_finalize<RED_SUM, 1>(acc, R, 0.0, [=](double (&t)[1]) { scal[S_RR] = t[0]; });
+}
+
+// restriction of the FP32 residual (full weighting, as the FP64 transfer);
+// starts the coarse smoother (d = cr0 D^-1 r, x = d), or, on the coarsest
+// level (start == 0), only clears x for the coarsest solve
+constexpr int MXB = 64, MYB = 4;
+__global__ __launch_bounds__(MXB * MYB) void mix_restrict_kernel(const LevelGeom Lc, const LevelGeom Lf,
+                                                                 const float* __restrict__ rf,
+                                                                 float* __restrict__ rc, float* __restrict__ dc,
+                                                                 float* __restrict__ xc,
+                                                                 const float* __restrict__ invD, float cr0,
+                                                                 int start) {
+  const int i = blockIdx.x * MXB + threadIdx.x;
+  const int j = blockIdx.y * MYB + threadIdx.y, k = blockIdx.z;
+  if (i >= Lc.nx || j >= Lc.ny) return;
+  const size_t Nc = Nof(Lc), Nf = Nof(Lf);
+  const int x = 2 * i, y = 2 * j, z = 2 * k;
+  float wx[3], wy[3], wz[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    wx[t] = (x + t - 1 < 0 || x + t - 1 > Lf.nx - 1) ? 0.0f : (t == 1 ? 1.0f : 0.5f);
+    wy[t] = (y + t - 1 < 0 || y + t - 1 > Lf.ny - 1) ? 0.0f : (t == 1 ? 1.0f : 0.5f);
+    wz[t] = (z + t - 1 < 0 || z + t - 1 > Lf.nz - 1) ? 0.0f : (t == 1 ? 1.0f : 0.5f);
+  }
+  const size_t qc = nidx(Lc, i, j, k);
+  const int dm = dir_mask(Lc, i, j, k);
+  for (int c = 0; c < 3; c++) {
+    const float* r = rf + c * Nf;
+    float acc = 0.0f;
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++) {
+      if (wz[dz] == 0.0f) continue;
+      float accy = 0.0f;
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) {
+        if (wy[dy] == 0.0f) continue;
+        const size_t row = nidx(Lf, 0, y + dy - 1, z + dz - 1);
+        float accx = 0.0f;
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+          if (wx[dx] != 0.0f) accx += wx[dx] * r[row + x + dx - 1];
+        accy += wy[dy] * accx;
+      }
+      acc += wz[dz] * accy;
+    }
+    if ((dm >> c) & 1) acc = 0.0f;
+    const size_t o = c * Nc + qc;
+    rc[o] = acc;
+    if (start) {
+      const float dv = cr0 * invD[o] * acc;
+      dc[o] = dv;
+      xc[o] = dv;
+    } else {
+      xc[o] = 0.0f;
+    }
+  }
+}
+
+// prolongation of the coarse correction: c = P e (zero on Dirichlet DOFs),
+// x_f += c, c kept as the stencil vector of the first post-smoothing step
+__global__ __launch_bounds__(MXB * MYB) void mix_prolong_kernel(const LevelGeom Lf, const LevelGeom Lc,
+                                                                float* __restrict__ xf, float* __restrict__ cf,
+                                                                const float* __restrict__ e) {
+  const int x = blockIdx.x * MXB + threadIdx.x;
+  const int y = blockIdx.y * MYB + threadIdx.y, z = blockIdx.z;
+  if (x >= Lf.nx || y >= Lf.ny) return;
+  const size_t Nc = Nof(Lc), Nf = Nof(Lf);
+  const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
+  const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
+  const float w = ((x & 1) ? 0.5f : 1.0f) * ((y & 1) ? 0.5f : 1.0f) * ((z & 1) ? 0.5f : 1.0f);
+  const size_t qf = nidx(Lf, x, y, z);
+  const int dm = dir_mask(Lf, x, y, z);
+  const size_t q00 = nidx(Lc, 0, j0, k0), q10 = nidx(Lc, 0, j1, k0), q01 = nidx(Lc, 0, j0, k1),
+               q11 = nidx(Lc, 0, j1, k1);
+  for (int c = 0; c < 3; c++) {
+    const float* ec = e + c * Nc;
+    const float a00 = ec[q00 + i0] + ((x & 1) ? ec[q00 + i1] : 0.0f);
+    const float a10 = (y & 1) ? ec[q10 + i0] + ((x & 1) ? ec[q10 + i1] : 0.0f) : 0.0f;
+    const float a01 = (z & 1) ? ec[q01 + i0] + ((x & 1) ? ec[q01 + i1] : 0.0f) : 0.0f;
+    const float a11 = ((y & 1) && (z & 1)) ? ec[q11 + i0] + ((x & 1) ? ec[q11 + i1] : 0.0f) : 0.0f;
+    float corr = w * (a00 + a10 + a01 + a11);
+    if ((dm >> c) & 1) corr = 0.0f;
+    const size_t o = c * Nf + qf;
+    xf[o] += corr;
+    cf[o] = corr;
+  }
+}
+
+__global__ void to_f32_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    b[q] = (float)a[q];
+}
+
+// deterministic start vector of the power iteration, zero on Dirichlet DOFs
+__global__ void power_init_kernel(const LevelGeom L, float* __restrict__ v) {
+  const unsigned N = (unsigned)Nof(L);
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+    const int dm = node_mask(L, n);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      unsigned h = (n * 3u + (unsigned)c) * 2654435761u;
+      h ^= h >> 15;
+      h *= 2246822519u;
+      h ^= h >> 13;
+      const float val = 0.5f + (float)(h & 0xffffu) * (1.0f / 65536.0f);
+      v[(size_t)c * N + n] = ((dm >> c) & 1) ? 0.0f : val;
+    }
+  }
+}
+
+void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, float* r32, float* d, float* x,
+                      const float* invD, float cr0) {
+  const size_t N = Nof(L);
+  hipLaunchKernelGGL(mix_start_kernel, dim3(mx_blocks(N, 4096)), dim3(MX_TPB), 0, st, (unsigned)N, r64, r32, d, x,
+                     invD, cr0);
+}
+void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, const double* p, const double* q,
+                  float* r32, float* d, float* x, const float* invD, float cr0, const Reduce& R) {
+  hipLaunchKernelGGL(fcg_b_kernel, dim3(passb_blocks(Nof(L))), dim3(MX_TPB), 0, st, L, u, r, p, q, r32, d, x, invD,
+                     cr0, R);
+}
+void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,
+                         float* dc, float* xc, const float* invD, float cr0, int start) {
+  const dim3 g((Lc.nx + MXB - 1) / MXB, (Lc.ny + MYB - 1) / MYB, Lc.nz);
+  hipLaunchKernelGGL(mix_restrict_kernel, g, dim3(MXB, MYB), 0, st, Lc, Lf, rf, rc, dc, xc, invD, cr0, start);
+}
+void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, float* xf, float* cf,
+                        const float* ec) {
+  const dim3 g((Lf.nx + MXB - 1) / MXB, (Lf.ny + MYB - 1) / MYB, Lf.nz);
+  hipLaunchKernelGGL(mix_prolong_kernel, g, dim3(MXB, MYB), 0, st, Lf, Lc, xf, cf, ec);
+}
+void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b) {
+  hipLaunchKernelGGL(to_f32_kernel, dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);
+}
+void launch_power_init(hipStream_t st, const LevelGeom& L, float* v) {
+  hipLaunchKernelGGL(power_init_kernel, dim3(mx_blocks(Nof(L), 4096)), dim3(256), 0, st, L, v);
+}
+
+}  // namespace hpcr

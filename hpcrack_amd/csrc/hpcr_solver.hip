@@ -8,6 +8,7 @@
 // residual norm for the stopping test.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -40,6 +41,15 @@ struct Lev {
   std::vector<int> kplanes;
 };
 
+// FP32 state of one level of the mixed-precision preconditioner
+struct MixLev {
+  float *x = nullptr, *r = nullptr, *dA = nullptr, *dB = nullptr, *invD = nullptr;
+  float *K = nullptr, *G = nullptr, *d = nullptr, *Ke = nullptr, *Ge = nullptr;
+  double lam = 0.0;
+  float cr0 = 0.0f;
+  std::vector<float> cd, cr;  // Chebyshev steps 1..m-1
+};
+
 struct TimingRec {
   int kernel, level;
   hipEvent_t a, b;
@@ -67,6 +77,9 @@ struct hpcr_ctx {
   long long smoother_iters = 0;
   bool diag_u_valid = false, diag_d_valid = false;
   int last_rr_level = -1;  // level whose residual norm scal[S_RR] holds
+  // mixed-precision solve (allocated on first use)
+  std::vector<MixLev> mix;
+  double* q2 = nullptr;  // second FP64 q buffer of the flexible CG
 };
 
 namespace {
@@ -307,6 +320,210 @@ int u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {
     st->cycles = cycles;
     st->r_norm_final = r_norm;
     st->smoother_iters = c->smoother_iters;
+  }
+  return HPCR_OK;
+}
+
+// ---- mixed-precision form of u_entire ---------------------------------------
+int mix_alloc(hpcr_ctx* c) {
+  if (!c->mix.empty()) return HPCR_OK;
+  std::vector<MixLev> mix(c->L + 1);
+  int rc;
+  for (int l = 0; l <= c->L; l++) {
+    const size_t N = c->lev[l].N;
+    MixLev& m = mix[l];
+    if ((rc = alloc_n(c, &m.x, 3 * N)) || (rc = alloc_n(c, &m.r, 3 * N))) return rc;
+    if (l == 0) continue;
+    if ((rc = alloc_n(c, &m.dA, 3 * N)) || (rc = alloc_n(c, &m.dB, 3 * N)) || (rc = alloc_n(c, &m.invD, 3 * N)))
+      return rc;
+    if (l == c->L) {
+      if ((rc = alloc_n(c, &m.K, N)) || (rc = alloc_n(c, &m.G, N)) || (rc = alloc_n(c, &m.d, N))) return rc;
+    } else {
+      if ((rc = alloc_n(c, &m.Ke, N)) || (rc = alloc_n(c, &m.Ge, N))) return rc;
+    }
+  }
+  if ((rc = alloc_n(c, &c->q2, 3 * c->lev[c->L].N))) return rc;
+  c->mix = std::move(mix);
+  return HPCR_OK;
+}
+
+MixCoef mixcoef(hpcr_ctx* c, int l) {
+  const MixLev& m = c->mix[l];
+  MixCoef k{};
+  k.fine = (l == c->L) ? 1 : 0;
+  k.K = m.K;
+  k.G = m.G;
+  k.d = m.d;
+  k.flag = c->lev[l].flag;
+  k.Ke = m.Ke;
+  k.Ge = m.Ge;
+  k.small_k = (float)c->par.small_k;
+  return k;
+}
+
+// FP32 coefficients and diagonals, power iteration and Chebyshev coefficients
+int mix_setup(hpcr_ctx* c, const hpcr_mixed_params& mp) {
+  int rc;
+  const int m = mp.cheb_degree;
+  for (int l = 1; l <= c->L; l++) {
+    Lev& v = c->lev[l];
+    MixLev& M = c->mix[l];
+    launch_to_f32(c->stream, 3 * v.N, v.invDu, M.invD);
+    if (l == c->L) {
+      launch_to_f32(c->stream, v.N, v.K, M.K);
+      launch_to_f32(c->stream, v.N, v.G, M.G);
+      launch_to_f32(c->stream, v.N, v.d, M.d);
+    } else {
+      launch_to_f32(c->stream, v.N, v.Ke, M.Ke);
+      launch_to_f32(c->stream, v.N, v.Ge, M.Ge);
+    }
+    // power iteration on D^-1 A: Rayleigh quotient of the last iterate
+    float* a = M.dA;
+    float* b = M.dB;
+    launch_power_init(c->stream, v.g, a);
+    for (int it = 0; it < mp.power_iters; it++) {
+      launch_mix_power(c->stream, v.g, mixcoef(c, l), a, b, M.invD, red(c));
+      std::swap(a, b);
+    }
+    if ((rc = check_launch(c, "mixed set-up"))) return rc;
+    if ((rc = read_scal(c))) return rc;
+    M.lam = c->host_scal[S_LAM];
+    if (!(M.lam > 0.0) || !std::isfinite(M.lam))
+      return set_err(c, HPCR_ERR_STATE, "power iteration gave no positive eigenvalue estimate");
+    const double hi = mp.cheb_upper * M.lam, lo = hi / mp.cheb_ratio;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    M.cr0 = (float)(1.0 / theta);
+    M.cd.assign(m, 0.0f);
+    M.cr.assign(m, 0.0f);
+    double rho_prev = 1.0 / sigma;
+    for (int j = 1; j < m; j++) {
+      const double rho = 1.0 / (2.0 * sigma - rho_prev);
+      M.cd[j] = (float)(rho * rho_prev);
+      M.cr[j] = (float)(2.0 * rho / delta);
+      rho_prev = rho;
+    }
+  }
+  return HPCR_OK;
+}
+
+// one FP32 V-cycle on the residual held in mix[L].r (smoother started);
+// the result is mix[L].x.  With qprev, x.qprev goes to scal[S_ZQ].
+int mix_vcycle(hpcr_ctx* c, int m, int nu0, const double* qprev) {
+  const int L = c->L;
+  for (int l = L; l >= 1; l--) {
+    MixLev& M = c->mix[l];
+    const LevelGeom& g = c->lev[l].g;
+    const MixCoef C = mixcoef(c, l);
+    float* sv = M.dA;
+    float* ov = M.dB;
+    for (int j = 1; j < m; j++) {
+      launch_mix_cheb(c->stream, g, C, sv, ov, M.r, M.x, M.invD, M.cd[j], M.cr[j], nullptr, red(c));
+      std::swap(sv, ov);
+    }
+    launch_mix_res(c->stream, g, C, sv, M.r);
+    MixLev& K = c->mix[l - 1];
+    if (l - 1 >= 1)
+      launch_mix_restrict(c->stream, c->lev[l - 1].g, g, M.r, K.r, K.dA, K.x, K.invD, K.cr0, 1);
+    else
+      launch_mix_restrict(c->stream, c->lev[0].g, g, M.r, K.r, nullptr, K.x, nullptr, 0.0f, 0);
+  }
+  {
+    Lev& v0 = c->lev[0];
+    if (small_level(v0.g)) {
+      launch_small_pcg_u_f32(c->stream, v0.g, v0.Ke, v0.Ge, c->mix[0].x, c->mix[0].r, v0.invDu, nu0,
+                             c->scal + S_SCRATCH);
+    } else {
+      return set_err(c, HPCR_ERR_ARG, "mixed precision needs a coarsest level of at most 512 nodes and 256 elements");
+    }
+  }
+  for (int l = 1; l <= L; l++) {
+    MixLev& M = c->mix[l];
+    const LevelGeom& g = c->lev[l].g;
+    const MixCoef C = mixcoef(c, l);
+    launch_mix_prolong(c->stream, g, c->lev[l - 1].g, M.x, M.dA, c->mix[l - 1].x);
+    float* sv = M.dA;
+    float* ov = M.dB;
+    launch_mix_cheb(c->stream, g, C, sv, ov, M.r, M.x, M.invD, 0.0f, M.cr0, (l == L && m == 1) ? qprev : nullptr,
+                    red(c));
+    std::swap(sv, ov);
+    for (int j = 1; j < m; j++) {
+      launch_mix_cheb(c->stream, g, C, sv, ov, M.r, M.x, M.invD, M.cd[j], M.cr[j],
+                      (l == L && j == m - 1) ? qprev : nullptr, red(c));
+      std::swap(sv, ov);
+    }
+  }
+  return check_launch(c, "mixed V-cycle");
+}
+
+int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mixed_params& mp, hpcr_stats* st) {
+  int rc;
+  const int L = c->L;
+  Lev& F = c->lev[L];
+  if (L < 1) return set_err(c, HPCR_ERR_ARG, "mixed precision needs at least two levels");
+  if (mp.cheb_degree < 1 || mp.cheb_degree > 8 || mp.power_iters < 1 || !(mp.cheb_ratio > 1.0) ||
+      !(mp.cheb_upper > 0.0) || mp.max_iter < 0)
+    return set_err(c, HPCR_ERR_ARG, "bad mixed-precision parameters");
+  if (!small_level(c->lev[0].g))
+    return set_err(c, HPCR_ERR_ARG, "mixed precision needs a coarsest level of at most 512 nodes and 256 elements");
+  if ((rc = mix_alloc(c))) return rc;
+  c->smoother_iters = 0;
+  if ((rc = mech_setup(c))) return rc;
+  if ((rc = mech_diag_all(c))) return rc;
+  if ((rc = mix_setup(c, mp))) return rc;
+  MixLev& MF = c->mix[L];
+  const int t0 = (t == 0) ? 1 : 0;
+  // residual from u with its boundary values; at t == 0 also f := A u on
+  // Dirichlet DOFs and f_norm = |f|^2, as the finest smoother does
+  auto true_residual = [&](double* rr) -> int {
+    launch_dirichlet_bd(c->stream, F.g, F.u, Ut, F.kplanes.data(), (int)F.kplanes.size());
+    launch_mech_resid(c->stream, F.g, mcoef(c, L), F.u, F.fu, F.ru, F.zu, F.invDu, t0, red(c));
+    int e = check_launch(c, "mixed residual");
+    if (e) return e;
+    if ((e = read_scal(c))) return e;
+    *rr = c->host_scal[S_RR];
+    if (t0) *f_norm = c->host_scal[S_FF];
+    launch_mix_start(c->stream, F.g, F.ru, MF.r, MF.dA, MF.x, MF.invD, MF.cr0);
+    return check_launch(c, "mixed start");
+  };
+  double rr = 0.0;
+  if ((rc = true_residual(&rr))) return rc;
+  if (st) st->r_norm_initial = rr;
+  double rel = (*f_norm != 0.0) ? rr / *f_norm : 1.0;
+  double* pb[2] = {F.p, F.p2};
+  double* qb[2] = {F.w, c->q2};
+  int it = 0;
+  const int m = mp.cheb_degree;
+  while (it < mp.max_iter && (rel > mp.tol || (t0 && it == 0))) {
+    if ((rc = mix_vcycle(c, m, c->par.mg_u.nu0, it > 0 ? qb[it & 1] : nullptr))) return rc;
+    launch_fcg_a(c->stream, F.g, mcoef(c, L), MF.x, pb[it & 1], pb[(it + 1) & 1], qb[(it + 1) & 1], F.ru, it == 0,
+                 red(c));
+    launch_fcg_b(c->stream, F.g, F.u, F.ru, pb[(it + 1) & 1], qb[(it + 1) & 1], MF.r, MF.dA, MF.x, MF.invD, MF.cr0,
+                 red(c));
+    if ((rc = check_launch(c, "flexible CG"))) return rc;
+    if ((rc = read_scal(c))) return rc;
+    rr = c->host_scal[S_RR];
+    it++;
+    rel = rr / *f_norm;
+    if (!std::isfinite(rel)) return set_err(c, HPCR_ERR_STATE, "mixed-precision iteration diverged");
+    const bool replace = (mp.replace_every > 0 && it % mp.replace_every == 0) || rel <= mp.tol;
+    if (replace) {
+      if ((rc = true_residual(&rr))) return rc;
+      rel = rr / *f_norm;
+    }
+    if (st && it <= HPCR_HIST_MAX) st->history[it - 1] = rr;
+  }
+  // externe_force_nodal (mechanical_right_hnd.cpp:18-38)
+  launch_dirichlet_bd(c->stream, F.g, F.u, Ut, F.kplanes.data(), (int)F.kplanes.size());
+  launch_mech_apply(c->stream, F.g, mcoef(c, L), F.u, F.w);
+  launch_dirichlet_copy(c->stream, F.g, F.fu, F.w, F.kplanes.data(), (int)F.kplanes.size());
+  if ((rc = check_launch(c, "externe_force_nodal"))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->diag_u_valid = false;
+  c->last_rr_level = -1;
+  if (st) {
+    st->cycles = it;
+    st->r_norm_final = rr;
+    st->smoother_iters = 0;
   }
   return HPCR_OK;
 }
@@ -716,6 +933,14 @@ int hpcr_u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st)
   (void)hipSetDevice(c->device);
   if (st) memset(st, 0, sizeof(*st));
   return u_entire(c, Ut, t, f_norm, st);
+}
+
+int hpcr_u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mixed_params* mp,
+                        hpcr_stats* st) {
+  if (!c || !f_norm || !mp) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (st) memset(st, 0, sizeof(*st));
+  return u_entire_mixed(c, Ut, t, f_norm, *mp, st);
 }
 
 int hpcr_phase_field(hpcr_ctx* c, int t, double* fd_norm, hpcr_stats* st) {

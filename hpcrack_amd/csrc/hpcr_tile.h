@@ -1,0 +1,597 @@
+// Device building blocks shared by the FP64 kernels (hpcr_kernels.hip) and
+// the FP32 multigrid preconditioner (hpcr_mixed.hip): index helpers,
+// deterministic reductions, the sum-factorised hex8 element algebra
+// (templated on the arithmetic type), the Dirichlet mask and the z-marching
+// tile kernel framework.  Included by .hip sources only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hpcr_internal.h"
+
+namespace hpcr {
+
+// L_i(g): 1-D linear shape function of node i at Gauss point g (xi = -+1/sqrt3)
+#define GA 0.78867513459481288225  // (1 + 1/sqrt(3)) / 2
+#define GB 0.21132486540518711775  // (1 - 1/sqrt(3)) / 2
+
+__device__ __forceinline__ double Lw(int i, int g) { return (i == g) ? GA : GB; }
+
+__device__ __forceinline__ size_t nidx(const LevelGeom& L, int i, int j, int k) {
+  return (size_t)i + (size_t)L.nx * ((size_t)j + (size_t)L.ny * (size_t)k);
+}
+__host__ __device__ __forceinline__ size_t Nof(const LevelGeom& L) { return (size_t)L.nx * L.ny * L.nz; }
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+enum RedOp { RED_SUM = 0, RED_MIN = 1, RED_MAX = 2 };
+
+template <int OP>
+__device__ __forceinline__ double red2(double a, double b) {
+  if (OP == RED_SUM) return a + b;
+  if (OP == RED_MIN) return (b < a) ? b : a;
+  return (b > a) ? b : a;
+}
+
+template <int OP>
+__device__ __forceinline__ double wave_red(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = red2<OP>(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// block of blockDim.x (multiple of 64, <= 1024) threads -> result in thread 0
+template <int OP, int NV>
+__device__ __forceinline__ void block_red(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_red<OP>(v[k]);
+  if (lane == 0)
+    for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; k++) {
+      double a = sh[k * 16];
+      for (int w = 1; w < nw; w++) a = red2<OP>(a, sh[k * 16 + w]);
+      v[k] = a;
+    }
+}
+
+__device__ __forceinline__ unsigned lin_block() {
+  return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+__device__ __forceinline__ unsigned num_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
+
+// Block partials -> last block -> fin(result[NV]) run by thread 0 of the last block.
+template <int OP, int NV, class Fin>
+__device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R, double init, Fin fin) {
+  __shared__ double sh[NV * 16];
+  __shared__ int is_last;
+  block_red<OP, NV>(v, sh);
+  const unsigned nb = num_blocks();
+  if (threadIdx.x == 0) {
+    const unsigned b = lin_block();
+    for (int k = 0; k < NV; k++)
+      __hip_atomic_store(&R.partials[(size_t)b * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned t = atomicAdd(R.counter, 1u);
+    is_last = (t == nb - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double acc[NV];
+  for (int k = 0; k < NV; k++) acc[k] = init;
+  for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
+    for (int k = 0; k < NV; k++)
+      acc[k] = red2<OP>(acc[k], __hip_atomic_load(&R.partials[(size_t)i * NV + k], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT));
+  block_red<OP, NV>(acc, sh);
+  if (threadIdx.x == 0) {
+    fin(acc);
+    __hip_atomic_store(R.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element algebra (sum factorisation)
+// ---------------------------------------------------------------------------
+// trilinear interpolation of 8 nodal values v[iz][iy][ix] to Gauss points o[gz][gy][gx]
+template <class T>
+__device__ __forceinline__ void interp8(const T (&v)[2][2][2], T (&o)[2][2][2]) {
+  const T ga = T(GA), gb = T(GB);
+  T t[2][2][2], s[2][2][2];
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int iy = 0; iy < 2; iy++) {
+      t[iz][iy][0] = ga * v[iz][iy][0] + gb * v[iz][iy][1];
+      t[iz][iy][1] = gb * v[iz][iy][0] + ga * v[iz][iy][1];
+    }
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) {
+      s[iz][0][gx] = ga * t[iz][0][gx] + gb * t[iz][1][gx];
+      s[iz][1][gx] = gb * t[iz][0][gx] + ga * t[iz][1][gx];
+    }
+#pragma unroll
+  for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) {
+      o[0][gy][gx] = ga * s[0][gy][gx] + gb * s[1][gy][gx];
+      o[1][gy][gx] = gb * s[0][gy][gx] + ga * s[1][gy][gx];
+    }
+}
+// transpose of interp8: nodal f[iz][iy][ix] += sum_g L L L q[gz][gy][gx]
+template <class T>
+__device__ __forceinline__ void interp8_t_add(const T (&q)[2][2][2], T (&f)[2][2][2]) {
+  const T ga = T(GA), gb = T(GB);
+  T s[2][2][2], t[2][2][2];
+#pragma unroll
+  for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) {
+      s[0][gy][gx] = ga * q[0][gy][gx] + gb * q[1][gy][gx];
+      s[1][gy][gx] = gb * q[0][gy][gx] + ga * q[1][gy][gx];
+    }
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) {
+      t[iz][0][gx] = ga * s[iz][0][gx] + gb * s[iz][1][gx];
+      t[iz][1][gx] = gb * s[iz][0][gx] + ga * s[iz][1][gx];
+    }
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int iy = 0; iy < 2; iy++) {
+      f[iz][iy][0] += ga * t[iz][iy][0] + gb * t[iz][iy][1];
+      f[iz][iy][1] += gb * t[iz][iy][0] + ga * t[iz][iy][1];
+    }
+}
+// bilinear interpolation of 4 edge values e[a][b] (a: outer axis, b: inner
+// axis) to Gauss pairs o[ga][gb], with the inner stage scaled by s
+template <class T>
+__device__ __forceinline__ void interp4(const T (&e)[2][2], T s, T (&o)[2][2]) {
+  const T ga = T(GA), gb = T(GB);
+  T t0 = s * (ga * e[0][0] + gb * e[0][1]);
+  T t1 = s * (gb * e[0][0] + ga * e[0][1]);
+  T u0 = s * (ga * e[1][0] + gb * e[1][1]);
+  T u1 = s * (gb * e[1][0] + ga * e[1][1]);
+  o[0][0] = ga * t0 + gb * u0;
+  o[0][1] = ga * t1 + gb * u1;
+  o[1][0] = gb * t0 + ga * u0;
+  o[1][1] = gb * t1 + ga * u1;
+}
+// transpose of interp4
+template <class T>
+__device__ __forceinline__ void interp4_t(const T (&S)[2][2], T s, T (&R)[2][2]) {
+  const T ga = T(GA), gb = T(GB);
+  T q00 = ga * S[0][0] + gb * S[1][0];  // a=0, gb=0
+  T q01 = ga * S[0][1] + gb * S[1][1];
+  T q10 = gb * S[0][0] + ga * S[1][0];  // a=1
+  T q11 = gb * S[0][1] + ga * S[1][1];
+  R[0][0] = s * (ga * q00 + gb * q01);
+  R[0][1] = s * (gb * q00 + ga * q01);
+  R[1][0] = s * (ga * q10 + gb * q11);
+  R[1][1] = s * (gb * q10 + ga * q11);
+}
+
+// Gradient of a nodal scalar v[iz][iy][ix] at the Gauss points:
+// gxv[gz][gy] (d/dx), gyv[gz][gx] (d/dy), gzv[gy][gx] (d/dz)
+template <class T>
+__device__ __forceinline__ void grad8(const T (&v)[2][2][2], T sx, T sy, T sz, T (&gxv)[2][2], T (&gyv)[2][2],
+                                      T (&gzv)[2][2]) {
+  T ex[2][2], ey[2][2], ez[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      ex[a][b] = v[a][b][1] - v[a][b][0];  // [iz][iy]
+      ey[a][b] = v[a][1][b] - v[a][0][b];  // [iz][ix]
+      ez[a][b] = v[1][a][b] - v[0][a][b];  // [iy][ix]
+    }
+  interp4(ex, sx, gxv);
+  interp4(ey, sy, gyv);
+  interp4(ez, sz, gzv);
+}
+// Transposed gradient: f[iz][iy][ix] += sum_g dN/dx_j * S_j
+template <class T>
+__device__ __forceinline__ void grad8_t_add(const T (&Sx)[2][2], const T (&Sy)[2][2], const T (&Sz)[2][2], T sx,
+                                            T sy, T sz, T (&f)[2][2][2]) {
+  T R[2][2];
+  interp4_t(Sx, sx, R);  // R[iz][iy]
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      f[a][b][1] += R[a][b];
+      f[a][b][0] -= R[a][b];
+    }
+  interp4_t(Sy, sy, R);  // R[iz][ix]
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      f[a][1][b] += R[a][b];
+      f[a][0][b] -= R[a][b];
+    }
+  interp4_t(Sz, sz, R);  // R[iy][ix]
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      f[1][a][b] += R[a][b];
+      f[0][a][b] -= R[a][b];
+    }
+}
+
+// gauss_point_MP (material_gauss_point.cpp:18-41): zero if any nodal value is 0
+template <class T>
+__device__ __forceinline__ void gp_mp(const T (&v)[2][2][2], T (&o)[2][2][2]) {
+  bool z = false;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) z |= (v[a][b][c] == T(0));
+  interp8(v, o);
+  if (z) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) o[a][b][c] = T(0);
+  }
+}
+// (1-k)(1-d)^2 + k with d clamped to [0,1] (material_gauss_point.cpp:50-56)
+template <class T>
+__device__ __forceinline__ T degr(T dg, T k) {
+  dg = dg < T(0) ? T(0) : (dg > T(1) ? T(1) : dg);
+  return (T(1) - k) * (T(1) - dg) * (T(1) - dg) + k;
+}
+
+// Gauss-point Lame coefficients (lambda = K_d - 2/3 G_d, mu = G_d) of a fine
+// element (MP_all_GAUSS_Point, mechanical_pcg.cpp:26-61)
+template <class T>
+__device__ __forceinline__ void fine_lame(const T (&Kn)[2][2][2], const T (&Gn)[2][2][2], const T (&dn)[2][2][2],
+                                          int flag, T small_k, T (&lam)[2][2][2], T (&mu)[2][2][2]) {
+  T Kg[2][2][2], Gg[2][2][2], dg[2][2][2];
+  gp_mp(Kn, Kg);
+  gp_mp(Gn, Gg);
+  gp_mp(dn, dg);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const T g = degr(dg[a][b][c], small_k);
+        const T Kd = (flag == 0) ? g * Kg[a][b][c] : Kg[a][b][c];
+        const T Gd = (flag == 2) ? Gg[a][b][c] : g * Gg[a][b][c];
+        lam[a][b][c] = Kd - (T(2) / T(3)) * Gd;
+        mu[a][b][c] = Gd;
+      }
+}
+
+// y_e = K_e u_e for one element: u[c][iz][iy][ix] -> f[c][iz][iy][ix]
+// (Internal_Force_element, mechanical_pcg.cpp:114-172)
+template <bool CONST_COEF, class T>
+__device__ __forceinline__ void mech_element(const T (&u)[3][2][2][2], const T (&lam)[2][2][2],
+                                             const T (&mu)[2][2][2], T lam0, T mu0, T sx, T sy, T sz, T tx, T ty,
+                                             T tz, T (&f)[3][2][2][2]) {
+  T dx[3][2][2], dy[3][2][2], dz[3][2][2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) grad8(u[c], sx, sy, sz, dx[c], dy[c], dz[c]);
+  T Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) Sx[c][a][b] = Sy[c][a][b] = Sz[c][a][b] = T(0);
+#pragma unroll
+  for (int gz = 0; gz < 2; gz++)
+#pragma unroll
+    for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+      for (int gx = 0; gx < 2; gx++) {
+        const T exx = dx[0][gz][gy], eyy = dy[1][gz][gx], ezz = dz[2][gy][gx];
+        const T gxy = dx[1][gz][gy] + dy[0][gz][gx];
+        const T gxz = dx[2][gz][gy] + dz[0][gy][gx];
+        const T gyz = dy[2][gz][gx] + dz[1][gy][gx];
+        const T tr = exx + eyy + ezz;
+        const T l = CONST_COEF ? lam0 : lam[gz][gy][gx];
+        const T m = CONST_COEF ? mu0 : mu[gz][gy][gx];
+        const T lt = l * tr, m2 = T(2) * m;
+        const T sxx = lt + m2 * exx, syy = lt + m2 * eyy, szz = lt + m2 * ezz;
+        const T sxy = m * gxy, sxz = m * gxz, syz = m * gyz;
+        Sx[0][gz][gy] += sxx; Sx[1][gz][gy] += sxy; Sx[2][gz][gy] += sxz;
+        Sy[0][gz][gx] += sxy; Sy[1][gz][gx] += syy; Sy[2][gz][gx] += syz;
+        Sz[0][gy][gx] += sxz; Sz[1][gy][gx] += syz; Sz[2][gy][gx] += szz;
+      }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = T(0);
+    grad8_t_add(Sx[c], Sy[c], Sz[c], tx, ty, tz, f[c]);
+  }
+}
+
+// Jacobi diagonal of one element (PreM_u_element, mechanical_pcg.cpp:63-100)
+__device__ __forceinline__ void mech_element_diag(const double (&lam)[2][2][2], const double (&mu)[2][2][2],
+                                                  double ihx2, double ihy2, double ihz2, double wg,
+                                                  double (&f)[3][2][2][2]) {
+#pragma unroll
+  for (int ka = 0; ka < 2; ka++)
+#pragma unroll
+    for (int ja = 0; ja < 2; ja++)
+#pragma unroll
+      for (int ia = 0; ia < 2; ia++) {
+        double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int gz = 0; gz < 2; gz++)
+#pragma unroll
+          for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+            for (int gx = 0; gx < 2; gx++) {
+              const double lx = Lw(ia, gx), ly = Lw(ja, gy), lz = Lw(ka, gz);
+              const double X = ihx2 * ly * ly * lz * lz;
+              const double Y = ihy2 * lx * lx * lz * lz;
+              const double Z = ihz2 * lx * lx * ly * ly;
+              const double l = lam[gz][gy][gx], m = mu[gz][gy][gx];
+              const double p = l + 2.0 * m;
+              s0 += p * X + m * (Y + Z);
+              s1 += p * Y + m * (X + Z);
+              s2 += p * Z + m * (X + Y);
+            }
+        f[0][ka][ja][ia] = wg * s0;
+        f[1][ka][ja][ia] = wg * s1;
+        f[2][ka][ja][ia] = wg * s2;
+      }
+}
+
+// Phase-field element (Fld_element, phase_field_pcg.cpp:61-91)
+__device__ __forceinline__ void pf_coefs(const double (&gcn)[2][2][2], double He, double lc, double wg,
+                                         double (&m)[2][2][2], double (&k)[2][2][2]) {
+  double gg[2][2][2];
+  gp_mp(gcn, gg);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        m[a][b][c] = wg * (2.0 * He + gg[a][b][c] / lc);
+        k[a][b][c] = wg * gg[a][b][c] * lc;
+      }
+}
+__device__ __forceinline__ void pf_element(const double (&v)[2][2][2], const double (&m)[2][2][2],
+                                           const double (&k)[2][2][2], double sx, double sy, double sz,
+                                           double (&f)[2][2][2]) {
+  double dv[2][2][2];
+  interp8(v, dv);
+  double q[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        q[a][b][c] = m[a][b][c] * dv[a][b][c];
+        f[a][b][c] = 0.0;
+      }
+  interp8_t_add(q, f);
+  double gxv[2][2], gyv[2][2], gzv[2][2];
+  grad8(v, sx, sy, sz, gxv, gyv, gzv);
+  double Sx[2][2], Sy[2][2], Sz[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      Sx[a][b] = gxv[a][b] * (k[a][b][0] + k[a][b][1]);  // [gz][gy], sum over gx
+      Sy[a][b] = gyv[a][b] * (k[a][0][b] + k[a][1][b]);  // [gz][gx], sum over gy
+      Sz[a][b] = gzv[a][b] * (k[0][a][b] + k[1][a][b]);  // [gy][gx], sum over gz
+    }
+  grad8_t_add(Sx, Sy, Sz, sx, sy, sz, f);
+}
+// Md_element (phase_field_pcg.cpp:20-47)
+__device__ __forceinline__ void pf_element_diag(const double (&m)[2][2][2], const double (&k)[2][2][2],
+                                                double ihx2, double ihy2, double ihz2, double (&f)[2][2][2]) {
+#pragma unroll
+  for (int ka = 0; ka < 2; ka++)
+#pragma unroll
+    for (int ja = 0; ja < 2; ja++)
+#pragma unroll
+      for (int ia = 0; ia < 2; ia++) {
+        double s = 0;
+#pragma unroll
+        for (int gz = 0; gz < 2; gz++)
+#pragma unroll
+          for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+            for (int gx = 0; gx < 2; gx++) {
+              const double lx = Lw(ia, gx), ly = Lw(ja, gy), lz = Lw(ka, gz);
+              const double N2 = lx * lx * ly * ly * lz * lz;
+              const double G2 = ihx2 * ly * ly * lz * lz + ihy2 * lx * lx * lz * lz + ihz2 * lx * lx * ly * ly;
+              s += m[gz][gy][gx] * N2 + k[gz][gy][gx] * G2;
+            }
+        f[ka][ja][ia] = s;
+      }
+}
+
+// Dirichlet DOF test of the reference's boundary conditions
+// (boundary_conditions.cpp:28-66): returns a mask, bit c set when component
+// c is prescribed; bit 3 marks u_z = Ut (top face) rather than 0.
+__device__ __forceinline__ int dir_mask(const LevelGeom& L, int i, int j, int k) {
+  const unsigned fz = L.fz[k];
+  if (fz == 0) return 0;
+  int m = 0;
+  const unsigned fx = L.fx[i], fy = L.fy[j];
+  if ((fz & 1) && (fx & 1) && (fy & 1)) m |= 1;
+  if ((fz & 1) && (fx & 2) && (fy & 2)) m |= 2;
+  if (fz & 1) m |= 4;
+  if (fz & 2) m |= 4 | 8;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernel framework
+// ---------------------------------------------------------------------------
+// lane i <- lane i-1 across the wavefront (DPP wave_shr:1); lane 0 gets 0.
+// Element (tx, ty) and its x-neighbour (tx-1, ty) are adjacent lanes because
+// TX divides the wavefront into whole rows; lanes with tx == 0 own no node.
+__device__ __forceinline__ float wave_shr1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double wave_shr1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so that tiles sharing halo rows share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb / 8, rem = nb % 8;
+  const int xcd = b % 8, j = b / 8;
+  return (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + j;
+}
+
+// optional occupancy target of the tile kernels (waves per SIMD), a tuning macro
+#ifdef HPCR_TILE_WAVES_PER_EU
+#define HPCR_TILE_ATTR __attribute__((amdgpu_waves_per_eu(HPCR_TILE_WAVES_PER_EU)))
+#else
+#define HPCR_TILE_ATTR
+#endif
+
+template <class Op>
+__global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op op) {
+  constexpr int NC = Op::NC;  // output components
+  constexpr int NV = Op::NV;  // values per node in LDS
+  constexpr int NR = Op::NR;  // raw values per node fetched from HBM
+  using T = typename Op::T;   // element arithmetic (double, or float in the preconditioner)
+  using TR = typename Op::TR; // type of the fetched raw values
+  __shared__ T sN[2][NV][PLANE];
+  __shared__ T sX[2 * NC][TPB];
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int ntx = (L.nx + OWNX - 1) / OWNX, nty = (L.ny + OWNY - 1) / OWNY;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx0 = (t % ntx) * OWNX, by0 = ((t / ntx) % nty) * OWNY;
+  const int z0 = (t / (ntx * nty)) * L.zc;
+  const int zend = min(z0 + L.zc, L.nz);
+  const int ex = bx0 - 1 + tx, ey = by0 - 1 + ty;
+  const bool owner = (tx >= 1) && (ty >= 1) && (ex < L.nx) && (ey < L.ny);
+  const bool exy = (ex >= 0) && (ey >= 0) && (ex <= L.nx - 2) && (ey <= L.ny - 2);
+  const typename Op::State st = op.state();
+
+  // node slots of this thread in a plane: tid and tid + TPB
+  const bool has2 = tid + TPB < PLANE;
+  const int q1 = tid, q2 = tid + TPB;
+  const int gx1 = bx0 - 1 + q1 % PX, gy1 = by0 - 1 + q1 / PX;
+  const int gx2 = bx0 - 1 + q2 % PX, gy2 = by0 - 1 + q2 / PX;
+  const bool in1 = gx1 >= 0 && gy1 >= 0 && gx1 < L.nx && gy1 < L.ny;
+  const bool in2 = has2 && gx2 >= 0 && gy2 >= 0 && gx2 < L.nx && gy2 < L.ny;
+  TR r1[NR], r2[NR];
+  auto fetch = [&](int pz) {
+    const bool zok = pz >= 0 && pz < L.nz;
+    op.fetch(L, gx1, gy1, pz, in1 && zok, st, r1);
+    op.fetch(L, gx2, gy2, pz, in2 && zok, st, r2);
+  };
+  auto commit = [&](int slot) {
+    T v[NV];
+    op.commit(r1, st, v);
+#pragma unroll
+    for (int k = 0; k < NV; k++) sN[slot][k][q1] = v[k];
+    if (has2) {
+      op.commit(r2, st, v);
+#pragma unroll
+      for (int k = 0; k < NV; k++) sN[slot][k][q2] = v[k];
+    }
+  };
+
+  typename Op::Acc acc;
+  T carry[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) carry[c] = T(0);
+
+  // planes z0-1 and z0 into LDS, plane z0+1 in flight
+  fetch(z0 - 1);
+  commit((z0 - 1) & 1);
+  fetch(z0);
+  commit(z0 & 1);
+  fetch(z0 + 1);
+  __syncthreads();
+  for (int kz = z0 - 1; kz < zend; ++kz) {
+    T f[NC][2][2][2];
+    if (exy && kz >= 0 && kz <= L.nz - 2) {
+      op.element(L, sN, kz & 1, tx, ty, ex, ey, kz, st, f);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = T(0);
+    }
+    // x-direction: node (tx, .) takes corner 0 of its own element and corner 1
+    // of element tx-1 (adjacent lane); g[c][iz][iy]
+    T g[NC][2][2];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+        for (int iy = 0; iy < 2; iy++) g[c][iz][iy] = f[c][iz][iy][0] + wave_shr1(f[c][iz][iy][1]);
+    // y-direction through LDS: the iy = 1 halves go to the row above
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int iz = 0; iz < 2; iz++) sX[c * 2 + iz][tid] = g[c][iz][1];
+    // values of this thread's own node of plane kz, read before the slot is reused
+    T own[NC];
+    if constexpr (Op::NEEDS_OWN) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) own[c] = sN[kz & 1][c][tx + PX * ty];
+    }
+    __syncthreads();
+    // plane kz is no longer read: it receives plane kz+2; plane kz+3 goes in flight
+    commit(kz & 1);
+    fetch(kz + 3);
+    if (owner) {
+      T vb[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        vb[c] = carry[c] + g[c][0][0] + sX[c * 2 + 0][tid - TX];
+        carry[c] = g[c][1][0] + sX[c * 2 + 1][tid - TX];
+      }
+      if (kz >= z0) op.epilogue(L, own, ex, ey, kz, vb, st, acc);
+    }
+    __syncthreads();
+  }
+  op.finish(acc);
+}
+
+// the 8 corner values of element (tx, ty) from the two LDS planes
+template <class T, int NV>
+__device__ __forceinline__ void gather8(const T (&sN)[2][NV][PLANE], int s0, int tx, int ty, int k,
+                                        T (&v)[2][2][2]) {
+  const int q = tx + PX * ty;
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++) {
+    const int s = s0 ^ iz;
+    v[iz][0][0] = sN[s][k][q];
+    v[iz][0][1] = sN[s][k][q + 1];
+    v[iz][1][0] = sN[s][k][q + PX];
+    v[iz][1][1] = sN[s][k][q + PX + 1];
+  }
+}
+
+}  // namespace hpcr

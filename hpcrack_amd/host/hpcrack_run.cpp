@@ -14,6 +14,8 @@
 //   --tol-d X           squared relative and absolute residual of d (default 1e-12)
 //   --du X              fixed top-displacement increment instead of the
 //                       reference's Ue-based schedule (first load = X)
+//   --du-ue F           fixed increment F*Ue (first load F*Ue), Ue from the
+//                       unit-load solution as in the reference schedule
 //   --stop-force F      stop when the reaction falls below F (default 0.5)
 //   --out PATH          F_U file (default F_U.dat)
 //   --device N          HIP device (default 0)
@@ -57,7 +59,7 @@ int main(int argc, char** argv) {
   std::string cfg_path = "config/config.txt", csv_path = "data/Material_Property.csv", out_path = "F_U.dat";
   int steps = 5000, device = 0;
   bool strict = true;
-  double small_k = 1e-5, tol_u = 1e-6, tol_d = 1e-12, du = 0.0, stop_force = 0.5;
+  double small_k = 1e-5, tol_u = 1e-6, tol_d = 1e-12, du = 0.0, du_ue = 0.0, stop_force = 0.5;
   int write_history = -1, history_every = 0, resume = -1;
   std::string history_dir = "Time_step_data";
   for (int a = 1; a < argc; a++) {
@@ -76,6 +78,7 @@ int main(int argc, char** argv) {
     else if (s == "--tol-u") tol_u = std::atof(next("--tol-u"));
     else if (s == "--tol-d") tol_d = std::atof(next("--tol-d"));
     else if (s == "--du") du = std::atof(next("--du"));
+    else if (s == "--du-ue") du_ue = std::atof(next("--du-ue"));
     else if (s == "--stop-force") stop_force = std::atof(next("--stop-force"));
     else if (s == "--out") out_path = next("--out");
     else if (s == "--device") device = std::atoi(next("--device"));
@@ -189,8 +192,8 @@ int main(int argc, char** argv) {
         delta_U = du;
       } else {
         if ((rc = hpcr_U_e(ctx, &Ue))) return fail("U_e", rc, ctx);
-        Ut = Ue;
-        delta_U = 1.5e-3 * Ue;
+        Ut = du_ue > 0.0 ? du_ue * Ue : Ue;
+        delta_U = du_ue > 0.0 ? du_ue * Ue : 1.5e-3 * Ue;
         std::cout << "Ue=" << Ue << std::endl;
       }
       u_prop = Ut / U_unit;

@@ -50,6 +50,7 @@ class Solver:
         self._h = h
         self.maxlevel = maxlevel
         self.small_k, self.lc = small_k, lc
+        self.tol_u = tol_u
         self.last_stats = None
 
     def close(self):
@@ -183,6 +184,19 @@ class Solver:
         fn = ctypes.c_double(f_norm)
         st = capi.Stats()
         self._chk(self._L.hpcr_u_entire(self._h, Ut, t, ctypes.byref(fn), ctypes.byref(st)))
+        self.last_stats = st
+        return fn.value, st.cycles
+
+    def u_entire_mixed(self, Ut, t, f_norm, tol=None, max_iter=100, cheb_degree=2, power_iters=10,
+                       cheb_ratio=10.0, cheb_upper=1.1, replace_every=10):
+        """Mixed-precision form of u_entire (FP64 flexible CG, FP32 V-cycle);
+        returns (f_norm, outer iterations).  tol defaults to the context's
+        tol_u (squared relative residual)."""
+        fn = ctypes.c_double(f_norm)
+        st = capi.Stats()
+        mp = capi.MixedParams(max_iter, self.tol_u if tol is None else tol, cheb_degree, power_iters, cheb_ratio,
+                              cheb_upper, replace_every)
+        self._chk(self._L.hpcr_u_entire_mixed(self._h, Ut, t, ctypes.byref(fn), ctypes.byref(mp), ctypes.byref(st)))
         self.last_stats = st
         return fn.value, st.cycles
 

@@ -40,6 +40,11 @@ class Workload:
     mg_u: tuple = (50, 4, 2, 1, 30)
     mg_d: tuple = (10, 2, 1, 1, 30)
     steps: int = 5
+    # preset phase field: planar penny crack at mid-height, d = exp(-dist/l)
+    penny_radius_frac: float = 0.0
+    penny_l: float = 2.0
+    # two-phase variant: inclusion Young's modulus (Pa) instead of the table's
+    incl_E: float = 0.0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -64,7 +69,26 @@ class Workload:
                                pore_fraction=self.pore_fraction, pore_r=self.pore_r, pore_label=2)
 
     def table(self):
-        return hio.material_table(GRAY, E_PA, NU, GC)
+        E = list(E_PA)
+        if self.incl_E > 0:
+            E[1] = self.incl_E
+        return hio.material_table(GRAY, E, NU, GC)
+
+    def damage(self):
+        """Preset d (nz, ny, nx) of a planar penny crack {z = z_mid, rho <=
+        R}, R = penny_radius_frac * (n - 1) voxels: d = exp(-dist / l) with
+        dist the distance to the disk; None without a crack."""
+        if self.penny_radius_frac <= 0:
+            return None
+        nx, ny, nz = self.dims
+        R = self.penny_radius_frac * (self.n - 1)
+        z = np.arange(nz, dtype=np.float64) - 0.5 * (nz - 1)
+        y = np.arange(ny, dtype=np.float64) - 0.5 * (ny - 1)
+        x = np.arange(nx, dtype=np.float64) - 0.5 * (nx - 1)
+        rho = np.sqrt(y[:, None] ** 2 + x[None, :] ** 2)
+        out = np.maximum(rho - R, 0.0)
+        dist = np.sqrt(out[None, :, :] ** 2 + z[:, None, None] ** 2)
+        return np.exp(-dist / self.penny_l)
 
     def materials(self):
         img, frac = self.image()
@@ -83,7 +107,13 @@ class Workload:
 C1 = Workload("C1-129-two-phase", 129, 1001, n_incl=30, incl_r=(4.0, 12.0), steps=5)
 C2 = Workload("C2-257-three-phase", 257, 1002, n_incl=200, incl_r=(3.0, 10.0), pore_fraction=0.05,
               pore_r=(2.0, 5.0), steps=20)
-WORKLOADS = {"C1": C1, "C2": C2}
+# 513^3 two-phase, inclusion/matrix stiffness ratio 50 (3 GPa / 150 GPa),
+# inclusions to ~20 % volume (count chosen for that fraction at radii 4-12)
+C3 = Workload("C3-513-two-phase-ratio50", 513, 1003, n_incl=10000, incl_r=(4.0, 12.0), incl_E=150e9, steps=3)
+# the C2 image with a preset penny crack (radius 0.3 L at mid-height, l = 2)
+C4 = Workload("C4-257-penny-crack", 257, 1002, n_incl=200, incl_r=(3.0, 10.0), pore_fraction=0.05,
+              pore_r=(2.0, 5.0), steps=1, penny_radius_frac=0.3, penny_l=2.0)
+WORKLOADS = {"C1": C1, "C2": C2, "C3": C3, "C4": C4}
 
 
 def scaled(w, n, name=None):
@@ -92,18 +122,41 @@ def scaled(w, n, name=None):
     return Workload(name or f"{w.name}@{n}", n, w.seed, n_incl=max(1, int(round(w.n_incl * f))),
                     incl_r=w.incl_r, pore_fraction=w.pore_fraction, pore_r=w.pore_r, element_nb=w.element_nb,
                     voxel_size=w.voxel_size, lc_voxels=w.lc_voxels, small_k=w.small_k, tol_u=w.tol_u,
-                    tol_d_rel=w.tol_d_rel, tol_d_abs=w.tol_d_abs, mg_u=w.mg_u, mg_d=w.mg_d, steps=w.steps)
+                    tol_d_rel=w.tol_d_rel, tol_d_abs=w.tol_d_abs, mg_u=w.mg_u, mg_d=w.mg_d, steps=w.steps,
+                    penny_radius_frac=w.penny_radius_frac, penny_l=w.penny_l, incl_E=w.incl_E)
 
 
 def upload(solver, w, K, G, gc):
-    """Finest-level materials, then coarse_VR on the device."""
+    """Finest-level materials, then coarse_VR on the device; the preset
+    phase field if the workload has one."""
     L = w.maxlevel
-    from .capi import GC as F_GC, G as F_G, K as F_K
+    from .capi import D as F_D, GC as F_GC, G as F_G, K as F_K
     solver.set(F_K, L, K)
     solver.set(F_G, L, G)
     solver.set(F_GC, L, gc)
     for f in (F_K, F_G, F_GC):
         solver.coarse_vr(f)
+    d = w.damage()
+    if d is not None:
+        solver.set(F_D, L, d)
+
+
+def rotation_mode(w, level=None):
+    """Flattened (3 x nodes) zero-energy mode of the reference's boundary
+    conditions on a level: the rotation about the vertical axis through
+    (x_b, y_a), which satisfies both pins and the u_z conditions.  Two
+    solvers may leave different amounts of it in u."""
+    L = w.maxlevel if level is None else level
+    dims = tuple(e * 2 ** L + 1 for e in w.element_nb)
+    dom = w.domain
+    hx = (dom[1] - dom[0]) / (dims[0] - 1)
+    hy = (dom[3] - dom[2]) / (dims[1] - 1)
+    x = dom[0] + np.arange(dims[0]) * hx
+    y = dom[2] + np.arange(dims[1]) * hy
+    shape = (dims[2], dims[1], dims[0])
+    mx = np.broadcast_to(-(y - dom[2])[None, :, None], shape)
+    my = np.broadcast_to((x - dom[1])[None, None, :], shape)
+    return np.concatenate([mx.ravel(), my.ravel(), np.zeros(int(np.prod(shape)))])
 
 
 def label_counts(img):

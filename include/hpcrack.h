@@ -135,6 +135,29 @@ int hpcr_u_entire(hpcr_ctx* ctx, double Ut, int t, double* f_norm, hpcr_stats* s
 /* phase_field (src/phase_field.cpp:15-57) */
 int hpcr_phase_field(hpcr_ctx* ctx, int t, double* fd_norm, hpcr_stats* stats);
 
+/* Mixed-precision form of u_entire (same problem, boundary values, f_norm
+ * convention, stopping test and reactions as hpcr_u_entire): FP64 flexible
+ * CG on the finest level preconditioned by one FP32 multigrid V-cycle with
+ * Chebyshev-Jacobi smoothing (degree cheb_degree before and after the coarse
+ * correction, bounds [cheb_upper*lam/cheb_ratio, cheb_upper*lam] from
+ * power_iters power iterations per level) and an FP64 coarsest solve of
+ * nu0_u iterations.  Stops when sum(r^2)/f_norm <= tol with the residual
+ * recomputed from u, or after max_iter iterations; the residual is also
+ * recomputed every replace_every iterations (0: only at the end).  stats
+ * counts outer iterations in `cycles`.  Allocates its FP32 buffers (about
+ * 64 B per finest node plus 48 B FP64) on the first call. */
+typedef struct hpcr_mixed_params {
+  int max_iter;
+  double tol;
+  int cheb_degree;
+  int power_iters;
+  double cheb_ratio;
+  double cheb_upper;
+  int replace_every;
+} hpcr_mixed_params;
+int hpcr_u_entire_mixed(hpcr_ctx* ctx, double Ut, int t, double* f_norm, const hpcr_mixed_params* mp,
+                        hpcr_stats* stats);
+
 /* ---- between-solve updates --------------------------------------------- */
 /* strain_history (src/strain_history.cpp:162-210), finest level */
 int hpcr_strain_history(hpcr_ctx* ctx, double u_proportion);

@@ -43,7 +43,7 @@ def driver_args(tmp_path, steps, out="F_U.dat"):
             "--stop-force", "-1e300"]
 
 
-@pytest.mark.parametrize("schedule", ["fixed", "reference"])
+@pytest.mark.parametrize("schedule", ["fixed", "reference", "ue-multiple"])
 def test_driver_matches_oracle_staggered_loop(tmp_path, schedule):
     require_gpu()
     img = write_case(tmp_path)
@@ -51,6 +51,8 @@ def test_driver_matches_oracle_staggered_loop(tmp_path, schedule):
     args = driver_args(tmp_path, steps)
     if schedule == "fixed":
         args += ["--du", "0.05"]
+    elif schedule == "ue-multiple":
+        args += ["--du-ue", "2"]
     say(f"[gpu] driver {schedule}")
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
@@ -62,7 +64,7 @@ def test_driver_matches_oracle_staggered_loop(tmp_path, schedule):
     o.image_mp(img, [0, 1], [3e9, 70e9], [0.3, 0.2], [100.0, 1000.0], 1e-6)
     for f in (O.K, O.G, O.GC):
         o.coarse_vr_nodal(f)
-    ref = staggered(o, steps, 1e-6, 2.0, schedule=schedule, du=0.05)
+    ref = staggered(o, steps, 1e-6, 2.0, schedule=schedule, du=2.0 if schedule == "ue-multiple" else 0.05)
     for g, e in zip(got, ref):
         say(f"  T={g['T']:.0f} U {g['U']:.6e}/{e['Ut']:.6e} Fw {g['Fw']:.6e}/{e['Fw']:.6e}")
         assert g["U"] == pytest.approx(e["Ut"], rel=1e-9)
@@ -80,8 +82,8 @@ def test_driver_resume_reproduces_uninterrupted_run(tmp_path):
     write_case(tmp_path)
     hist = tmp_path / "hist"
     say("[gpu] driver history + resume")
-    # a fixed load increment large enough that H and d grow from step 0
-    r = subprocess.run(driver_args(tmp_path, 4, "full.dat") + ["--du", "0.05", "--write-history",
+    # fixed load increments of 2 Ue: H and d grow from step 0
+    r = subprocess.run(driver_args(tmp_path, 4, "full.dat") + ["--du-ue", "2", "--write-history",
                                                                 "--history-every", "1", "--history-dir", str(hist)],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
@@ -95,7 +97,7 @@ def test_driver_resume_reproduces_uninterrupted_run(tmp_path):
     assert st["t"] == 1 and st["f_norm"] > 0
     full = open(tmp_path / "full.dat").read().splitlines()
     (tmp_path / "resumed.dat").write_text("\n".join(full[:2]) + "\n")
-    r = subprocess.run(driver_args(tmp_path, 4, "resumed.dat") + ["--du", "0.05", "--resume", "1",
+    r = subprocess.run(driver_args(tmp_path, 4, "resumed.dat") + ["--du-ue", "2", "--resume", "1",
                                                                    "--history-dir", str(hist)],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]

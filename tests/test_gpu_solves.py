@@ -31,6 +31,11 @@ def staggered(obj, steps, voxel_size, lc, schedule="reference", du=None, log=Non
                 Ue = obj.U_e()
                 Ut = Ue
                 delta_U = 1.5e-3 * Ue
+            elif schedule == "ue-multiple":
+                # fixed increments of du * Ue: the strain history exceeds the
+                # threshold from step 0 and the phase field grows
+                Ue = obj.U_e()
+                Ut, delta_U = du * Ue, du * Ue
             else:
                 Ut, delta_U = du, du
             u_prop = Ut
@@ -91,15 +96,19 @@ def test_u_entire_and_phase_field(HA, ml, tol_u):
     assert rel_l2(s.get(O.D, L), o.field(O.D, L)) < 1e-9
 
 
-def test_staggered_steps_match_oracle(HA):
+@pytest.mark.parametrize("schedule,du", [("fixed", 0.05), ("ue-multiple", 2.0)], ids=["fixed", "2Ue"])
+def test_staggered_steps_match_oracle(HA, schedule, du):
     ml = 2
     dom = (0.0, 16.0, 0.0, 16.0, 0.0, 16.0)
-    say("[gpu] staggered 17^3, 6 steps")
+    say(f"[gpu] staggered 17^3, 6 steps, {schedule} {du}")
     kw = dict(lc=2.0, small_k=1e-6, tol_u=1e-16, mg_u=(50, 4, 2, 1, 30), mg_d=(10, 2, 1, 1, 30),
               tol_d_rel=1e-16, tol_d_abs=1e-30)
     o, s, _ = make_pair(HA, ml, (4, 4, 4), dom, seed=21, lab_incl=3, **kw)
-    ro = staggered(o, 6, 1e-6, 2.0, schedule="fixed", du=0.05)
-    rs = staggered(s, 6, 1e-6, 2.0, schedule="fixed", du=0.05)
+    ro = staggered(o, 6, 1e-6, 2.0, schedule=schedule, du=du)
+    rs = staggered(s, 6, 1e-6, 2.0, schedule=schedule, du=du)
+    if schedule == "ue-multiple":
+        # loads of 2 Ue .. 12 Ue: the phase field is active and grows
+        assert max(r["cyc_d"] for r in ro) > 0 and o.field(O.D, ml).max() > 0.1
     for a, b in zip(ro, rs):
         say(f"  t={a['t']} cyc_u {a['cyc_u']}/{b['cyc_u']} cyc_d {a['cyc_d']}/{b['cyc_d']} Fw {a['Fw']:.6e}/{b['Fw']:.6e}")
         assert abs(a["cyc_u"] - b["cyc_u"]) <= 1 and abs(a["cyc_d"] - b["cyc_d"]) <= 1
