@@ -81,6 +81,14 @@ struct MixCoef {
   float small_k;
 };
 
+// Material table in device memory (layout of hpcr_material_table)
+constexpr int MAT_MAX = 256;
+struct MatTable {
+  int n;
+  int gray[MAT_MAX];
+  double E[MAT_MAX], nu[MAT_MAX], gc[MAT_MAX];
+};
+
 // Phase-field coefficients of one level.
 struct PfCoef {
   const double* H;   // element-indexed
@@ -166,6 +174,10 @@ void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc
 void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b);
 void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, float* x,
                             const float* b, const double* invD, int NB, double* scal);
+
+// image_MP: gray levels -> K, G, gc; bad[0] := 1 for a level outside the table
+void launch_image_mp(hipStream_t s, size_t n, const unsigned char* img, const MatTable* tab, double vs, double* K,
+                     double* G, double* gc, int* bad);
 
 // Dirichlet_BD on planes with z flags; `kplanes` lists the flagged k indices.
 void launch_dirichlet_bd(hipStream_t s, const LevelGeom& L, double* u, double Ut, const int* kplanes_host,

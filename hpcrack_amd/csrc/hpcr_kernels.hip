@@ -914,6 +914,54 @@ void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, cons
 }
 
 // ---------------------------------------------------------------------------
+// image_MP on the device (image_input.cpp:120-142)
+// ---------------------------------------------------------------------------
+// gray level -> (K, G, gc) with the reference's integer-division
+// interpolation between table rows, the same FP64 operations in the same
+// order as the host restatement (no contraction), so results are identical
+__global__ __launch_bounds__(256) void image_mp_kernel(size_t n, const unsigned char* __restrict__ img,
+                                                       const MatTable* __restrict__ tab, double vs,
+                                                       double* __restrict__ K, double* __restrict__ G,
+                                                       double* __restrict__ gc, int* __restrict__ bad) {
+#pragma clang fp contract(off)
+  __shared__ int sg[MAT_MAX];
+  __shared__ double sE[MAT_MAX], snu[MAT_MAX], sgc[MAT_MAX];
+  const int nt = tab->n;
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+    sg[i] = tab->gray[i];
+    sE[i] = tab->E[i];
+    snu[i] = tab->nu[i];
+    sgc[i] = tab->gc[i];
+  }
+  __syncthreads();
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const int x = img[q];
+    int pos = -1;
+    // findPosition (csv_reader.cpp:74-82)
+    for (int i = 0; i < nt - 1; ++i)
+      if (x >= sg[i] && x <= sg[i + 1]) {
+        pos = i;
+        break;
+      }
+    if (pos < 0 || sg[pos + 1] == sg[pos]) {
+      bad[0] = 1;
+      continue;
+    }
+    const int fac = (x - sg[pos]) / (sg[pos + 1] - sg[pos]);
+    double E = sE[pos] + (double)fac * (sE[pos + 1] - sE[pos]);
+    const double nu = snu[pos] + (double)fac * (snu[pos + 1] - snu[pos]);
+    gc[q] = sgc[pos] + (double)fac * (sgc[pos + 1] - sgc[pos]);
+    E = vs * E;
+    K[q] = 1.0 / (3.0 * (1.0 - 2.0 * nu)) * E;
+    G[q] = 1.0 / (2.0 * (1.0 + nu)) * E;
+  }
+}
+void launch_image_mp(hipStream_t s, size_t n, const unsigned char* img, const MatTable* tab, double vs, double* K,
+                     double* G, double* gc, int* bad) {
+  hipLaunchKernelGGL(image_mp_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, img, tab, vs, K, G, gc, bad);
+}
+
+// ---------------------------------------------------------------------------
 // Transfers (multigrid_intergrid.cpp)
 // ---------------------------------------------------------------------------
 // solution_injection + Dirichlet_BD + copy to uold (multigrid_intergrid.cpp:25-121)

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/hpcrack.h"
+#include "../../include/hpcrack_io.h"
 #include "hpcr_internal.h"
 
 using namespace hpcr;
@@ -926,6 +927,48 @@ int hpcr_coarse_vr(hpcr_ctx* c, int f) {
   c->diag_u_valid = c->diag_d_valid = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c, "coarse_vr");
+}
+
+int hpcr_set_image(hpcr_ctx* c, const unsigned char* img, const hpcr_material_table* tab, double voxel_size) {
+  if (!c || !img || !tab || tab->n < 2 || tab->n > HPCR_MAX_PHASES) return HPCR_ERR_ARG;
+  static_assert(HPCR_MAX_PHASES == MAT_MAX, "material table size");
+  (void)hipSetDevice(c->device);
+  Lev& F = c->lev[c->L];
+  MatTable h{};
+  h.n = tab->n;
+  for (int i = 0; i < tab->n; i++) {
+    h.gray[i] = tab->gray[i];
+    h.E[i] = tab->E[i];
+    h.nu[i] = tab->nu[i];
+    h.gc[i] = tab->gc[i];
+  }
+  unsigned char* dimg = nullptr;
+  MatTable* dtab = nullptr;
+  int* dbad = nullptr;
+  HIPCHK(c, hipMalloc((void**)&dimg, F.N));
+  hipError_t e = hipMalloc((void**)&dtab, sizeof(MatTable) + 256);
+  if (e == hipSuccess) {
+    dbad = (int*)((char*)dtab + sizeof(MatTable));
+    e = hipMemcpyAsync(dimg, img, F.N, hipMemcpyHostToDevice, c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(dtab, &h, sizeof(MatTable), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(dbad, 0, sizeof(int), c->stream);
+  if (e == hipSuccess) {
+    launch_image_mp(c->stream, F.N, dimg, dtab, voxel_size, F.K, F.G, F.gc, dbad);
+    e = hipGetLastError();
+  }
+  int bad = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dimg);
+  if (dtab) (void)hipFree(dtab);
+  if (e != hipSuccess) return set_err(c, HPCR_ERR_DEVICE, std::string("set_image: ") + hipGetErrorString(e));
+  if (bad) return set_err(c, HPCR_ERR_ARG, "gray level outside the material table");
+  for (int f : {HPCR_K, HPCR_G, HPCR_GC}) {
+    int rc = hpcr_coarse_vr(c, f);
+    if (rc) return rc;
+  }
+  return HPCR_OK;
 }
 
 int hpcr_u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {

@@ -121,9 +121,6 @@ int main(int argc, char** argv) {
     std::cerr << "Could not open the file" << std::endl;
     return fail("reading the material table", rc);
   }
-  std::vector<double> K(N), G(N), gc(N);
-  if ((rc = hpcr_image_mp(&tab, img.data(), N, cfg.voxel_size, K.data(), G.data(), gc.data())))
-    return fail("mapping gray levels to materials", rc);
 
   hpcr_params par{};
   par.small_k = small_k;
@@ -136,11 +133,8 @@ int main(int argc, char** argv) {
   hpcr_ctx* ctx = nullptr;
   if ((rc = hpcr_ctx_create(&geo, &par, device, &ctx))) return fail("creating the device context", rc);
   const int L = geo.maxlevel;
-  if ((rc = hpcr_set_field(ctx, HPCR_K, L, 0, K.data())) || (rc = hpcr_set_field(ctx, HPCR_G, L, 0, G.data())) ||
-      (rc = hpcr_set_field(ctx, HPCR_GC, L, 0, gc.data())))
-    return fail("uploading materials", rc, ctx);
-  for (int f : {HPCR_K, HPCR_G, HPCR_GC})
-    if ((rc = hpcr_coarse_vr(ctx, f))) return fail("coarse_VR", rc, ctx);
+  // image_MP and coarse_VR on the device (input_total, src/image_input.cpp:256-282)
+  if ((rc = hpcr_set_image(ctx, img.data(), &tab, cfg.voxel_size))) return fail("mapping gray levels to materials", rc, ctx);
 
   // history output (outTdata / Tdatafrequence, src/crack_propagation.cpp:64,101)
   if (write_history < 0) write_history = (((cfg.seen >> HPCR_KEY_OUTTDATA) & 1ULL) && cfg.outTdata == 1) ? 1 : 0;

@@ -93,6 +93,13 @@ int hpcr_read_step_state(const char* dir, int t, hpcr_step_state* s);
 int hpcr_image_mp(const hpcr_material_table* tab, const unsigned char* img, size_t n, double voxel_size,
                   double* K, double* G, double* gc);
 
+/* The material set-up of input_total (src/image_input.cpp:256-282) on the
+ * device: the finest level's uint8 image (nodes of level maxlevel, x-fastest)
+ * is uploaded (1 byte per node), image_MP runs on the device with results
+ * identical to hpcr_image_mp, and coarse_VR fills K, G and gc on every coarse
+ * level.  HPCR_ERR_ARG for a gray level outside the table. */
+int hpcr_set_image(hpcr_ctx* ctx, const unsigned char* img, const hpcr_material_table* tab, double voxel_size);
+
 /* Seeded synthetic voxel image: non-overlapping spheres on integer node
  * centres placed by rejection (xoshiro256** seeded through splitmix64).
  * Inclusions first (count n_incl, radii U(incl_rmin, incl_rmax)), then pores
