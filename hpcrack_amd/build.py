@@ -10,10 +10,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libhpcrack_amd.so")
 DRIVER = os.path.join(HERE, "hpcrack_run")
+SERVER = os.path.join(HERE, "hpcrack_serve")
 
 HIP_SOURCES = ["csrc/hpcr_kernels.hip", "csrc/hpcr_mixed.hip", "csrc/hpcr_solver.hip"]
 HOST_SOURCES = ["csrc/hpcr_io.cpp"]
 DRIVER_SOURCES = ["host/hpcrack_run.cpp"]
+SERVER_SOURCES = ["host/hpcrack_serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -40,6 +42,10 @@ def build(verbose=False):
     drv = [s for s in DRIVER_SOURCES if os.path.exists(os.path.join(HERE, s))]
     if drv:
         _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", DRIVER] + drv +
+             ["-L", HERE, "-lhpcrack_amd", "-Wl,-rpath,$ORIGIN"])
+    srv = [s for s in SERVER_SOURCES if os.path.exists(os.path.join(HERE, s))]
+    if srv:
+        _run(["g++", "-O2", "-std=c++17", "-pthread", "-I", os.path.join(ROOT, "include"), "-o", SERVER] + srv +
              ["-L", HERE, "-lhpcrack_amd", "-Wl,-rpath,$ORIGIN"])
     if verbose:
         print("built", LIB)

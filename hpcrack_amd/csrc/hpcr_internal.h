@@ -172,6 +172,7 @@ void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& L
 void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, float* xf, float* cf,
                         const float* ec);
 void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b);
+void launch_to_f64(hipStream_t st, size_t n, const float* a, double* b);
 void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, float* x,
                             const float* b, const double* invD, int NB, double* scal);
 

@@ -464,6 +464,10 @@ __global__ __launch_bounds__(MXB * MYB) void mix_prolong_kernel(const LevelGeom 
   }
 }
 
+__global__ void to_f64_kernel(size_t n, const float* __restrict__ a, double* __restrict__ b) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    b[q] = (double)a[q];
+}
 __global__ void to_f32_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
     b[q] = (float)a[q];
@@ -509,6 +513,9 @@ void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc
 }
 void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b) {
   hipLaunchKernelGGL(to_f32_kernel, dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);
+}
+void launch_to_f64(hipStream_t st, size_t n, const float* a, double* b) {
+  hipLaunchKernelGGL(to_f64_kernel, dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);
 }
 void launch_power_init(hipStream_t st, const LevelGeom& L, float* v) {
   hipLaunchKernelGGL(power_init_kernel, dim3(mx_blocks(Nof(L), 4096)), dim3(256), 0, st, L, v);

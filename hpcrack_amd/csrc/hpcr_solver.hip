@@ -106,18 +106,21 @@ double coord(double origin, double h, int i) {
   return origin + p;
 }
 
+// Stream-ordered allocation on the context's stream: creating and destroying
+// contexts does not synchronise the device, so contexts of other requests
+// keep running meanwhile.
 int dev_alloc(hpcr_ctx* c, void** p, size_t bytes) {
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = hipMallocAsync(p, bytes, c->stream);
   if (e != hipSuccess) {
-    c->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+    c->err = std::string("hipMallocAsync failed: ") + hipGetErrorString(e);
     *p = nullptr;
     return HPCR_ERR_MEMORY;
   }
   c->allocs.push_back(*p);
   c->bytes += (long long)bytes;
-  e = hipMemset(*p, 0, bytes);
+  e = hipMemsetAsync(*p, 0, bytes, c->stream);
   if (e != hipSuccess) {
-    c->err = std::string("hipMemset failed: ") + hipGetErrorString(e);
+    c->err = std::string("hipMemsetAsync failed: ") + hipGetErrorString(e);
     return HPCR_ERR_DEVICE;
   }
   return HPCR_OK;
@@ -434,7 +437,13 @@ int mix_vcycle(hpcr_ctx* c, int m, int nu0, const double* qprev) {
       launch_small_pcg_u_f32(c->stream, v0.g, v0.Ke, v0.Ge, c->mix[0].x, c->mix[0].r, v0.invDu, nu0,
                              c->scal + S_SCRATCH);
     } else {
-      return set_err(c, HPCR_ERR_ARG, "mixed precision needs a coarsest level of at most 512 nodes and 256 elements");
+      // larger coarsest level: the FP64 smoother of the FAS path on the
+      // level's own buffers (zero start, homogeneous boundary values)
+      launch_to_f64(c->stream, 3 * v0.N, c->mix[0].r, v0.fu);
+      launch_fill(c->stream, 3 * v0.N, v0.u, 0.0);
+      int rc = pcg_u(c, 0, nu0, 0.0, 1);
+      if (rc) return rc;
+      launch_to_f32(c->stream, 3 * v0.N, v0.u, c->mix[0].x);
     }
   }
   for (int l = 1; l <= L; l++) {
@@ -464,8 +473,6 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
   if (mp.cheb_degree < 1 || mp.cheb_degree > 8 || mp.power_iters < 1 || !(mp.cheb_ratio > 1.0) ||
       !(mp.cheb_upper > 0.0) || mp.max_iter < 0)
     return set_err(c, HPCR_ERR_ARG, "bad mixed-precision parameters");
-  if (!small_level(c->lev[0].g))
-    return set_err(c, HPCR_ERR_ARG, "mixed precision needs a coarsest level of at most 512 nodes and 256 elements");
   if ((rc = mix_alloc(c))) return rc;
   c->smoother_iters = 0;
   if ((rc = mech_setup(c))) return rc;
@@ -742,9 +749,10 @@ int build_levels(hpcr_ctx* c) {
     if (v.kplanes.size() > 4) return set_err(c, HPCR_ERR_ARG, "more than 4 Dirichlet z-planes on one level");
     if ((rc = alloc_n(c, &v.fx, v.g.nx)) || (rc = alloc_n(c, &v.fy, v.g.ny)) || (rc = alloc_n(c, &v.fz, v.g.nz)))
       return rc;
-    HIPCHK(c, hipMemcpy(v.fx, fx.data(), fx.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(v.fy, fy.data(), fy.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(v.fz, fz.data(), fz.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpyAsync(v.fx, fx.data(), fx.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(v.fy, fy.data(), fy.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(v.fz, fz.data(), fz.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // the host vectors end with this iteration
     v.g.fx = v.fx;
     v.g.fy = v.fy;
     v.g.fz = v.fz;
@@ -760,7 +768,34 @@ int build_levels(hpcr_ctx* c) {
   if ((rc = alloc_n(c, &c->counter, 1))) return rc;
   if ((rc = alloc_n(c, &c->scal, S_COUNT))) return rc;
   HIPCHK(c, hipHostMalloc((void**)&c->host_scal, S_COUNT * sizeof(double), hipHostMallocDefault));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return HPCR_OK;
+}
+
+// device bytes that build_levels (and, with mixed, mix_alloc) allocate
+long long estimate_bytes(const hpcr_geometry& G, bool mixed) {
+  const int L = G.maxlevel;
+  long long total = 0, pb = 4096;
+  long ii = G.element_nb[0], jj = G.element_nb[1], kk = G.element_nb[2];
+  for (int l = 0; l <= L; l++) {
+    const long long nx = ii + 1, ny = jj + 1, nz = kk + 1, N = nx * ny * nz;
+    total += 8 * 3 * N * 8 + (l < L ? 3 * N * 8 : 0) + 14 * N * 8 + (l < L ? N * 8 : 0);
+    if (l == L) total += N + N * 8;
+    total += nx + ny + nz;
+    const long tb = tile_blocks((int)nx, (int)ny, (int)nz, choose_zchunk((int)nx, (int)ny, (int)nz));
+    if (tb > pb) pb = tb;
+    if (passb_blocks((size_t)N) > pb) pb = passb_blocks((size_t)N);
+    if (mixed) {
+      total += 2 * 3 * N * 4;
+      if (l > 0) total += 3 * 3 * N * 4 + (l == L ? 3 * N * 4 : 2 * N * 4);
+      if (l == L) total += 3 * N * 8;
+    }
+    ii *= 2;
+    jj *= 2;
+    kk *= 2;
+  }
+  total += pb * 4 * 8 + 4 + S_COUNT * 8;
+  return total;
 }
 
 void destroy(hpcr_ctx* c) {
@@ -771,7 +806,10 @@ void destroy(hpcr_ctx* c) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
-  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) {
+    for (void* p : c->allocs) (void)hipFreeAsync(p, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+  }
   if (c->host_scal) (void)hipHostFree(c->host_scal);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -927,6 +965,14 @@ int hpcr_coarse_vr(hpcr_ctx* c, int f) {
   c->diag_u_valid = c->diag_d_valid = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c, "coarse_vr");
+}
+
+int hpcr_estimate_bytes(const hpcr_geometry* geo, int mixed, long long* bytes) {
+  if (!geo || !bytes || geo->maxlevel < 0 || geo->maxlevel > 12 || geo->element_nb[0] < 1 ||
+      geo->element_nb[1] < 1 || geo->element_nb[2] < 1)
+    return HPCR_ERR_ARG;
+  *bytes = estimate_bytes(*geo, mixed != 0);
+  return HPCR_OK;
 }
 
 int hpcr_set_image(hpcr_ctx* c, const unsigned char* img, const hpcr_material_table* tab, double voxel_size) {

@@ -28,6 +28,21 @@ def device_count():
     return n.value if rc == 0 else 0
 
 
+def estimate_bytes(maxlevel, element_nb, domain, mixed=False):
+    """Device bytes a Solver of this geometry holds (no device needed)."""
+    L = capi.load()
+    geo = capi.Geometry()
+    geo.maxlevel = maxlevel
+    for i in range(3):
+        geo.element_nb[i] = int(element_nb[i])
+    geo.xa, geo.xb, geo.ya, geo.yb, geo.za, geo.zb = [float(v) for v in domain]
+    out = ctypes.c_longlong(0)
+    rc = L.hpcr_estimate_bytes(ctypes.byref(geo), 1 if mixed else 0, ctypes.byref(out))
+    if rc != 0:
+        raise HpcrError(rc, "bad geometry")
+    return out.value
+
+
 class Solver:
     def __init__(self, maxlevel, element_nb, domain, small_k=1e-5, lc=3.0,
                  mg_u=(50, 4, 2, 1, 12), mg_d=(10, 2, 1, 1, 12),

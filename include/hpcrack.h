@@ -113,6 +113,10 @@ const char* hpcr_last_error(const hpcr_ctx* ctx); /* NULL ctx: last creation err
 int hpcr_level_dims(const hpcr_ctx* ctx, int level, int dims[3]);
 int hpcr_level_h(const hpcr_ctx* ctx, int level, double h[3]);
 long long hpcr_device_bytes(const hpcr_ctx* ctx); /* device memory held */
+/* device bytes a context of this geometry holds (with mixed != 0: plus the
+ * buffers of the mixed-precision solve), without allocating; for admission
+ * control of several contexts on one device */
+int hpcr_estimate_bytes(const hpcr_geometry* geo, int mixed, long long* bytes);
 int hpcr_synchronize(hpcr_ctx* ctx);
 
 /* Host <-> device field transfer: N doubles, x-fastest.  HPCR_FLAG takes
@@ -141,7 +145,7 @@ int hpcr_phase_field(hpcr_ctx* ctx, int t, double* fd_norm, hpcr_stats* stats);
  * Chebyshev-Jacobi smoothing (degree cheb_degree before and after the coarse
  * correction, bounds [cheb_upper*lam/cheb_ratio, cheb_upper*lam] from
  * power_iters power iterations per level) and an FP64 coarsest solve of
- * nu0_u iterations.  Stops when sum(r^2)/f_norm <= tol with the residual
+ * nu0_u Jacobi-PCG iterations.  Stops when sum(r^2)/f_norm <= tol with the residual
  * recomputed from u, or after max_iter iterations; the residual is also
  * recomputed every replace_every iterations (0: only at the end).  stats
  * counts outer iterations in `cycles`.  Allocates its FP32 buffers (about
