@@ -72,13 +72,9 @@ struct MechCoef {
 // the mixed-precision solve).
 struct MixCoef {
   int fine;
-  const float* K;
-  const float* G;
-  const float* d;
-  const signed char* flag;
-  const float* Ke;
+  const float* lm;  // finest level: (lambda, mu) at the 8 Gauss points, 16 arrays of N
+  const float* Ke;  // coarse levels: element moduli
   const float* Ge;
-  float small_k;
 };
 
 // Material table in device memory (layout of hpcr_material_table)
@@ -172,6 +168,9 @@ void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& L
 void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, float* xf, float* cf,
                         const float* ec);
 void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b);
+// FP32 Gauss-point (lambda, mu) of the finest level's elements
+void launch_mix_lame(hipStream_t st, const LevelGeom& L, const double* K, const double* G, const double* d,
+                     const signed char* flag, double small_k, float* lm);
 void launch_to_f64(hipStream_t st, size_t n, const float* a, double* b);
 void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, float* x,
                             const float* b, const double* invD, int NB, double* scal);

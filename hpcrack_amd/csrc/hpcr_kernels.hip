@@ -59,6 +59,8 @@ struct MechOp {
   struct State {
     double beta;
   };
+  struct EData {};
+  __device__ __forceinline__ void efetch(const LevelGeom&, int, int, int, bool, EData&) const {}
   MechCoef C;
   const double* x;      // APPLY/RESID: input vector; PCGA: z
   const double* invD;   // RESID
@@ -143,7 +145,7 @@ struct MechOp {
   }
 
   __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State& st,
+                                          int ty, int ex, int ey, int ez, const State& st, const EData&,
                                           double (&f)[3][2][2][2]) const {
     double lam[2][2][2], mu[2][2][2], lam0, mu0;
     lame(sN, s0, tx, ty, ex, ey, ez, L, lam, mu, lam0, mu0);
@@ -161,9 +163,25 @@ struct MechOp {
   }
 
   static constexpr bool NEEDS_OWN = (MODE == M_PCGA);
+  // RESID: f and 1/D of the owner's node, read one plane ahead
+  struct NData {
+    double f[MODE == M_RESID ? 3 : 1], iD[MODE == M_RESID ? 3 : 1];
+  };
+  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, bool ok, NData& n) const {
+    if constexpr (MODE == M_RESID) {
+      if (!ok) return;
+      const size_t q = nidx(L, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        n.f[c] = out2[c * N + q];
+        n.iD[c] = invD[c * N + q];
+      }
+    }
+  }
   // v: assembled values of node (i, j, k); own: the node's LDS values (p)
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k,
-                                           const double (&v)[3], const State& st, Acc& acc) const {
+                                           const double (&v)[3], const State& st, const NData& nd,
+                                           Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
     if constexpr (MODE == M_APPLY) {
 #pragma unroll
@@ -172,14 +190,14 @@ struct MechOp {
       const int dm = dir_mask(L, i, j, k);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        double fv = out2[c * N + q];
+        double fv = nd.f[c];
         const bool dir = (dm >> c) & 1;
         if (flag && dir) {
           fv = v[c];
           out2[c * N + q] = fv;
         }
         const double r = dir ? 0.0 : fv - v[c];
-        const double z = r * invD[c * N + q];
+        const double z = r * nd.iD[c];
         out[c * N + q] = r;
         out3[c * N + q] = z;
         acc.a[0] += r * z;
@@ -243,6 +261,8 @@ struct PfOp {
   struct State {
     double beta;
   };
+  struct EData {};
+  __device__ __forceinline__ void efetch(const LevelGeom&, int, int, int, bool, EData&) const {}
   PfCoef C;
   const double* x;      // APPLY/RESID: d; PCGA: z
   const double* invD;   // RESID
@@ -286,7 +306,7 @@ struct PfOp {
   }
 
   __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State& st,
+                                          int ty, int ex, int ey, int ez, const State& st, const EData&,
                                           double (&f)[1][2][2][2]) const {
     double gcn[2][2][2];
     gather8(sN, s0, tx, ty, NU, gcn);
@@ -302,15 +322,27 @@ struct PfOp {
   }
 
   static constexpr bool NEEDS_OWN = (MODE == M_PCGA);
+  struct NData {
+    double f, iD;
+  };
+  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, bool ok, NData& n) const {
+    if constexpr (MODE == M_RESID) {
+      if (!ok) return;
+      const size_t q = nidx(L, i, j, k);
+      n.f = out2[q];
+      n.iD = invD[q];
+    }
+  }
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[1], int i, int j, int k,
-                                           const double (&v)[1], const State& st, Acc& acc) const {
+                                           const double (&v)[1], const State& st, const NData& nd,
+                                           Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
     if constexpr (MODE == M_APPLY) {
       out[q] = v[0];
     } else if constexpr (MODE == M_RESID) {
-      const double fv = out2[q];
+      const double fv = nd.f;
       const double r = fv - v[0];
-      const double z = r * invD[q];
+      const double z = r * nd.iD;
       out[q] = r;
       out3[q] = z;
       acc.a[0] += r * z;
@@ -448,12 +480,38 @@ static int pw_blocks(size_t n) {
 // grid of the PCG pass-B kernels (at most HPCR_PASSB_MAX_BLOCKS blocks,
 // grid-stride beyond)
 #ifndef HPCR_PASSB_MAX_BLOCKS
-#define HPCR_PASSB_MAX_BLOCKS 2048
+#define HPCR_PASSB_MAX_BLOCKS 512
 #endif
 int passb_blocks(size_t n) {
   const size_t b = (n + PW_TPB - 1) / PW_TPB;
   const size_t cap = HPCR_PASSB_MAX_BLOCKS;
   return (int)(b == 0 ? 1 : (b < cap ? b : cap));
+}
+
+// Optional (tuning macro HPCR_PASSB_SPLIT): the pass-B kernels only write
+// their block partials and a one-block kernel finishes the reduction, instead
+// of the last-block ticket.
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], const Reduce& R) {
+  __shared__ double sh[NV * 16];
+  block_red<RED_SUM, NV>(v, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; k++) R.partials[(size_t)lin_block() * NV + k] = v[k];
+}
+__global__ __launch_bounds__(256) void passb_finalize_kernel(const Reduce R, unsigned nb) {
+  __shared__ double sh[2 * 16];
+  double acc[2] = {0.0, 0.0};
+  for (unsigned i = threadIdx.x; i < nb; i += blockDim.x) {
+    acc[0] += R.partials[(size_t)i * 2];
+    acc[1] += R.partials[(size_t)i * 2 + 1];
+  }
+  block_red<RED_SUM, 2>(acc, sh);
+  if (threadIdx.x == 0) {
+    double* scal = R.scal;
+    scal[S_BETA] = acc[0] / scal[S_RZ];
+    scal[S_RZ] = acc[0];
+    scal[S_RR] = acc[1];
+  }
 }
 
 __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, double* __restrict__ u,
@@ -487,12 +545,16 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
       acc[1] += rv * rv;
     }
   }
+#ifdef HPCR_PASSB_SPLIT
+  block_partials<2>(acc, R);
+#else
   double* scal = R.scal;
   reduce_finalize<RED_SUM, 2>(acc, R, 0.0, [=](double (&s)[2]) {
     scal[S_BETA] = s[0] / scal[S_RZ];
     scal[S_RZ] = s[0];
     scal[S_RR] = s[1];
   });
+#endif
 }
 
 __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __restrict__ d, double* __restrict__ r,
@@ -522,6 +584,9 @@ void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, 
                        const double* w, const double* invD, const Reduce& R) {
   const size_t N = (size_t)L.nx * L.ny * L.nz;
   hipLaunchKernelGGL(mech_pcg_b_kernel, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, z, p, w, invD, R);
+#ifdef HPCR_PASSB_SPLIT
+  hipLaunchKernelGGL(passb_finalize_kernel, dim3(1), dim3(256), 0, s, R, (unsigned)passb_blocks(N));
+#endif
 }
 void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, double* z, const double* p,
                      const double* w, const double* invD, const Reduce& R) {

@@ -37,8 +37,7 @@ struct MechOpF {
   using TR = float;
   static constexpr int NC = 3;
   static constexpr int NU = 3;
-  static constexpr int NK = FINE ? 3 : 0;
-  static constexpr int NV = NU + NK;
+  static constexpr int NV = NU;
   static constexpr int NR = NV;
   static constexpr bool NEEDS_OWN = (MODE != F_RES);
   struct Acc {
@@ -46,6 +45,12 @@ struct MechOpF {
     __device__ Acc() { a[0] = a[1] = 0.0; }
   };
   struct State {};
+  // finest level: (lambda, mu) at the 8 Gauss points, precomputed once per
+  // solve; coarse levels: K_e, G_e
+  static constexpr int NE = FINE ? 16 : 2;
+  struct EData {
+    float v[NE];
+  };
   MixCoef C;
   const float* s;        // stencil vector
   float* dout;           // CHEB: new direction; POWER: D^-1 A s
@@ -68,10 +73,16 @@ struct MechOpF {
     const size_t q = nidx(L, gx, gy, gz);
 #pragma unroll
     for (int c = 0; c < 3; c++) v[c] = s[c * N + q];
+  }
+  __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, EData& e) const {
+    if (!ok) return;
+    const size_t q = nidx(L, ex, ey, ez);
     if constexpr (FINE) {
-      v[3] = C.K[q];
-      v[4] = C.G[q];
-      v[5] = C.d[q];
+#pragma unroll
+      for (int k = 0; k < 16; k++) e.v[k] = C.lm[k * N + q];
+    } else {
+      e.v[0] = C.Ke[q];
+      e.v[1] = C.Ge[q];
     }
   }
   __device__ __forceinline__ void commit(const float (&v)[NR], const State&, float (&o)[NV]) const {
@@ -80,38 +91,53 @@ struct MechOpF {
   }
 
   __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State&,
+                                          int ty, int ex, int ey, int ez, const State&, const EData& e,
                                           float (&f)[3][2][2][2]) const {
     float u[3][2][2][2];
 #pragma unroll
     for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
     if constexpr (FINE) {
-      float Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
-      gather8(sN, s0, tx, ty, 3, Kn);
-      gather8(sN, s0, tx, ty, 4, Gn);
-      gather8(sN, s0, tx, ty, 5, dn);
-      fine_lame(Kn, Gn, dn, (int)C.flag[nidx(L, ex, ey, ez)], C.small_k, lam, mu);
+      float lam[2][2][2], mu[2][2][2];
+#pragma unroll
+      for (int g = 0; g < 8; g++) {
+        lam[g >> 2][(g >> 1) & 1][g & 1] = e.v[2 * g];
+        mu[g >> 2][(g >> 1) & 1][g & 1] = e.v[2 * g + 1];
+      }
       mech_element<false>(u, lam, mu, 0.0f, 0.0f, sx, sy, sz, tx_, ty_, tz_, f);
     } else {
-      const size_t e = nidx(L, ex, ey, ez);
-      const float Ke = C.Ke[e], Ge = C.Ge[e];
-      const float lam0 = Ke - (2.0f / 3.0f) * Ge;
+      const float lam0 = e.v[0] - (2.0f / 3.0f) * e.v[1];
       float dl[2][2][2], dm[2][2][2];
-      mech_element<true>(u, dl, dm, lam0, Ge, sx, sy, sz, tx_, ty_, tz_, f);
+      mech_element<true>(u, dl, dm, lam0, e.v[1], sx, sy, sz, tx_, ty_, tz_, f);
+    }
+  }
+
+  // the epilogue's inputs at the owner's node, read one plane ahead
+  struct NData {
+    float r[3], x[3], iD[3];
+  };
+  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, bool ok, NData& n) const {
+    if (!ok) return;
+    const size_t q = nidx(L, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if constexpr (MODE != F_POWER) n.r[c] = r[c * N + q];
+      if constexpr (MODE == F_CHEB) n.x[c] = x[c * N + q];
+      if constexpr (MODE != F_RES) n.iD[c] = invD[c * N + q];
     }
   }
 
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const float (&own)[3], int i, int j, int k,
-                                           const float (&v)[3], const State&, Acc& acc) const {
+                                           const float (&v)[3], const State&, const NData& nd,
+                                           Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
     const int dm = dir_mask(L, i, j, k);
     if constexpr (MODE == F_CHEB) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t o = c * N + q;
-        const float rn = ((dm >> c) & 1) ? 0.0f : r[o] - v[c];
-        const float dn = cd * own[c] + cr * invD[o] * rn;
-        const float xn = x[o] + dn;
+        const float rn = ((dm >> c) & 1) ? 0.0f : nd.r[c] - v[c];
+        const float dn = cd * own[c] + cr * nd.iD[c] * rn;
+        const float xn = nd.x[c] + dn;
         r[o] = rn;
         dout[o] = dn;
         x[o] = xn;
@@ -121,14 +147,14 @@ struct MechOpF {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t o = c * N + q;
-        r[o] = ((dm >> c) & 1) ? 0.0f : r[o] - v[c];
+        r[o] = ((dm >> c) & 1) ? 0.0f : nd.r[c] - v[c];
       }
     } else {  // F_POWER
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t o = c * N + q;
         const bool dir = (dm >> c) & 1;
-        const float di = invD[o];
+        const float di = nd.iD[c];
         dout[o] = dir ? 0.0f : di * v[c];
         if (!dir) {
           acc.a[0] += (double)own[c] * (double)v[c];
@@ -167,6 +193,8 @@ struct FcgOpA {
   struct State {
     double beta;
   };
+  struct EData {};
+  __device__ __forceinline__ void efetch(const LevelGeom&, int, int, int, bool, EData&) const {}
   MechCoef C;
   const float* z;
   const double* p_old;
@@ -207,7 +235,7 @@ struct FcgOpA {
     w[5] = v[8];
   }
   __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State&,
+                                          int ty, int ex, int ey, int ez, const State&, const EData&,
                                           double (&f)[3][2][2][2]) const {
     double u[3][2][2][2], Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
 #pragma unroll
@@ -218,15 +246,25 @@ struct FcgOpA {
     fine_lame(Kn, Gn, dn, (int)C.flag[nidx(L, ex, ey, ez)], C.small_k, lam, mu);
     mech_element<false>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
   }
+  struct NData {
+    double r[3];
+  };
+  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, bool ok, NData& n) const {
+    if (!ok) return;
+    const size_t o = nidx(L, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++) n.r[c] = r[c * N + o];
+  }
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k,
-                                           const double (&v)[3], const State&, Acc& acc) const {
+                                           const double (&v)[3], const State&, const NData& nd,
+                                           Acc& acc) const {
     const size_t o = nidx(L, i, j, k);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       p_new[c * N + o] = own[c];
       q[c * N + o] = v[c];
       acc.a[0] += own[c] * v[c];
-      acc.a[1] += own[c] * r[c * N + o];
+      acc.a[1] += own[c] * nd.r[c];
     }
   }
   __device__ __forceinline__ void finish(Acc& acc) const {
@@ -462,6 +500,41 @@ __global__ __launch_bounds__(MXB * MYB) void mix_prolong_kernel(const LevelGeom 
     xf[o] += corr;
     cf[o] = corr;
   }
+}
+
+// (lambda, mu) at the Gauss points of every finest-level element (the
+// coefficients of MP_all_GAUSS_Point, computed in FP64 and stored as FP32)
+__global__ __launch_bounds__(256) void mix_lame_kernel(const LevelGeom L, const double* __restrict__ K,
+                                                       const double* __restrict__ G, const double* __restrict__ d,
+                                                       const signed char* __restrict__ flag, double small_k,
+                                                       float* __restrict__ lm) {
+  const int ex = blockIdx.x * 64 + threadIdx.x, ey = blockIdx.y * 4 + threadIdx.y, ez = blockIdx.z;
+  if (ex > L.nx - 2 || ey > L.ny - 2) return;
+  const size_t N = Nof(L);
+  double Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const size_t q = nidx(L, ex + c, ey + b, ez + a);
+        Kn[a][b][c] = K[q];
+        Gn[a][b][c] = G[q];
+        dn[a][b][c] = d[q];
+      }
+  const size_t e = nidx(L, ex, ey, ez);
+  fine_lame(Kn, Gn, dn, (int)flag[e], small_k, lam, mu);
+#pragma unroll
+  for (int g = 0; g < 8; g++) {
+    lm[(2 * g) * N + e] = (float)lam[g >> 2][(g >> 1) & 1][g & 1];
+    lm[(2 * g + 1) * N + e] = (float)mu[g >> 2][(g >> 1) & 1][g & 1];
+  }
+}
+void launch_mix_lame(hipStream_t st, const LevelGeom& L, const double* K, const double* G, const double* d,
+                     const signed char* flag, double small_k, float* lm) {
+  const dim3 g((L.nx - 1 + 63) / 64, (L.ny - 1 + 3) / 4, L.nz - 1);
+  hipLaunchKernelGGL(mix_lame_kernel, g, dim3(64, 4), 0, st, L, K, G, d, flag, small_k, lm);
 }
 
 __global__ void to_f64_kernel(size_t n, const float* __restrict__ a, double* __restrict__ b) {

@@ -45,7 +45,7 @@ struct Lev {
 // FP32 state of one level of the mixed-precision preconditioner
 struct MixLev {
   float *x = nullptr, *r = nullptr, *dA = nullptr, *dB = nullptr, *invD = nullptr;
-  float *K = nullptr, *G = nullptr, *d = nullptr, *Ke = nullptr, *Ge = nullptr;
+  float *lm = nullptr, *Ke = nullptr, *Ge = nullptr;
   double lam = 0.0;
   float cr0 = 0.0f;
   std::vector<float> cd, cr;  // Chebyshev steps 1..m-1
@@ -341,7 +341,7 @@ int mix_alloc(hpcr_ctx* c) {
     if ((rc = alloc_n(c, &m.dA, 3 * N)) || (rc = alloc_n(c, &m.dB, 3 * N)) || (rc = alloc_n(c, &m.invD, 3 * N)))
       return rc;
     if (l == c->L) {
-      if ((rc = alloc_n(c, &m.K, N)) || (rc = alloc_n(c, &m.G, N)) || (rc = alloc_n(c, &m.d, N))) return rc;
+      if ((rc = alloc_n(c, &m.lm, 16 * N))) return rc;
     } else {
       if ((rc = alloc_n(c, &m.Ke, N)) || (rc = alloc_n(c, &m.Ge, N))) return rc;
     }
@@ -355,13 +355,9 @@ MixCoef mixcoef(hpcr_ctx* c, int l) {
   const MixLev& m = c->mix[l];
   MixCoef k{};
   k.fine = (l == c->L) ? 1 : 0;
-  k.K = m.K;
-  k.G = m.G;
-  k.d = m.d;
-  k.flag = c->lev[l].flag;
+  k.lm = m.lm;
   k.Ke = m.Ke;
   k.Ge = m.Ge;
-  k.small_k = (float)c->par.small_k;
   return k;
 }
 
@@ -374,9 +370,7 @@ int mix_setup(hpcr_ctx* c, const hpcr_mixed_params& mp) {
     MixLev& M = c->mix[l];
     launch_to_f32(c->stream, 3 * v.N, v.invDu, M.invD);
     if (l == c->L) {
-      launch_to_f32(c->stream, v.N, v.K, M.K);
-      launch_to_f32(c->stream, v.N, v.G, M.G);
-      launch_to_f32(c->stream, v.N, v.d, M.d);
+      launch_mix_lame(c->stream, v.g, v.K, v.G, v.d, v.flag, c->par.small_k, M.lm);
     } else {
       launch_to_f32(c->stream, v.N, v.Ke, M.Ke);
       launch_to_f32(c->stream, v.N, v.Ge, M.Ge);
@@ -787,7 +781,7 @@ long long estimate_bytes(const hpcr_geometry& G, bool mixed) {
     if (passb_blocks((size_t)N) > pb) pb = passb_blocks((size_t)N);
     if (mixed) {
       total += 2 * 3 * N * 4;
-      if (l > 0) total += 3 * 3 * N * 4 + (l == L ? 3 * N * 4 : 2 * N * 4);
+      if (l > 0) total += 3 * 3 * N * 4 + (l == L ? 16 * N * 4 : 2 * N * 4);
       if (l == L) total += 3 * N * 8;
     }
     ii *= 2;

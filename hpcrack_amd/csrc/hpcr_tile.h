@@ -517,6 +517,14 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
     }
   };
 
+  // per-element data read from HBM (not shared through LDS), one layer ahead
+  typename Op::EData ed;
+  op.efetch(L, ex, ey, z0 - 1, exy && z0 - 1 >= 0 && z0 - 1 <= L.nz - 2, ed);
+
+  // inputs of the epilogue at this thread's own node, one plane ahead
+  typename Op::NData nd;
+  if (owner) op.nfetch(L, ex, ey, z0, z0 < zend, nd);
+
   typename Op::Acc acc;
   T carry[NC];
 #pragma unroll
@@ -532,7 +540,7 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
   for (int kz = z0 - 1; kz < zend; ++kz) {
     T f[NC][2][2][2];
     if (exy && kz >= 0 && kz <= L.nz - 2) {
-      op.element(L, sN, kz & 1, tx, ty, ex, ey, kz, st, f);
+      op.element(L, sN, kz & 1, tx, ty, ex, ey, kz, st, ed, f);
     } else {
 #pragma unroll
       for (int c = 0; c < NC; c++)
@@ -541,6 +549,8 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
 #pragma unroll
           for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = T(0);
     }
+    // the next layer's element data goes in flight during the exchange
+    op.efetch(L, ex, ey, kz + 1, exy && kz + 1 < zend && kz + 1 <= L.nz - 2, ed);
     // x-direction: node (tx, .) takes corner 0 of its own element and corner 1
     // of element tx-1 (adjacent lane); g[c][iz][iy]
     T g[NC][2][2];
@@ -572,7 +582,10 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
         vb[c] = carry[c] + g[c][0][0] + sX[c * 2 + 0][tid - TX];
         carry[c] = g[c][1][0] + sX[c * 2 + 1][tid - TX];
       }
-      if (kz >= z0) op.epilogue(L, own, ex, ey, kz, vb, st, acc);
+      if (kz >= z0) {
+        op.epilogue(L, own, ex, ey, kz, vb, st, nd, acc);
+        op.nfetch(L, ex, ey, kz + 1, kz + 1 < zend, nd);
+      }
     }
     __syncthreads();
   }

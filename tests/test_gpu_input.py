@@ -29,11 +29,18 @@ def test_set_image_matches_host_image_mp_and_oracle():
     o.image_mp(img, list(W.GRAY), list(tab.E[:3]), list(W.NU), list(W.GC), w.voxel_size)
     for f in (O.K, O.G, O.GC):
         o.coarse_vr_nodal(f)
-    for l in range(w.maxlevel + 1):
+    L = w.maxlevel
+    for l in range(L + 1):
         for f in (O.K, O.G, O.GC):
             a, b = s.get(f, l), s2.get(f, l)
             assert np.array_equal(a, b), (l, f)
-            assert np.array_equal(a, o.field(f, l)), (l, f)
+            ref = o.field(f, l)
+            if l == L:
+                # image_MP: the same FP64 operations in the same order
+                assert np.array_equal(a, ref), (l, f)
+            else:
+                # coarse_VR: 27-point means and square roots, summed in another order
+                assert np.max(np.abs(a - ref) / np.abs(ref)) < 1e-14, (l, f)
 
 
 def test_set_image_rejects_unknown_gray_level():
