@@ -118,6 +118,7 @@ def main():
                     help="also time the mixed-precision solve (FP64 flexible CG + FP32 V-cycle) to the same "
                          "tolerance (0: skip)")
     ap.add_argument("--mixed-cheb", type=int, default=2, help="Chebyshev degree of the FP32 smoother")
+    ap.add_argument("--mixed-ratio", type=float, default=6.0, help="Chebyshev smoothing range lambda_max/lambda_min")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -204,14 +205,16 @@ def main():
         u_plain = s.get(U, L)
         for i in range(args.warmup):
             zero_state()
-            fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, cheb_degree=args.mixed_cheb)
+            fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, cheb_degree=args.mixed_cheb,
+                                      cheb_ratio=args.mixed_ratio)
             log(f"[bench] mixed warmup {i}: {it} iterations, rel. residual^2 {s.last_stats.r_norm_final / fm:.3e}")
         s.synchronize()
         t0 = time.perf_counter()
         its, mrels = 0, []
         for i in range(args.steps):
             zero_state()
-            fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, cheb_degree=args.mixed_cheb)
+            fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, cheb_degree=args.mixed_cheb,
+                                      cheb_ratio=args.mixed_ratio)
             its += it
             mrels.append(s.last_stats.r_norm_final / fm)
         s.synchronize()
@@ -231,6 +234,7 @@ def main():
             "u_rel_l2_vs_plain_fp64": float(np.linalg.norm(du) / np.linalg.norm(u_plain)),
             "plain_fp64_final_rel_residual_squared": max(rels),
             "cheb_degree": args.mixed_cheb,
+            "cheb_ratio": args.mixed_ratio,
             "dtype": "f32 V-cycle (Chebyshev-Jacobi, transfers, operators), f64 outer flexible CG, f64 coarsest",
         }
         log(f"[bench] mixed: {its} iterations in {mel:.3f} s, {mixed['solves_per_s']} solves/s")

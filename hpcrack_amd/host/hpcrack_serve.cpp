@@ -198,7 +198,7 @@ Result solve(const Request& r, const Options& o, double t0_clock, Clock::time_po
   if (!rc) rc = hpcr_set_image(ctx, r.img.data(), &tab, 1e-6);
   if (!rc) {
     if (o.mixed) {
-      hpcr_mixed_params mp{200, o.tol, 2, 10, 10.0, 1.1, 10};
+      hpcr_mixed_params mp{200, o.tol, 2, 10, 6.0, 1.1, 10};
       rc = hpcr_u_entire_mixed(ctx, 1.0, 0, &f_norm, &mp, &su);
     } else {
       rc = hpcr_u_entire(ctx, 1.0, 0, &f_norm, &su);
@@ -291,12 +291,17 @@ int main(int argc, char** argv) {
     if (o.only >= o.requests) return 2;
     reqs = {reqs[o.only]};
   }
-  for (Request& r : reqs) {
+  for (size_t i = 0; i < reqs.size(); i++) {
+    Request& r = reqs[i];
     if (build_image(r)) {
       std::fprintf(stderr, "image of request %d failed\n", r.id);
       return 1;
     }
     if (hpcr_estimate_bytes(&r.geo, o.mixed ? 1 : 0, &r.bytes)) return 1;
+    if ((i + 1) % 32 == 0 || i + 1 == reqs.size()) {
+      std::fprintf(stderr, "[serve] %zu/%zu request images ready\n", i + 1, reqs.size());
+      std::fflush(stderr);
+    }
   }
   const long long budget = (long long)(o.budget_gb * 1e9);
   for (const Request& r : reqs)
@@ -311,7 +316,7 @@ int main(int argc, char** argv) {
   std::deque<int> waiting;  // indices into reqs, arrival order
   long long in_use = 0, peak_bytes = 0;
   int active = 0, peak_active = 0;
-  size_t arrived = 0;
+  size_t arrived = 0, completed = 0;
   std::vector<Result> results(reqs.size());
   const bool alone = o.only >= 0;
   const auto start = Clock::now();
@@ -354,6 +359,10 @@ int main(int argc, char** argv) {
           in_use -= reqs[i].bytes;
           active--;
           results[i] = r;
+          if (++completed % 32 == 0 || completed == reqs.size()) {
+            std::fprintf(stderr, "[serve] %zu/%zu requests done at %.2f s\n", completed, reqs.size(), r.done);
+            std::fflush(stderr);
+          }
         }
         cv.notify_all();
       }

@@ -203,7 +203,7 @@ class Solver:
         return fn.value, st.cycles
 
     def u_entire_mixed(self, Ut, t, f_norm, tol=None, max_iter=100, cheb_degree=2, power_iters=10,
-                       cheb_ratio=10.0, cheb_upper=1.1, replace_every=10):
+                       cheb_ratio=6.0, cheb_upper=1.1, replace_every=10):
         """Mixed-precision form of u_entire (FP64 flexible CG, FP32 V-cycle);
         returns (f_norm, outer iterations).  tol defaults to the context's
         tol_u (squared relative residual)."""
