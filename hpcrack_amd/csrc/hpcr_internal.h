@@ -11,11 +11,15 @@ namespace hpcr {
 // one element per thread, marching in z; a block owns the 31x7 interior node
 // columns of its tile (neighbouring tiles overlap by one element).  32-wide
 // rows keep the tile's global rows long (33 doubles) for cache-line use.
-constexpr int TX = 32, TY = 8;
+#ifndef HPCR_TILE_TX
+#define HPCR_TILE_TX 32
+#endif
+constexpr int TX = HPCR_TILE_TX, TY = 256 / HPCR_TILE_TX;
 constexpr int OWNX = TX - 1, OWNY = TY - 1;
 constexpr int TPB = TX * TY;                  // 256 threads
 constexpr int PX = TX + 1;                    // node row length in LDS
-constexpr int PLANE = (TX + 1) * (TY + 1);    // 297 nodes per LDS plane
+constexpr int PLANE = (TX + 1) * (TY + 1);    // 297 nodes per LDS plane (32x8 tiles)
+static_assert(64 % TX == 0 && PLANE <= 2 * TX * TY, "tile rows must divide the wavefront");
 constexpr int ZCHUNK_MAX = 32;                // node planes per block (fine levels)
 // levels small enough for the one-workgroup smoother (small_pcg_kernel)
 constexpr int SMALL_NODES = 512, SMALL_ELEMS = 256;

@@ -256,22 +256,48 @@ __device__ __forceinline__ T degr(T dg, T k) {
 
 // Gauss-point Lame coefficients (lambda = K_d - 2/3 G_d, mu = G_d) of a fine
 // element (MP_all_GAUSS_Point, mechanical_pcg.cpp:26-61)
+// An element with a node at d == 0 has d = 0 at every Gauss point (the zero
+// rule), i.e. the undamaged factor degr(0, k) everywhere: that common case
+// skips the interpolation of d.
 template <class T>
 __device__ __forceinline__ void fine_lame(const T (&Kn)[2][2][2], const T (&Gn)[2][2][2], const T (&dn)[2][2][2],
                                           int flag, T small_k, T (&lam)[2][2][2], T (&mu)[2][2][2]) {
-  T Kg[2][2][2], Gg[2][2][2], dg[2][2][2];
+  T Kg[2][2][2], Gg[2][2][2], g[2][2][2];
   gp_mp(Kn, Kg);
   gp_mp(Gn, Gg);
-  gp_mp(dn, dg);
+  bool dzero = false;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) dzero |= (dn[a][b][c] == T(0));
+  if (dzero) {
+    const T g0 = degr(T(0), small_k);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) g[a][b][c] = g0;
+  } else {
+    T dg[2][2][2];
+    interp8(dn, dg);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) g[a][b][c] = degr(dg[a][b][c], small_k);
+  }
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int c = 0; c < 2; c++) {
-        const T g = degr(dg[a][b][c], small_k);
-        const T Kd = (flag == 0) ? g * Kg[a][b][c] : Kg[a][b][c];
-        const T Gd = (flag == 2) ? Gg[a][b][c] : g * Gg[a][b][c];
+        const T Kd = (flag == 0) ? g[a][b][c] * Kg[a][b][c] : Kg[a][b][c];
+        const T Gd = (flag == 2) ? Gg[a][b][c] : g[a][b][c] * Gg[a][b][c];
         lam[a][b][c] = Kd - (T(2) / T(3)) * Gd;
         mu[a][b][c] = Gd;
       }
