@@ -74,6 +74,7 @@ PROTOTYPES = {
     "hpcr_phase_field": (_i, [_vp, _i, _dp, ctypes.POINTER(Stats)]),
     "hpcr_u_entire_mixed": (_i, [_vp, _d, _i, _dp, ctypes.POINTER(MixedParams), ctypes.POINTER(Stats)]),
     "hpcr_estimate_bytes": (_i, [ctypes.POINTER(Geometry), _i, ctypes.POINTER(ctypes.c_longlong)]),
+    "hpcr_set_graphs": (_i, [_vp, _i]),
     "hpcr_strain_history": (_i, [_vp, _d]),
     "hpcr_voigt_all_level": (_i, [_vp, _i]),
     "hpcr_vr_all_level": (_i, [_vp, _i]),

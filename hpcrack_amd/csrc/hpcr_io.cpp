@@ -1,6 +1,7 @@
 // Host-side input formats (include/hpcrack_io.h): config.txt, material CSV,
 // uint8 .raw images, the level geometry they imply, and the seeded synthetic
 // image generator.  Plain C++, no device calls.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -360,13 +361,31 @@ int hpcr_synth_image(const hpcr_synth_spec* sp, unsigned char* out, double reali
   Rng rng(sp->seed);
   std::vector<Sphere> placed;
   const int max_tries = sp->max_tries > 0 ? sp->max_tries : 1000000;
+  // Placed spheres are binned by centre on a grid of cells no smaller than
+  // the largest possible centre distance of two overlapping spheres, so a
+  // candidate is tested against the 27 cells around its centre only (the
+  // same accept/reject decisions as testing every placed sphere).
+  const double rmax = std::max(sp->n_incl > 0 ? sp->incl_rmax : 0.0, sp->pore_fraction > 0 ? sp->pore_rmax : 0.0);
+  const int cell = std::max(1, (int)std::ceil(2.0 * rmax + sp->gap));
+  const int gx = (nx + cell - 1) / cell, gy = (ny + cell - 1) / cell, gz = (nz + cell - 1) / cell;
+  std::vector<std::vector<int>> bins((size_t)gx * gy * gz);
+  auto bin_of = [&](int cx, int cy, int cz) { return (size_t)(cx / cell) + (size_t)gx * ((cy / cell) + (size_t)gy * (cz / cell)); };
   auto fits = [&](const Sphere& s) {
-    for (const Sphere& o : placed) {
-      const double dx = s.cx - o.cx, dy = s.cy - o.cy, dz = s.cz - o.cz;
-      const double lim = s.r + o.r + sp->gap;
-      if (dx * dx + dy * dy + dz * dz < lim * lim) return false;
-    }
+    const int bx = s.cx / cell, by = s.cy / cell, bz = s.cz / cell;
+    for (int kz = std::max(0, bz - 1); kz <= std::min(gz - 1, bz + 1); kz++)
+      for (int ky = std::max(0, by - 1); ky <= std::min(gy - 1, by + 1); ky++)
+        for (int kx = std::max(0, bx - 1); kx <= std::min(gx - 1, bx + 1); kx++)
+          for (int idx : bins[(size_t)kx + (size_t)gx * (ky + (size_t)gy * kz)]) {
+            const Sphere& o = placed[idx];
+            const double dx = s.cx - o.cx, dy = s.cy - o.cy, dz = s.cz - o.cz;
+            const double lim = s.r + o.r + sp->gap;
+            if (dx * dx + dy * dy + dz * dz < lim * lim) return false;
+          }
     return true;
+  };
+  auto place = [&](const Sphere& s) {
+    bins[bin_of(s.cx, s.cy, s.cz)].push_back((int)placed.size());
+    placed.push_back(s);
   };
   auto stamp = [&](const Sphere& s, int label) {
     size_t cnt = 0;
@@ -396,7 +415,7 @@ int hpcr_synth_image(const hpcr_synth_spec* sp, unsigned char* out, double reali
     Sphere s = draw(sp->incl_rmin, sp->incl_rmax);
     tries++;
     if (!fits(s)) continue;
-    placed.push_back(s);
+    place(s);
     incl_vox += stamp(s, sp->incl_label);
     q++;
   }
@@ -405,7 +424,7 @@ int hpcr_synth_image(const hpcr_synth_spec* sp, unsigned char* out, double reali
     Sphere s = draw(sp->pore_rmin, sp->pore_rmax);
     tries++;
     if (!fits(s)) continue;
-    placed.push_back(s);
+    place(s);
     pore_vox += stamp(s, sp->pore_label);
   }
   if (realised) {

@@ -78,6 +78,13 @@ struct hpcr_ctx {
   long long smoother_iters = 0;
   bool diag_u_valid = false, diag_d_valid = false;
   int last_rr_level = -1;  // level whose residual norm scal[S_RR] holds
+  // HIP graphs of whole multigrid cycles (optional): captured once per
+  // (load value, first-step flag) and replayed, one launch per cycle
+  bool graphs = false;
+  hipGraphExec_t g_u = nullptr, g_d = nullptr;
+  double g_u_Ut = 0.0;
+  int g_u_t0 = -1;
+  long long g_u_iters = 0, g_d_iters = 0;  // smoother iterations per captured cycle
   // mixed-precision solve (allocated on first use)
   std::vector<MixLev> mix;
   double* q2 = nullptr;  // second FP64 q buffer of the flexible CG
@@ -270,6 +277,67 @@ int mech_setup(hpcr_ctx* c) {
   return check_launch(c, "mech_setup");
 }
 
+// Capture the kernel sequence of `body` on the context's stream into an
+// executable graph.  The body must not synchronise or read back.
+template <class F>
+int capture_graph(hpcr_ctx* c, hipGraphExec_t* exec, F body) {
+  if (*exec) {
+    (void)hipGraphExecDestroy(*exec);
+    *exec = nullptr;
+  }
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = body();
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return set_err(c, HPCR_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    *exec = nullptr;
+    return set_err(c, HPCR_ERR_DEVICE, std::string("graph instantiation: ") + hipGetErrorString(e));
+  }
+  return HPCR_OK;
+}
+
+// one mechanical cycle on the finest level, as a graph replay when enabled
+// (kernel timing records events per launch, so it runs the cycle eagerly)
+int run_cycle_u(hpcr_ctx* c, double Ut, int t) {
+  if (!c->graphs || c->timing) return cycle_u(c, c->L, Ut, t);
+  const int t0 = (t == 0) ? 1 : 0;
+  if (!c->g_u || c->g_u_Ut != Ut || c->g_u_t0 != t0) {
+    const long long it0 = c->smoother_iters;
+    int rc = capture_graph(c, &c->g_u, [&] { return cycle_u(c, c->L, Ut, t); });
+    c->g_u_iters = c->smoother_iters - it0;
+    c->smoother_iters = it0;
+    if (rc) return rc;
+    c->g_u_Ut = Ut;
+    c->g_u_t0 = t0;
+  }
+  HIPCHK(c, hipGraphLaunch(c->g_u, c->stream));
+  c->smoother_iters += c->g_u_iters;
+  c->last_rr_level = c->L;
+  return HPCR_OK;
+}
+int cycle_d(hpcr_ctx* c, int l);
+int run_cycle_d(hpcr_ctx* c) {
+  if (!c->graphs || c->timing) return cycle_d(c, c->L);
+  if (!c->g_d) {
+    const long long it0 = c->smoother_iters;
+    int rc = capture_graph(c, &c->g_d, [&] { return cycle_d(c, c->L); });
+    c->g_d_iters = c->smoother_iters - it0;
+    c->smoother_iters = it0;
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipGraphLaunch(c->g_d, c->stream));
+  c->smoother_iters += c->g_d_iters;
+  c->last_rr_level = c->L;
+  return HPCR_OK;
+}
+
 // finest-level squared residual norm after a cycle
 int finest_rr(hpcr_ctx* c, bool mech, double* out) {
   int rc;
@@ -304,7 +372,7 @@ int u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {
   int cycles = 0;
   for (int j = 1; j <= c->par.mg_u.ncy; j++) {
     if (rel > c->par.tol_u) {
-      if ((rc = cycle_u(c, L, Ut, t))) return rc;
+      if ((rc = run_cycle_u(c, Ut, t))) return rc;
       cycles++;
       if ((rc = finest_rr(c, true, &r_norm))) return rc;
       if (t == 0) *f_norm = c->host_scal[S_FF];
@@ -622,7 +690,7 @@ int phase_field(hpcr_ctx* c, int t, double* fd_norm, hpcr_stats* st) {
   for (int j = 1; j <= c->par.mg_d.ncy; j++) {
     if (r_norm > c->par.tol_d_abs) {
       if (sres > c->par.tol_d_rel) {
-        if ((rc = cycle_d(c, L))) return rc;
+        if ((rc = run_cycle_d(c))) return rc;
         cycles++;
       }
       if ((rc = finest_rr(c, false, &r_norm))) return rc;
@@ -800,6 +868,8 @@ void destroy(hpcr_ctx* c) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
+  if (c->g_u) (void)hipGraphExecDestroy(c->g_u);
+  if (c->g_d) (void)hipGraphExecDestroy(c->g_d);
   if (c->stream) {
     for (void* p : c->allocs) (void)hipFreeAsync(p, c->stream);
     (void)hipStreamSynchronize(c->stream);
@@ -959,6 +1029,12 @@ int hpcr_coarse_vr(hpcr_ctx* c, int f) {
   c->diag_u_valid = c->diag_d_valid = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c, "coarse_vr");
+}
+
+int hpcr_set_graphs(hpcr_ctx* c, int on) {
+  if (!c) return HPCR_ERR_ARG;
+  c->graphs = on != 0;
+  return HPCR_OK;
 }
 
 int hpcr_estimate_bytes(const hpcr_geometry* geo, int mixed, long long* bytes) {

@@ -19,6 +19,8 @@
 //   --stop-force F      stop when the reaction falls below F (default 0.5)
 //   --out PATH          F_U file (default F_U.dat)
 //   --device N          HIP device (default 0)
+//   --mixed             mechanical solves in the mixed-precision form
+//   --graphs            replay the multigrid cycles as HIP graphs
 //   --write-history     write u, v, w, H, d every --history-every steps
 //                       (default: when the config sets outTdata=1, every
 //                       Tdatafrequence steps), as write_vector_3D does
@@ -61,6 +63,7 @@ int main(int argc, char** argv) {
   bool strict = true;
   double small_k = 1e-5, tol_u = 1e-6, tol_d = 1e-12, du = 0.0, du_ue = 0.0, stop_force = 0.5;
   int write_history = -1, history_every = 0, resume = -1;
+  bool mixed = false, graphs = false;
   std::string history_dir = "Time_step_data";
   for (int a = 1; a < argc; a++) {
     std::string s = argv[a];
@@ -82,6 +85,8 @@ int main(int argc, char** argv) {
     else if (s == "--stop-force") stop_force = std::atof(next("--stop-force"));
     else if (s == "--out") out_path = next("--out");
     else if (s == "--device") device = std::atoi(next("--device"));
+    else if (s == "--mixed") mixed = true;
+    else if (s == "--graphs") graphs = true;
     else if (s == "--write-history") write_history = 1;
     else if (s == "--history-every") history_every = std::atoi(next("--history-every"));
     else if (s == "--history-dir") history_dir = next("--history-dir");
@@ -132,6 +137,7 @@ int main(int argc, char** argv) {
   par.mg_d = cfg.mg_d;
   hpcr_ctx* ctx = nullptr;
   if ((rc = hpcr_ctx_create(&geo, &par, device, &ctx))) return fail("creating the device context", rc);
+  if (graphs && (rc = hpcr_set_graphs(ctx, 1))) return fail("enabling graphs", rc, ctx);
   const int L = geo.maxlevel;
   // image_MP and coarse_VR on the device (input_total, src/image_input.cpp:256-282)
   if ((rc = hpcr_set_image(ctx, img.data(), &tab, cfg.voxel_size))) return fail("mapping gray levels to materials", rc, ctx);
@@ -179,7 +185,12 @@ int main(int argc, char** argv) {
   for (int t = t_begin; t < steps; t++) {
     std::cout << "***** At Time step " << t << std::endl;
     hpcr_stats su, sd;
-    if ((rc = hpcr_u_entire(ctx, U_unit, t, &f_norm, &su))) return fail("u_entire", rc, ctx);
+    if (mixed) {
+      const hpcr_mixed_params mp{200, tol_u, 2, 10, 6.0, 1.1, 10};
+      if ((rc = hpcr_u_entire_mixed(ctx, U_unit, t, &f_norm, &mp, &su))) return fail("u_entire (mixed)", rc, ctx);
+    } else if ((rc = hpcr_u_entire(ctx, U_unit, t, &f_norm, &su))) {
+      return fail("u_entire", rc, ctx);
+    }
     if (t == 0) {
       if (du > 0.0) {
         Ut = du;

@@ -26,6 +26,8 @@
 //   --slots K           concurrent requests (default 8)
 //   --budget-gb B       HBM admission budget (default 200)
 //   --mixed             mechanical solve in the mixed-precision form
+//   --no-graphs         launch the multigrid cycles kernel by kernel instead of
+//                       replaying them as HIP graphs
 //   --load-factor F     u_prop = F * Ue (default 2)
 //   --tol X             squared relative residual of both solves (default 1e-16)
 //   --only ID           run request ID alone (same inputs), nothing else
@@ -82,6 +84,7 @@ struct Options {
   int slots = 8;
   double budget_gb = 200.0;
   bool mixed = false;
+  bool graphs = true;
   double load_factor = 2.0;
   double tol = 1e-16;
   int only = -1;
@@ -195,6 +198,7 @@ Result solve(const Request& r, const Options& o, double t0_clock, Clock::time_po
   const int L = r.geo.maxlevel;
   double f_norm = 0.0, fd_norm = 0.0, FeW = 0.0, Ue = 0.0;
   hpcr_stats su{}, sd{};
+  if (!rc && o.graphs) rc = hpcr_set_graphs(ctx, 1);
   if (!rc) rc = hpcr_set_image(ctx, r.img.data(), &tab, 1e-6);
   if (!rc) {
     if (o.mixed) {
@@ -262,6 +266,7 @@ int main(int argc, char** argv) {
     } else if (s == "--slots") o.slots = std::atoi(next());
     else if (s == "--budget-gb") o.budget_gb = std::atof(next());
     else if (s == "--mixed") o.mixed = true;
+    else if (s == "--no-graphs") o.graphs = false;
     else if (s == "--load-factor") o.load_factor = std::atof(next());
     else if (s == "--tol") o.tol = std::atof(next());
     else if (s == "--only") o.only = std::atoi(next());

@@ -202,6 +202,10 @@ class Solver:
         self.last_stats = st
         return fn.value, st.cycles
 
+    def set_graphs(self, on):
+        """Replay whole multigrid cycles as HIP graphs (identical results)."""
+        self._chk(self._L.hpcr_set_graphs(self._h, 1 if on else 0))
+
     def u_entire_mixed(self, Ut, t, f_norm, tol=None, max_iter=100, cheb_degree=2, power_iters=10,
                        cheb_ratio=6.0, cheb_upper=1.1, replace_every=10):
         """Mixed-precision form of u_entire (FP64 flexible CG, FP32 V-cycle);

@@ -113,6 +113,11 @@ const char* hpcr_last_error(const hpcr_ctx* ctx); /* NULL ctx: last creation err
 int hpcr_level_dims(const hpcr_ctx* ctx, int level, int dims[3]);
 int hpcr_level_h(const hpcr_ctx* ctx, int level, double h[3]);
 long long hpcr_device_bytes(const hpcr_ctx* ctx); /* device memory held */
+/* Replay whole multigrid cycles of hpcr_u_entire / hpcr_phase_field as HIP
+ * graphs (captured on first use per load value, one launch per cycle) instead
+ * of launching their kernels one by one; results are identical.  Off by
+ * default; kernel timing (hpcr_timing_enable) runs cycles eagerly. */
+int hpcr_set_graphs(hpcr_ctx* ctx, int on);
 /* device bytes a context of this geometry holds (with mixed != 0: plus the
  * buffers of the mixed-precision solve), without allocating; for admission
  * control of several contexts on one device */
