@@ -488,6 +488,17 @@ int passb_blocks(size_t n) {
   return (int)(b == 0 ? 1 : (b < cap ? b : cap));
 }
 
+// Tuning macros of the pass-B kernels: HPCR_PASSB_NT (non-temporal loads and
+// stores of the streamed vectors), HPCR_PASSB_UNROLL2 (two nodes per thread
+// and iteration).
+#ifdef HPCR_PASSB_NT
+#define PB_LD(x) __builtin_nontemporal_load(&(x))
+#define PB_ST(v, x) __builtin_nontemporal_store((v), &(x))
+#else
+#define PB_LD(x) (x)
+#define PB_ST(v, x) ((x) = (v))
+#endif
+
 // Optional (tuning macro HPCR_PASSB_SPLIT): the pass-B kernels only write
 // their block partials and a one-block kernel finishes the reduction, instead
 // of the last-block ticket.
@@ -524,7 +535,7 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
   const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
   const double alpha = R.scal[S_ALPHA];
   double acc[2] = {0.0, 0.0};
-  for (unsigned q = blockIdx.x * PW_TPB + threadIdx.x; q < N; q += gridDim.x * PW_TPB) {
+  auto node = [&](unsigned q) {
     const unsigned k = q / plane;
     int dm = 0;
     if (L.fz[k]) {
@@ -535,16 +546,25 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t o = (size_t)c * N + q;
-      u[o] = u[o] + alpha * p[o];
-      double rv = r[o] - alpha * w[o];
+      PB_ST(PB_LD(u[o]) + alpha * PB_LD(p[o]), u[o]);
+      double rv = PB_LD(r[o]) - alpha * PB_LD(w[o]);
       if ((dm >> c) & 1) rv = 0.0;
-      r[o] = rv;
-      const double zv = rv * invD[o];
-      z[o] = zv;
+      PB_ST(rv, r[o]);
+      const double zv = rv * PB_LD(invD[o]);
+      PB_ST(zv, z[o]);
       acc[0] += rv * zv;
       acc[1] += rv * rv;
     }
+  };
+  const unsigned stride = gridDim.x * PW_TPB;
+#ifdef HPCR_PASSB_UNROLL2
+  for (unsigned q = blockIdx.x * PW_TPB + threadIdx.x; q < N; q += 2 * stride) {
+    node(q);
+    if (q + stride < N) node(q + stride);
   }
+#else
+  for (unsigned q = blockIdx.x * PW_TPB + threadIdx.x; q < N; q += stride) node(q);
+#endif
 #ifdef HPCR_PASSB_SPLIT
   block_partials<2>(acc, R);
 #else
@@ -1054,18 +1074,28 @@ void launch_inject(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int 
 
 // 2-D blocks of 64 x 4 threads over (x, y), one z-plane per blockIdx.z
 constexpr int XB = 64, YB = 4;
-static dim3 node_grid(const LevelGeom& L) { return dim3((L.nx + XB - 1) / XB, (L.ny + YB - 1) / YB, L.nz); }
 
 // coarsen_f_standard_routine (multigrid_intergrid.cpp:124-342): full weighting
 // with out-of-range neighbours counted as 0, written in its separable form
 // (1-D weights 1/2, 1, 1/2 per axis)
-__global__ __launch_bounds__(XB * YB) void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
-                                                           const double* __restrict__ rf,
-                                                           const double* __restrict__ Fic,
-                                                           double* __restrict__ fc) {
-  const int i = blockIdx.x * XB + threadIdx.x;
-  const int j = blockIdx.y * YB + threadIdx.y, k = blockIdx.z;
-  if (i >= Lc.nx || j >= Lc.ny) return;
+// one thread per node, x fastest: (i, j, k) of node q of a level
+__device__ __forceinline__ void node_ijk(const LevelGeom& L, unsigned q, int& i, int& j, int& k) {
+  const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
+  const unsigned kk = q / plane, rem = q - kk * plane;
+  const unsigned jj = rem / (unsigned)L.nx;
+  i = (int)(rem - jj * (unsigned)L.nx);
+  j = (int)jj;
+  k = (int)kk;
+}
+
+__global__ __launch_bounds__(256) void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
+                                                       const double* __restrict__ rf,
+                                                       const double* __restrict__ Fic,
+                                                       double* __restrict__ fc) {
+  const unsigned qn = blockIdx.x * 256 + threadIdx.x;
+  if (qn >= (unsigned)Nof(Lc)) return;
+  int i, j, k;
+  node_ijk(Lc, qn, i, j, k);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const int x = 2 * i, y = 2 * j, z = 2 * k;
   double wx[3], wy[3], wz[3];
@@ -1100,7 +1130,8 @@ __global__ __launch_bounds__(XB * YB) void restrict_kernel(const LevelGeom Lc, c
 }
 void launch_restrict(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* rf,
                      const double* Fic, double* fc) {
-  hipLaunchKernelGGL(restrict_kernel, node_grid(Lc), dim3(XB, YB), 0, s, Lc, Lf, ncomp, rf, Fic, fc);
+  hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((Nof(Lc) + 255) / 256)), dim3(256), 0, s, Lc, Lf, ncomp, rf,
+                     Fic, fc);
 }
 
 // e_c = u_c - u_c,old (the coarse-grid correction, multigrid_intergrid.cpp:484-518)
@@ -1112,12 +1143,13 @@ __global__ void diff_kernel(size_t n, const double* __restrict__ a, const double
 
 // refine_standard_routine + Dirichlet_BD (multigrid_intergrid.cpp:416-549):
 // trilinear interpolation of the correction e_c, one writer per fine node
-__global__ __launch_bounds__(XB * YB) void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp,
-                                                          double* __restrict__ uf, const double* __restrict__ e,
-                                                          int apply_bc, double Ut) {
-  const int x = blockIdx.x * XB + threadIdx.x;
-  const int y = blockIdx.y * YB + threadIdx.y, z = blockIdx.z;
-  if (x >= Lf.nx || y >= Lf.ny) return;
+__global__ __launch_bounds__(256) void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp,
+                                                      double* __restrict__ uf, const double* __restrict__ e,
+                                                      int apply_bc, double Ut) {
+  const unsigned qn = blockIdx.x * 256 + threadIdx.x;
+  if (qn >= (unsigned)Nof(Lf)) return;
+  int x, y, z;
+  node_ijk(Lf, qn, x, y, z);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
   const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
@@ -1149,7 +1181,8 @@ void launch_prolong(hipStream_t s, const LevelGeom& Lf, const LevelGeom& Lc, int
                     const double* uc, const double* ucold, int apply_bc, double Ut, double* scratch) {
   const size_t n = (size_t)ncomp * Nof(Lc);
   hipLaunchKernelGGL(diff_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, uc, ucold, scratch);
-  hipLaunchKernelGGL(prolong_kernel, node_grid(Lf), dim3(XB, YB), 0, s, Lf, Lc, ncomp, uf, scratch, apply_bc, Ut);
+  hipLaunchKernelGGL(prolong_kernel, dim3((unsigned)((Nof(Lf) + 255) / 256)), dim3(256), 0, s, Lf, Lc, ncomp, uf,
+                     scratch, apply_bc, Ut);
 }
 
 // ---------------------------------------------------------------------------

@@ -419,16 +419,23 @@ __global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double
 // restriction of the FP32 residual (full weighting, as the FP64 transfer);
 // starts the coarse smoother (d = cr0 D^-1 r, x = d), or, on the coarsest
 // level (start == 0), only clears x for the coarsest solve
-constexpr int MXB = 64, MYB = 4;
-__global__ __launch_bounds__(MXB * MYB) void mix_restrict_kernel(const LevelGeom Lc, const LevelGeom Lf,
-                                                                 const float* __restrict__ rf,
-                                                                 float* __restrict__ rc, float* __restrict__ dc,
-                                                                 float* __restrict__ xc,
-                                                                 const float* __restrict__ invD, float cr0,
-                                                                 int start) {
-  const int i = blockIdx.x * MXB + threadIdx.x;
-  const int j = blockIdx.y * MYB + threadIdx.y, k = blockIdx.z;
-  if (i >= Lc.nx || j >= Lc.ny) return;
+__device__ __forceinline__ void mix_ijk(const LevelGeom& L, unsigned q, int& i, int& j, int& k) {
+  const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
+  const unsigned kk = q / plane, rem = q - kk * plane;
+  const unsigned jj = rem / (unsigned)L.nx;
+  i = (int)(rem - jj * (unsigned)L.nx);
+  j = (int)jj;
+  k = (int)kk;
+}
+
+__global__ __launch_bounds__(256) void mix_restrict_kernel(const LevelGeom Lc, const LevelGeom Lf,
+                                                           const float* __restrict__ rf, float* __restrict__ rc,
+                                                           float* __restrict__ dc, float* __restrict__ xc,
+                                                           const float* __restrict__ invD, float cr0, int start) {
+  const unsigned qn = blockIdx.x * 256 + threadIdx.x;
+  if (qn >= (unsigned)Nof(Lc)) return;
+  int i, j, k;
+  mix_ijk(Lc, qn, i, j, k);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const int x = 2 * i, y = 2 * j, z = 2 * k;
   float wx[3], wy[3], wz[3];
@@ -474,12 +481,13 @@ __global__ __launch_bounds__(MXB * MYB) void mix_restrict_kernel(const LevelGeom
 
 // prolongation of the coarse correction: c = P e (zero on Dirichlet DOFs),
 // x_f += c, c kept as the stencil vector of the first post-smoothing step
-__global__ __launch_bounds__(MXB * MYB) void mix_prolong_kernel(const LevelGeom Lf, const LevelGeom Lc,
-                                                                float* __restrict__ xf, float* __restrict__ cf,
-                                                                const float* __restrict__ e) {
-  const int x = blockIdx.x * MXB + threadIdx.x;
-  const int y = blockIdx.y * MYB + threadIdx.y, z = blockIdx.z;
-  if (x >= Lf.nx || y >= Lf.ny) return;
+__global__ __launch_bounds__(256) void mix_prolong_kernel(const LevelGeom Lf, const LevelGeom Lc,
+                                                          float* __restrict__ xf, float* __restrict__ cf,
+                                                          const float* __restrict__ e) {
+  const unsigned qn = blockIdx.x * 256 + threadIdx.x;
+  if (qn >= (unsigned)Nof(Lf)) return;
+  int x, y, z;
+  mix_ijk(Lf, qn, x, y, z);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
   const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
@@ -576,13 +584,13 @@ void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, cons
 }
 void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,
                          float* dc, float* xc, const float* invD, float cr0, int start) {
-  const dim3 g((Lc.nx + MXB - 1) / MXB, (Lc.ny + MYB - 1) / MYB, Lc.nz);
-  hipLaunchKernelGGL(mix_restrict_kernel, g, dim3(MXB, MYB), 0, st, Lc, Lf, rf, rc, dc, xc, invD, cr0, start);
+  hipLaunchKernelGGL(mix_restrict_kernel, dim3((unsigned)((Nof(Lc) + 255) / 256)), dim3(256), 0, st, Lc, Lf, rf, rc,
+                     dc, xc, invD, cr0, start);
 }
 void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, float* xf, float* cf,
                         const float* ec) {
-  const dim3 g((Lf.nx + MXB - 1) / MXB, (Lf.ny + MYB - 1) / MYB, Lf.nz);
-  hipLaunchKernelGGL(mix_prolong_kernel, g, dim3(MXB, MYB), 0, st, Lf, Lc, xf, cf, ec);
+  hipLaunchKernelGGL(mix_prolong_kernel, dim3((unsigned)((Nof(Lf) + 255) / 256)), dim3(256), 0, st, Lf, Lc, xf, cf,
+                     ec);
 }
 void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b) {
   hipLaunchKernelGGL(to_f32_kernel, dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);

@@ -45,6 +45,8 @@ class Workload:
     penny_l: float = 2.0
     # two-phase variant: inclusion Young's modulus (Pa) instead of the table's
     incl_E: float = 0.0
+    # rejection-sampling budget of the sphere placement
+    max_tries: int = 2000000
     extra: dict = field(default_factory=dict)
 
     @property
@@ -66,7 +68,8 @@ class Workload:
 
     def image(self):
         return hio.synth_image(self.dims, self.seed, n_incl=self.n_incl, incl_r=self.incl_r, incl_label=1,
-                               pore_fraction=self.pore_fraction, pore_r=self.pore_r, pore_label=2)
+                               pore_fraction=self.pore_fraction, pore_r=self.pore_r, pore_label=2,
+                               max_tries=self.max_tries)
 
     def table(self):
         E = list(E_PA)
@@ -109,7 +112,8 @@ C2 = Workload("C2-257-three-phase", 257, 1002, n_incl=200, incl_r=(3.0, 10.0), p
               pore_r=(2.0, 5.0), steps=20)
 # 513^3 two-phase, inclusion/matrix stiffness ratio 50 (3 GPa / 150 GPa),
 # inclusions to ~20 % volume (count chosen for that fraction at radii 4-12)
-C3 = Workload("C3-513-two-phase-ratio50", 513, 1003, n_incl=10000, incl_r=(4.0, 12.0), incl_E=150e9, steps=3)
+C3 = Workload("C3-513-two-phase-ratio50", 513, 1003, n_incl=17000, incl_r=(4.0, 12.0), incl_E=150e9, steps=3,
+              max_tries=50000000)
 # the C2 image with a preset penny crack (radius 0.3 L at mid-height, l = 2)
 C4 = Workload("C4-257-penny-crack", 257, 1002, n_incl=200, incl_r=(3.0, 10.0), pore_fraction=0.05,
               pore_r=(2.0, 5.0), steps=1, penny_radius_frac=0.3, penny_l=2.0)
@@ -123,7 +127,8 @@ def scaled(w, n, name=None):
                     incl_r=w.incl_r, pore_fraction=w.pore_fraction, pore_r=w.pore_r, element_nb=w.element_nb,
                     voxel_size=w.voxel_size, lc_voxels=w.lc_voxels, small_k=w.small_k, tol_u=w.tol_u,
                     tol_d_rel=w.tol_d_rel, tol_d_abs=w.tol_d_abs, mg_u=w.mg_u, mg_d=w.mg_d, steps=w.steps,
-                    penny_radius_frac=w.penny_radius_frac, penny_l=w.penny_l, incl_E=w.incl_E)
+                    penny_radius_frac=w.penny_radius_frac, penny_l=w.penny_l, incl_E=w.incl_E,
+                    max_tries=w.max_tries)
 
 
 def upload(solver, w, K, G, gc):
