@@ -21,6 +21,13 @@ constexpr int PX = TX + 1;                    // node row length in LDS
 constexpr int PLANE = (TX + 1) * (TY + 1);    // 297 nodes per LDS plane (32x8 tiles)
 static_assert(64 % TX == 0 && PLANE <= 2 * TX * TY, "tile rows must divide the wavefront");
 constexpr int ZCHUNK_MAX = 32;                // node planes per block (fine levels)
+// node planes held in LDS by a tile kernel: 3 lets one barrier per layer
+// order the plane and exchange buffers, 2 needs two (a tuning macro)
+#ifndef HPCR_TILE_SLOTS
+#define HPCR_TILE_SLOTS 3
+#endif
+constexpr int NSLOT = HPCR_TILE_SLOTS;
+static_assert(NSLOT == 2 || NSLOT == 3, "two or three plane slots");
 // levels small enough for the one-workgroup smoother (small_pcg_kernel)
 constexpr int SMALL_NODES = 512, SMALL_ELEMS = 256;
 

@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "hpcr_internal.h"
 #include "hpcr_tile.h"
@@ -59,8 +60,10 @@ struct MechOp {
   struct State {
     double beta;
   };
-  struct EData {};
-  __device__ __forceinline__ void efetch(const LevelGeom&, int, int, int, bool, EData&) const {}
+  // per-element data, staged through LDS with the node planes: the split
+  // flag on the finest level, K_e and G_e on coarse levels
+  using TE = std::conditional_t<FINE, int, double>;
+  static constexpr int NE = FINE ? 1 : 2;
   MechCoef C;
   const double* x;      // APPLY/RESID: input vector; PCGA: z
   const double* invD;   // RESID
@@ -104,6 +107,19 @@ struct MechOp {
     }
   }
 
+  __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, TE (&e)[NE]) const {
+#pragma unroll
+    for (int k = 0; k < NE; k++) e[k] = TE(0);
+    if (!ok) return;
+    const size_t q = nidx(L, ex, ey, ez);
+    if constexpr (FINE) {
+      e[0] = C.flag[q];
+    } else {
+      e[0] = C.Ke[q];
+      e[1] = C.Ge[q];
+    }
+  }
+
   __device__ __forceinline__ void commit(const double (&r)[NR], const State& st, double (&v)[NV]) const {
     if constexpr (MODE == M_PCGA) {
 #pragma unroll
@@ -116,22 +132,19 @@ struct MechOp {
     }
   }
 
-  __device__ __forceinline__ void lame(const double (&sN)[2][NV][PLANE], int s0, int tx, int ty, int ex, int ey,
-                                       int ez, const LevelGeom& L, double (&lam)[2][2][2], double (&mu)[2][2][2],
-                                       double& lam0, double& mu0) const {
+  __device__ __forceinline__ void lame(const double (&sN)[NSLOT][NV][PLANE], int s0, int tx, int ty, const TE (&e)[NE],
+                                       double (&lam)[2][2][2], double (&mu)[2][2][2], double& lam0,
+                                       double& mu0) const {
     if constexpr (FINE) {
       double Kn[2][2][2], Gn[2][2][2], dn[2][2][2];
       gather8(sN, s0, tx, ty, NU + 0, Kn);
       gather8(sN, s0, tx, ty, NU + 1, Gn);
       gather8(sN, s0, tx, ty, NU + 2, dn);
-      const int fl = C.flag[nidx(L, ex, ey, ez)];
-      fine_lame(Kn, Gn, dn, fl, C.small_k, lam, mu);
+      fine_lame(Kn, Gn, dn, e[0], C.small_k, lam, mu);
       lam0 = mu0 = 0.0;
     } else {
-      const size_t e = nidx(L, ex, ey, ez);
-      const double Ke = C.Ke[e], Ge = C.Ge[e];
-      lam0 = Ke - (2.0 / 3.0) * Ge;
-      mu0 = Ge;
+      lam0 = e[0] - (2.0 / 3.0) * e[1];
+      mu0 = e[1];
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -144,11 +157,11 @@ struct MechOp {
     }
   }
 
-  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State& st, const EData&,
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSLOT][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, const State& st, const TE (&e)[NE],
                                           double (&f)[3][2][2][2]) const {
     double lam[2][2][2], mu[2][2][2], lam0, mu0;
-    lame(sN, s0, tx, ty, ex, ey, ez, L, lam, mu, lam0, mu0);
+    lame(sN, s0, tx, ty, e, lam, mu, lam0, mu0);
     if constexpr (MODE == M_DIAG) {
       mech_element_diag(lam, mu, ihx2, ihy2, ihz2, wg, f);
     } else {
@@ -261,8 +274,12 @@ struct PfOp {
   struct State {
     double beta;
   };
-  struct EData {};
-  __device__ __forceinline__ void efetch(const LevelGeom&, int, int, int, bool, EData&) const {}
+  // the element's strain history H, staged through LDS with the node planes
+  using TE = double;
+  static constexpr int NE = 1;
+  __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, double (&e)[1]) const {
+    e[0] = ok ? C.H[nidx(L, ex, ey, ez)] : 0.0;
+  }
   PfCoef C;
   const double* x;      // APPLY/RESID: d; PCGA: z
   const double* invD;   // RESID
@@ -305,13 +322,13 @@ struct PfOp {
     }
   }
 
-  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State& st, const EData&,
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSLOT][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, const State& st, const double (&e)[1],
                                           double (&f)[1][2][2][2]) const {
     double gcn[2][2][2];
     gather8(sN, s0, tx, ty, NU, gcn);
     double m[2][2][2], k[2][2][2];
-    pf_coefs(gcn, C.H[nidx(L, ex, ey, ez)], C.lc, wg, m, k);
+    pf_coefs(gcn, e[0], C.lc, wg, m, k);
     if constexpr (MODE == M_DIAG) {
       pf_element_diag(m, k, ihx2, ihy2, ihz2, f[0]);
     } else {

@@ -47,10 +47,8 @@ struct MechOpF {
   struct State {};
   // finest level: (lambda, mu) at the 8 Gauss points, precomputed once per
   // solve; coarse levels: K_e, G_e
+  using TE = float;
   static constexpr int NE = FINE ? 16 : 2;
-  struct EData {
-    float v[NE];
-  };
   MixCoef C;
   const float* s;        // stencil vector
   float* dout;           // CHEB: new direction; POWER: D^-1 A s
@@ -74,15 +72,17 @@ struct MechOpF {
 #pragma unroll
     for (int c = 0; c < 3; c++) v[c] = s[c * N + q];
   }
-  __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, EData& e) const {
+  __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, float (&e)[NE]) const {
+#pragma unroll
+    for (int k = 0; k < NE; k++) e[k] = 0.0f;
     if (!ok) return;
     const size_t q = nidx(L, ex, ey, ez);
     if constexpr (FINE) {
 #pragma unroll
-      for (int k = 0; k < 16; k++) e.v[k] = C.lm[k * N + q];
+      for (int k = 0; k < 16; k++) e[k] = C.lm[k * N + q];
     } else {
-      e.v[0] = C.Ke[q];
-      e.v[1] = C.Ge[q];
+      e[0] = C.Ke[q];
+      e[1] = C.Ge[q];
     }
   }
   __device__ __forceinline__ void commit(const float (&v)[NR], const State&, float (&o)[NV]) const {
@@ -90,8 +90,8 @@ struct MechOpF {
     for (int k = 0; k < NV; k++) o[k] = v[k];
   }
 
-  __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State&, const EData& e,
+  __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[NSLOT][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, const State&, const float (&e)[NE],
                                           float (&f)[3][2][2][2]) const {
     float u[3][2][2][2];
 #pragma unroll
@@ -100,14 +100,14 @@ struct MechOpF {
       float lam[2][2][2], mu[2][2][2];
 #pragma unroll
       for (int g = 0; g < 8; g++) {
-        lam[g >> 2][(g >> 1) & 1][g & 1] = e.v[2 * g];
-        mu[g >> 2][(g >> 1) & 1][g & 1] = e.v[2 * g + 1];
+        lam[g >> 2][(g >> 1) & 1][g & 1] = e[2 * g];
+        mu[g >> 2][(g >> 1) & 1][g & 1] = e[2 * g + 1];
       }
       mech_element<false>(u, lam, mu, 0.0f, 0.0f, sx, sy, sz, tx_, ty_, tz_, f);
     } else {
-      const float lam0 = e.v[0] - (2.0f / 3.0f) * e.v[1];
+      const float lam0 = e[0] - (2.0f / 3.0f) * e[1];
       float dl[2][2][2], dm[2][2][2];
-      mech_element<true>(u, dl, dm, lam0, e.v[1], sx, sy, sz, tx_, ty_, tz_, f);
+      mech_element<true>(u, dl, dm, lam0, e[1], sx, sy, sz, tx_, ty_, tz_, f);
     }
   }
 
@@ -193,8 +193,9 @@ struct FcgOpA {
   struct State {
     double beta;
   };
-  struct EData {};
-  __device__ __forceinline__ void efetch(const LevelGeom&, int, int, int, bool, EData&) const {}
+  // the element's split flag, staged through LDS with the node planes
+  using TE = int;
+  static constexpr int NE = 1;
   MechCoef C;
   const float* z;
   const double* p_old;
@@ -227,6 +228,9 @@ struct FcgOpA {
     v[7] = C.G[o];
     v[8] = C.d[o];
   }
+  __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, int (&e)[1]) const {
+    e[0] = ok ? (int)C.flag[nidx(L, ex, ey, ez)] : 0;
+  }
   __device__ __forceinline__ void commit(const double (&v)[NR], const State& st, double (&w)[NV]) const {
 #pragma unroll
     for (int c = 0; c < 3; c++) w[c] = v[c] + st.beta * v[3 + c];
@@ -234,8 +238,8 @@ struct FcgOpA {
     w[4] = v[7];
     w[5] = v[8];
   }
-  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[2][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State&, const EData&,
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSLOT][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, const State&, const int (&e)[1],
                                           double (&f)[3][2][2][2]) const {
     double u[3][2][2][2], Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
 #pragma unroll
@@ -243,7 +247,7 @@ struct FcgOpA {
     gather8(sN, s0, tx, ty, 3, Kn);
     gather8(sN, s0, tx, ty, 4, Gn);
     gather8(sN, s0, tx, ty, 5, dn);
-    fine_lame(Kn, Gn, dn, (int)C.flag[nidx(L, ex, ey, ez)], C.small_k, lam, mu);
+    fine_lame(Kn, Gn, dn, e[0], C.small_k, lam, mu);
     mech_element<false>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
   }
   struct NData {

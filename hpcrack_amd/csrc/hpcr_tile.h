@@ -6,6 +6,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hpcr_internal.h"
 
 namespace hpcr {
@@ -96,86 +98,121 @@ __device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R
 // ---------------------------------------------------------------------------
 // Element algebra (sum factorisation)
 // ---------------------------------------------------------------------------
+// Interpolation of a node pair (a, b) to the two Gauss points:
+// (GA a + GB b, GB a + GA b).  With GA + GB = 1 this is b + GA (a - b) and
+// a - GA (a - b): three operations instead of four (the "delta form").  The
+// FP32 preconditioner uses it; the FP64 path keeps the two-product form by
+// default, whose rounding the oracle comparisons were pinned with
+// (HPCR_DELTA_INTERP_F64=1 selects the delta form there too: about 5 % off the
+// fine-level pass A, at a different rounding).
+#ifndef HPCR_DELTA_INTERP_F64
+#define HPCR_DELTA_INTERP_F64 0
+#endif
+template <class T>
+constexpr bool kDelta = !std::is_same<T, double>::value || HPCR_DELTA_INTERP_F64;
+
+template <class T>
+__device__ __forceinline__ void lerp2(T a, T b, T& o0, T& o1) {
+  if constexpr (kDelta<T>) {
+    const T d = a - b;
+    o0 = fma(T(GA), d, b);
+    o1 = fma(T(-GA), d, a);
+  } else {
+    o0 = T(GA) * a + T(GB) * b;
+    o1 = T(GB) * a + T(GA) * b;
+  }
+}
+
 // trilinear interpolation of 8 nodal values v[iz][iy][ix] to Gauss points o[gz][gy][gx]
 template <class T>
 __device__ __forceinline__ void interp8(const T (&v)[2][2][2], T (&o)[2][2][2]) {
-  const T ga = T(GA), gb = T(GB);
   T t[2][2][2], s[2][2][2];
 #pragma unroll
   for (int iz = 0; iz < 2; iz++)
 #pragma unroll
-    for (int iy = 0; iy < 2; iy++) {
-      t[iz][iy][0] = ga * v[iz][iy][0] + gb * v[iz][iy][1];
-      t[iz][iy][1] = gb * v[iz][iy][0] + ga * v[iz][iy][1];
-    }
+    for (int iy = 0; iy < 2; iy++) lerp2(v[iz][iy][0], v[iz][iy][1], t[iz][iy][0], t[iz][iy][1]);
 #pragma unroll
   for (int iz = 0; iz < 2; iz++)
 #pragma unroll
-    for (int gx = 0; gx < 2; gx++) {
-      s[iz][0][gx] = ga * t[iz][0][gx] + gb * t[iz][1][gx];
-      s[iz][1][gx] = gb * t[iz][0][gx] + ga * t[iz][1][gx];
-    }
+    for (int gx = 0; gx < 2; gx++) lerp2(t[iz][0][gx], t[iz][1][gx], s[iz][0][gx], s[iz][1][gx]);
 #pragma unroll
   for (int gy = 0; gy < 2; gy++)
 #pragma unroll
-    for (int gx = 0; gx < 2; gx++) {
-      o[0][gy][gx] = ga * s[0][gy][gx] + gb * s[1][gy][gx];
-      o[1][gy][gx] = gb * s[0][gy][gx] + ga * s[1][gy][gx];
-    }
+    for (int gx = 0; gx < 2; gx++) lerp2(s[0][gy][gx], s[1][gy][gx], o[0][gy][gx], o[1][gy][gx]);
 }
 // transpose of interp8: nodal f[iz][iy][ix] += sum_g L L L q[gz][gy][gx]
 template <class T>
 __device__ __forceinline__ void interp8_t_add(const T (&q)[2][2][2], T (&f)[2][2][2]) {
-  const T ga = T(GA), gb = T(GB);
   T s[2][2][2], t[2][2][2];
 #pragma unroll
   for (int gy = 0; gy < 2; gy++)
 #pragma unroll
-    for (int gx = 0; gx < 2; gx++) {
-      s[0][gy][gx] = ga * q[0][gy][gx] + gb * q[1][gy][gx];
-      s[1][gy][gx] = gb * q[0][gy][gx] + ga * q[1][gy][gx];
-    }
+    for (int gx = 0; gx < 2; gx++) lerp2(q[0][gy][gx], q[1][gy][gx], s[0][gy][gx], s[1][gy][gx]);
 #pragma unroll
   for (int iz = 0; iz < 2; iz++)
 #pragma unroll
-    for (int gx = 0; gx < 2; gx++) {
-      t[iz][0][gx] = ga * s[iz][0][gx] + gb * s[iz][1][gx];
-      t[iz][1][gx] = gb * s[iz][0][gx] + ga * s[iz][1][gx];
-    }
+    for (int gx = 0; gx < 2; gx++) lerp2(s[iz][0][gx], s[iz][1][gx], t[iz][0][gx], t[iz][1][gx]);
 #pragma unroll
   for (int iz = 0; iz < 2; iz++)
 #pragma unroll
     for (int iy = 0; iy < 2; iy++) {
-      f[iz][iy][0] += ga * t[iz][iy][0] + gb * t[iz][iy][1];
-      f[iz][iy][1] += gb * t[iz][iy][0] + ga * t[iz][iy][1];
+      T a, b;
+      lerp2(t[iz][iy][0], t[iz][iy][1], a, b);
+      f[iz][iy][0] += a;
+      f[iz][iy][1] += b;
     }
 }
 // bilinear interpolation of 4 edge values e[a][b] (a: outer axis, b: inner
-// axis) to Gauss pairs o[ga][gb], with the inner stage scaled by s
+// axis) to Gauss pairs o[ga][gb], scaled by s (the delta form scales the
+// result, the two-product form the inner stage)
 template <class T>
 __device__ __forceinline__ void interp4(const T (&e)[2][2], T s, T (&o)[2][2]) {
-  const T ga = T(GA), gb = T(GB);
-  T t0 = s * (ga * e[0][0] + gb * e[0][1]);
-  T t1 = s * (gb * e[0][0] + ga * e[0][1]);
-  T u0 = s * (ga * e[1][0] + gb * e[1][1]);
-  T u1 = s * (gb * e[1][0] + ga * e[1][1]);
-  o[0][0] = ga * t0 + gb * u0;
-  o[0][1] = ga * t1 + gb * u1;
-  o[1][0] = gb * t0 + ga * u0;
-  o[1][1] = gb * t1 + ga * u1;
+  if constexpr (kDelta<T>) {
+    T t0, t1, u0, u1;
+    lerp2(e[0][0], e[0][1], t0, t1);
+    lerp2(e[1][0], e[1][1], u0, u1);
+    lerp2(t0, u0, o[0][0], o[1][0]);
+    lerp2(t1, u1, o[0][1], o[1][1]);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) o[a][b] *= s;
+  } else {
+    const T ga = T(GA), gb = T(GB);
+    const T t0 = s * (ga * e[0][0] + gb * e[0][1]);
+    const T t1 = s * (gb * e[0][0] + ga * e[0][1]);
+    const T u0 = s * (ga * e[1][0] + gb * e[1][1]);
+    const T u1 = s * (gb * e[1][0] + ga * e[1][1]);
+    o[0][0] = ga * t0 + gb * u0;
+    o[0][1] = ga * t1 + gb * u1;
+    o[1][0] = gb * t0 + ga * u0;
+    o[1][1] = gb * t1 + ga * u1;
+  }
 }
 // transpose of interp4
 template <class T>
 __device__ __forceinline__ void interp4_t(const T (&S)[2][2], T s, T (&R)[2][2]) {
-  const T ga = T(GA), gb = T(GB);
-  T q00 = ga * S[0][0] + gb * S[1][0];  // a=0, gb=0
-  T q01 = ga * S[0][1] + gb * S[1][1];
-  T q10 = gb * S[0][0] + ga * S[1][0];  // a=1
-  T q11 = gb * S[0][1] + ga * S[1][1];
-  R[0][0] = s * (ga * q00 + gb * q01);
-  R[0][1] = s * (gb * q00 + ga * q01);
-  R[1][0] = s * (ga * q10 + gb * q11);
-  R[1][1] = s * (gb * q10 + ga * q11);
+  if constexpr (kDelta<T>) {
+    T q00, q01, q10, q11;
+    lerp2(S[0][0], S[1][0], q00, q10);
+    lerp2(S[0][1], S[1][1], q01, q11);
+    lerp2(q00, q01, R[0][0], R[0][1]);
+    lerp2(q10, q11, R[1][0], R[1][1]);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) R[a][b] *= s;
+  } else {
+    const T ga = T(GA), gb = T(GB);
+    const T q00 = ga * S[0][0] + gb * S[1][0];  // a=0, gb=0
+    const T q01 = ga * S[0][1] + gb * S[1][1];
+    const T q10 = gb * S[0][0] + ga * S[1][0];  // a=1
+    const T q11 = gb * S[0][1] + ga * S[1][1];
+    R[0][0] = s * (ga * q00 + gb * q01);
+    R[0][1] = s * (gb * q00 + ga * q01);
+    R[1][0] = s * (ga * q10 + gb * q11);
+    R[1][1] = s * (gb * q10 + ga * q11);
+  }
 }
 
 // Gradient of a nodal scalar v[iz][iy][ix] at the Gauss points:
@@ -504,8 +541,15 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
   constexpr int NR = Op::NR;  // raw values per node fetched from HBM
   using T = typename Op::T;   // element arithmetic (double, or float in the preconditioner)
   using TR = typename Op::TR; // type of the fetched raw values
-  __shared__ T sN[2][NV][PLANE];
-  __shared__ T sX[2 * NC][TPB];
+  constexpr int NE = Op::NE;  // per-element values (split flag, K_e/G_e, H, Gauss-point moduli)
+  using TE = typename Op::TE;
+  static_assert(NE >= 1, "tile ops carry per-element data");
+  __shared__ T sN[NSLOT][NV][PLANE];
+  // per-element data travels with the node planes: layer p sits in slot(p)
+  __shared__ TE sE[NSLOT][NE][TPB];
+  // y-exchange buffers; with three plane slots they alternate between layers
+  constexpr int NXB = (NSLOT == 3) ? 2 : 1;
+  __shared__ T sX[NXB][2 * NC][TPB];
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int ntx = (L.nx + OWNX - 1) / OWNX, nty = (L.ny + OWNY - 1) / OWNY;
@@ -526,12 +570,17 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
   const bool in1 = gx1 >= 0 && gy1 >= 0 && gx1 < L.nx && gy1 < L.ny;
   const bool in2 = has2 && gx2 >= 0 && gy2 >= 0 && gx2 < L.nx && gy2 < L.ny;
   TR r1[NR], r2[NR];
+  TE re[NE];
+  // node plane pz and the element layer above it (planes past zend are not needed)
   auto fetch = [&](int pz) {
-    const bool zok = pz >= 0 && pz < L.nz;
+    const bool zok = pz >= 0 && pz < L.nz && pz <= zend;
+    op.efetch(L, ex, ey, pz, exy && zok && pz < zend && pz <= L.nz - 2, re);
     op.fetch(L, gx1, gy1, pz, in1 && zok, st, r1);
     op.fetch(L, gx2, gy2, pz, in2 && zok, st, r2);
   };
   auto commit = [&](int slot) {
+#pragma unroll
+    for (int k = 0; k < NE; k++) sE[slot][k][tid] = re[k];
     T v[NV];
     op.commit(r1, st, v);
 #pragma unroll
@@ -543,9 +592,8 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
     }
   };
 
-  // per-element data read from HBM (not shared through LDS), one layer ahead
-  typename Op::EData ed;
-  op.efetch(L, ex, ey, z0 - 1, exy && z0 - 1 >= 0 && z0 - 1 <= L.nz - 2, ed);
+  // LDS slot of node plane p >= -1
+  auto slot = [](int p) { return (NSLOT == 3) ? (p + 3) % 3 : (p & 1); };
 
   // inputs of the epilogue at this thread's own node, one plane ahead
   typename Op::NData nd;
@@ -556,17 +604,26 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
 #pragma unroll
   for (int c = 0; c < NC; c++) carry[c] = T(0);
 
-  // planes z0-1 and z0 into LDS, plane z0+1 in flight
+  // planes z0-1 .. z0+NSLOT-2 into LDS, the next one in flight
   fetch(z0 - 1);
-  commit((z0 - 1) & 1);
+  commit(slot(z0 - 1));
   fetch(z0);
-  commit(z0 & 1);
+  commit(slot(z0));
   fetch(z0 + 1);
+  if constexpr (NSLOT == 3) {
+    commit(slot(z0 + 1));
+    fetch(z0 + 2);
+  }
   __syncthreads();
   for (int kz = z0 - 1; kz < zend; ++kz) {
+    const int s0 = slot(kz);
+    const int xb = (NXB == 2) ? (kz & 1) : 0;
     T f[NC][2][2][2];
     if (exy && kz >= 0 && kz <= L.nz - 2) {
-      op.element(L, sN, kz & 1, tx, ty, ex, ey, kz, st, ed, f);
+      TE e[NE];
+#pragma unroll
+      for (int k = 0; k < NE; k++) e[k] = sE[s0][k][tid];
+      op.element(L, sN, s0, tx, ty, ex, ey, kz, st, e, f);
     } else {
 #pragma unroll
       for (int c = 0; c < NC; c++)
@@ -575,8 +632,6 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
 #pragma unroll
           for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = T(0);
     }
-    // the next layer's element data goes in flight during the exchange
-    op.efetch(L, ex, ey, kz + 1, exy && kz + 1 < zend && kz + 1 <= L.nz - 2, ed);
     // x-direction: node (tx, .) takes corner 0 of its own element and corner 1
     // of element tx-1 (adjacent lane); g[c][iz][iy]
     T g[NC][2][2];
@@ -590,42 +645,47 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll
-      for (int iz = 0; iz < 2; iz++) sX[c * 2 + iz][tid] = g[c][iz][1];
+      for (int iz = 0; iz < 2; iz++) sX[xb][c * 2 + iz][tid] = g[c][iz][1];
     // values of this thread's own node of plane kz, read before the slot is reused
     T own[NC];
     if constexpr (Op::NEEDS_OWN) {
 #pragma unroll
-      for (int c = 0; c < NC; c++) own[c] = sN[kz & 1][c][tx + PX * ty];
+      for (int c = 0; c < NC; c++) own[c] = sN[s0][c][tx + PX * ty];
     }
     __syncthreads();
-    // plane kz is no longer read: it receives plane kz+2; plane kz+3 goes in flight
-    commit(kz & 1);
-    fetch(kz + 3);
+    // plane kz is no longer read: its slot receives plane kz+NSLOT and the
+    // next plane goes in flight.  With three slots no thread reads this slot
+    // before the next barrier, and the exchange buffer written in the next
+    // layer is the other one, so that barrier is the only one per layer.
+    commit(s0);
+    fetch(kz + NSLOT + 1);
     if (owner) {
       T vb[NC];
 #pragma unroll
       for (int c = 0; c < NC; c++) {
-        vb[c] = carry[c] + g[c][0][0] + sX[c * 2 + 0][tid - TX];
-        carry[c] = g[c][1][0] + sX[c * 2 + 1][tid - TX];
+        vb[c] = carry[c] + g[c][0][0] + sX[xb][c * 2 + 0][tid - TX];
+        carry[c] = g[c][1][0] + sX[xb][c * 2 + 1][tid - TX];
       }
       if (kz >= z0) {
         op.epilogue(L, own, ex, ey, kz, vb, st, nd, acc);
         op.nfetch(L, ex, ey, kz + 1, kz + 1 < zend, nd);
       }
     }
-    __syncthreads();
+    if constexpr (NSLOT == 2) __syncthreads();
   }
   op.finish(acc);
 }
 
-// the 8 corner values of element (tx, ty) from the two LDS planes
+// the 8 corner values of element (tx, ty) from the LDS planes of its bottom
+// (slot s0) and top (the next slot) nodes
 template <class T, int NV>
-__device__ __forceinline__ void gather8(const T (&sN)[2][NV][PLANE], int s0, int tx, int ty, int k,
+__device__ __forceinline__ void gather8(const T (&sN)[NSLOT][NV][PLANE], int s0, int tx, int ty, int k,
                                         T (&v)[2][2][2]) {
   const int q = tx + PX * ty;
+  const int s1 = (NSLOT == 3) ? (s0 == 2 ? 0 : s0 + 1) : (s0 ^ 1);
 #pragma unroll
   for (int iz = 0; iz < 2; iz++) {
-    const int s = s0 ^ iz;
+    const int s = iz ? s1 : s0;
     v[iz][0][0] = sN[s][k][q];
     v[iz][0][1] = sN[s][k][q + 1];
     v[iz][1][0] = sN[s][k][q + PX];
