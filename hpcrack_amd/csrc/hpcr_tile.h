@@ -264,16 +264,26 @@ __device__ __forceinline__ void grad8_t_add(const T (&Sx)[2][2], const T (&Sy)[2
     }
 }
 
+// true when any of the 8 values is 0 (+0 or -0)
+template <class T>
+__device__ __forceinline__ bool any_zero(const T (&v)[2][2][2]) {
+  bool z = false;
+#pragma unroll
+  for (int i = 0; i < 8; i++) z |= (v[i >> 2][(i >> 1) & 1][i & 1] == T(0));
+  return z;
+}
+
+// a product the compiler may not fuse into a neighbouring addition
+template <class T>
+__device__ __forceinline__ T mul_nc(T a, T b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 // gauss_point_MP (material_gauss_point.cpp:18-41): zero if any nodal value is 0
 template <class T>
 __device__ __forceinline__ void gp_mp(const T (&v)[2][2][2], T (&o)[2][2][2]) {
-  bool z = false;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int c = 0; c < 2; c++) z |= (v[a][b][c] == T(0));
+  const bool z = any_zero(v);
   interp8(v, o);
   if (z) {
 #pragma unroll
@@ -292,31 +302,37 @@ __device__ __forceinline__ T degr(T dg, T k) {
 }
 
 // Gauss-point Lame coefficients (lambda = K_d - 2/3 G_d, mu = G_d) of a fine
-// element (MP_all_GAUSS_Point, mechanical_pcg.cpp:26-61)
-// An element with a node at d == 0 has d = 0 at every Gauss point (the zero
-// rule), i.e. the undamaged factor degr(0, k) everywhere: that common case
-// skips the interpolation of d.
+// element (MP_all_GAUSS_Point, mechanical_pcg.cpp:26-61).
+// K_d = (aK + bK g) K_g and G_d = (aG + bG g) G_g with a, b in {0, 1}, so the
+// factors are exactly g, 1 or 0: the split flag picks which modulus degrades
+// by g = degr(d_g), and the zero rule (a nodal K or G of 0) makes the modulus 0
+// at every Gauss point.  The products are kept out of fused multiply-adds, so
+// the values equal the reference's selected ones.  An element with a node at
+// d == 0 has d = 0 at every Gauss point (the zero rule on d), i.e. the
+// undamaged factor degr(0, k) everywhere: that common case skips the
+// interpolation of d.
 template <class T>
 __device__ __forceinline__ void fine_lame(const T (&Kn)[2][2][2], const T (&Gn)[2][2][2], const T (&dn)[2][2][2],
                                           int flag, T small_k, T (&lam)[2][2][2], T (&mu)[2][2][2]) {
-  T Kg[2][2][2], Gg[2][2][2], g[2][2][2];
-  gp_mp(Kn, Kg);
-  gp_mp(Gn, Gg);
-  bool dzero = false;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int c = 0; c < 2; c++) dzero |= (dn[a][b][c] == T(0));
-  if (dzero) {
+  const bool zK = any_zero(Kn), zG = any_zero(Gn);
+  const T bK = (!zK && flag == 0) ? T(1) : T(0), aK = (!zK && flag != 0) ? T(1) : T(0);
+  const T bG = (!zG && flag != 2) ? T(1) : T(0), aG = (!zG && flag == 2) ? T(1) : T(0);
+  T Kg[2][2][2], Gg[2][2][2];
+  interp8(Kn, Kg);
+  interp8(Gn, Gg);
+  if (any_zero(dn)) {
     const T g0 = degr(T(0), small_k);
+    const T gK = aK + bK * g0, gG = aG + bG * g0;
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
       for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int c = 0; c < 2; c++) g[a][b][c] = g0;
+        for (int c = 0; c < 2; c++) {
+          const T Gd = mul_nc(gG, Gg[a][b][c]);
+          lam[a][b][c] = mul_nc(gK, Kg[a][b][c]) - (T(2) / T(3)) * Gd;
+          mu[a][b][c] = Gd;
+        }
   } else {
     T dg[2][2][2];
     interp8(dn, dg);
@@ -325,19 +341,13 @@ __device__ __forceinline__ void fine_lame(const T (&Kn)[2][2][2], const T (&Gn)[
 #pragma unroll
       for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int c = 0; c < 2; c++) g[a][b][c] = degr(dg[a][b][c], small_k);
+        for (int c = 0; c < 2; c++) {
+          const T g = degr(dg[a][b][c], small_k);
+          const T Gd = mul_nc(aG + bG * g, Gg[a][b][c]);
+          lam[a][b][c] = mul_nc(aK + bK * g, Kg[a][b][c]) - (T(2) / T(3)) * Gd;
+          mu[a][b][c] = Gd;
+        }
   }
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int c = 0; c < 2; c++) {
-        const T Kd = (flag == 0) ? g[a][b][c] * Kg[a][b][c] : Kg[a][b][c];
-        const T Gd = (flag == 2) ? Gg[a][b][c] : g[a][b][c] * Gg[a][b][c];
-        lam[a][b][c] = Kd - (T(2) / T(3)) * Gd;
-        mu[a][b][c] = Gd;
-      }
 }
 
 // y_e = K_e u_e for one element: u[c][iz][iy][ix] -> f[c][iz][iy][ix]
@@ -506,16 +516,17 @@ __device__ __forceinline__ int dir_mask(const LevelGeom& L, int i, int j, int k)
 // ---------------------------------------------------------------------------
 // Tile kernel framework
 // ---------------------------------------------------------------------------
-// lane i <- lane i-1 across the wavefront (DPP wave_shr:1); lane 0 gets 0.
-// Element (tx, ty) and its x-neighbour (tx-1, ty) are adjacent lanes because
-// TX divides the wavefront into whole rows; lanes with tx == 0 own no node.
+// lane i <- lane i-1 across the wavefront (DPP wave_shr:1); lane 0 gets 0
+// (bound_ctrl, so no old value has to be materialised).  Element (tx, ty) and
+// its x-neighbour (tx-1, ty) are adjacent lanes because TX divides the
+// wavefront into whole rows; lanes with tx == 0 own no node.
 __device__ __forceinline__ float wave_shr1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ double wave_shr1(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -618,19 +629,17 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
   for (int kz = z0 - 1; kz < zend; ++kz) {
     const int s0 = slot(kz);
     const int xb = (NXB == 2) ? (kz & 1) : 0;
+    // Every thread computes its element.  One outside the grid (a node off
+    // the grid, or layer -1 / nz-1) finds zeros in LDS for the off-grid nodes
+    // and for its element data, which zeroes all of its coefficients (the
+    // zero rule of gauss_point_MP on K, G and gc; K_e = G_e = H = 0 on coarse
+    // levels), so its contributions are exactly 0 without a branch.
     T f[NC][2][2][2];
-    if (exy && kz >= 0 && kz <= L.nz - 2) {
+    {
       TE e[NE];
 #pragma unroll
       for (int k = 0; k < NE; k++) e[k] = sE[s0][k][tid];
       op.element(L, sN, s0, tx, ty, ex, ey, kz, st, e, f);
-    } else {
-#pragma unroll
-      for (int c = 0; c < NC; c++)
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-          for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = T(0);
     }
     // x-direction: node (tx, .) takes corner 0 of its own element and corner 1
     // of element tx-1 (adjacent lane); g[c][iz][iy]
