@@ -83,7 +83,7 @@ struct MechCoef {
 // the mixed-precision solve).
 struct MixCoef {
   int fine;
-  const float* lm;  // finest level: (lambda, mu) at the 8 Gauss points, 16 arrays of N
+  const float* lm;  // finest level: (lambda, mu) at the 8 Gauss points, 16 floats per element (element-major)
   const float* Ke;  // coarse levels: element moduli
   const float* Ge;
 };

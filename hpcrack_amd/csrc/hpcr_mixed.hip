@@ -23,6 +23,7 @@
 // u += alpha p, r -= alpha q; the residual is recomputed from u every few
 // iterations and before convergence is accepted.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "hpcr_internal.h"
 #include "hpcr_tile.h"
@@ -47,8 +48,10 @@ struct MechOpF {
   struct State {};
   // finest level: (lambda, mu) at the 8 Gauss points, precomputed once per
   // solve; coarse levels: K_e, G_e
-  using TE = float;
-  static constexpr int NE = FINE ? 16 : 2;
+  // finest level: four float4 per element, (lambda, mu) of Gauss points
+  // 2k and 2k+1 in float4 k (16-byte loads and LDS accesses)
+  using TE = std::conditional_t<FINE, float4, float>;
+  static constexpr int NE = FINE ? 4 : 2;
   MixCoef C;
   const float* s;        // stencil vector
   float* dout;           // CHEB: new direction; POWER: D^-1 A s
@@ -72,14 +75,15 @@ struct MechOpF {
 #pragma unroll
     for (int c = 0; c < 3; c++) v[c] = s[c * N + q];
   }
-  __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, float (&e)[NE]) const {
+  __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, TE (&e)[NE]) const {
 #pragma unroll
-    for (int k = 0; k < NE; k++) e[k] = 0.0f;
+    for (int k = 0; k < NE; k++) e[k] = TE{};
     if (!ok) return;
     const size_t q = nidx(L, ex, ey, ez);
     if constexpr (FINE) {
+      const float4* p = reinterpret_cast<const float4*>(C.lm) + 4 * q;
 #pragma unroll
-      for (int k = 0; k < 16; k++) e[k] = C.lm[k * N + q];
+      for (int k = 0; k < 4; k++) e[k] = p[k];
     } else {
       e[0] = C.Ke[q];
       e[1] = C.Ge[q];
@@ -91,7 +95,7 @@ struct MechOpF {
   }
 
   __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[NSLOT][NV][PLANE], int s0, int tx,
-                                          int ty, int ex, int ey, int ez, const State&, const float (&e)[NE],
+                                          int ty, int ex, int ey, int ez, const State&, const TE (&e)[NE],
                                           float (&f)[3][2][2][2]) const {
     float u[3][2][2][2];
 #pragma unroll
@@ -99,9 +103,12 @@ struct MechOpF {
     if constexpr (FINE) {
       float lam[2][2][2], mu[2][2][2];
 #pragma unroll
-      for (int g = 0; g < 8; g++) {
-        lam[g >> 2][(g >> 1) & 1][g & 1] = e[2 * g];
-        mu[g >> 2][(g >> 1) & 1][g & 1] = e[2 * g + 1];
+      for (int k = 0; k < 4; k++) {
+        const int g0 = 2 * k, g1 = 2 * k + 1;
+        lam[g0 >> 2][(g0 >> 1) & 1][g0 & 1] = e[k].x;
+        mu[g0 >> 2][(g0 >> 1) & 1][g0 & 1] = e[k].y;
+        lam[g1 >> 2][(g1 >> 1) & 1][g1 & 1] = e[k].z;
+        mu[g1 >> 2][(g1 >> 1) & 1][g1 & 1] = e[k].w;
       }
       mech_element<false>(u, lam, mu, 0.0f, 0.0f, sx, sy, sz, tx_, ty_, tz_, f);
     } else {
@@ -522,7 +529,6 @@ __global__ __launch_bounds__(256) void mix_lame_kernel(const LevelGeom L, const 
                                                        float* __restrict__ lm) {
   const int ex = blockIdx.x * 64 + threadIdx.x, ey = blockIdx.y * 4 + threadIdx.y, ez = blockIdx.z;
   if (ex > L.nx - 2 || ey > L.ny - 2) return;
-  const size_t N = Nof(L);
   double Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -537,10 +543,13 @@ __global__ __launch_bounds__(256) void mix_lame_kernel(const LevelGeom L, const 
       }
   const size_t e = nidx(L, ex, ey, ez);
   fine_lame(Kn, Gn, dn, (int)flag[e], small_k, lam, mu);
+  // element-major: 16 floats per element, Gauss points 2k and 2k+1 in float4 k
+  float4* o = reinterpret_cast<float4*>(lm) + 4 * e;
 #pragma unroll
-  for (int g = 0; g < 8; g++) {
-    lm[(2 * g) * N + e] = (float)lam[g >> 2][(g >> 1) & 1][g & 1];
-    lm[(2 * g + 1) * N + e] = (float)mu[g >> 2][(g >> 1) & 1][g & 1];
+  for (int k = 0; k < 4; k++) {
+    const int g0 = 2 * k, g1 = 2 * k + 1;
+    o[k] = make_float4((float)lam[g0 >> 2][(g0 >> 1) & 1][g0 & 1], (float)mu[g0 >> 2][(g0 >> 1) & 1][g0 & 1],
+                       (float)lam[g1 >> 2][(g1 >> 1) & 1][g1 & 1], (float)mu[g1 >> 2][(g1 >> 1) & 1][g1 & 1]);
   }
 }
 void launch_mix_lame(hipStream_t st, const LevelGeom& L, const double* K, const double* G, const double* d,
