@@ -22,12 +22,19 @@ constexpr int PLANE = (TX + 1) * (TY + 1);    // 297 nodes per LDS plane (32x8 t
 static_assert(64 % TX == 0 && PLANE <= 2 * TX * TY, "tile rows must divide the wavefront");
 constexpr int ZCHUNK_MAX = 32;                // node planes per block (fine levels)
 // node planes held in LDS by a tile kernel: 3 lets one barrier per layer
-// order the plane and exchange buffers, 2 needs two (a tuning macro)
+// order the plane and exchange buffers, 2 needs two but less LDS (tuning
+// macros: the FP64 passes, the FP32 preconditioner passes)
 #ifndef HPCR_TILE_SLOTS
 #define HPCR_TILE_SLOTS 3
 #endif
+#ifndef HPCR_MIXF_SLOTS
+#define HPCR_MIXF_SLOTS 2
+#endif
+// minimum waves per SIMD the FP32 preconditioner passes are compiled for
+#ifndef HPCR_MIXF_WAVES
+#define HPCR_MIXF_WAVES 1
+#endif
 constexpr int NSLOT = HPCR_TILE_SLOTS;
-static_assert(NSLOT == 2 || NSLOT == 3, "two or three plane slots");
 // levels small enough for the one-workgroup smoother (small_pcg_kernel)
 constexpr int SMALL_NODES = 512, SMALL_ELEMS = 256;
 

@@ -53,6 +53,7 @@ struct MechOp {
   static constexpr int NK = FINE ? 3 : 0;              // K, G, d
   static constexpr int NV = (NU + NK) > 0 ? NU + NK : 1;
   static constexpr int NR = (MODE == M_PCGA) ? 6 + NK : NV;  // PCGA fetches z and p_old
+  static constexpr int NSL = NSLOT, MIN_WAVES = 1;  // LDS plane slots, occupancy target
   struct Acc {
     double a[3];
     __device__ Acc() { a[0] = a[1] = a[2] = 0.0; }
@@ -132,7 +133,7 @@ struct MechOp {
     }
   }
 
-  __device__ __forceinline__ void lame(const double (&sN)[NSLOT][NV][PLANE], int s0, int tx, int ty, const TE (&e)[NE],
+  __device__ __forceinline__ void lame(const double (&sN)[NSL][NV][PLANE], int s0, int tx, int ty, const TE (&e)[NE],
                                        double (&lam)[2][2][2], double (&mu)[2][2][2], double& lam0,
                                        double& mu0) const {
     if constexpr (FINE) {
@@ -157,7 +158,7 @@ struct MechOp {
     }
   }
 
-  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSLOT][NV][PLANE], int s0, int tx,
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSL][NV][PLANE], int s0, int tx,
                                           int ty, int ex, int ey, int ez, const State& st, const TE (&e)[NE],
                                           double (&f)[3][2][2][2]) const {
     double lam[2][2][2], mu[2][2][2], lam0, mu0;
@@ -267,6 +268,7 @@ struct PfOp {
   static constexpr int NU = (MODE == M_DIAG) ? 0 : 1;
   static constexpr int NV = NU + 1;  // + gc
   static constexpr int NR = (MODE == M_PCGA) ? 3 : NV;
+  static constexpr int NSL = NSLOT, MIN_WAVES = 1;
   struct Acc {
     double a[3];
     __device__ Acc() { a[0] = a[1] = a[2] = 0.0; }
@@ -322,7 +324,7 @@ struct PfOp {
     }
   }
 
-  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSLOT][NV][PLANE], int s0, int tx,
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSL][NV][PLANE], int s0, int tx,
                                           int ty, int ex, int ey, int ez, const State& st, const double (&e)[1],
                                           double (&f)[1][2][2][2]) const {
     double gcn[2][2][2];

@@ -41,6 +41,7 @@ struct MechOpF {
   static constexpr int NV = NU;
   static constexpr int NR = NV;
   static constexpr bool NEEDS_OWN = (MODE != F_RES);
+  static constexpr int NSL = HPCR_MIXF_SLOTS, MIN_WAVES = HPCR_MIXF_WAVES;
   struct Acc {
     double a[2];
     __device__ Acc() { a[0] = a[1] = 0.0; }
@@ -94,7 +95,7 @@ struct MechOpF {
     for (int k = 0; k < NV; k++) o[k] = v[k];
   }
 
-  __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[NSLOT][NV][PLANE], int s0, int tx,
+  __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[NSL][NV][PLANE], int s0, int tx,
                                           int ty, int ex, int ey, int ez, const State&, const TE (&e)[NE],
                                           float (&f)[3][2][2][2]) const {
     float u[3][2][2][2];
@@ -193,6 +194,7 @@ struct FcgOpA {
   static constexpr int NV = 6;
   static constexpr int NR = 9;
   static constexpr bool NEEDS_OWN = true;
+  static constexpr int NSL = NSLOT, MIN_WAVES = 1;
   struct Acc {
     double a[2];
     __device__ Acc() { a[0] = a[1] = 0.0; }
@@ -245,7 +247,7 @@ struct FcgOpA {
     w[4] = v[7];
     w[5] = v[8];
   }
-  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSLOT][NV][PLANE], int s0, int tx,
+  __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSL][NV][PLANE], int s0, int tx,
                                           int ty, int ex, int ey, int ez, const State&, const int (&e)[1],
                                           double (&f)[3][2][2][2]) const {
     double u[3][2][2][2], Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];

@@ -546,7 +546,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 #endif
 
 template <class Op>
-__global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeom L, const Op op) {
+__global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel(const LevelGeom L, const Op op) {
   constexpr int NC = Op::NC;  // output components
   constexpr int NV = Op::NV;  // values per node in LDS
   constexpr int NR = Op::NR;  // raw values per node fetched from HBM
@@ -555,11 +555,13 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
   constexpr int NE = Op::NE;  // per-element values (split flag, K_e/G_e, H, Gauss-point moduli)
   using TE = typename Op::TE;
   static_assert(NE >= 1, "tile ops carry per-element data");
-  __shared__ T sN[NSLOT][NV][PLANE];
+  constexpr int NS = Op::NSL;  // node-plane slots in LDS (2 or 3)
+  static_assert(NS == 2 || NS == 3, "two or three plane slots");
+  __shared__ T sN[NS][NV][PLANE];
   // per-element data travels with the node planes: layer p sits in slot(p)
-  __shared__ TE sE[NSLOT][NE][TPB];
+  __shared__ TE sE[NS][NE][TPB];
   // y-exchange buffers; with three plane slots they alternate between layers
-  constexpr int NXB = (NSLOT == 3) ? 2 : 1;
+  constexpr int NXB = (NS == 3) ? 2 : 1;
   __shared__ T sX[NXB][2 * NC][TPB];
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
@@ -604,7 +606,7 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
   };
 
   // LDS slot of node plane p >= -1
-  auto slot = [](int p) { return (NSLOT == 3) ? (p + 3) % 3 : (p & 1); };
+  auto slot = [](int p) { return (NS == 3) ? (p + 3) % 3 : (p & 1); };
 
   // inputs of the epilogue at this thread's own node, one plane ahead
   typename Op::NData nd;
@@ -615,13 +617,13 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
 #pragma unroll
   for (int c = 0; c < NC; c++) carry[c] = T(0);
 
-  // planes z0-1 .. z0+NSLOT-2 into LDS, the next one in flight
+  // planes z0-1 .. z0+NS-2 into LDS, the next one in flight
   fetch(z0 - 1);
   commit(slot(z0 - 1));
   fetch(z0);
   commit(slot(z0));
   fetch(z0 + 1);
-  if constexpr (NSLOT == 3) {
+  if constexpr (NS == 3) {
     commit(slot(z0 + 1));
     fetch(z0 + 2);
   }
@@ -662,12 +664,12 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
       for (int c = 0; c < NC; c++) own[c] = sN[s0][c][tx + PX * ty];
     }
     __syncthreads();
-    // plane kz is no longer read: its slot receives plane kz+NSLOT and the
+    // plane kz is no longer read: its slot receives plane kz+NS and the
     // next plane goes in flight.  With three slots no thread reads this slot
     // before the next barrier, and the exchange buffer written in the next
     // layer is the other one, so that barrier is the only one per layer.
     commit(s0);
-    fetch(kz + NSLOT + 1);
+    fetch(kz + NS + 1);
     if (owner) {
       T vb[NC];
 #pragma unroll
@@ -680,18 +682,18 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB) void tile_kernel(const LevelGeo
         op.nfetch(L, ex, ey, kz + 1, kz + 1 < zend, nd);
       }
     }
-    if constexpr (NSLOT == 2) __syncthreads();
+    if constexpr (NS == 2) __syncthreads();
   }
   op.finish(acc);
 }
 
 // the 8 corner values of element (tx, ty) from the LDS planes of its bottom
 // (slot s0) and top (the next slot) nodes
-template <class T, int NV>
-__device__ __forceinline__ void gather8(const T (&sN)[NSLOT][NV][PLANE], int s0, int tx, int ty, int k,
+template <class T, int NS, int NV>
+__device__ __forceinline__ void gather8(const T (&sN)[NS][NV][PLANE], int s0, int tx, int ty, int k,
                                         T (&v)[2][2][2]) {
   const int q = tx + PX * ty;
-  const int s1 = (NSLOT == 3) ? (s0 == 2 ? 0 : s0 + 1) : (s0 ^ 1);
+  const int s1 = (NS == 3) ? (s0 == 2 ? 0 : s0 + 1) : (s0 ^ 1);
 #pragma unroll
   for (int iz = 0; iz < 2; iz++) {
     const int s = iz ? s1 : s0;
