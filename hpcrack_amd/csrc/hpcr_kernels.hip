@@ -193,7 +193,7 @@ struct MechOp {
     }
   }
   // v: assembled values of node (i, j, k); own: the node's LDS values (p)
-  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k,
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k, int dm,
                                            const double (&v)[3], const State& st, const NData& nd,
                                            Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
@@ -201,7 +201,6 @@ struct MechOp {
 #pragma unroll
       for (int c = 0; c < 3; c++) out[c * N + q] = v[c];
     } else if constexpr (MODE == M_RESID) {
-      const int dm = dir_mask(L, i, j, k);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         double fv = nd.f[c];
@@ -352,7 +351,7 @@ struct PfOp {
       n.iD = invD[q];
     }
   }
-  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[1], int i, int j, int k,
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[1], int i, int j, int k, int dm,
                                            const double (&v)[1], const State& st, const NData& nd,
                                            Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
@@ -544,11 +543,21 @@ __global__ __launch_bounds__(256) void passb_finalize_kernel(const Reduce R, uns
   }
 }
 
+// LDSF: the Dirichlet axis flags are staged in LDS (levels with
+// nx + ny + nz <= DIRF_MAX), so a node's mask needs no dependent global load,
+// which would also wait on the stores still in flight.
+template <bool LDSF>
 __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, double* __restrict__ u,
                                                             double* __restrict__ r, double* __restrict__ z,
                                                             const double* __restrict__ p,
                                                             const double* __restrict__ w,
                                                             const double* __restrict__ invD, const Reduce R) {
+  __shared__ unsigned char sflags[LDSF ? DIRF_MAX : 1];
+  const DirFlags F{sflags};
+  if constexpr (LDSF) {
+    F.stage(L);
+    __syncthreads();
+  }
   // node counts stay below 2^32 (a 1625^3 grid would not fit in HBM)
   const unsigned N = (unsigned)Nof(L);
   const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
@@ -557,7 +566,11 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
   auto node = [&](unsigned q) {
     const unsigned k = q / plane;
     int dm = 0;
-    if (L.fz[k]) {
+    if constexpr (LDSF) {
+      const unsigned rem = q - k * plane;
+      const unsigned j = rem / (unsigned)L.nx, i = rem - j * (unsigned)L.nx;
+      dm = F.mask(L, (int)i, (int)j, (int)k);
+    } else if (L.fz[k]) {
       const unsigned rem = q - k * plane;
       const unsigned j = rem / (unsigned)L.nx, i = rem - j * (unsigned)L.nx;
       dm = dir_mask(L, (int)i, (int)j, (int)k);
@@ -622,7 +635,11 @@ __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __re
 void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const double* p,
                        const double* w, const double* invD, const Reduce& R) {
   const size_t N = (size_t)L.nx * L.ny * L.nz;
-  hipLaunchKernelGGL(mech_pcg_b_kernel, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, z, p, w, invD, R);
+  if (L.nx + L.ny + L.nz <= DIRF_MAX)
+    hipLaunchKernelGGL(mech_pcg_b_kernel<true>, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, z, p, w, invD, R);
+  else
+    hipLaunchKernelGGL(mech_pcg_b_kernel<false>, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, z, p, w, invD,
+                       R);
 #ifdef HPCR_PASSB_SPLIT
   hipLaunchKernelGGL(passb_finalize_kernel, dim3(1), dim3(256), 0, s, R, (unsigned)passb_blocks(N));
 #endif

@@ -134,11 +134,10 @@ struct MechOpF {
     }
   }
 
-  __device__ __forceinline__ void epilogue(const LevelGeom& L, const float (&own)[3], int i, int j, int k,
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const float (&own)[3], int i, int j, int k, int dm,
                                            const float (&v)[3], const State&, const NData& nd,
                                            Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
-    const int dm = dir_mask(L, i, j, k);
     if constexpr (MODE == F_CHEB) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -268,7 +267,7 @@ struct FcgOpA {
 #pragma unroll
     for (int c = 0; c < 3; c++) n.r[c] = r[c * N + o];
   }
-  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k,
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k, int dm,
                                            const double (&v)[3], const State&, const NData& nd,
                                            Acc& acc) const {
     const size_t o = nidx(L, i, j, k);
@@ -399,17 +398,33 @@ __global__ __launch_bounds__(MX_TPB) void mix_start_kernel(unsigned N, const dou
 }
 
 // FCG pass B (FP64): u += alpha p; r -= alpha q (masked); r.r; then the start
-// of the next preconditioner application from the new r
+// of the next preconditioner application from the new r.  LDSF: the
+// Dirichlet axis flags are staged in LDS (nx + ny + nz <= DIRF_MAX).
+template <bool LDSF>
 __global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double* __restrict__ u,
                                                        double* __restrict__ r, const double* __restrict__ p,
                                                        const double* __restrict__ q, float* __restrict__ r32,
                                                        float* __restrict__ d, float* __restrict__ x,
                                                        const float* __restrict__ invD, float cr0, const Reduce R) {
+  __shared__ unsigned char sflags[LDSF ? DIRF_MAX : 1];
+  const DirFlags F{sflags};
+  if constexpr (LDSF) {
+    F.stage(L);
+    __syncthreads();
+  }
   const unsigned N = (unsigned)Nof(L);
+  const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
   const double alpha = R.scal[S_FALPHA];
   double acc[1] = {0.0};
   for (unsigned n = blockIdx.x * MX_TPB + threadIdx.x; n < N; n += gridDim.x * MX_TPB) {
-    const int dm = node_mask(L, n);
+    int dm;
+    if constexpr (LDSF) {
+      const unsigned k = n / plane, rem = n - k * plane;
+      const unsigned j = rem / (unsigned)L.nx, i = rem - j * (unsigned)L.nx;
+      dm = F.mask(L, (int)i, (int)j, (int)k);
+    } else {
+      dm = node_mask(L, n);
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t o = (size_t)c * N + n;
@@ -594,8 +609,12 @@ void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, flo
 }
 void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, const double* p, const double* q,
                   float* r32, float* d, float* x, const float* invD, float cr0, const Reduce& R) {
-  hipLaunchKernelGGL(fcg_b_kernel, dim3(passb_blocks(Nof(L))), dim3(MX_TPB), 0, st, L, u, r, p, q, r32, d, x, invD,
-                     cr0, R);
+  if (L.nx + L.ny + L.nz <= DIRF_MAX)
+    hipLaunchKernelGGL(fcg_b_kernel<true>, dim3(passb_blocks(Nof(L))), dim3(MX_TPB), 0, st, L, u, r, p, q, r32, d, x,
+                       invD, cr0, R);
+  else
+    hipLaunchKernelGGL(fcg_b_kernel<false>, dim3(passb_blocks(Nof(L))), dim3(MX_TPB), 0, st, L, u, r, p, q, r32, d,
+                       x, invD, cr0, R);
 }
 void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,
                          float* dc, float* xc, const float* invD, float cr0, int start) {

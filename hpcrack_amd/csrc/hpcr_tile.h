@@ -501,17 +501,37 @@ __device__ __forceinline__ void pf_element_diag(const double (&m)[2][2][2], cons
 // Dirichlet DOF test of the reference's boundary conditions
 // (boundary_conditions.cpp:28-66): returns a mask, bit c set when component
 // c is prescribed; bit 3 marks u_z = Ut (top face) rather than 0.
-__device__ __forceinline__ int dir_mask(const LevelGeom& L, int i, int j, int k) {
-  const unsigned fz = L.fz[k];
-  if (fz == 0) return 0;
+// From the node's three axis flags:
+__device__ __forceinline__ int dir_mask_bits(unsigned fz, unsigned fx, unsigned fy) {
   int m = 0;
-  const unsigned fx = L.fx[i], fy = L.fy[j];
   if ((fz & 1) && (fx & 1) && (fy & 1)) m |= 1;
   if ((fz & 1) && (fx & 2) && (fy & 2)) m |= 2;
   if (fz & 1) m |= 4;
   if (fz & 2) m |= 4 | 8;
   return m;
 }
+__device__ __forceinline__ int dir_mask(const LevelGeom& L, int i, int j, int k) {
+  const unsigned fz = L.fz[k];
+  if (fz == 0) return 0;
+  return dir_mask_bits(fz, L.fx[i], L.fy[j]);
+}
+
+// The axis flags of a level staged in LDS (nx + ny + nz bytes), so that a
+// node's mask costs LDS reads rather than dependent global loads.  All
+// threads of the block call stage(); the caller synchronises.
+constexpr int DIRF_MAX = 12288;
+struct DirFlags {
+  unsigned char* s;
+  __device__ __forceinline__ void stage(const LevelGeom& L) const {
+    const int n = L.nx + L.ny + L.nz;
+    for (int t = threadIdx.x; t < n; t += blockDim.x)
+      s[t] = t < L.nx ? L.fx[t] : (t < L.nx + L.ny ? L.fy[t - L.nx] : L.fz[t - L.nx - L.ny]);
+  }
+  __device__ __forceinline__ int mask(const LevelGeom& L, int i, int j, int k) const {
+    const unsigned fz = s[L.nx + L.ny + k];
+    return fz ? dir_mask_bits(fz, s[i], s[L.nx + j]) : 0;
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Tile kernel framework
@@ -605,6 +625,12 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel
     }
   };
 
+  // Dirichlet flags of the owner node: x and y once, the chunk's z flags in
+  // LDS (visible after the prologue's barrier)
+  __shared__ unsigned char sFz[ZCHUNK_MAX];
+  if (tid < zend - z0) sFz[tid] = L.fz[z0 + tid];
+  const unsigned fxo = owner ? L.fx[ex] : 0u, fyo = owner ? L.fy[ey] : 0u;
+
   // LDS slot of node plane p >= -1
   auto slot = [](int p) { return (NS == 3) ? (p + 3) % 3 : (p & 1); };
 
@@ -678,7 +704,9 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel
         carry[c] = g[c][1][0] + sX[xb][c * 2 + 1][tid - TX];
       }
       if (kz >= z0) {
-        op.epilogue(L, own, ex, ey, kz, vb, st, nd, acc);
+        const unsigned fz = sFz[kz - z0];
+        const int dm = fz ? dir_mask_bits(fz, fxo, fyo) : 0;
+        op.epilogue(L, own, ex, ey, kz, dm, vb, st, nd, acc);
         op.nfetch(L, ex, ey, kz + 1, kz + 1 < zend, nd);
       }
     }
