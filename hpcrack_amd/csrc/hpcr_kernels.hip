@@ -53,6 +53,7 @@ struct MechOp {
   static constexpr int NK = FINE ? 3 : 0;              // K, G, d
   static constexpr int NV = (NU + NK) > 0 ? NU + NK : 1;
   static constexpr int NR = (MODE == M_PCGA) ? 6 + NK : NV;  // PCGA fetches z and p_old
+  using Raw = double[NR];
   static constexpr int NSL = NSLOT, MIN_WAVES = 1;  // LDS plane slots, occupancy target
   struct Acc {
     double a[3];
@@ -181,9 +182,8 @@ struct MechOp {
   struct NData {
     double f[MODE == M_RESID ? 3 : 1], iD[MODE == M_RESID ? 3 : 1];
   };
-  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, bool ok, NData& n) const {
+  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
     if constexpr (MODE == M_RESID) {
-      if (!ok) return;
       const size_t q = nidx(L, i, j, k);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -267,6 +267,7 @@ struct PfOp {
   static constexpr int NU = (MODE == M_DIAG) ? 0 : 1;
   static constexpr int NV = NU + 1;  // + gc
   static constexpr int NR = (MODE == M_PCGA) ? 3 : NV;
+  using Raw = double[NR];
   static constexpr int NSL = NSLOT, MIN_WAVES = 1;
   struct Acc {
     double a[3];
@@ -343,9 +344,8 @@ struct PfOp {
   struct NData {
     double f, iD;
   };
-  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, bool ok, NData& n) const {
+  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
     if constexpr (MODE == M_RESID) {
-      if (!ok) return;
       const size_t q = nidx(L, i, j, k);
       n.f = out2[q];
       n.iD = invD[q];

@@ -30,7 +30,8 @@
 
 namespace hpcr {
 
-enum FMode { F_CHEB = 0, F_RES = 1, F_POWER = 2 };
+// F_CHEBQ: a Chebyshev step that also accumulates x.q_prev into S_ZQ
+enum FMode { F_CHEB = 0, F_RES = 1, F_POWER = 2, F_CHEBQ = 3 };
 
 template <int MODE, bool FINE>
 struct MechOpF {
@@ -42,6 +43,7 @@ struct MechOpF {
   static constexpr int NR = NV;
   static constexpr bool NEEDS_OWN = (MODE != F_RES);
   static constexpr int NSL = HPCR_MIXF_SLOTS, MIN_WAVES = HPCR_MIXF_WAVES;
+  using Raw = float[NR];
   struct Acc {
     double a[2];
     __device__ Acc() { a[0] = a[1] = 0.0; }
@@ -120,17 +122,19 @@ struct MechOpF {
   }
 
   // the epilogue's inputs at the owner's node, read one plane ahead
+  static constexpr bool CHEB = (MODE == F_CHEB || MODE == F_CHEBQ);
   struct NData {
     float r[3], x[3], iD[3];
+    double qp[MODE == F_CHEBQ ? 3 : 1];
   };
-  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, bool ok, NData& n) const {
-    if (!ok) return;
+  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
     const size_t q = nidx(L, i, j, k);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       if constexpr (MODE != F_POWER) n.r[c] = r[c * N + q];
-      if constexpr (MODE == F_CHEB) n.x[c] = x[c * N + q];
+      if constexpr (CHEB) n.x[c] = x[c * N + q];
       if constexpr (MODE != F_RES) n.iD[c] = invD[c * N + q];
+      if constexpr (MODE == F_CHEBQ) n.qp[c] = qprev[c * N + q];
     }
   }
 
@@ -138,7 +142,7 @@ struct MechOpF {
                                            const float (&v)[3], const State&, const NData& nd,
                                            Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
-    if constexpr (MODE == F_CHEB) {
+    if constexpr (CHEB) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t o = c * N + q;
@@ -148,7 +152,7 @@ struct MechOpF {
         r[o] = rn;
         dout[o] = dn;
         x[o] = xn;
-        if (qprev) acc.a[0] += (double)xn * qprev[o];
+        if constexpr (MODE == F_CHEBQ) acc.a[0] += (double)xn * nd.qp[c];
       }
     } else if constexpr (MODE == F_RES) {
 #pragma unroll
@@ -173,8 +177,7 @@ struct MechOpF {
 
   __device__ __forceinline__ void finish(Acc& acc) const {
     double* scal = R.scal;
-    if constexpr (MODE == F_CHEB) {
-      if (!qprev) return;
+    if constexpr (MODE == F_CHEBQ) {
       double v1[1] = {acc.a[0]};
       reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&t)[1]) { scal[S_ZQ] = t[0]; });
     } else if constexpr (MODE == F_POWER) {
@@ -194,6 +197,12 @@ struct FcgOpA {
   static constexpr int NR = 9;
   static constexpr bool NEEDS_OWN = true;
   static constexpr int NSL = NSLOT, MIN_WAVES = 1;
+  // raw node values in flight: z stays FP32 until the commit, so that no
+  // conversion consumes a load before the layer that needs it
+  struct Raw {
+    float z[3];
+    double p[3], K, G, d;
+  };
   struct Acc {
     double a[2];
     __device__ Acc() { a[0] = a[1] = 0.0; }
@@ -221,30 +230,34 @@ struct FcgOpA {
     return st;
   }
   __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State&,
-                                        double (&v)[NR]) const {
+                                        Raw& v) const {
 #pragma unroll
-    for (int k = 0; k < NR; k++) v[k] = 0.0;
+    for (int c = 0; c < 3; c++) {
+      v.z[c] = 0.0f;
+      v.p[c] = 0.0;
+    }
+    v.K = v.G = v.d = 0.0;
     if (!ok) return;
     const size_t o = nidx(L, gx, gy, gz);
 #pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = (double)z[c * N + o];
+    for (int c = 0; c < 3; c++) v.z[c] = z[c * N + o];
     if (!first) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) v[3 + c] = p_old[c * N + o];
+      for (int c = 0; c < 3; c++) v.p[c] = p_old[c * N + o];
     }
-    v[6] = C.K[o];
-    v[7] = C.G[o];
-    v[8] = C.d[o];
+    v.K = C.K[o];
+    v.G = C.G[o];
+    v.d = C.d[o];
   }
   __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, int (&e)[1]) const {
     e[0] = ok ? (int)C.flag[nidx(L, ex, ey, ez)] : 0;
   }
-  __device__ __forceinline__ void commit(const double (&v)[NR], const State& st, double (&w)[NV]) const {
+  __device__ __forceinline__ void commit(const Raw& v, const State& st, double (&w)[NV]) const {
 #pragma unroll
-    for (int c = 0; c < 3; c++) w[c] = v[c] + st.beta * v[3 + c];
-    w[3] = v[6];
-    w[4] = v[7];
-    w[5] = v[8];
+    for (int c = 0; c < 3; c++) w[c] = (double)v.z[c] + st.beta * v.p[c];
+    w[3] = v.K;
+    w[4] = v.G;
+    w[5] = v.d;
   }
   __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSL][NV][PLANE], int s0, int tx,
                                           int ty, int ex, int ey, int ez, const State&, const int (&e)[1],
@@ -261,8 +274,7 @@ struct FcgOpA {
   struct NData {
     double r[3];
   };
-  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, bool ok, NData& n) const {
-    if (!ok) return;
+  __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
     const size_t o = nidx(L, i, j, k);
 #pragma unroll
     for (int c = 0; c < 3; c++) n.r[c] = r[c * N + o];
@@ -328,7 +340,10 @@ static void run_f32_level(hipStream_t st, const LevelGeom& L, const MixCoef& C, 
 
 void launch_mix_cheb(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout, float* r,
                      float* x, const float* invD, float cd, float cr, const double* qprev, const Reduce& R) {
-  run_f32_level<F_CHEB>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+  if (qprev)
+    run_f32_level<F_CHEBQ>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+  else
+    run_f32_level<F_CHEB>(st, L, C, s, dout, r, x, invD, cd, cr, nullptr, R);
 }
 void launch_mix_res(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* r) {
   Reduce R{};

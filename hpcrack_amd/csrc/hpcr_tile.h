@@ -64,6 +64,9 @@ __device__ __forceinline__ unsigned lin_block() {
 }
 __device__ __forceinline__ unsigned num_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
 
+#ifndef HPCR_TICKET_FENCE
+#define HPCR_TICKET_FENCE 0
+#endif
 // Block partials -> last block -> fin(result[NV]) run by thread 0 of the last block.
 template <int OP, int NV, class Fin>
 __device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R, double init, Fin fin) {
@@ -75,7 +78,15 @@ __device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R
     const unsigned b = lin_block();
     for (int k = 0; k < NV; k++)
       __hip_atomic_store(&R.partials[(size_t)b * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#if HPCR_TICKET_FENCE
     __threadfence();
+#else
+    // The partials are agent-scope atomic stores, which write through the
+    // XCD's L2; waiting for their completion orders them before the ticket.
+    // A release fence here would also write back every dirty line of the L2
+    // (all of the pass's streaming stores), once per block.
+    __builtin_amdgcn_s_waitcnt(0);
+#endif
     unsigned t = atomicAdd(R.counter, 1u);
     is_last = (t == nb - 1);
   }
@@ -602,7 +613,7 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel
   const int gx2 = bx0 - 1 + q2 % PX, gy2 = by0 - 1 + q2 / PX;
   const bool in1 = gx1 >= 0 && gy1 >= 0 && gx1 < L.nx && gy1 < L.ny;
   const bool in2 = has2 && gx2 >= 0 && gy2 >= 0 && gx2 < L.nx && gy2 < L.ny;
-  TR r1[NR], r2[NR];
+  typename Op::Raw r1, r2;  // raw node values in flight (Op::Raw is TR[NR] or an op's own record)
   TE re[NE];
   // node plane pz and the element layer above it (planes past zend are not needed)
   auto fetch = [&](int pz) {
@@ -634,9 +645,13 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel
   // LDS slot of node plane p >= -1
   auto slot = [](int p) { return (NS == 3) ? (p + 3) % 3 : (p & 1); };
 
-  // inputs of the epilogue at this thread's own node, one plane ahead
+  // Inputs of the epilogue at this thread's own node, read one plane ahead.
+  // The load is unconditional, from a clamped (always valid) node, so the
+  // registers carried into the next layer never need a merge copy that
+  // would wait for the load; threads and planes without an epilogue ignore
+  // what they read.
   typename Op::NData nd;
-  if (owner) op.nfetch(L, ex, ey, z0, z0 < zend, nd);
+  const int exc = min(max(ex, 0), L.nx - 1), eyc = min(max(ey, 0), L.ny - 1);
 
   typename Op::Acc acc;
   T carry[NC];
@@ -707,9 +722,9 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel
         const unsigned fz = sFz[kz - z0];
         const int dm = fz ? dir_mask_bits(fz, fxo, fyo) : 0;
         op.epilogue(L, own, ex, ey, kz, dm, vb, st, nd, acc);
-        op.nfetch(L, ex, ey, kz + 1, kz + 1 < zend, nd);
       }
     }
+    op.nfetch(L, exc, eyc, min(kz + 1, L.nz - 1), nd);
     if constexpr (NS == 2) __syncthreads();
   }
   op.finish(acc);
