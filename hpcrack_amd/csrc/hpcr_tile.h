@@ -92,7 +92,11 @@ __device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R
   }
   __syncthreads();
   if (!is_last) return;
+#if HPCR_TICKET_FENCE
   __threadfence();
+#endif
+  // the partial loads below are agent-scope atomic loads, issued after the
+  // last ticket: every block's partials completed before its ticket
   double acc[NV];
   for (int k = 0; k < NV; k++) acc[k] = init;
   for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
