@@ -155,8 +155,9 @@ __device__ __forceinline__ void interp8(const T (&v)[2][2][2], T (&o)[2][2][2]) 
 #pragma unroll
     for (int gx = 0; gx < 2; gx++) lerp2(s[0][gy][gx], s[1][gy][gx], o[0][gy][gx], o[1][gy][gx]);
 }
-// transpose of interp8: nodal f[iz][iy][ix] += sum_g L L L q[gz][gy][gx]
-template <class T>
+// transpose of interp8: nodal f[iz][iy][ix] (+)= sum_g L L L q[gz][gy][gx]
+// (SET assigns instead of adding to f)
+template <bool SET = false, class T>
 __device__ __forceinline__ void interp8_t_add(const T (&q)[2][2][2], T (&f)[2][2][2]) {
   T s[2][2][2], t[2][2][2];
 #pragma unroll
@@ -173,8 +174,13 @@ __device__ __forceinline__ void interp8_t_add(const T (&q)[2][2][2], T (&f)[2][2
     for (int iy = 0; iy < 2; iy++) {
       T a, b;
       lerp2(t[iz][iy][0], t[iz][iy][1], a, b);
-      f[iz][iy][0] += a;
-      f[iz][iy][1] += b;
+      if constexpr (SET) {
+        f[iz][iy][0] = a;
+        f[iz][iy][1] = b;
+      } else {
+        f[iz][iy][0] += a;
+        f[iz][iy][1] += b;
+      }
     }
 }
 // bilinear interpolation of 4 edge values e[a][b] (a: outer axis, b: inner
@@ -248,18 +254,26 @@ __device__ __forceinline__ void grad8(const T (&v)[2][2][2], T sx, T sy, T sz, T
   interp4(ey, sy, gyv);
   interp4(ez, sz, gzv);
 }
-// Transposed gradient: f[iz][iy][ix] += sum_g dN/dx_j * S_j
-template <class T>
-__device__ __forceinline__ void grad8_t_add(const T (&Sx)[2][2], const T (&Sy)[2][2], const T (&Sz)[2][2], T sx,
-                                            T sy, T sz, T (&f)[2][2][2]) {
+// Transposed gradient: f[iz][iy][ix] = sum_g dN/dx_j * S_j (SET) or
+// f += (SET false).  Assigning the x part instead of adding it to 0 saves
+// the additions of zero, which the compiler keeps (0 + x is +0 for x = -0);
+// every other value is the same.
+template <bool SET, class T>
+__device__ __forceinline__ void grad8_t(const T (&Sx)[2][2], const T (&Sy)[2][2], const T (&Sz)[2][2], T sx, T sy,
+                                        T sz, T (&f)[2][2][2]) {
   T R[2][2];
   interp4_t(Sx, sx, R);  // R[iz][iy]
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) {
-      f[a][b][1] += R[a][b];
-      f[a][b][0] -= R[a][b];
+      if constexpr (SET) {
+        f[a][b][1] = R[a][b];
+        f[a][b][0] = -R[a][b];
+      } else {
+        f[a][b][1] += R[a][b];
+        f[a][b][0] -= R[a][b];
+      }
     }
   interp4_t(Sy, sy, R);  // R[iz][ix]
 #pragma unroll
@@ -277,6 +291,11 @@ __device__ __forceinline__ void grad8_t_add(const T (&Sx)[2][2], const T (&Sy)[2
       f[1][a][b] += R[a][b];
       f[0][a][b] -= R[a][b];
     }
+}
+template <class T>
+__device__ __forceinline__ void grad8_t_add(const T (&Sx)[2][2], const T (&Sy)[2][2], const T (&Sz)[2][2], T sx,
+                                            T sy, T sz, T (&f)[2][2][2]) {
+  grad8_t<false>(Sx, Sy, Sz, sx, sy, sz, f);
 }
 
 // true when any of the 8 values is 0 (+0 or -0)
@@ -374,13 +393,12 @@ __device__ __forceinline__ void mech_element(const T (&u)[3][2][2][2], const T (
   T dx[3][2][2], dy[3][2][2], dz[3][2][2];
 #pragma unroll
   for (int c = 0; c < 3; c++) grad8(u[c], sx, sy, sz, dx[c], dy[c], dz[c]);
+  // Sx sums over gx, Sy over gy, Sz over gz: the first Gauss point of each
+  // sum assigns, the second adds (no additions of zero)
   T Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++) Sx[c][a][b] = Sy[c][a][b] = Sz[c][a][b] = T(0);
+  auto put = [](T& s, T v, bool first) {
+    if (first) s = v; else s += v;
+  };
 #pragma unroll
   for (int gz = 0; gz < 2; gz++)
 #pragma unroll
@@ -397,18 +415,12 @@ __device__ __forceinline__ void mech_element(const T (&u)[3][2][2][2], const T (
         const T lt = l * tr, m2 = T(2) * m;
         const T sxx = lt + m2 * exx, syy = lt + m2 * eyy, szz = lt + m2 * ezz;
         const T sxy = m * gxy, sxz = m * gxz, syz = m * gyz;
-        Sx[0][gz][gy] += sxx; Sx[1][gz][gy] += sxy; Sx[2][gz][gy] += sxz;
-        Sy[0][gz][gx] += sxy; Sy[1][gz][gx] += syy; Sy[2][gz][gx] += syz;
-        Sz[0][gy][gx] += sxz; Sz[1][gy][gx] += syz; Sz[2][gy][gx] += szz;
+        put(Sx[0][gz][gy], sxx, gx == 0); put(Sx[1][gz][gy], sxy, gx == 0); put(Sx[2][gz][gy], sxz, gx == 0);
+        put(Sy[0][gz][gx], sxy, gy == 0); put(Sy[1][gz][gx], syy, gy == 0); put(Sy[2][gz][gx], syz, gy == 0);
+        put(Sz[0][gy][gx], sxz, gz == 0); put(Sz[1][gy][gx], syz, gz == 0); put(Sz[2][gy][gx], szz, gz == 0);
       }
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++) f[c][a][b][0] = f[c][a][b][1] = T(0);
-    grad8_t_add(Sx[c], Sy[c], Sz[c], tx, ty, tz, f[c]);
-  }
+  for (int c = 0; c < 3; c++) grad8_t<true>(Sx[c], Sy[c], Sz[c], tx, ty, tz, f[c]);
 }
 
 // Jacobi diagonal of one element (PreM_u_element, mechanical_pcg.cpp:63-100)
@@ -470,11 +482,8 @@ __device__ __forceinline__ void pf_element(const double (&v)[2][2][2], const dou
 #pragma unroll
     for (int b = 0; b < 2; b++)
 #pragma unroll
-      for (int c = 0; c < 2; c++) {
-        q[a][b][c] = m[a][b][c] * dv[a][b][c];
-        f[a][b][c] = 0.0;
-      }
-  interp8_t_add(q, f);
+      for (int c = 0; c < 2; c++) q[a][b][c] = m[a][b][c] * dv[a][b][c];
+  interp8_t_add<true>(q, f);
   double gxv[2][2], gyv[2][2], gzv[2][2];
   grad8(v, sx, sy, sz, gxv, gyv, gzv);
   double Sx[2][2], Sy[2][2], Sz[2][2];
