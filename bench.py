@@ -35,6 +35,12 @@ PASS_B_BYTES_PER_NODE = 192.0
 # smoother residual r = f - A u, z = r/D: read u, f, 1/D (3 x 24), K, G, d, flag
 # (25); write r, z (2 x 24)
 RESID_BYTES_PER_NODE = 145.0
+# canonical bytes of one Jacobi-PCG iteration (pass A + pass B, 313 B per fine
+# node) and of one mechanical V-cycle with nu_u = (50, 4, 2): 2144 B per fine
+# node on the finest level (6 iterations, 2 residual set-ups, transfers) plus
+# 322 on all coarser levels
+PCG_ITERATION_BYTES_PER_NODE = 313.0
+VCYCLE_BYTES_PER_FINE_NODE = 2466.0
 METRIC = "MDOF·iter/s + solves/s, FP32-MG vs FP64 @1e-8 residual; p50/p99 serve latency"
 
 
@@ -198,6 +204,18 @@ def main():
         "algorithmic_bytes_per_launch": PASS_A_BYTES_PER_NODE * N,
         "fine_level_share": round(sum(k["total_ms"] for k in kern.values()) / (elapsed * 1e3), 3),
     }
+    # the same accounting over a whole PCG iteration (pass A + pass B) and over
+    # the whole V-cycle (canonical bytes over the measured time per cycle,
+    # set-up of each solve included)
+    b = kern["mech_pcg_b"]
+    if b["avg_ms"]:
+        it_gbs = PCG_ITERATION_BYTES_PER_NODE * N / ((a["avg_ms"] + b["avg_ms"]) / 1e3) / 1e9
+        roofline["pcg_iteration"] = {"bytes_per_node": PCG_ITERATION_BYTES_PER_NODE, "achieved": round(it_gbs, 1),
+                                     "frac": round(it_gbs / HBM_PEAK_GBS, 4)}
+    if cycles:
+        vc_gbs = VCYCLE_BYTES_PER_FINE_NODE * N / (elapsed / cycles) / 1e9
+        roofline["vcycle"] = {"bytes_per_fine_node": VCYCLE_BYTES_PER_FINE_NODE, "achieved": round(vc_gbs, 1),
+                              "frac": round(vc_gbs / HBM_PEAK_GBS, 4), "ms_per_cycle": round(elapsed / cycles * 1e3, 3)}
     value = 3 * N * cycles / elapsed / 1e6
 
     mixed = None

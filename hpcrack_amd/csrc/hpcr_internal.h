@@ -108,6 +108,7 @@ struct PfCoef {
   const double* H;   // element-indexed
   const double* gc;  // nodal
   double lc;
+  double rlc;        // 1 / lc
 };
 
 // z-chunk of the tile kernels: the longest chunk that still gives >= 1024

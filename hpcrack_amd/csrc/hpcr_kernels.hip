@@ -330,7 +330,7 @@ struct PfOp {
     double gcn[2][2][2];
     gather8(sN, s0, tx, ty, NU, gcn);
     double m[2][2][2], k[2][2][2];
-    pf_coefs(gcn, e[0], C.lc, wg, m, k);
+    pf_coefs(gcn, e[0], C.lc, C.rlc, wg, m, k);
     if constexpr (MODE == M_DIAG) {
       pf_element_diag(m, k, ihx2, ihy2, ihz2, f[0]);
     } else {
@@ -736,7 +736,7 @@ struct SmallArgsT {
   const double* Ge;
   const double* H;
   const double* gc;
-  double lc;
+  double lc, rlc;      // lc and 1 / lc
   TIO* x;              // u (3 comps) or d
   const TIO* f;        // right-hand side
   const double* invD;  // inverse Jacobi diagonal
@@ -787,7 +787,7 @@ __global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgsT<T
         for (int iy = 0; iy < 2; iy++)
 #pragma unroll
           for (int ix = 0; ix < 2; ix++) gcn[iz][iy][ix] = a.gc[nidx(L, ei + ix, ej + iy, ek + iz)];
-      pf_coefs(gcn, a.H[e], a.lc, wg, pm, pk);
+      pf_coefs(gcn, a.H[e], a.lc, a.rlc, wg, pm, pk);
     }
   }
 
@@ -1022,6 +1022,7 @@ void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, cons
   a.H = H;
   a.gc = gc;
   a.lc = lc;
+  a.rlc = 1.0 / lc;
   a.x = d;
   a.f = fd;
   a.invD = invD;

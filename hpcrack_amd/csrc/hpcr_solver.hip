@@ -153,7 +153,7 @@ MechCoef mcoef(hpcr_ctx* c, int l) {
   m.small_k = c->par.small_k;
   return m;
 }
-PfCoef pcoef(hpcr_ctx* c, int l) { return PfCoef{c->lev[l].H, c->lev[l].gc, c->par.lc}; }
+PfCoef pcoef(hpcr_ctx* c, int l) { return PfCoef{c->lev[l].H, c->lev[l].gc, c->par.lc, 1.0 / c->par.lc}; }
 
 int read_scal(hpcr_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->host_scal, c->scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
