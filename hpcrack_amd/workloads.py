@@ -166,3 +166,41 @@ def rotation_mode(w, level=None):
 
 def label_counts(img):
     return {int(v): int(c) for v, c in zip(*np.unique(img, return_counts=True))}
+
+
+def write_case(w, outdir):
+    """The workload as input files of the C++ driver (hpcrack_run): the image
+    as x-fastest uint8 img.raw, the material table as mat.csv (the reference's
+    Material_Property.csv layout) and config.txt (the reference's config
+    keys, config_reader order)."""
+    import os
+    os.makedirs(outdir, exist_ok=True)
+    img, _ = w.image()
+    hio.write_raw(os.path.join(outdir, "img.raw"), img)
+    E = list(E_PA)
+    if w.incl_E > 0:
+        E[1] = w.incl_E
+    with open(os.path.join(outdir, "mat.csv"), "w") as f:
+        f.write("Gray level," + ",".join(str(g) for g in GRAY) + "\n")
+        f.write("E/Pa," + ",".join(repr(float(e)) for e in E) + "\n")
+        f.write("nu," + ",".join(repr(float(v)) for v in NU) + "\n")
+        f.write("gc/(N/m)," + ",".join(repr(float(g)) for g in GC) + "\n")
+    nx, ny, nz = w.dims
+    mu, md = w.mg_u, w.mg_d
+    with open(os.path.join(outdir, "config.txt"), "w") as f:
+        f.write("D=3\nnp=1,1,1\nX_start=0,0,0\nIMGname=%s\n" % os.path.abspath(os.path.join(outdir, "img.raw")))
+        f.write("voxel_nb_X=%d\nvoxel_nb_Y=%d\nvoxel_nb_Z=%d\n" % (nx, ny, nz))
+        f.write("ROI_start_X=0\nROI_start_Y=0\nROI_start_Z=0\nvoxel_size=%r\n" % w.voxel_size)
+        f.write("element_nb=%d,%d,%d\n" % tuple(w.element_nb))
+        f.write("ncy_u=%d\nnu0_u=%d\nnu1_u=%d\nnu2_u=%d\ngamma_u=%d\n" % (mu[4], mu[0], mu[1], mu[2], mu[3]))
+        f.write("ncy_d=%d\nnu0_d=%d\nnu1_d=%d\nnu2_d=%d\ngamma_d=%d\n" % (md[4], md[0], md[1], md[2], md[3]))
+        f.write("lc=%d\n" % int(w.lc_voxels))
+    return img
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) != 4 or sys.argv[1] != "write-case" or sys.argv[2] not in WORKLOADS:
+        print("usage: python -m hpcrack_amd.workloads write-case {%s} OUTDIR" % ",".join(WORKLOADS), file=sys.stderr)
+        sys.exit(2)
+    write_case(WORKLOADS[sys.argv[2]], sys.argv[3])
