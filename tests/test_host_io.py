@@ -174,3 +174,34 @@ def test_step_state_round_trip_is_exact(hio, tmp_path):
     (tmp_path / "state_t_12.txt").write_text("t 12\nUt 1\n")
     with pytest.raises(hio.IoError):
         hio.read_step_state(str(tmp_path), 12)
+
+
+def config_seen_all(hio, c):
+    return {"D", "np", "X_start", "IMGname", "voxel_size", "element_nb", "ncy_u", "ncy_d", "lc"} <= hio.config_seen(c)
+
+
+def test_workload_case_files_read_back_as_the_workload(tmp_path, hio):
+    """write_case (python -m hpcrack_amd.workloads write-case) writes what the
+    driver's readers parse back into the workload's geometry, multigrid
+    parameters, material table and image."""
+    import hpcrack_amd.workloads as W
+    w = W.scaled(W.C2, 33, "C2@33")
+    img = W.write_case(w, str(tmp_path))
+    c = hio.read_config(str(tmp_path / "config.txt"))  # strict: every key parsed
+    g, lc = hio.config_geometry(c)
+    assert g.maxlevel == w.maxlevel and list(g.element_nb) == list(w.element_nb)
+    assert (g.xa, g.xb, g.ya, g.yb, g.za, g.zb) == w.domain
+    assert lc == w.lc_voxels
+    mg = lambda m: (m.nu0, m.nu1, m.nu2, m.gamma, m.ncy)
+    assert mg(c.mg_u) == tuple(w.mg_u) and mg(c.mg_d) == tuple(w.mg_d)
+    assert config_seen_all(hio, c)
+    t = hio.read_material_csv(str(tmp_path / "mat.csv"))
+    ref = w.table()
+    assert t.n == ref.n
+    for k in range(t.n):
+        assert (t.gray[k], t.E[k], t.nu[k], t.gc[k]) == (ref.gray[k], ref.E[k], ref.nu[k], ref.gc[k])
+    got = hio.read_raw(str(tmp_path / "img.raw"), w.dims, (0, 0, 0), w.dims)
+    assert np.array_equal(got, img)
+    K0, G0, gc0 = hio.image_mp(ref, img, w.voxel_size)
+    K1, G1, gc1 = hio.image_mp(t, got, c.voxel_size)
+    assert np.array_equal(K0, K1) and np.array_equal(G0, G1) and np.array_equal(gc0, gc1)
