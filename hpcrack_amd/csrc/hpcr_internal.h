@@ -30,6 +30,10 @@ constexpr int ZCHUNK_MAX = 32;                // node planes per block (fine lev
 #ifndef HPCR_MIXF_SLOTS
 #define HPCR_MIXF_SLOTS 2
 #endif
+// minimum waves per SIMD the FP64 mechanical tile passes are compiled for
+#ifndef HPCR_MECH_WAVES
+#define HPCR_MECH_WAVES 1
+#endif
 // minimum waves per SIMD the FP32 preconditioner passes are compiled for
 #ifndef HPCR_MIXF_WAVES
 #define HPCR_MIXF_WAVES 1
@@ -113,10 +117,13 @@ struct PfCoef {
 
 // z-chunk of the tile kernels: the longest chunk that still gives >= 1024
 // blocks, so that small levels are not serialised along z
+#ifndef HPCR_ZC_MIN_BLOCKS
+#define HPCR_ZC_MIN_BLOCKS 1024
+#endif
 inline int choose_zchunk(int nx, int ny, int nz) {
   const long xy = (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY);
   for (int zc = ZCHUNK_MAX; zc > 2; zc /= 2)
-    if (xy * ((nz + zc - 1) / zc) >= 1024) return zc;
+    if (xy * ((nz + zc - 1) / zc) >= HPCR_ZC_MIN_BLOCKS) return zc;
   return 2;
 }
 inline long tile_blocks(int nx, int ny, int nz, int zc) {

@@ -54,7 +54,7 @@ struct MechOp {
   static constexpr int NV = (NU + NK) > 0 ? NU + NK : 1;
   static constexpr int NR = (MODE == M_PCGA) ? 6 + NK : NV;  // PCGA fetches z and p_old
   using Raw = double[NR];
-  static constexpr int NSL = NSLOT, MIN_WAVES = 1;  // LDS plane slots, occupancy target
+  static constexpr int NSL = NSLOT, MIN_WAVES = HPCR_MECH_WAVES;  // LDS plane slots, occupancy target
   struct Acc {
     double a[3];
     __device__ Acc() { a[0] = a[1] = a[2] = 0.0; }
