@@ -282,3 +282,27 @@ def test_cycles(HA, case):
     fs = s.cycle_d(L)
     assert fs == pytest.approx(fo, rel=1e-13)
     assert rel_l2(s.get(O.D, L), o.field(O.D, L)) < 1e-10
+
+
+def test_pcg_smoother_on_a_long_grid(HA):
+    """12289 x 5 x 5 nodes: nx + ny + nz exceeds what pass B stages in LDS, so
+    its Dirichlet masks come from the axis flags in global memory (the other
+    branch of the same pass)."""
+    ml, enb, dom = 1, (6144, 2, 2), (0.0, 12288.0, 0.0, 4.0, 0.0, 4.0)
+    say(f"[gpu] pcg on a long grid {enb}")
+    o, s, _ = make_pair(HA, ml, enb, dom, seed=14, lab_incl=0, lc=2.0)
+    L = ml
+    assert sum(s.dims(L)) > 12288
+    random_state(o, s, L, 15, d_zero_frac=0.0)
+    rng = np.random.default_rng(16)
+    dims = o.dims(L)
+    u0 = rng.standard_normal((3,) + dims[::-1]) * 0.1
+    f0 = rng.standard_normal((3,) + dims[::-1]) * 1e3
+    for obj in (o, s):
+        obj.set(O.U, L, u0)
+        obj.set(O.FU, L, f0)
+    fo = o.pcg_u(L, 3, 0.3, 0, 0.0)
+    fs = s.pcg_u(L, 3, 0.3, 0, 0.0)
+    assert fs == pytest.approx(fo, rel=1e-12)
+    assert rel_l2(s.get(O.U, L), o.vec(O.U, L)) < 1e-11
+    assert rel_l2(s.get(O.RU, L), o.vec(O.RU, L)) < 1e-10
