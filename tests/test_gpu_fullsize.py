@@ -1,0 +1,160 @@
+"""Checks at the benchmark sizes.
+
+C1 (129^3 two-phase image) runs against the oracle itself: two V-cycles of
+u_entire, about 4 s of oracle time.  At C2 (257^3 three-phase image, 51M
+DOFs) the oracle is too slow, so the checks are properties that do not
+depend on size:
+- patch test: an affine displacement on a homogeneous material leaves zero
+  force on every interior node (any wrong neighbour or weight breaks the
+  cancellation);
+- symmetry of the heterogeneous degraded operator: y.(A x) = x.(A y);
+- repeated solves are bit-identical;
+- the mixed-precision solve's reported residual equals an independent
+  recomputation from u and the reactions, and reaches the tolerance."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_common import rel_l2, require_gpu, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def HA():
+    return require_gpu()
+
+
+@pytest.fixture(scope="module")
+def W():
+    import hpcrack_amd.workloads as W
+    return W
+
+
+def test_c1_full_size_matches_oracle(HA, W):
+    w = W.C1
+    img, frac, K, G, gc = w.materials()
+    L = w.maxlevel
+    kw = w.solver_kwargs()
+    kw["mg_u"] = (50, 4, 2, 1, 2)
+    say(f"[gpu] C1 {w.dims} vs oracle, 2 V-cycles")
+    o = O.Oracle(L, w.element_nb, w.domain, **kw)
+    s = HA.Solver(L, w.element_nb, w.domain, **kw)
+    for f, a in ((O.K, K), (O.G, G), (O.GC, gc)):
+        o.set(f, L, a)
+        s.set(f, L, a)
+    for f in (O.K, O.G, O.GC):
+        o.coarse_vr_nodal(f)
+        s.coarse_vr(f)
+    fo, co = o.u_entire(1.0, 0, 0.0)
+    fs, cs = s.u_entire(1.0, 0, 0.0)
+    eu, ef = rel_l2(s.get(O.U, L), o.vec(O.U, L)), rel_l2(s.get(O.FU, L), o.vec(O.FU, L))
+    say(f"  cycles {co}/{cs}, u {eu:.3e}, reactions {ef:.3e}")
+    assert cs == co == 2
+    assert fs == pytest.approx(fo, rel=1e-9)
+    assert eu < 1e-9 and ef < 1e-9
+
+
+@pytest.fixture(scope="module")
+def c2(HA, W):
+    w = W.C2
+    img, frac, K, G, gc = w.materials()
+    s = HA.Solver(w.maxlevel, w.element_nb, w.domain, **w.solver_kwargs())
+    W.upload(s, w, K, G, gc)
+    return w, s, K, G, gc
+
+
+def test_c2_patch_test_homogeneous_affine_field(c2):
+    w, s, K, G, gc = c2
+    L = w.maxlevel
+    nx, ny, nz = w.dims
+    say(f"[gpu] C2 {w.dims} patch test")
+    s.set(O.K, L, np.full((nz, ny, nx), K.flat[0]))
+    s.set(O.G, L, np.full((nz, ny, nx), G.flat[0]))
+    s.fill(O.D, L, 0.0)
+    s.set(O.FLAG, L, np.zeros((nz, ny, nx)))
+    z, y, x = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                          np.arange(nx, dtype=np.float64), indexing="ij")
+    u = np.stack([1e-3 * x - 2e-3 * y + 0.5e-3 * z + 0.1, 0.7e-3 * x + 1e-3 * y - 1.2e-3 * z - 0.3,
+                  -0.4e-3 * x + 0.9e-3 * y + 2e-3 * z + 0.2])
+    s.set(O.P, L, u)
+    s.internal_force(L, O.P, O.W)
+    f = s.get(O.W, L)
+    interior = np.abs(f[:, 1:-1, 1:-1, 1:-1]).max()
+    boundary = np.abs(f).max()
+    say(f"  max |f| interior {interior:.3e}, overall {boundary:.3e}")
+    assert boundary > 0
+    assert interior <= 1e-12 * boundary
+    # restore the image's moduli for the other tests
+    s.set(O.K, L, K)
+    s.set(O.G, L, G)
+
+
+def test_c2_degraded_operator_is_symmetric(c2):
+    w, s, K, G, gc = c2
+    L = w.maxlevel
+    nx, ny, nz = w.dims
+    say("[gpu] C2 operator symmetry (damage, all split flags)")
+    rng = np.random.default_rng(7)
+    s.set(O.D, L, rng.uniform(-0.05, 0.95, (nz, ny, nx)) * (rng.random((nz, ny, nx)) < 0.5))
+    s.set(O.FLAG, L, rng.integers(0, 3, (nz, ny, nx)).astype(np.float64))
+    x = rng.standard_normal((3, nz, ny, nx))
+    y = rng.standard_normal((3, nz, ny, nx))
+    s.set(O.P, L, x)
+    s.internal_force(L, O.P, O.W)
+    ax = s.get(O.W, L)
+    s.set(O.P, L, y)
+    s.internal_force(L, O.P, O.W)
+    ay = s.get(O.W, L)
+    yax, xay = float(np.vdot(y, ax)), float(np.vdot(x, ay))
+    scale = np.linalg.norm(x) * np.linalg.norm(ay)
+    say(f"  y.Ax {yax:.12e}  x.Ay {xay:.12e}  |diff|/scale {abs(yax - xay) / scale:.2e}")
+    assert abs(yax - xay) <= 1e-13 * scale
+    s.fill(O.D, L, 0.0)
+    s.set(O.FLAG, L, np.zeros((nz, ny, nx)))
+
+
+def test_c2_solves_repeat_bitwise_and_mixed_residual_is_true(c2):
+    w, s, K, G, gc = c2
+    L = w.maxlevel
+
+    def zero():
+        for c in range(3):
+            s.fill(O.U, L, 0.0, c)
+            s.fill(O.FU, L, 0.0, c)
+
+    say("[gpu] C2 u_entire twice, then the mixed solve")
+    runs = []
+    for _ in range(2):
+        zero()
+        fn, cyc = s.u_entire(1.0, 0, 0.0)
+        runs.append((fn, cyc, s.get(O.U, L)))
+    assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][2], runs[1][2])
+    zero()
+    fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=1e-16, max_iter=200)
+    rep = s.last_stats.r_norm_final
+    # reactions fu = A u on Dirichlet DOFs and 0 elsewhere, so fu - A u is the
+    # residual of the free DOFs and exactly 0 on the Dirichlet ones
+    u = s.get(O.U, L)
+    fu = s.get(O.FU, L)
+    s.set(O.P, L, u)
+    s.internal_force(L, O.P, O.W)
+    r = fu - s.get(O.W, L)
+    rr = float(np.vdot(r, r))
+    say(f"  mixed: {it} iterations, reported r.r {rep:.6e}, recomputed {rr:.6e}, f_norm {fm:.6e}")
+    assert rr == pytest.approx(rep, rel=1e-6)
+    assert rr / fm <= 1e-16 * (1 + 1e-6)
+    # against the plain FP64 solve (capped at 30 cycles): the difference is
+    # the plain solve's remaining error, a few 1e-3 modulo the rotation mode
+    m = W_rotation(w)
+    du = (u - runs[0][2]).ravel()
+    du -= (du @ m) / (m @ m) * m
+    diff = np.linalg.norm(du) / np.linalg.norm(u)
+    say(f"  |u_mixed - u_plain| / |u| modulo rotation: {diff:.3e}")
+    assert diff < 3e-2
+
+
+def W_rotation(w):
+    import hpcrack_amd.workloads as W
+    return W.rotation_mode(w)
