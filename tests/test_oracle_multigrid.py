@@ -352,3 +352,32 @@ def test_cycle_from_rough_start_is_rounding_sensitive():
     fb, ub = run(1e-14)
     assert np.linalg.norm(ua - ub) / np.linalg.norm(ua) > 1e-8
     assert abs(fa - fb) / fa > 1e-6
+
+
+def test_solve_at_17_cubed_amplifies_a_one_ulp_load_change():
+    """The physical-start counterpart of the property above, on the 17^3 and
+    33^3 two-phase cases of the GPU solve-parity tests: u_entire with Ut = 1
+    and with Ut = 1 + 2^-52 (an exact-arithmetic change of 1e-16).  At 17^3
+    the result moves by a few 1e-10, which is the floor against which any
+    re-implementation of the algorithm can be compared there; at 33^3 it
+    moves by a few 1e-11."""
+    from gpu_common import material_from_labels, rel_l2, spheres_labels
+
+    def run(ml, Ut):
+        dom = (0.0, 4.0 * 2 ** ml, 0.0, 4.0 * 2 ** ml, 0.0, 4.0 * 2 ** ml)
+        o = make(ml, (4, 4, 4), dom, lc=2.0, small_k=1e-6, tol_u=1e-6, mg_u=(50, 4, 2, 1, 30),
+                 mg_d=(10, 2, 1, 1, 30))
+        dims = o.dims(ml)
+        K, G, gc = material_from_labels(spheres_labels(dims, 4, 1.5, max(2.0, min(dims) / 5.0), 20))
+        for f, a in ((O.K, K), (O.G, G), (O.GC, gc)):
+            o.set(f, ml, a)
+        for f in (O.K, O.G, O.GC):
+            o.coarse_vr_nodal(f)
+        o.u_entire(Ut, 0, 0.0)
+        return o.vec(O.U, ml) / Ut
+
+    ut = np.nextafter(1.0, 2.0)
+    d17 = rel_l2(run(2, ut), run(2, 1.0))
+    d33 = rel_l2(run(3, ut), run(3, 1.0))
+    assert 1e-11 < d17 < 1e-8
+    assert d33 < d17 / 3
