@@ -39,6 +39,15 @@ constexpr int ZCHUNK_MAX = 32;                // node planes per block (fine lev
 #define HPCR_MIXF_WAVES 1
 #endif
 constexpr int NSLOT = HPCR_TILE_SLOTS;
+// A CG smoother call applies its x updates x += alpha_i p_i in groups: the
+// pass B of iterations 0..CG_GROUP-2 of a group leaves the update out and
+// keeps alpha_i, and the pass B that closes the group (or the call) applies
+// the group's updates in iteration order.  The group's directions sit in
+// CG_GROUP buffers, p_i in buffer i % CG_GROUP.
+constexpr int CG_GROUP = 4;
+struct PGroup {
+  const double* p[CG_GROUP];
+};
 // levels small enough for the one-workgroup smoother (small_pcg_kernel)
 constexpr int SMALL_NODES = 512, SMALL_ELEMS = 256;
 
@@ -59,9 +68,12 @@ enum Scal {
   S_PR = 10,     // p.r
   S_FALPHA = 11, // p.r / p.q
   S_LAM = 12,    // Rayleigh quotient of the power iteration
+  S_AH = 13,     // CG_GROUP - 1 slots: alpha_i of the updates a smoother's group defers
   S_SCRATCH = 16,  // 8 slots written by the coarsest solve of the preconditioner
-  S_COUNT = 24
+  S_APREV = 24,    // alpha of the flexible CG iteration whose x update is deferred
+  S_COUNT = 25
 };
+static_assert(S_AH + CG_GROUP - 1 <= S_SCRATCH, "scalar slots of the deferred updates");
 
 struct LevelGeom {
   int nx, ny, nz;
@@ -142,9 +154,12 @@ int passb_blocks(size_t n);
 // p_new = z + beta p_old (first: p_new = z); w = A p_new; p.w -> alpha
 void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* z,
                        const double* p_old, double* p_new, double* w, int first, const Reduce& R);
-// u += alpha p; r -= alpha w; mask r; z = r*invD; r.z, r.r -> beta, rz, rr
-void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const double* p,
-                       const double* w, const double* invD, const Reduce& R);
+// r -= alpha w; mask r; z = r*invD; r.z, r.r -> beta, rz, rr.  x updates:
+// np == 0 keeps alpha in scal[S_AH + slot]; np >= 1 closes a group,
+// u = fma(alpha_m, P.p[m], u) for m < np, alpha_m from scal[S_AH + m] and
+// the last one the current alpha.
+void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
+                       const double* w, const double* invD, int np, int slot, const Reduce& R);
 // invD = 1 / diag(A); zero entries counted into scal[S_ZERO]; diag optionally kept
 void launch_mech_diag(hipStream_t s, const LevelGeom& L, const MechCoef& C, double* diag_or_null,
                       double* invD, const Reduce& R);
@@ -153,10 +168,11 @@ void launch_pf_apply(hipStream_t s, const LevelGeom& L, const PfCoef& C, const d
 // R = fd - A d, z = R*invD; r.z, r.r, fd.fd
 void launch_pf_resid(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* d, const double* fd,
                      double* r, double* z, const double* invD, const Reduce& R);
+// the scalar forms of launch_mech_pcg_a / launch_mech_pcg_b
 void launch_pf_pcg_a(hipStream_t s, const LevelGeom& L, const PfCoef& C, const double* z, const double* p_old,
                      double* p_new, double* w, int first, const Reduce& R);
-void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, double* z, const double* p,
-                     const double* w, const double* invD, const Reduce& R);
+void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, double* z, const PGroup& P,
+                     const double* w, const double* invD, int np, int slot, const Reduce& R);
 void launch_pf_diag(hipStream_t s, const LevelGeom& L, const PfCoef& C, double* diag_or_null, double* invD,
                     const Reduce& R);
 
@@ -181,12 +197,17 @@ void launch_mix_res(hipStream_t st, const LevelGeom& L, const MixCoef& C, const 
 void launch_mix_power(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* v, float* w,
                       const float* invD, const Reduce& R);
 void launch_power_init(hipStream_t st, const LevelGeom& L, float* v);
-// flexible CG pass A (FP64): p = z + beta p_old, q = A p, p.q, p.r -> alpha
+// flexible CG pass A (FP64): p = z + beta p_old, q = A p, p.q, p.r -> alpha.
+// With u_lag (not first): also u += alpha_prev p_old, the update a lean
+// pass B left out.
 void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
-                  double* p_new, double* q, const double* r, int first, const Reduce& R);
-// flexible CG pass B (FP64) + start of the next preconditioner application
+                  double* p_new, double* q, const double* r, int first, double* u_lag, const Reduce& R);
+// the update a lean pass B left out, when no pass A follows: u += alpha_prev p
+void launch_fcg_flush(hipStream_t st, const LevelGeom& L, double* u, const double* p, const Reduce& R);
+// flexible CG pass B (FP64) + start of the next preconditioner application;
+// lean: u and p are left alone and alpha is kept in scal[S_APREV]
 void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, const double* p, const double* q,
-                  float* r32, float* d, float* x, const float* invD, float cr0, const Reduce& R);
+                  float* r32, float* d, float* x, const float* invD, float cr0, bool lean, const Reduce& R);
 void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, float* r32, float* d, float* x,
                       const float* invD, float cr0);
 void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,

@@ -527,7 +527,7 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], const Reduce& R)
   if (threadIdx.x == 0)
     for (int k = 0; k < NV; k++) R.partials[(size_t)lin_block() * NV + k] = v[k];
 }
-__global__ __launch_bounds__(256) void passb_finalize_kernel(const Reduce R, unsigned nb) {
+__global__ __launch_bounds__(256) void passb_finalize_kernel(const Reduce R, unsigned nb, int save_slot) {
   __shared__ double sh[2 * 16];
   double acc[2] = {0.0, 0.0};
   for (unsigned i = threadIdx.x; i < nb; i += blockDim.x) {
@@ -540,18 +540,37 @@ __global__ __launch_bounds__(256) void passb_finalize_kernel(const Reduce R, uns
     scal[S_BETA] = acc[0] / scal[S_RZ];
     scal[S_RZ] = acc[0];
     scal[S_RR] = acc[1];
+    if (save_slot >= 0) scal[S_AH + save_slot] = scal[S_ALPHA];
   }
 }
 
 // LDSF: the Dirichlet axis flags are staged in LDS (levels with
 // nx + ny + nz <= DIRF_MAX), so a node's mask needs no dependent global load,
 // which would also wait on the stores still in flight.
-template <bool LDSF>
+// NP: x updates of the CG group applied here (0: this iteration's is left
+// out and its alpha kept in scal[S_AH + slot]).  A group's updates come in
+// iteration order, each as the one fma of the undeferred form.
+template <int NP>
+struct GroupAlpha {
+  double a[NP > 0 ? NP : 1];
+  __device__ __forceinline__ GroupAlpha(const double* scal) {
+#pragma unroll
+    for (int m = 0; m + 1 < NP; m++) a[m] = scal[S_AH + m];
+    a[NP > 0 ? NP - 1 : 0] = scal[S_ALPHA];
+  }
+  __device__ __forceinline__ double apply(const PGroup& P, size_t o, double x) const {
+#pragma unroll
+    for (int m = 0; m < NP; m++) x = fma(a[m], PB_LD(P.p[m][o]), x);
+    return x;
+  }
+};
+
+template <bool LDSF, int NP>
 __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, double* __restrict__ u,
                                                             double* __restrict__ r, double* __restrict__ z,
-                                                            const double* __restrict__ p,
-                                                            const double* __restrict__ w,
-                                                            const double* __restrict__ invD, const Reduce R) {
+                                                            const PGroup P, const double* __restrict__ w,
+                                                            const double* __restrict__ invD, int slot,
+                                                            const Reduce R) {
   __shared__ unsigned char sflags[LDSF ? DIRF_MAX : 1];
   const DirFlags F{sflags};
   if constexpr (LDSF) {
@@ -562,6 +581,7 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
   const unsigned N = (unsigned)Nof(L);
   const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
   const double alpha = R.scal[S_ALPHA];
+  const GroupAlpha<NP> ga(R.scal);
   double acc[2] = {0.0, 0.0};
   auto node = [&](unsigned q) {
     const unsigned k = q / plane;
@@ -578,7 +598,7 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t o = (size_t)c * N + q;
-      PB_ST(PB_LD(u[o]) + alpha * PB_LD(p[o]), u[o]);
+      if constexpr (NP > 0) PB_ST(ga.apply(P, o, PB_LD(u[o])), u[o]);
       double rv = PB_LD(r[o]) - alpha * PB_LD(w[o]);
       if ((dm >> c) & 1) rv = 0.0;
       PB_ST(rv, r[o]);
@@ -605,18 +625,22 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
     scal[S_BETA] = s[0] / scal[S_RZ];
     scal[S_RZ] = s[0];
     scal[S_RR] = s[1];
+    if constexpr (NP == 0) scal[S_AH + slot] = scal[S_ALPHA];
   });
 #endif
 }
 
+template <int NP>
 __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __restrict__ d, double* __restrict__ r,
-                                                          double* __restrict__ z, const double* __restrict__ p,
+                                                          double* __restrict__ z, const PGroup P,
                                                           const double* __restrict__ w,
-                                                          const double* __restrict__ invD, const Reduce R) {
+                                                          const double* __restrict__ invD, int slot,
+                                                          const Reduce R) {
   const double alpha = R.scal[S_ALPHA];
+  const GroupAlpha<NP> ga(R.scal);
   double acc[2] = {0.0, 0.0};
   for (size_t q = (size_t)blockIdx.x * PW_TPB + threadIdx.x; q < N; q += (size_t)gridDim.x * PW_TPB) {
-    d[q] = d[q] + alpha * p[q];
+    if constexpr (NP > 0) d[q] = ga.apply(P, q, d[q]);
     const double rv = r[q] - alpha * w[q];
     r[q] = rv;
     const double zv = rv * invD[q];
@@ -629,25 +653,50 @@ __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __re
     scal[S_BETA] = s[0] / scal[S_RZ];
     scal[S_RZ] = s[0];
     scal[S_RR] = s[1];
+    if constexpr (NP == 0) scal[S_AH + slot] = scal[S_ALPHA];
   });
 }
 
-void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const double* p,
-                       const double* w, const double* invD, const Reduce& R) {
-  const size_t N = (size_t)L.nx * L.ny * L.nz;
+template <bool LDSF, int NP>
+static void mech_pcg_b_np(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
+                          const double* w, const double* invD, int slot, const Reduce& R) {
+  hipLaunchKernelGGL((mech_pcg_b_kernel<LDSF, NP>), dim3(passb_blocks(Nof(L))), dim3(PW_TPB), 0, s, L, u, r, z, P, w,
+                     invD, slot, R);
+}
+template <bool LDSF>
+static void mech_pcg_b_ldsf(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
+                            const double* w, const double* invD, int np, int slot, const Reduce& R) {
+  static_assert(CG_GROUP == 4, "one instantiation per group position");
+  switch (np) {
+    case 0: mech_pcg_b_np<LDSF, 0>(s, L, u, r, z, P, w, invD, slot, R); break;
+    case 1: mech_pcg_b_np<LDSF, 1>(s, L, u, r, z, P, w, invD, slot, R); break;
+    case 2: mech_pcg_b_np<LDSF, 2>(s, L, u, r, z, P, w, invD, slot, R); break;
+    case 3: mech_pcg_b_np<LDSF, 3>(s, L, u, r, z, P, w, invD, slot, R); break;
+    default: mech_pcg_b_np<LDSF, 4>(s, L, u, r, z, P, w, invD, slot, R); break;
+  }
+}
+void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
+                       const double* w, const double* invD, int np, int slot, const Reduce& R) {
   if (L.nx + L.ny + L.nz <= DIRF_MAX)
-    hipLaunchKernelGGL(mech_pcg_b_kernel<true>, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, z, p, w, invD, R);
+    mech_pcg_b_ldsf<true>(s, L, u, r, z, P, w, invD, np, slot, R);
   else
-    hipLaunchKernelGGL(mech_pcg_b_kernel<false>, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, L, u, r, z, p, w, invD,
-                       R);
+    mech_pcg_b_ldsf<false>(s, L, u, r, z, P, w, invD, np, slot, R);
 #ifdef HPCR_PASSB_SPLIT
-  hipLaunchKernelGGL(passb_finalize_kernel, dim3(1), dim3(256), 0, s, R, (unsigned)passb_blocks(N));
+  hipLaunchKernelGGL(passb_finalize_kernel, dim3(1), dim3(256), 0, s, R, (unsigned)passb_blocks(Nof(L)),
+                     np == 0 ? slot : -1);
 #endif
 }
-void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, double* z, const double* p,
-                     const double* w, const double* invD, const Reduce& R) {
-  const size_t N = (size_t)L.nx * L.ny * L.nz;
-  hipLaunchKernelGGL(pf_pcg_b_kernel, dim3(passb_blocks(N)), dim3(PW_TPB), 0, s, N, d, r, z, p, w, invD, R);
+void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, double* z, const PGroup& P,
+                     const double* w, const double* invD, int np, int slot, const Reduce& R) {
+  const size_t N = Nof(L);
+  const dim3 g(passb_blocks(N)), b(PW_TPB);
+  switch (np) {
+    case 0: hipLaunchKernelGGL(pf_pcg_b_kernel<0>, g, b, 0, s, N, d, r, z, P, w, invD, slot, R); break;
+    case 1: hipLaunchKernelGGL(pf_pcg_b_kernel<1>, g, b, 0, s, N, d, r, z, P, w, invD, slot, R); break;
+    case 2: hipLaunchKernelGGL(pf_pcg_b_kernel<2>, g, b, 0, s, N, d, r, z, P, w, invD, slot, R); break;
+    case 3: hipLaunchKernelGGL(pf_pcg_b_kernel<3>, g, b, 0, s, N, d, r, z, P, w, invD, slot, R); break;
+    default: hipLaunchKernelGGL(pf_pcg_b_kernel<4>, g, b, 0, s, N, d, r, z, P, w, invD, slot, R); break;
+  }
 }
 
 // ---------------------------------------------------------------------------

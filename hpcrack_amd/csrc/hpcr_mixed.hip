@@ -187,7 +187,10 @@ struct MechOpF {
 };
 
 // FCG pass A (FP64, finest level): p = z + beta p_prev with z the FP32
-// preconditioner output; q = A p; p.q, p.r -> alpha
+// preconditioner output; q = A p; p.q, p.r -> alpha.  LAG: also
+// u += alpha_prev p_prev at the own nodes, the update of the previous
+// iteration that its lean pass B left out (the same fma, one kernel later).
+template <bool LAG>
 struct FcgOpA {
   using T = double;
   using TR = double;
@@ -208,7 +211,7 @@ struct FcgOpA {
     __device__ Acc() { a[0] = a[1] = 0.0; }
   };
   struct State {
-    double beta;
+    double beta, aprev;
   };
   // the element's split flag, staged through LDS with the node planes
   using TE = int;
@@ -219,6 +222,7 @@ struct FcgOpA {
   double* p_new;
   double* q;
   const double* r;
+  double* u;  // LAG
   int first;
   double sx, sy, sz, tx_, ty_, tz_;
   Reduce R;
@@ -227,6 +231,7 @@ struct FcgOpA {
   __device__ __forceinline__ State state() const {
     State st;
     st.beta = first ? 0.0 : -R.scal[S_ZQ] / R.scal[S_PQ];
+    st.aprev = LAG ? R.scal[S_APREV] : 0.0;
     return st;
   }
   __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State&,
@@ -273,14 +278,22 @@ struct FcgOpA {
   }
   struct NData {
     double r[3];
+    double u[LAG ? 3 : 1], p[LAG ? 3 : 1];
   };
   __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
     const size_t o = nidx(L, i, j, k);
 #pragma unroll
     for (int c = 0; c < 3; c++) n.r[c] = r[c * N + o];
+    if constexpr (LAG) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        n.u[c] = u[c * N + o];
+        n.p[c] = p_old[c * N + o];
+      }
+    }
   }
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k, int dm,
-                                           const double (&v)[3], const State&, const NData& nd,
+                                           const double (&v)[3], const State& st, const NData& nd,
                                            Acc& acc) const {
     const size_t o = nidx(L, i, j, k);
 #pragma unroll
@@ -289,6 +302,7 @@ struct FcgOpA {
       q[c * N + o] = v[c];
       acc.a[0] += own[c] * v[c];
       acc.a[1] += own[c] * nd.r[c];
+      if constexpr (LAG) u[c * N + o] = fma(st.aprev, nd.p[c], nd.u[c]);
     }
   }
   __device__ __forceinline__ void finish(Acc& acc) const {
@@ -354,10 +368,12 @@ void launch_mix_power(hipStream_t st, const LevelGeom& L, const MixCoef& C, cons
   run_f32_level<F_POWER>(st, L, C, v, w, nullptr, nullptr, invD, 0.0f, 0.0f, nullptr, R);
 }
 
-void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
-                  double* p_new, double* q, const double* r, int first, const Reduce& R) {
-  FcgOpA op{};
+template <bool LAG>
+static void run_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
+                      double* p_new, double* q, const double* r, int first, double* u, const Reduce& R) {
+  FcgOpA<LAG> op{};
   op.C = C;
+  op.u = u;
   op.z = z;
   op.p_old = p_old;
   op.p_new = p_new;
@@ -373,7 +389,14 @@ void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const f
   op.tz_ = wg / L.hz;
   op.R = R;
   op.N = (size_t)L.nx * L.ny * L.nz;
-  tile_kernel<FcgOpA><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
+  tile_kernel<FcgOpA<LAG>><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
+}
+void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
+                  double* p_new, double* q, const double* r, int first, double* u_lag, const Reduce& R) {
+  if (u_lag && !first)
+    run_fcg_a<true>(st, L, C, z, p_old, p_new, q, r, 0, u_lag, R);
+  else
+    run_fcg_a<false>(st, L, C, z, p_old, p_new, q, r, first, nullptr, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -415,7 +438,9 @@ __global__ __launch_bounds__(MX_TPB) void mix_start_kernel(unsigned N, const dou
 // FCG pass B (FP64): u += alpha p; r -= alpha q (masked); r.r; then the start
 // of the next preconditioner application from the new r.  LDSF: the
 // Dirichlet axis flags are staged in LDS (nx + ny + nz <= DIRF_MAX).
-template <bool LDSF>
+// LEAN: u and p are left alone (the next pass A, or launch_fcg_flush,
+// applies u += alpha p) and alpha moves to scal[S_APREV].
+template <bool LDSF, bool LEAN>
 __global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double* __restrict__ u,
                                                        double* __restrict__ r, const double* __restrict__ p,
                                                        const double* __restrict__ q, float* __restrict__ r32,
@@ -443,7 +468,7 @@ __global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t o = (size_t)c * N + n;
-      u[o] = u[o] + alpha * p[o];
+      if constexpr (!LEAN) u[o] = fma(alpha, p[o], u[o]);
       double rv = r[o] - alpha * q[o];
       if ((dm >> c) & 1) rv = 0.0;
       r[o] = rv;
@@ -456,7 +481,18 @@ __global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double
     }
   }
   double* scal = R.scal;
-  reduce_finalize<RED_SUM, 1>(acc, R, 0.0, [=](double (&t)[1]) { scal[S_RR] = t[0]; });
+  reduce_finalize<RED_SUM, 1>(acc, R, 0.0, [=](double (&t)[1]) {
+    scal[S_RR] = t[0];
+    if constexpr (LEAN) scal[S_APREV] = scal[S_FALPHA];
+  });
+}
+
+__global__ __launch_bounds__(MX_TPB) void fcg_flush_kernel(size_t n, double* __restrict__ u,
+                                                           const double* __restrict__ p,
+                                                           const double* __restrict__ scal) {
+  const double alpha = scal[S_APREV];
+  for (size_t o = (size_t)blockIdx.x * MX_TPB + threadIdx.x; o < n; o += (size_t)gridDim.x * MX_TPB)
+    u[o] = fma(alpha, p[o], u[o]);
 }
 
 // restriction of the FP32 residual (full weighting, as the FP64 transfer);
@@ -623,13 +659,22 @@ void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, flo
                      invD, cr0);
 }
 void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, const double* p, const double* q,
-                  float* r32, float* d, float* x, const float* invD, float cr0, const Reduce& R) {
-  if (L.nx + L.ny + L.nz <= DIRF_MAX)
-    hipLaunchKernelGGL(fcg_b_kernel<true>, dim3(passb_blocks(Nof(L))), dim3(MX_TPB), 0, st, L, u, r, p, q, r32, d, x,
-                       invD, cr0, R);
+                  float* r32, float* d, float* x, const float* invD, float cr0, bool lean, const Reduce& R) {
+  const dim3 g(passb_blocks(Nof(L))), b(MX_TPB);
+  const bool ldsf = L.nx + L.ny + L.nz <= DIRF_MAX;
+  if (ldsf && lean)
+    hipLaunchKernelGGL((fcg_b_kernel<true, true>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
+  else if (ldsf)
+    hipLaunchKernelGGL((fcg_b_kernel<true, false>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
+  else if (lean)
+    hipLaunchKernelGGL((fcg_b_kernel<false, true>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
   else
-    hipLaunchKernelGGL(fcg_b_kernel<false>, dim3(passb_blocks(Nof(L))), dim3(MX_TPB), 0, st, L, u, r, p, q, r32, d,
-                       x, invD, cr0, R);
+    hipLaunchKernelGGL((fcg_b_kernel<false, false>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
+}
+void launch_fcg_flush(hipStream_t st, const LevelGeom& L, double* u, const double* p, const Reduce& R) {
+  const size_t n = 3 * Nof(L);
+  hipLaunchKernelGGL(fcg_flush_kernel, dim3(mx_blocks(n, 4096)), dim3(MX_TPB), 0, st, n, u, p,
+                     static_cast<const double*>(R.scal));
 }
 void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,
                          float* dc, float* xc, const float* invD, float cr0, int start) {

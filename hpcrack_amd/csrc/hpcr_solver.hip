@@ -31,9 +31,11 @@ struct Lev {
   // mechanics (3 components each)
   double *u = nullptr, *fu = nullptr, *ru = nullptr, *uold = nullptr;
   double *p = nullptr, *p2 = nullptr, *w = nullptr, *invDu = nullptr, *zu = nullptr;
+  double *p3 = nullptr, *p4 = nullptr;  // with p and p2: the directions of a CG group
   // phase field
   double *d = nullptr, *fd = nullptr, *rd = nullptr, *dold = nullptr;
   double *pd = nullptr, *pd2 = nullptr, *wd = nullptr, *invDd = nullptr, *zd = nullptr;
+  double *pd3 = nullptr, *pd4 = nullptr;
   // coefficients
   double *H = nullptr, *K = nullptr, *G = nullptr, *gc = nullptr, *Ke = nullptr, *Ge = nullptr;
   signed char* flag = nullptr;
@@ -212,9 +214,9 @@ int pcg_u(hpcr_ctx* c, int l, int NB, double Ut, int t) {
     if (c->host_scal[S_ZERO] != 0.0)
       return set_err(c, HPCR_ERR_ZERO_DIAGONAL, "Division by zero in grid /= operation (mechanical diagonal)");
   }
-  double* pb[2] = {v.p, v.p2};
   if (l < c->L && small_level(v.g)) {
-    launch_small_pcg_u(c->stream, v.g, v.Ke, v.Ge, v.u, v.fu, v.invDu, v.ru, v.zu, pb[NB & 1], v.w, Ut, NB, c->scal);
+    launch_small_pcg_u(c->stream, v.g, v.Ke, v.Ge, v.u, v.fu, v.invDu, v.ru, v.zu, (NB & 1) ? v.p2 : v.p, v.w, Ut, NB,
+                       c->scal);
     c->smoother_iters += NB;
     c->last_rr_level = l;
     return check_launch(c, "pcg_u (small level)");
@@ -225,16 +227,20 @@ int pcg_u(hpcr_ctx* c, int l, int NB, double Ut, int t) {
     TBracket tb(c, HPCR_K_MECH_RESID, l);
     launch_mech_resid(c->stream, v.g, mcoef(c, l), v.u, v.fu, v.ru, v.zu, v.invDu, t0fine, red(c));
   }
+  double* pb[CG_GROUP] = {v.p, v.p2, v.p3, v.p4};
+  const PGroup P{{v.p, v.p2, v.p3, v.p4}};
   for (int i = 0; i < NB; i++) {
-    double* p_old = pb[i & 1];
-    double* p_new = pb[(i + 1) & 1];
+    const int m = i % CG_GROUP;
     {
       TBracket tb(c, HPCR_K_MECH_PCG_A, l);
-      launch_mech_pcg_a(c->stream, v.g, mcoef(c, l), v.zu, p_old, p_new, v.w, i == 0, red(c));
+      launch_mech_pcg_a(c->stream, v.g, mcoef(c, l), v.zu, pb[(m + CG_GROUP - 1) % CG_GROUP], pb[m], v.w, i == 0,
+                        red(c));
     }
     {
+      // u += alpha_i p_i waits for the pass B that closes the group
+      const int np = (m == CG_GROUP - 1 || i == NB - 1) ? m + 1 : 0;
       TBracket tb(c, HPCR_K_MECH_PCG_B, l);
-      launch_mech_pcg_b(c->stream, v.g, v.u, v.ru, v.zu, p_new, v.w, v.invDu, red(c));
+      launch_mech_pcg_b(c->stream, v.g, v.u, v.ru, v.zu, P, v.w, v.invDu, np, m, red(c));
     }
     c->smoother_iters++;
   }
@@ -563,12 +569,21 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
   double* qb[2] = {F.w, c->q2};
   int it = 0;
   const int m = mp.cheb_degree;
+  // Pass B leaves u += alpha p to the next pass A; `lag` says that the update
+  // of the last iteration (direction pb[it & 1]) is still to be applied,
+  // which `flush` does where u is read before another pass A runs.
+  bool lag = false;
+  auto flush = [&]() {
+    if (lag) launch_fcg_flush(c->stream, F.g, F.u, pb[it & 1], red(c));
+    lag = false;
+  };
   while (it < mp.max_iter && (rel > mp.tol || (t0 && it == 0))) {
     if ((rc = mix_vcycle(c, m, c->par.mg_u.nu0, it > 0 ? qb[it & 1] : nullptr))) return rc;
     launch_fcg_a(c->stream, F.g, mcoef(c, L), MF.x, pb[it & 1], pb[(it + 1) & 1], qb[(it + 1) & 1], F.ru, it == 0,
-                 red(c));
+                 lag ? F.u : nullptr, red(c));
     launch_fcg_b(c->stream, F.g, F.u, F.ru, pb[(it + 1) & 1], qb[(it + 1) & 1], MF.r, MF.dA, MF.x, MF.invD, MF.cr0,
-                 red(c));
+                 true, red(c));
+    lag = true;
     if ((rc = check_launch(c, "flexible CG"))) return rc;
     if ((rc = read_scal(c))) return rc;
     rr = c->host_scal[S_RR];
@@ -577,11 +592,14 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
     if (!std::isfinite(rel)) return set_err(c, HPCR_ERR_STATE, "mixed-precision iteration diverged");
     const bool replace = (mp.replace_every > 0 && it % mp.replace_every == 0) || rel <= mp.tol;
     if (replace) {
+      flush();
       if ((rc = true_residual(&rr))) return rc;
       rel = rr / *f_norm;
     }
     if (st && it <= HPCR_HIST_MAX) st->history[it - 1] = rr;
   }
+  flush();
+  if ((rc = check_launch(c, "flexible CG update"))) return rc;
   // externe_force_nodal (mechanical_right_hnd.cpp:18-38)
   launch_dirichlet_bd(c->stream, F.g, F.u, Ut, F.kplanes.data(), (int)F.kplanes.size());
   launch_mech_apply(c->stream, F.g, mcoef(c, L), F.u, F.w);
@@ -614,25 +632,28 @@ int pcg_d(hpcr_ctx* c, int l, int NB) {
     if (c->host_scal[S_ZERO] != 0.0)
       return set_err(c, HPCR_ERR_ZERO_DIAGONAL, "Division by zero in grid /= operation (phase-field diagonal)");
   }
-  double* pb[2] = {v.pd, v.pd2};
   if (l < c->L && small_level(v.g)) {
-    launch_small_pcg_d(c->stream, v.g, v.H, v.gc, c->par.lc, v.d, v.fd, v.invDd, v.rd, v.zd, pb[NB & 1], v.wd, NB,
-                       c->scal);
+    launch_small_pcg_d(c->stream, v.g, v.H, v.gc, c->par.lc, v.d, v.fd, v.invDd, v.rd, v.zd, (NB & 1) ? v.pd2 : v.pd,
+                       v.wd, NB, c->scal);
     c->smoother_iters += NB;
     c->last_rr_level = l;
     return check_launch(c, "pcg_d (small level)");
   }
   launch_pf_resid(c->stream, v.g, pcoef(c, l), v.d, v.fd, v.rd, v.zd, v.invDd, red(c));
+  double* pb[CG_GROUP] = {v.pd, v.pd2, v.pd3, v.pd4};
+  const PGroup P{{v.pd, v.pd2, v.pd3, v.pd4}};
   for (int i = 0; i < NB; i++) {
-    double* p_old = pb[i & 1];
-    double* p_new = pb[(i + 1) & 1];
+    const int m = i % CG_GROUP;
     {
       TBracket tb(c, HPCR_K_PF_PCG_A, l);
-      launch_pf_pcg_a(c->stream, v.g, pcoef(c, l), v.zd, p_old, p_new, v.wd, i == 0, red(c));
+      launch_pf_pcg_a(c->stream, v.g, pcoef(c, l), v.zd, pb[(m + CG_GROUP - 1) % CG_GROUP], pb[m], v.wd, i == 0,
+                      red(c));
     }
     {
+      // d += alpha_i p_i waits for the pass B that closes the group
+      const int np = (m == CG_GROUP - 1 || i == NB - 1) ? m + 1 : 0;
       TBracket tb(c, HPCR_K_PF_PCG_B, l);
-      launch_pf_pcg_b(c->stream, v.g, v.d, v.rd, v.zd, p_new, v.wd, v.invDd, red(c));
+      launch_pf_pcg_b(c->stream, v.g, v.d, v.rd, v.zd, P, v.wd, v.invDd, np, m, red(c));
     }
     c->smoother_iters++;
   }
@@ -780,11 +801,12 @@ int build_levels(hpcr_ctx* c) {
   for (int l = 0; l <= L; l++) {
     Lev& v = c->lev[l];
     const size_t N = v.N;
-    double** vec3[] = {&v.u, &v.fu, &v.ru, &v.p, &v.p2, &v.w, &v.invDu, &v.zu};
+    double** vec3[] = {&v.u, &v.fu, &v.ru, &v.p, &v.p2, &v.p3, &v.p4, &v.w, &v.invDu, &v.zu};
     for (double** pp : vec3)
       if ((rc = alloc_n(c, pp, 3 * N))) return rc;
     if (l < L && (rc = alloc_n(c, &v.uold, 3 * N))) return rc;
-    double** sc[] = {&v.d, &v.fd, &v.rd, &v.pd, &v.pd2, &v.wd, &v.invDd, &v.zd, &v.H, &v.K, &v.G, &v.gc, &v.Ke, &v.Ge};
+    double** sc[] = {&v.d,  &v.fd, &v.rd, &v.pd, &v.pd2, &v.pd3, &v.pd4, &v.wd,
+                     &v.invDd, &v.zd, &v.H, &v.K, &v.G, &v.gc, &v.Ke, &v.Ge};
     for (double** pp : sc)
       if ((rc = alloc_n(c, pp, N))) return rc;
     if (l < L && (rc = alloc_n(c, &v.dold, N))) return rc;
@@ -841,7 +863,7 @@ long long estimate_bytes(const hpcr_geometry& G, bool mixed) {
   long ii = G.element_nb[0], jj = G.element_nb[1], kk = G.element_nb[2];
   for (int l = 0; l <= L; l++) {
     const long long nx = ii + 1, ny = jj + 1, nz = kk + 1, N = nx * ny * nz;
-    total += 8 * 3 * N * 8 + (l < L ? 3 * N * 8 : 0) + 14 * N * 8 + (l < L ? N * 8 : 0);
+    total += 10 * 3 * N * 8 + (l < L ? 3 * N * 8 : 0) + 16 * N * 8 + (l < L ? N * 8 : 0);
     if (l == L) total += N + N * 8;
     total += nx + ny + nz;
     const long tb = tile_blocks((int)nx, (int)ny, (int)nz, choose_zchunk((int)nx, (int)ny, (int)nz));
