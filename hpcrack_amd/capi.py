@@ -57,6 +57,13 @@ class MixedParams(ctypes.Structure):
                 ("replace_every", ctypes.c_int)]
 
 
+class FuserStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_longlong), ("groups", ctypes.c_longlong), ("cycles", ctypes.c_longlong),
+                ("launches", ctypes.c_longlong), ("problem_launches", ctypes.c_longlong),
+                ("max_problems", ctypes.c_int), ("graph_captures", ctypes.c_longlong),
+                ("graph_replays", ctypes.c_longlong)]
+
+
 PROTOTYPES = {
     "hpcr_device_count": (_i, [_ip]),
     "hpcr_ctx_create": (_i, [ctypes.POINTER(Geometry), ctypes.POINTER(Params), _i, ctypes.POINTER(_vp)]),
@@ -75,6 +82,11 @@ PROTOTYPES = {
     "hpcr_u_entire_mixed": (_i, [_vp, _d, _i, _dp, ctypes.POINTER(MixedParams), ctypes.POINTER(Stats)]),
     "hpcr_estimate_bytes": (_i, [ctypes.POINTER(Geometry), _i, ctypes.POINTER(ctypes.c_longlong)]),
     "hpcr_set_graphs": (_i, [_vp, _i]),
+    "hpcr_fuser_create": (_i, [_i, _i, ctypes.POINTER(_vp)]),
+    "hpcr_fuser_destroy": (_i, [_vp]),
+    "hpcr_fuser_set_graphs": (_i, [_vp, _i]),
+    "hpcr_fuser_get_stats": (_i, [_vp, ctypes.POINTER(FuserStats)]),
+    "hpcr_set_fuser": (_i, [_vp, _vp]),
     "hpcr_strain_history": (_i, [_vp, _d]),
     "hpcr_voigt_all_level": (_i, [_vp, _i]),
     "hpcr_vr_all_level": (_i, [_vp, _i]),

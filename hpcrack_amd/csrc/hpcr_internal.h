@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace hpcr {
 
@@ -89,6 +90,25 @@ struct Reduce {
   unsigned int* counter; // zero between kernels
   double* scal;          // S_COUNT doubles
 };
+
+// Multi-problem launches.  While a recorder is set on the calling thread the
+// launch functions record their kernels instead of issuing them; the fusion
+// engine (hpcr_fuser) records one multigrid cycle of each of several contexts
+// and issues position i of all their sequences as one grid, in which every
+// problem keeps its own grid and argument record.
+struct Rec;
+using MultiFn = void (*)(hipStream_t, const Rec* const*, int);
+constexpr int REC_ARG_BYTES = 448;
+struct Rec {
+  MultiFn fn;       // issues n records of one kernel type as one launch
+  unsigned g[3];    // the problem's own grid
+  alignas(16) unsigned char a[REC_ARG_BYTES];  // the kernel's argument record
+};
+struct Recorder {
+  std::vector<Rec> recs;
+};
+// recorder of the calling thread (null: the launch functions launch)
+extern thread_local Recorder* t_recorder;
 
 // Mechanical coefficients of one level.
 struct MechCoef {

@@ -38,8 +38,11 @@
 
 #include "hpcr_internal.h"
 #include "hpcr_tile.h"
+#include "hpcr_multi.h"
 
 namespace hpcr {
+
+thread_local Recorder* t_recorder = nullptr;
 
 // ---- mechanical tile ops --------------------------------------------------
 enum MechMode { M_APPLY = 0, M_RESID = 1, M_PCGA = 2, M_DIAG = 3 };
@@ -235,25 +238,25 @@ struct MechOp {
     }
   }
 
-  __device__ __forceinline__ void finish(Acc& acc) const {
+  __device__ __forceinline__ void finish(Acc& acc, const Blk& bk) const {
     if constexpr (MODE == M_APPLY) return;
     double* scal = R.scal;
     if constexpr (MODE == M_RESID) {
       const int t0 = flag;
-      reduce_finalize<RED_SUM, 3>(acc.a, R, 0.0, [=](double (&s)[3]) {
+      reduce_finalize<RED_SUM, 3>(bk, acc.a, R, 0.0, [=](double (&s)[3]) {
         scal[S_RZ] = s[0];
         scal[S_RR] = s[1];
         if (t0) scal[S_FF] = s[2];
       });
     } else if constexpr (MODE == M_PCGA) {
       double v1[1] = {acc.a[0]};
-      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) {
+      reduce_finalize<RED_SUM, 1>(bk, v1, R, 0.0, [=](double (&s)[1]) {
         scal[S_PW] = s[0];
         scal[S_ALPHA] = scal[S_RZ] / s[0];
       });
     } else {
       double v1[1] = {acc.a[0]};
-      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) { scal[S_ZERO] = s[0]; });
+      reduce_finalize<RED_SUM, 1>(bk, v1, R, 0.0, [=](double (&s)[1]) { scal[S_ZERO] = s[0]; });
     }
   }
 };
@@ -378,24 +381,24 @@ struct PfOp {
     }
   }
 
-  __device__ __forceinline__ void finish(Acc& acc) const {
+  __device__ __forceinline__ void finish(Acc& acc, const Blk& bk) const {
     if constexpr (MODE == M_APPLY) return;
     double* scal = R.scal;
     if constexpr (MODE == M_RESID) {
-      reduce_finalize<RED_SUM, 3>(acc.a, R, 0.0, [=](double (&s)[3]) {
+      reduce_finalize<RED_SUM, 3>(bk, acc.a, R, 0.0, [=](double (&s)[3]) {
         scal[S_RZ] = s[0];
         scal[S_RR] = s[1];
         scal[S_FF] = s[2];
       });
     } else if constexpr (MODE == M_PCGA) {
       double v1[1] = {acc.a[0]};
-      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) {
+      reduce_finalize<RED_SUM, 1>(bk, v1, R, 0.0, [=](double (&s)[1]) {
         scal[S_PW] = s[0];
         scal[S_ALPHA] = scal[S_RZ] / s[0];
       });
     } else {
       double v1[1] = {acc.a[0]};
-      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&s)[1]) { scal[S_ZERO] = s[0]; });
+      reduce_finalize<RED_SUM, 1>(bk, v1, R, 0.0, [=](double (&s)[1]) { scal[S_ZERO] = s[0]; });
     }
   }
 };
@@ -427,11 +430,13 @@ static void run_mech(hipStream_t s, const LevelGeom& L, const MechCoef& C, const
     auto op = mech_op<MODE, true>(L, C);
     op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
     op.flag = flag; op.R = R;
+    if (record<TileK<MechOp<MODE, true>>>(tile_grid(L), {L, op})) return;
     tile_kernel<MechOp<MODE, true>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
   } else {
     auto op = mech_op<MODE, false>(L, C);
     op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
     op.flag = flag; op.R = R;
+    if (record<TileK<MechOp<MODE, false>>>(tile_grid(L), {L, op})) return;
     tile_kernel<MechOp<MODE, false>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
   }
 }
@@ -467,6 +472,7 @@ static void run_pf(hipStream_t s, const LevelGeom& L, const PfCoef& C, const dou
   op.ihx2 = 1.0 / (L.hx * L.hx);
   op.ihy2 = 1.0 / (L.hy * L.hy);
   op.ihz2 = 1.0 / (L.hz * L.hz);
+  if (record<TileK<PfOp<MODE>>>(tile_grid(L), {L, op})) return;
   tile_kernel<PfOp<MODE>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
 }
 
@@ -525,7 +531,7 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], const Reduce& R)
   __shared__ double sh[NV * 16];
   block_red<RED_SUM, NV>(v, sh);
   if (threadIdx.x == 0)
-    for (int k = 0; k < NV; k++) R.partials[(size_t)lin_block() * NV + k] = v[k];
+    for (int k = 0; k < NV; k++) R.partials[(size_t)Blk::hw().lin() * NV + k] = v[k];
 }
 __global__ __launch_bounds__(256) void passb_finalize_kernel(const Reduce R, unsigned nb, int save_slot) {
   __shared__ double sh[2 * 16];
@@ -566,11 +572,10 @@ struct GroupAlpha {
 };
 
 template <bool LDSF, int NP>
-__global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, double* __restrict__ u,
-                                                            double* __restrict__ r, double* __restrict__ z,
-                                                            const PGroup P, const double* __restrict__ w,
-                                                            const double* __restrict__ invD, int slot,
-                                                            const Reduce R) {
+__device__ __forceinline__ void mech_pcg_b_body(const LevelGeom& L, double* __restrict__ u, double* __restrict__ r,
+                                                double* __restrict__ z, const PGroup& P,
+                                                const double* __restrict__ w, const double* __restrict__ invD,
+                                                int slot, const Reduce& R, const Blk& bk) {
   __shared__ unsigned char sflags[LDSF ? DIRF_MAX : 1];
   const DirFlags F{sflags};
   if constexpr (LDSF) {
@@ -608,20 +613,20 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
       acc[1] += rv * rv;
     }
   };
-  const unsigned stride = gridDim.x * PW_TPB;
+  const unsigned stride = bk.gx * PW_TPB;
 #ifdef HPCR_PASSB_UNROLL2
-  for (unsigned q = blockIdx.x * PW_TPB + threadIdx.x; q < N; q += 2 * stride) {
+  for (unsigned q = bk.x * PW_TPB + threadIdx.x; q < N; q += 2 * stride) {
     node(q);
     if (q + stride < N) node(q + stride);
   }
 #else
-  for (unsigned q = blockIdx.x * PW_TPB + threadIdx.x; q < N; q += stride) node(q);
+  for (unsigned q = bk.x * PW_TPB + threadIdx.x; q < N; q += stride) node(q);
 #endif
 #ifdef HPCR_PASSB_SPLIT
   block_partials<2>(acc, R);
 #else
   double* scal = R.scal;
-  reduce_finalize<RED_SUM, 2>(acc, R, 0.0, [=](double (&s)[2]) {
+  reduce_finalize<RED_SUM, 2>(bk, acc, R, 0.0, [=](double (&s)[2]) {
     scal[S_BETA] = s[0] / scal[S_RZ];
     scal[S_RZ] = s[0];
     scal[S_RR] = s[1];
@@ -630,16 +635,46 @@ __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, d
 #endif
 }
 
+template <bool LDSF, int NP>
+__global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, double* __restrict__ u,
+                                                            double* __restrict__ r, double* __restrict__ z,
+                                                            const PGroup P, const double* __restrict__ w,
+                                                            const double* __restrict__ invD, int slot,
+                                                            const Reduce R) {
+  mech_pcg_b_body<LDSF, NP>(L, u, r, z, P, w, invD, slot, R, Blk::hw());
+}
+
+struct PassBArgs {
+  LevelGeom L;
+  size_t N;
+  double* x;  // u or d
+  double* r;
+  double* z;
+  PGroup P;
+  const double* w;
+  const double* invD;
+  int slot;
+  Reduce R;
+};
+template <bool LDSF, int NP>
+struct MechPassBK {
+  using Args = PassBArgs;
+  static constexpr int NT = PW_TPB, MINW = 1;
+  static dim3 block() { return dim3(PW_TPB); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    mech_pcg_b_body<LDSF, NP>(a.L, a.x, a.r, a.z, a.P, a.w, a.invD, a.slot, a.R, bk);
+  }
+};
+
 template <int NP>
-__global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __restrict__ d, double* __restrict__ r,
-                                                          double* __restrict__ z, const PGroup P,
-                                                          const double* __restrict__ w,
-                                                          const double* __restrict__ invD, int slot,
-                                                          const Reduce R) {
+__device__ __forceinline__ void pf_pcg_b_body(size_t N, double* __restrict__ d, double* __restrict__ r,
+                                              double* __restrict__ z, const PGroup& P, const double* __restrict__ w,
+                                              const double* __restrict__ invD, int slot, const Reduce& R,
+                                              const Blk& bk) {
   const double alpha = R.scal[S_ALPHA];
   const GroupAlpha<NP> ga(R.scal);
   double acc[2] = {0.0, 0.0};
-  for (size_t q = (size_t)blockIdx.x * PW_TPB + threadIdx.x; q < N; q += (size_t)gridDim.x * PW_TPB) {
+  for (size_t q = (size_t)bk.x * PW_TPB + threadIdx.x; q < N; q += (size_t)bk.gx * PW_TPB) {
     if constexpr (NP > 0) d[q] = ga.apply(P, q, d[q]);
     const double rv = r[q] - alpha * w[q];
     r[q] = rv;
@@ -649,7 +684,7 @@ __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __re
     acc[1] += rv * rv;
   }
   double* scal = R.scal;
-  reduce_finalize<RED_SUM, 2>(acc, R, 0.0, [=](double (&s)[2]) {
+  reduce_finalize<RED_SUM, 2>(bk, acc, R, 0.0, [=](double (&s)[2]) {
     scal[S_BETA] = s[0] / scal[S_RZ];
     scal[S_RZ] = s[0];
     scal[S_RR] = s[1];
@@ -657,9 +692,31 @@ __global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __re
   });
 }
 
+template <int NP>
+__global__ __launch_bounds__(PW_TPB) void pf_pcg_b_kernel(size_t N, double* __restrict__ d, double* __restrict__ r,
+                                                          double* __restrict__ z, const PGroup P,
+                                                          const double* __restrict__ w,
+                                                          const double* __restrict__ invD, int slot,
+                                                          const Reduce R) {
+  pf_pcg_b_body<NP>(N, d, r, z, P, w, invD, slot, R, Blk::hw());
+}
+template <int NP>
+struct PfPassBK {
+  using Args = PassBArgs;
+  static constexpr int NT = PW_TPB, MINW = 1;
+  static dim3 block() { return dim3(PW_TPB); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    pf_pcg_b_body<NP>(a.N, a.x, a.r, a.z, a.P, a.w, a.invD, a.slot, a.R, bk);
+  }
+};
+
 template <bool LDSF, int NP>
 static void mech_pcg_b_np(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
                           const double* w, const double* invD, int slot, const Reduce& R) {
+#ifndef HPCR_PASSB_SPLIT
+  if (record<MechPassBK<LDSF, NP>>(dim3(passb_blocks(Nof(L))), PassBArgs{L, Nof(L), u, r, z, P, w, invD, slot, R}))
+    return;
+#endif
   hipLaunchKernelGGL((mech_pcg_b_kernel<LDSF, NP>), dim3(passb_blocks(Nof(L))), dim3(PW_TPB), 0, s, L, u, r, z, P, w,
                      invD, slot, R);
 }
@@ -690,6 +747,16 @@ void launch_pf_pcg_b(hipStream_t s, const LevelGeom& L, double* d, double* r, do
                      const double* w, const double* invD, int np, int slot, const Reduce& R) {
   const size_t N = Nof(L);
   const dim3 g(passb_blocks(N)), b(PW_TPB);
+  const PassBArgs a{L, N, d, r, z, P, w, invD, slot, R};
+  bool rec = false;
+  switch (np) {
+    case 0: rec = record<PfPassBK<0>>(g, a); break;
+    case 1: rec = record<PfPassBK<1>>(g, a); break;
+    case 2: rec = record<PfPassBK<2>>(g, a); break;
+    case 3: rec = record<PfPassBK<3>>(g, a); break;
+    default: rec = record<PfPassBK<4>>(g, a); break;
+  }
+  if (rec) return;
   switch (np) {
     case 0: hipLaunchKernelGGL(pf_pcg_b_kernel<0>, g, b, 0, s, N, d, r, z, P, w, invD, slot, R); break;
     case 1: hipLaunchKernelGGL(pf_pcg_b_kernel<1>, g, b, 0, s, N, d, r, z, P, w, invD, slot, R); break;
@@ -707,10 +774,10 @@ struct Planes {
   int n;
 };
 
-__global__ void dirichlet_bd_kernel(const LevelGeom L, double* u, double Ut, Planes P) {
+__device__ __forceinline__ void dirichlet_bd_body(const LevelGeom& L, double* u, double Ut, const Planes& P, const Blk& bk) {
   const size_t N = Nof(L);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
+  const int i = bk.x * blockDim.x + threadIdx.x;
+  const int j = bk.y;
   if (i >= L.nx) return;
   for (int t = 0; t < P.n; t++) {
     const int k = P.k[t];
@@ -721,6 +788,22 @@ __global__ void dirichlet_bd_kernel(const LevelGeom L, double* u, double Ut, Pla
     if (dm & 4) u[2 * N + q] = (dm & 8) ? Ut : 0.0;
   }
 }
+__global__ void dirichlet_bd_kernel(const LevelGeom L, double* u, double Ut, Planes P) {
+  dirichlet_bd_body(L, u, Ut, P, Blk::hw());
+}
+struct DirichletBdK {
+  struct Args {
+    LevelGeom L;
+    double* u;
+    double Ut;
+    Planes P;
+  };
+  static constexpr int NT = 64, MINW = 1;
+  static dim3 block() { return dim3(64); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    dirichlet_bd_body(a.L, a.u, a.Ut, a.P, bk);
+  }
+};
 __global__ void dirichlet_copy_kernel(const LevelGeom L, double* f, const double* Fi, Planes P) {
   const size_t N = Nof(L);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -743,6 +826,7 @@ static Planes planes_of(const int* kp, int n) {
 }
 void launch_dirichlet_bd(hipStream_t s, const LevelGeom& L, double* u, double Ut, const int* kp, int n) {
   if (n <= 0) return;
+  if (record<DirichletBdK>(dim3((L.nx + 63) / 64, L.ny), {L, u, Ut, planes_of(kp, n)})) return;
   hipLaunchKernelGGL(dirichlet_bd_kernel, dim3((L.nx + 63) / 64, L.ny), dim3(64), 0, s, L, u, Ut, planes_of(kp, n));
 }
 void launch_dirichlet_copy(hipStream_t s, const LevelGeom& L, double* f, const double* Fi, const int* kp, int n) {
@@ -801,7 +885,7 @@ using SmallArgs = SmallArgsT<double>;
 
 // TIO: type of the vectors in HBM; the iteration itself is FP64
 template <int NC, class TIO>
-__global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgsT<TIO> a) {
+__device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
   const LevelGeom& L = a.L;
   __shared__ double sP[NC][SMALL_NODES];
   __shared__ double sE[NC * 8][SMALL_ELEMS];
@@ -1020,6 +1104,18 @@ __global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgsT<T
   }
 }
 
+template <int NC, class TIO>
+__global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgsT<TIO> a) {
+  small_pcg_body<NC, TIO>(a);
+}
+template <int NC, class TIO>
+struct SmallPcgK {
+  using Args = SmallArgsT<TIO>;
+  static constexpr int NT = SMALL_TPB, MINW = 1;
+  static dim3 block() { return dim3(SMALL_TPB); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk&) { small_pcg_body<NC, TIO>(a); }
+};
+
 // coarsest solve of the FP32 preconditioner: zero start, homogeneous
 // Dirichlet values, FP64 iteration on float right-hand side and result
 void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, float* x,
@@ -1034,6 +1130,7 @@ void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke,
   a.Ut = 0.0;
   a.NB = NB;
   a.scal = scal;
+  if (record<SmallPcgK<3, float>>(dim3(1), a)) return;
   small_pcg_kernel<3, float><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
 }
 
@@ -1060,6 +1157,7 @@ void launch_small_pcg_u(hipStream_t s, const LevelGeom& L, const double* Ke, con
   a.Ut = Ut;
   a.NB = NB;
   a.scal = scal;
+  if (record<SmallPcgK<3, double>>(dim3(1), a)) return;
   small_pcg_kernel<3, double><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
 }
 
@@ -1081,6 +1179,7 @@ void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, cons
   a.w = w;
   a.NB = NB;
   a.scal = scal;
+  if (record<SmallPcgK<1, double>>(dim3(1), a)) return;
   small_pcg_kernel<1, double><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
 }
 
@@ -1136,11 +1235,9 @@ void launch_image_mp(hipStream_t s, size_t n, const unsigned char* img, const Ma
 // Transfers (multigrid_intergrid.cpp)
 // ---------------------------------------------------------------------------
 // solution_injection + Dirichlet_BD + copy to uold (multigrid_intergrid.cpp:25-121)
-__global__ __launch_bounds__(256) void inject_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
-                                                     const double* __restrict__ uf, double* __restrict__ uc,
-                                                     double* __restrict__ ucold, int apply_bc, double Ut) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+__device__ __forceinline__ void inject_body(const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* __restrict__ uf, double* __restrict__ uc, double* __restrict__ ucold, int apply_bc, double Ut, const Blk& bk) {
+  const int i = bk.x * 64 + threadIdx.x;
+  const int j = bk.y * 4 + threadIdx.y, k = bk.z;
   if (i >= Lc.nx || j >= Lc.ny) return;
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const size_t qc = nidx(Lc, i, j, k), qf = nidx(Lf, 2 * i, 2 * j, 2 * k);
@@ -1152,8 +1249,30 @@ __global__ __launch_bounds__(256) void inject_kernel(const LevelGeom Lc, const L
     if (ucold) ucold[c * Nc + qc] = v;
   }
 }
+__global__ __launch_bounds__(256) void inject_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp, const double* __restrict__ uf, double* __restrict__ uc, double* __restrict__ ucold, int apply_bc, double Ut) {
+  inject_body(Lc, Lf, ncomp, uf, uc, ucold, apply_bc, Ut, Blk::hw());
+}
+struct InjectK {
+  struct Args {
+    LevelGeom Lc;
+    LevelGeom Lf;
+    int ncomp;
+    const double* uf;
+    double* uc;
+    double* ucold;
+    int apply_bc;
+    double Ut;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(64, 4); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    inject_body(a.Lc, a.Lf, a.ncomp, a.uf, a.uc, a.ucold, a.apply_bc, a.Ut, bk);
+  }
+};
 void launch_inject(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* uf,
                    double* uc, double* ucold, int apply_bc, double Ut) {
+  if (record<InjectK>(dim3((Lc.nx + 63) / 64, (Lc.ny + 3) / 4, Lc.nz), {Lc, Lf, ncomp, uf, uc, ucold, apply_bc, Ut}))
+    return;
   hipLaunchKernelGGL(inject_kernel, dim3((Lc.nx + 63) / 64, (Lc.ny + 3) / 4, Lc.nz), dim3(64, 4), 0, s, Lc, Lf,
                      ncomp, uf, uc, ucold, apply_bc, Ut);
 }
@@ -1174,11 +1293,8 @@ __device__ __forceinline__ void node_ijk(const LevelGeom& L, unsigned q, int& i,
   k = (int)kk;
 }
 
-__global__ __launch_bounds__(256) void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
-                                                       const double* __restrict__ rf,
-                                                       const double* __restrict__ Fic,
-                                                       double* __restrict__ fc) {
-  const unsigned qn = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void restrict_body(const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* __restrict__ rf, const double* __restrict__ Fic, double* __restrict__ fc, const Blk& bk) {
+  const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lc)) return;
   int i, j, k;
   node_ijk(Lc, qn, i, j, k);
@@ -1214,25 +1330,57 @@ __global__ __launch_bounds__(256) void restrict_kernel(const LevelGeom Lc, const
     fc[c * Nc + qc] = Fic[c * Nc + qc] + acc;
   }
 }
+__global__ __launch_bounds__(256) void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp, const double* __restrict__ rf, const double* __restrict__ Fic, double* __restrict__ fc) {
+  restrict_body(Lc, Lf, ncomp, rf, Fic, fc, Blk::hw());
+}
+struct RestrictK {
+  struct Args {
+    LevelGeom Lc;
+    LevelGeom Lf;
+    int ncomp;
+    const double* rf;
+    const double* Fic;
+    double* fc;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(256); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    restrict_body(a.Lc, a.Lf, a.ncomp, a.rf, a.Fic, a.fc, bk);
+  }
+};
 void launch_restrict(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* rf,
                      const double* Fic, double* fc) {
+  if (record<RestrictK>(dim3((unsigned)((Nof(Lc) + 255) / 256)), {Lc, Lf, ncomp, rf, Fic, fc})) return;
   hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((Nof(Lc) + 255) / 256)), dim3(256), 0, s, Lc, Lf, ncomp, rf,
                      Fic, fc);
 }
 
 // e_c = u_c - u_c,old (the coarse-grid correction, multigrid_intergrid.cpp:484-518)
-__global__ void diff_kernel(size_t n, const double* __restrict__ a, const double* __restrict__ b,
-                            double* __restrict__ e) {
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void diff_body(size_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ e, const Blk& bk) {
+  for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
     e[q] = a[q] - b[q];
 }
+__global__ void diff_kernel(size_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ e) {
+  diff_body(n, a, b, e, Blk::hw());
+}
+struct DiffK {
+  struct Args {
+    size_t n;
+    const double* a;
+    const double* b;
+    double* e;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(256); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    diff_body(a.n, a.a, a.b, a.e, bk);
+  }
+};
 
 // refine_standard_routine + Dirichlet_BD (multigrid_intergrid.cpp:416-549):
 // trilinear interpolation of the correction e_c, one writer per fine node
-__global__ __launch_bounds__(256) void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp,
-                                                      double* __restrict__ uf, const double* __restrict__ e,
-                                                      int apply_bc, double Ut) {
-  const unsigned qn = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, int ncomp, double* __restrict__ uf, const double* __restrict__ e, int apply_bc, double Ut, const Blk& bk) {
+  const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lf)) return;
   int x, y, z;
   node_ijk(Lf, qn, x, y, z);
@@ -1263,9 +1411,32 @@ __global__ __launch_bounds__(256) void prolong_kernel(const LevelGeom Lf, const 
     uf[c * Nf + qf] = v;
   }
 }
+__global__ __launch_bounds__(256) void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp, double* __restrict__ uf, const double* __restrict__ e, int apply_bc, double Ut) {
+  prolong_body(Lf, Lc, ncomp, uf, e, apply_bc, Ut, Blk::hw());
+}
+struct ProlongK {
+  struct Args {
+    LevelGeom Lf;
+    LevelGeom Lc;
+    int ncomp;
+    double* uf;
+    const double* e;
+    int apply_bc;
+    double Ut;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(256); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    prolong_body(a.Lf, a.Lc, a.ncomp, a.uf, a.e, a.apply_bc, a.Ut, bk);
+  }
+};
 void launch_prolong(hipStream_t s, const LevelGeom& Lf, const LevelGeom& Lc, int ncomp, double* uf,
                     const double* uc, const double* ucold, int apply_bc, double Ut, double* scratch) {
   const size_t n = (size_t)ncomp * Nof(Lc);
+  if (record<DiffK>(dim3(pw_blocks(n)), {n, uc, ucold, scratch})) {
+    record<ProlongK>(dim3((unsigned)((Nof(Lf) + 255) / 256)), {Lf, Lc, ncomp, uf, scratch, apply_bc, Ut});
+    return;
+  }
   hipLaunchKernelGGL(diff_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, uc, ucold, scratch);
   hipLaunchKernelGGL(prolong_kernel, dim3((unsigned)((Nof(Lf) + 255) / 256)), dim3(256), 0, s, Lf, Lc, ncomp, uf,
                      scratch, apply_bc, Ut);

@@ -175,13 +175,13 @@ struct MechOpF {
     }
   }
 
-  __device__ __forceinline__ void finish(Acc& acc) const {
+  __device__ __forceinline__ void finish(Acc& acc, const Blk& bk) const {
     double* scal = R.scal;
     if constexpr (MODE == F_CHEBQ) {
       double v1[1] = {acc.a[0]};
-      reduce_finalize<RED_SUM, 1>(v1, R, 0.0, [=](double (&t)[1]) { scal[S_ZQ] = t[0]; });
+      reduce_finalize<RED_SUM, 1>(bk, v1, R, 0.0, [=](double (&t)[1]) { scal[S_ZQ] = t[0]; });
     } else if constexpr (MODE == F_POWER) {
-      reduce_finalize<RED_SUM, 2>(acc.a, R, 0.0, [=](double (&t)[2]) { scal[S_LAM] = t[0] / t[1]; });
+      reduce_finalize<RED_SUM, 2>(bk, acc.a, R, 0.0, [=](double (&t)[2]) { scal[S_LAM] = t[0] / t[1]; });
     }
   }
 };
@@ -305,9 +305,9 @@ struct FcgOpA {
       if constexpr (LAG) u[c * N + o] = fma(st.aprev, nd.p[c], nd.u[c]);
     }
   }
-  __device__ __forceinline__ void finish(Acc& acc) const {
+  __device__ __forceinline__ void finish(Acc& acc, const Blk& bk) const {
     double* scal = R.scal;
-    reduce_finalize<RED_SUM, 2>(acc.a, R, 0.0, [=](double (&t)[2]) {
+    reduce_finalize<RED_SUM, 2>(bk, acc.a, R, 0.0, [=](double (&t)[2]) {
       scal[S_PQ] = t[0];
       scal[S_PR] = t[1];
       scal[S_FALPHA] = t[1] / t[0];

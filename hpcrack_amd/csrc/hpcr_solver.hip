@@ -9,10 +9,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hpcrack.h"
@@ -90,6 +95,9 @@ struct hpcr_ctx {
   // mixed-precision solve (allocated on first use)
   std::vector<MixLev> mix;
   double* q2 = nullptr;  // second FP64 q buffer of the flexible CG
+  // fusion engine this context's multigrid cycles go through (optional)
+  hpcr_fuser* fuser = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering with the fuser's stream
 };
 
 namespace {
@@ -158,6 +166,7 @@ MechCoef mcoef(hpcr_ctx* c, int l) {
 PfCoef pcoef(hpcr_ctx* c, int l) { return PfCoef{c->lev[l].H, c->lev[l].gc, c->par.lc, 1.0 / c->par.lc}; }
 
 int read_scal(hpcr_ctx* c) {
+  if (t_recorder) return set_err(c, HPCR_ERR_STATE, "host read inside a recorded multigrid cycle");
   HIPCHK(c, hipMemcpyAsync(c->host_scal, c->scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return HPCR_OK;
@@ -169,13 +178,13 @@ struct TBracket {
   int kernel, level;
   hipEvent_t a = nullptr, b = nullptr;
   TBracket(hpcr_ctx* c_, int k, int l) : c(c_), kernel(k), level(l) {
-    if (!c->timing || l != c->L) return;
+    if (!c->timing || l != c->L || t_recorder) return;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     (void)hipEventRecord(a, c->stream);
   }
   ~TBracket() {
-    if (!c->timing || level != c->L) return;
+    if (!a) return;
     (void)hipEventRecord(b, c->stream);
     c->trec.push_back(TimingRec{kernel, level, a, b});
   }
@@ -309,9 +318,301 @@ int capture_graph(hpcr_ctx* c, hipGraphExec_t* exec, F body) {
   return HPCR_OK;
 }
 
+int cycle_d(hpcr_ctx* c, int l);
+
+}  // namespace
+
+// ---- fusion engine ------------------------------------------------------------
+// Multigrid cycles of several contexts run as multi-problem launches on one
+// stream.  A context attached to a fuser hands each cycle of its solve loop
+// to the fuser's thread and waits; the thread gathers the cycles submitted
+// within a short window (or until every attached context inside a cycle loop
+// has submitted), records each context's launch sequence (the same launch
+// functions, with a recorder set), and issues position i of the sequences of
+// contexts with the same cycle structure as one launch per kernel type.  Every
+// problem keeps its own grid, argument record, reduction buffers and device
+// scalars, so its results are bit-identical to a cycle of its own.  A batch's
+// launch sequence is captured as a HIP graph and replayed while the same
+// contexts come back with the same load.
+struct FuseJob {
+  hpcr_ctx* c = nullptr;
+  int kind = 0;  // 0: cycle_u, 1: cycle_d
+  double Ut = 0.0;
+  int t = 0;
+  int rc = 0;
+  long long iters = 0;
+  bool done = false;
+};
+
+struct hpcr_fuser {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int window_us = 200;
+  bool graphs = true;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::vector<FuseJob*> pending;
+  int in_loop = 0;  // attached contexts inside a solve's cycle loop
+  int attached = 0;
+  bool stop = false;
+  std::thread th;
+  struct Plan {
+    std::vector<hpcr_ctx*> members;
+    std::vector<long long> iters;
+    hipGraphExec_t exec = nullptr;
+    long long launches = 0, problem_launches = 0;
+    int max_problems = 0;
+    unsigned long long used = 0;
+  };
+  std::mutex plan_mu;  // plans: the fuser's thread and detaching contexts
+  std::map<std::string, Plan> plans;
+  unsigned long long tick = 0;
+  hpcr_fuser_stats st{};
+};
+
+namespace {
+
+constexpr size_t FUSE_PLAN_MAX = 64;  // cached batch graphs per fuser
+constexpr int FUSE_GROUP_MAX = 8;     // problems per launch (MULTI_MAX of the kernels)
+
+// Contexts with equal signatures record cycles whose positions correspond:
+// same kind, level count, cycle parameters, and one-workgroup coarse levels.
+std::string cycle_signature(hpcr_ctx* c, int kind) {
+  const hpcr_mg& m = kind == 0 ? c->par.mg_u : c->par.mg_d;
+  std::string s = std::to_string(kind) + ":" + std::to_string(c->L) + ":" + std::to_string(m.nu0) + "," +
+                  std::to_string(m.nu1) + "," + std::to_string(m.nu2) + "," + std::to_string(m.gamma) + ":";
+  for (int l = 0; l < c->L; l++) s += small_level(c->lev[l].g) ? 'S' : 'T';
+  return s;
+}
+
+std::string plan_key(const std::vector<FuseJob*>& g) {
+  std::string k;
+  for (FuseJob* j : g) {
+    char b[96];
+    std::snprintf(b, sizeof b, "%p/%d/%a/%d;", (void*)j->c, j->kind, j->kind == 0 ? j->Ut : 0.0,
+                  (j->kind == 0 && j->t == 0) ? 1 : 0);
+    k += b;
+  }
+  return k;
+}
+
+void drop_plan(hpcr_fuser::Plan& p) {
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  p.exec = nullptr;
+}
+
+// Issue the recorded sequences: position by position, one launch per kernel
+// type present at that position.
+void issue(hpcr_fuser* F, std::vector<Recorder>& recs, hpcr_fuser::Plan& plan) {
+  const size_t n = recs[0].recs.size();
+  std::vector<const Rec*> ptr;
+  std::vector<char> used(recs.size());
+  for (size_t pos = 0; pos < n; pos++) {
+    std::fill(used.begin(), used.end(), 0);
+    for (size_t i = 0; i < recs.size(); i++) {
+      if (used[i]) continue;
+      const MultiFn fn = recs[i].recs[pos].fn;
+      ptr.clear();
+      for (size_t k = i; k < recs.size(); k++)
+        if (!used[k] && recs[k].recs[pos].fn == fn) {
+          used[k] = 1;
+          ptr.push_back(&recs[k].recs[pos]);
+        }
+      fn(F->stream, ptr.data(), (int)ptr.size());
+      plan.launches += ((long long)ptr.size() + FUSE_GROUP_MAX - 1) / FUSE_GROUP_MAX;
+      plan.problem_launches += (long long)ptr.size();
+      plan.max_problems = std::max(plan.max_problems, std::min((int)ptr.size(), FUSE_GROUP_MAX));
+    }
+  }
+}
+
+// One batch of contexts with the same cycle signature (at most FUSE_GROUP_MAX).
+void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g) {
+  const std::string key = plan_key(g);
+  F->tick++;
+  if (F->graphs) {
+    auto it = F->plans.find(key);
+    if (it != F->plans.end()) {
+      hpcr_fuser::Plan& p = it->second;
+      hipError_t e = hipGraphLaunch(p.exec, F->stream);
+      for (size_t i = 0; i < g.size(); i++) {
+        g[i]->iters = p.iters[i];
+        if (e != hipSuccess) {
+          g[i]->c->err = std::string("fused graph launch: ") + hipGetErrorString(e);
+          g[i]->rc = HPCR_ERR_DEVICE;
+        }
+      }
+      p.used = F->tick;
+      F->st.graph_replays++;
+      F->st.launches += p.launches;
+      F->st.problem_launches += p.problem_launches;
+      F->st.max_problems = std::max(F->st.max_problems, p.max_problems);
+      return;
+    }
+  }
+  // record every member's cycle
+  std::vector<Recorder> recs(g.size());
+  hpcr_fuser::Plan plan;
+  for (size_t i = 0; i < g.size(); i++) {
+    hpcr_ctx* c = g[i]->c;
+    const long long it0 = c->smoother_iters;
+    t_recorder = &recs[i];
+    const int rc = g[i]->kind == 0 ? cycle_u(c, c->L, g[i]->Ut, g[i]->t) : cycle_d(c, c->L);
+    t_recorder = nullptr;
+    g[i]->iters = c->smoother_iters - it0;
+    c->smoother_iters = it0;
+    g[i]->rc = rc;
+    plan.members.push_back(c);
+    plan.iters.push_back(g[i]->iters);
+  }
+  bool ok = true;
+  for (size_t i = 0; i < g.size(); i++)
+    if (g[i]->rc || recs[i].recs.size() != recs[0].recs.size() || recs[i].recs.empty()) ok = false;
+  if (!ok) {
+    // a failed recording, or sequences that do not line up: each member alone
+    for (size_t i = 0; i < g.size(); i++) {
+      if (g[i]->rc || recs[i].recs.empty()) continue;
+      std::vector<Recorder> one(1);
+      one[0].recs.swap(recs[i].recs);
+      hpcr_fuser::Plan p1;
+      issue(F, one, p1);
+      F->st.launches += p1.launches;
+      F->st.problem_launches += p1.problem_launches;
+    }
+    return;
+  }
+  hipGraph_t graph = nullptr;
+  bool capture = F->graphs;
+  if (capture && hipStreamBeginCapture(F->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) capture = false;
+  issue(F, recs, plan);
+  if (capture) {
+    hipError_t e = hipStreamEndCapture(F->stream, &graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&plan.exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e == hipSuccess) e = hipGraphLaunch(plan.exec, F->stream);
+    if (e != hipSuccess) {
+      drop_plan(plan);
+      for (FuseJob* j : g) {
+        j->c->err = std::string("fused graph: ") + hipGetErrorString(e);
+        j->rc = HPCR_ERR_DEVICE;
+      }
+      return;
+    }
+    F->st.graph_captures++;
+    if (F->plans.size() >= FUSE_PLAN_MAX) {
+      auto lru = F->plans.begin();
+      for (auto it = F->plans.begin(); it != F->plans.end(); ++it)
+        if (it->second.used < lru->second.used) lru = it;
+      drop_plan(lru->second);
+      F->plans.erase(lru);
+    }
+    plan.used = F->tick;
+    F->plans[key] = plan;
+  } else {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      for (FuseJob* j : g) {
+        j->c->err = std::string("fused launch: ") + hipGetErrorString(e);
+        j->rc = HPCR_ERR_DEVICE;
+      }
+  }
+  F->st.launches += plan.launches;
+  F->st.problem_launches += plan.problem_launches;
+  F->st.max_problems = std::max(F->st.max_problems, plan.max_problems);
+}
+
+void run_batch(hpcr_fuser* F, std::vector<FuseJob*>& jobs) {
+  std::lock_guard<std::mutex> plk(F->plan_mu);
+  for (FuseJob* j : jobs)
+    if (hipStreamWaitEvent(F->stream, j->c->ev_in, 0) != hipSuccess) {
+      j->c->err = "fuser: cannot order after the context's stream";
+      j->rc = HPCR_ERR_DEVICE;
+    }
+  // group by cycle signature, in submission order
+  std::vector<std::string> sig(jobs.size());
+  std::vector<char> taken(jobs.size(), 0);
+  for (size_t i = 0; i < jobs.size(); i++) sig[i] = cycle_signature(jobs[i]->c, jobs[i]->kind);
+  for (size_t i = 0; i < jobs.size(); i++) {
+    if (taken[i] || jobs[i]->rc) continue;
+    std::vector<FuseJob*> g;
+    for (size_t k = i; k < jobs.size() && (int)g.size() < FUSE_GROUP_MAX; k++)
+      if (!taken[k] && !jobs[k]->rc && sig[k] == sig[i]) {
+        taken[k] = 1;
+        g.push_back(jobs[k]);
+      }
+    run_group(F, g);
+    F->st.groups++;
+  }
+  for (FuseJob* j : jobs)
+    if (hipEventRecord(j->c->ev_out, F->stream) != hipSuccess && !j->rc) {
+      j->c->err = "fuser: cannot record the batch's end";
+      j->rc = HPCR_ERR_DEVICE;
+    }
+  F->st.batches++;
+  F->st.cycles += (long long)jobs.size();
+}
+
+void fuser_loop(hpcr_fuser* F) {
+  (void)hipSetDevice(F->device);
+  std::unique_lock<std::mutex> lk(F->mu);
+  for (;;) {
+    F->cv_work.wait(lk, [&] { return F->stop || !F->pending.empty(); });
+    if (F->pending.empty()) break;  // stop requested, nothing left
+    const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(F->window_us);
+    F->cv_work.wait_until(lk, until, [&] { return F->stop || (int)F->pending.size() >= F->in_loop; });
+    std::vector<FuseJob*> jobs;
+    jobs.swap(F->pending);
+    lk.unlock();
+    run_batch(F, jobs);
+    lk.lock();
+    for (FuseJob* j : jobs) j->done = true;
+    F->cv_done.notify_all();
+  }
+}
+
+// a context's cycle through its fuser (the caller waits for the batch to be
+// issued; the context's stream then waits for the batch on the device)
+int fused_cycle(hpcr_ctx* c, int kind, double Ut, int t) {
+  hpcr_fuser* F = c->fuser;
+  HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
+  FuseJob j;
+  j.c = c;
+  j.kind = kind;
+  j.Ut = Ut;
+  j.t = t;
+  {
+    std::unique_lock<std::mutex> lk(F->mu);
+    F->pending.push_back(&j);
+    F->cv_work.notify_all();
+    F->cv_done.wait(lk, [&] { return j.done; });
+  }
+  if (j.rc) return j.rc;
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
+  c->smoother_iters += j.iters;
+  c->last_rr_level = c->L;
+  return HPCR_OK;
+}
+
+// marks an attached context as inside a solve's cycle loop
+struct FuseLoop {
+  hpcr_fuser* F;
+  explicit FuseLoop(hpcr_ctx* c) : F(c->fuser) {
+    if (!F) return;
+    std::lock_guard<std::mutex> lk(F->mu);
+    F->in_loop++;
+  }
+  ~FuseLoop() {
+    if (!F) return;
+    std::lock_guard<std::mutex> lk(F->mu);
+    F->in_loop--;
+    F->cv_work.notify_all();
+  }
+};
+
 // one mechanical cycle on the finest level, as a graph replay when enabled
 // (kernel timing records events per launch, so it runs the cycle eagerly)
 int run_cycle_u(hpcr_ctx* c, double Ut, int t) {
+  if (c->fuser) return fused_cycle(c, 0, Ut, t);
   if (!c->graphs || c->timing) return cycle_u(c, c->L, Ut, t);
   const int t0 = (t == 0) ? 1 : 0;
   if (!c->g_u || c->g_u_Ut != Ut || c->g_u_t0 != t0) {
@@ -328,8 +629,8 @@ int run_cycle_u(hpcr_ctx* c, double Ut, int t) {
   c->last_rr_level = c->L;
   return HPCR_OK;
 }
-int cycle_d(hpcr_ctx* c, int l);
 int run_cycle_d(hpcr_ctx* c) {
+  if (c->fuser) return fused_cycle(c, 1, 0.0, 0);
   if (!c->graphs || c->timing) return cycle_d(c, c->L);
   if (!c->g_d) {
     const long long it0 = c->smoother_iters;
@@ -376,16 +677,19 @@ int u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {
   double rel = 1.0;
   if (t >= 1 && *f_norm != 0) rel = r_norm / *f_norm;
   int cycles = 0;
-  for (int j = 1; j <= c->par.mg_u.ncy; j++) {
-    if (rel > c->par.tol_u) {
-      if ((rc = run_cycle_u(c, Ut, t))) return rc;
-      cycles++;
-      if ((rc = finest_rr(c, true, &r_norm))) return rc;
-      if (t == 0) *f_norm = c->host_scal[S_FF];
-      rel = r_norm / *f_norm;
-      if (st && cycles <= HPCR_HIST_MAX) st->history[cycles - 1] = r_norm;
+  {
+    FuseLoop in_loop(c);
+    for (int j = 1; j <= c->par.mg_u.ncy; j++) {
+      if (rel > c->par.tol_u) {
+        if ((rc = run_cycle_u(c, Ut, t))) return rc;
+        cycles++;
+        if ((rc = finest_rr(c, true, &r_norm))) return rc;
+        if (t == 0) *f_norm = c->host_scal[S_FF];
+        rel = r_norm / *f_norm;
+        if (st && cycles <= HPCR_HIST_MAX) st->history[cycles - 1] = r_norm;
+      }
+      if (rel <= c->par.tol_u) break;
     }
-    if (rel <= c->par.tol_u) break;
   }
   // externe_force_nodal (mechanical_right_hnd.cpp:18-38)
   launch_dirichlet_bd(c->stream, F.g, F.u, Ut, F.kplanes.data(), (int)F.kplanes.size());
@@ -708,19 +1012,22 @@ int phase_field(hpcr_ctx* c, int t, double* fd_norm, hpcr_stats* st) {
   if (st) st->r_norm_initial = r_norm;
   double sres = 1.0;
   int cycles = 0;
-  for (int j = 1; j <= c->par.mg_d.ncy; j++) {
-    if (r_norm > c->par.tol_d_abs) {
-      if (sres > c->par.tol_d_rel) {
-        if ((rc = run_cycle_d(c))) return rc;
-        cycles++;
+  {
+    FuseLoop in_loop(c);
+    for (int j = 1; j <= c->par.mg_d.ncy; j++) {
+      if (r_norm > c->par.tol_d_abs) {
+        if (sres > c->par.tol_d_rel) {
+          if ((rc = run_cycle_d(c))) return rc;
+          cycles++;
+        }
+        if ((rc = finest_rr(c, false, &r_norm))) return rc;
+        *fd_norm = c->host_scal[S_FF];
+        sres = r_norm / *fd_norm;
+        if (st && cycles >= 1 && cycles <= HPCR_HIST_MAX) st->history[cycles - 1] = r_norm;
+        if (sres <= c->par.tol_d_rel) break;
+      } else {
+        break;
       }
-      if ((rc = finest_rr(c, false, &r_norm))) return rc;
-      *fd_norm = c->host_scal[S_FF];
-      sres = r_norm / *fd_norm;
-      if (st && cycles >= 1 && cycles <= HPCR_HIST_MAX) st->history[cycles - 1] = r_norm;
-      if (sres <= c->par.tol_d_rel) break;
-    } else {
-      break;
     }
   }
   c->diag_d_valid = false;
@@ -882,9 +1189,34 @@ long long estimate_bytes(const hpcr_geometry& G, bool mixed) {
   return total;
 }
 
+// detach a context from its fuser: cached batch graphs that hold it go
+void fuser_detach(hpcr_ctx* c) {
+  hpcr_fuser* F = c->fuser;
+  if (!F) return;
+  {
+    std::lock_guard<std::mutex> plk(F->plan_mu);
+    for (auto it = F->plans.begin(); it != F->plans.end();) {
+      if (std::find(it->second.members.begin(), it->second.members.end(), c) != it->second.members.end()) {
+        drop_plan(it->second);
+        it = F->plans.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(F->mu);
+    F->attached--;
+  }
+  c->fuser = nullptr;
+}
+
 void destroy(hpcr_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  fuser_detach(c);
+  if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+  if (c->ev_out) (void)hipEventDestroy(c->ev_out);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& r : c->trec) {
     (void)hipEventDestroy(r.a);
@@ -1056,6 +1388,75 @@ int hpcr_coarse_vr(hpcr_ctx* c, int f) {
 int hpcr_set_graphs(hpcr_ctx* c, int on) {
   if (!c) return HPCR_ERR_ARG;
   c->graphs = on != 0;
+  return HPCR_OK;
+}
+
+int hpcr_fuser_create(int device, int window_us, hpcr_fuser** out) {
+  if (!out || window_us < 0) return HPCR_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return HPCR_ERR_DEVICE;
+  hpcr_fuser* F = new hpcr_fuser();
+  F->device = device;
+  F->window_us = window_us;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete F;
+    return HPCR_ERR_DEVICE;
+  }
+  F->th = std::thread(fuser_loop, F);
+  *out = F;
+  return HPCR_OK;
+}
+
+int hpcr_fuser_destroy(hpcr_fuser* F) {
+  if (!F) return HPCR_OK;
+  {
+    std::lock_guard<std::mutex> lk(F->mu);
+    if (F->attached > 0) return HPCR_ERR_STATE;
+    F->stop = true;
+    F->cv_work.notify_all();
+  }
+  F->th.join();
+  (void)hipSetDevice(F->device);
+  (void)hipStreamSynchronize(F->stream);
+  for (auto& kv : F->plans) drop_plan(kv.second);
+  (void)hipStreamDestroy(F->stream);
+  delete F;
+  return HPCR_OK;
+}
+
+int hpcr_fuser_set_graphs(hpcr_fuser* F, int on) {
+  if (!F) return HPCR_ERR_ARG;
+  std::lock_guard<std::mutex> plk(F->plan_mu);
+  F->graphs = on != 0;
+  if (!F->graphs) {
+    for (auto& kv : F->plans) drop_plan(kv.second);
+    F->plans.clear();
+  }
+  return HPCR_OK;
+}
+
+int hpcr_fuser_get_stats(hpcr_fuser* F, hpcr_fuser_stats* out) {
+  if (!F || !out) return HPCR_ERR_ARG;
+  std::lock_guard<std::mutex> lk(F->mu);
+  std::lock_guard<std::mutex> plk(F->plan_mu);
+  *out = F->st;
+  return HPCR_OK;
+}
+
+int hpcr_set_fuser(hpcr_ctx* c, hpcr_fuser* F) {
+  if (!c) return HPCR_ERR_ARG;
+  if (F && F->device != c->device) return set_err(c, HPCR_ERR_ARG, "fuser and context are on different devices");
+  fuser_detach(c);
+  if (!F) return HPCR_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->ev_in) HIPCHK(c, hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+  if (!c->ev_out) HIPCHK(c, hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+  {
+    std::lock_guard<std::mutex> lk(F->mu);
+    F->attached++;
+  }
+  c->fuser = F;
   return HPCR_OK;
 }
 

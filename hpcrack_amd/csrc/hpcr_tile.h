@@ -59,23 +59,31 @@ __device__ __forceinline__ void block_red(double (&v)[NV], double* sh) {
     }
 }
 
-__device__ __forceinline__ unsigned lin_block() {
-  return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-}
-__device__ __forceinline__ unsigned num_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
+// Block index and grid of a kernel body: the hardware's in a plain launch, the
+// problem's own in a multi-problem launch (each problem keeps its grid, so its
+// blocks compute, reduce and order exactly as in a launch of its own).
+struct Blk {
+  unsigned x, y, z, gx, gy, gz;
+  __device__ __forceinline__ static Blk hw() {
+    return Blk{blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.y, gridDim.z};
+  }
+  __device__ __forceinline__ unsigned lin() const { return x + gx * (y + gy * z); }
+  __device__ __forceinline__ unsigned count() const { return gx * gy * gz; }
+};
 
 #ifndef HPCR_TICKET_FENCE
 #define HPCR_TICKET_FENCE 0
 #endif
 // Block partials -> last block -> fin(result[NV]) run by thread 0 of the last block.
 template <int OP, int NV, class Fin>
-__device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R, double init, Fin fin) {
+__device__ __forceinline__ void reduce_finalize(const Blk& bk, double (&v)[NV], const Reduce& R, double init,
+                                                Fin fin) {
   __shared__ double sh[NV * 16];
   __shared__ int is_last;
   block_red<OP, NV>(v, sh);
-  const unsigned nb = num_blocks();
+  const unsigned nb = bk.count();
   if (threadIdx.x == 0) {
-    const unsigned b = lin_block();
+    const unsigned b = bk.lin();
     for (int k = 0; k < NV; k++)
       __hip_atomic_store(&R.partials[(size_t)b * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #if HPCR_TICKET_FENCE
@@ -108,6 +116,10 @@ __device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R
     fin(acc);
     __hip_atomic_store(R.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+template <int OP, int NV, class Fin>
+__device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R, double init, Fin fin) {
+  reduce_finalize<OP, NV>(Blk::hw(), v, R, init, fin);
 }
 
 // ---------------------------------------------------------------------------
@@ -600,7 +612,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 #endif
 
 template <class Op>
-__global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel(const LevelGeom L, const Op op) {
+__device__ __forceinline__ void tile_body(const LevelGeom& L, const Op& op, const Blk& bk) {
   constexpr int NC = Op::NC;  // output components
   constexpr int NV = Op::NV;  // values per node in LDS
   constexpr int NR = Op::NR;  // raw values per node fetched from HBM
@@ -620,7 +632,7 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int ntx = (L.nx + OWNX - 1) / OWNX, nty = (L.ny + OWNY - 1) / OWNY;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int t = xcd_tile(bk.x, bk.gx);
   const int bx0 = (t % ntx) * OWNX, by0 = ((t / ntx) % nty) * OWNY;
   const int z0 = (t / (ntx * nty)) * L.zc;
   const int zend = min(z0 + L.zc, L.nz);
@@ -750,7 +762,12 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel
     op.nfetch(L, exc, eyc, min(kz + 1, L.nz - 1), nd);
     if constexpr (NS == 2) __syncthreads();
   }
-  op.finish(acc);
+  op.finish(acc, bk);
+}
+
+template <class Op>
+__global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel(const LevelGeom L, const Op op) {
+  tile_body<Op>(L, op, Blk::hw());
 }
 
 // the 8 corner values of element (tx, ty) from the LDS planes of its bottom

@@ -28,6 +28,12 @@
 //   --mixed             mechanical solve in the mixed-precision form
 //   --no-graphs         launch the multigrid cycles kernel by kernel instead of
 //                       replaying them as HIP graphs
+//   --fuse [WINDOW_US]  run the multigrid cycles of all admitted requests
+//                       through one fusion engine (hpcr_fuser): cycles
+//                       submitted within WINDOW_US (default 200) run as
+//                       multi-problem launches, same-level sweeps of requests
+//                       with the same level structure in one launch, each batch
+//                       replayed as a HIP graph unless --no-graphs
 //   --load-factor F     u_prop = F * Ue (default 2)
 //   --tol X             squared relative residual of both solves (default 1e-16)
 //   --only ID           run request ID alone (same inputs), nothing else
@@ -85,6 +91,7 @@ struct Options {
   double budget_gb = 200.0;
   bool mixed = false;
   bool graphs = true;
+  int fuse_window_us = -1;  // < 0: no fusion
   double load_factor = 2.0;
   double tol = 1e-16;
   int only = -1;
@@ -188,7 +195,7 @@ int build_image(Request& r) {
 }
 
 // one load step of crack_propagation on a fresh context
-Result solve(const Request& r, const Options& o, double t0_clock, Clock::time_point start) {
+Result solve(const Request& r, const Options& o, hpcr_fuser* fz, Clock::time_point start) {
   Result res;
   res.id = r.id;
   const hpcr_params par = params_for(o);
@@ -198,7 +205,8 @@ Result solve(const Request& r, const Options& o, double t0_clock, Clock::time_po
   const int L = r.geo.maxlevel;
   double f_norm = 0.0, fd_norm = 0.0, FeW = 0.0, Ue = 0.0;
   hpcr_stats su{}, sd{};
-  if (!rc && o.graphs) rc = hpcr_set_graphs(ctx, 1);
+  if (!rc && fz) rc = hpcr_set_fuser(ctx, fz);
+  if (!rc && o.graphs && !fz) rc = hpcr_set_graphs(ctx, 1);
   if (!rc) rc = hpcr_set_image(ctx, r.img.data(), &tab, 1e-6);
   if (!rc) {
     if (o.mixed) {
@@ -228,7 +236,6 @@ Result solve(const Request& r, const Options& o, double t0_clock, Clock::time_po
   res.cyc_d = sd.cycles;
   res.rel_u = f_norm != 0.0 ? su.r_norm_final / f_norm : 0.0;
   res.done = std::chrono::duration<double>(Clock::now() - start).count();
-  (void)t0_clock;
   return res;
 }
 
@@ -267,6 +274,10 @@ int main(int argc, char** argv) {
     else if (s == "--budget-gb") o.budget_gb = std::atof(next());
     else if (s == "--mixed") o.mixed = true;
     else if (s == "--no-graphs") o.graphs = false;
+    else if (s == "--fuse") {
+      o.fuse_window_us = 200;
+      if (a + 1 < argc && argv[a + 1][0] != '-') o.fuse_window_us = std::atoi(argv[++a]);
+    }
     else if (s == "--load-factor") o.load_factor = std::atof(next());
     else if (s == "--tol") o.tol = std::atof(next());
     else if (s == "--only") o.only = std::atoi(next());
@@ -315,6 +326,14 @@ int main(int argc, char** argv) {
       return 1;
     }
 
+  hpcr_fuser* fz = nullptr;
+  if (o.fuse_window_us >= 0) {
+    if (hpcr_fuser_create(0, o.fuse_window_us, &fz) != HPCR_OK || hpcr_fuser_set_graphs(fz, o.graphs ? 1 : 0)) {
+      std::fprintf(stderr, "cannot create the fusion engine\n");
+      return 1;
+    }
+  }
+
   // ---- continuous batching ----
   std::mutex mu;
   std::condition_variable cv;
@@ -357,7 +376,7 @@ int main(int argc, char** argv) {
           peak_active = std::max(peak_active, active);
         }
         const double admit = std::chrono::duration<double>(Clock::now() - start).count();
-        Result r = solve(reqs[i], o, 0.0, start);
+        Result r = solve(reqs[i], o, fz, start);
         r.admit = admit;
         {
           std::lock_guard<std::mutex> lk(mu);
@@ -375,6 +394,11 @@ int main(int argc, char** argv) {
   }
   arrivals.join();
   for (auto& t : workers) t.join();
+  hpcr_fuser_stats fst{};
+  if (fz) {
+    hpcr_fuser_get_stats(fz, &fst);
+    hpcr_fuser_destroy(fz);
+  }
 
   // ---- report ----
   std::vector<double> lat, wait, service;
@@ -410,10 +434,13 @@ int main(int argc, char** argv) {
       "{\"requests\": %zu, \"failed\": %d, \"form\": \"%s\", \"slots\": %d, \"budget_gb\": %.1f, \"rate_per_s\": %.2f, "
       "\"solves_per_s\": %.3f, \"latency_ms_p50\": %.1f, \"latency_ms_p99\": %.1f, \"latency_ms_max\": %.1f, "
       "\"wait_ms_p50\": %.1f, \"service_ms_p50\": %.1f, \"peak_admitted_gb\": %.2f, \"peak_concurrent\": %d, "
-      "\"mean_cycles_u\": %.2f, \"mean_cycles_d\": %.2f, \"makespan_s\": %.3f}\n",
+      "\"mean_cycles_u\": %.2f, \"mean_cycles_d\": %.2f, \"makespan_s\": %.3f, \"fused\": %s, "
+      "\"fused_batches\": %lld, \"fused_cycles\": %lld, \"fused_launches\": %lld, \"fused_problem_launches\": %lld, "
+      "\"fused_max_problems\": %d, \"fused_graph_replays\": %lld}\n",
       reqs.size(), failed, o.mixed ? "mixed" : "fp64", nslots, o.budget_gb, o.rate,
       reqs.size() / std::max(1e-9, last - first), 1e3 * percentile(lat, 0.5), 1e3 * percentile(lat, 0.99),
       1e3 * percentile(lat, 1.0), 1e3 * percentile(wait, 0.5), 1e3 * percentile(service, 0.5), peak_bytes / 1e9,
-      peak_active, (double)cu / reqs.size(), (double)cd / reqs.size(), last);
+      peak_active, (double)cu / reqs.size(), (double)cd / reqs.size(), last, fz ? "true" : "false", fst.batches,
+      fst.cycles, fst.launches, fst.problem_launches, fst.max_problems, fst.graph_replays);
   return failed ? 1 : 0;
 }

@@ -43,6 +43,39 @@ def estimate_bytes(maxlevel, element_nb, domain, mixed=False):
     return out.value
 
 
+class Fuser:
+    """Fusion engine (hpcr_fuser): the multigrid cycles of the Solvers
+    attached to it run as multi-problem launches on one stream, each problem
+    over its own grid (bit-identical results).  Solves of different Solvers
+    may run from different threads (ctypes releases the GIL)."""
+
+    def __init__(self, window_us=200, graphs=True, device=0):
+        L = capi.load()
+        self._L = L
+        h = ctypes.c_void_p()
+        rc = L.hpcr_fuser_create(device, int(window_us), ctypes.byref(h))
+        if rc != 0:
+            raise HpcrError(rc, "cannot create the fuser")
+        self._h = h
+        if not graphs:
+            L.hpcr_fuser_set_graphs(h, 0)
+
+    def stats(self):
+        st = capi.FuserStats()
+        rc = self._L.hpcr_fuser_get_stats(self._h, ctypes.byref(st))
+        if rc != 0:
+            raise HpcrError(rc, "fuser stats")
+        return {k: getattr(st, k) for k, _ in capi.FuserStats._fields_}
+
+    def close(self):
+        """Destroys the fuser; every attached Solver must be detached or closed first."""
+        if getattr(self, "_h", None):
+            rc = self._L.hpcr_fuser_destroy(self._h)
+            if rc != 0:
+                raise HpcrError(rc, "fuser still has attached contexts")
+            self._h = None
+
+
 class Solver:
     def __init__(self, maxlevel, element_nb, domain, small_k=1e-5, lc=3.0,
                  mg_u=(50, 4, 2, 1, 12), mg_d=(10, 2, 1, 1, 12),
@@ -201,6 +234,10 @@ class Solver:
         self._chk(self._L.hpcr_u_entire(self._h, Ut, t, ctypes.byref(fn), ctypes.byref(st)))
         self.last_stats = st
         return fn.value, st.cycles
+
+    def set_fuser(self, fuser):
+        """Run this Solver's multigrid cycles through `fuser` (None detaches)."""
+        self._chk(self._L.hpcr_set_fuser(self._h, fuser._h if fuser is not None else None))
 
     def set_graphs(self, on):
         """Replay whole multigrid cycles as HIP graphs (identical results)."""

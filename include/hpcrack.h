@@ -118,6 +118,37 @@ long long hpcr_device_bytes(const hpcr_ctx* ctx); /* device memory held */
  * of launching their kernels one by one; results are identical.  Off by
  * default; kernel timing (hpcr_timing_enable) runs cycles eagerly. */
 int hpcr_set_graphs(hpcr_ctx* ctx, int on);
+
+/* ---- fusion of several contexts' multigrid cycles ----------------------- */
+/*
+ * No counterpart in the reference (one MPI job solves one image).  For a
+ * server holding many independent problems on one GPU: contexts attached to
+ * one fuser hand each multigrid cycle of hpcr_u_entire / hpcr_phase_field to
+ * the fuser's thread, which runs the cycles submitted together (within
+ * window_us, or once every attached context inside a cycle loop has
+ * submitted) as multi-problem launches: position i of the cycles of contexts
+ * with the same level structure is one launch per kernel type, every problem
+ * over its own grid.  Results are bit-identical to unfused solves.  With
+ * graphs on (the default) a batch's launch sequence is captured as a HIP
+ * graph and replayed while the same contexts return with the same load.
+ * Calls on different contexts may come from different threads; a context's
+ * own calls still come from one thread.
+ */
+typedef struct hpcr_fuser hpcr_fuser;
+typedef struct {
+  long long batches;          /* gathered batches                                */
+  long long groups;           /* same-structure groups (one launch sequence each)  */
+  long long cycles;           /* context cycles run through the fuser              */
+  long long launches;         /* multi-problem launches issued                     */
+  long long problem_launches; /* sum over those launches of their problems        */
+  int max_problems;           /* most problems in one launch                       */
+  long long graph_captures, graph_replays;
+} hpcr_fuser_stats;
+int hpcr_fuser_create(int device, int window_us, hpcr_fuser** out);
+int hpcr_fuser_destroy(hpcr_fuser* fuser); /* HPCR_ERR_STATE while contexts are attached */
+int hpcr_fuser_set_graphs(hpcr_fuser* fuser, int on);
+int hpcr_fuser_get_stats(hpcr_fuser* fuser, hpcr_fuser_stats* out);
+int hpcr_set_fuser(hpcr_ctx* ctx, hpcr_fuser* fuser); /* NULL detaches */
 /* device bytes a context of this geometry holds (with mixed != 0: plus the
  * buffers of the mixed-precision solve), without allocating; for admission
  * control of several contexts on one device */

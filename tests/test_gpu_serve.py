@@ -58,6 +58,35 @@ def test_results_identical_alone_and_in_the_mix(tmp_path, mixed):
             assert float(mix[i]["rel_u"]) <= 1e-16
 
 
+@pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
+def test_fused_serving_identical_to_requests_alone(tmp_path, mixed):
+    """--fuse: the cycles of concurrently admitted requests run as
+    multi-problem launches through one fusion engine (the mixed form fuses its
+    phase-field cycles); every request's result is that of the request alone."""
+    require_gpu()
+    base = [SERVER, "--requests", "12", "--rate", "400", "--sizes", "17,33", "--seed", "11", "--budget-gb", "50"]
+    if mixed:
+        base.append("--mixed")
+    runs = {}
+    for slots, extra in ((6, ["--fuse", "2000"]), (1, [])):
+        out = tmp_path / f"res_{slots}.txt"
+        say(f"[gpu] server {'mixed' if mixed else 'fp64'} slots={slots} {' '.join(extra)}")
+        r = subprocess.run(base + ["--slots", str(slots), "--out", str(out)] + extra, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        summary = json.loads(r.stdout.strip().splitlines()[-1])
+        say(f"  {summary}")
+        assert summary["failed"] == 0 and summary["requests"] == 12
+        runs[slots] = (summary, results(out))
+    fused = runs[6][0]
+    assert fused["fused"] is True and fused["fused_cycles"] > 0
+    assert fused["fused_problem_launches"] >= fused["fused_launches"] > 0
+    mix, alone = runs[6][1], runs[1][1]
+    assert mix.keys() == alone.keys()
+    for i in mix:
+        assert mix[i] == alone[i], (i, mix[i], alone[i])
+
+
 def test_byte_estimate_matches_context():
     HA = require_gpu()
     for ml, enb in ((2, (4, 4, 4)), (3, (4, 8, 4)), (1, (2, 6, 4))):
