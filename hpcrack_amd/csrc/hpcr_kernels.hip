@@ -774,12 +774,16 @@ struct Planes {
   int n;
 };
 
-__device__ __forceinline__ void dirichlet_bd_body(const LevelGeom& L, double* u, double Ut, const Planes& P, const Blk& bk) {
+__device__ __forceinline__ void dirichlet_bd_body(const LevelGeom& L, double* u, double Ut, const Planes& P,
+                                                  const Blk& bk) {
   const size_t N = Nof(L);
   const int i = bk.x * blockDim.x + threadIdx.x;
   const int j = bk.y;
   if (i >= L.nx) return;
-  for (int t = 0; t < P.n; t++) {
+  // unrolled: the plane list is indexed with constants (no private copy)
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    if (t >= P.n) break;
     const int k = P.k[t];
     const int dm = dir_mask(L, i, j, k);
     const size_t q = nidx(L, i, j, k);
@@ -1776,10 +1780,25 @@ __global__ void double_to_flag_kernel(size_t n, const double* in, signed char* f
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
     f[q] = (signed char)in[q];
 }
-__global__ void fill_kernel(size_t n, double* p, double v) {
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void fill_body(size_t n, double* p, double v, const Blk& bk) {
+  for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
     p[q] = v;
 }
+__global__ void fill_kernel(size_t n, double* p, double v) {
+  fill_body(n, p, v, Blk::hw());
+}
+struct FillK {
+  struct Args {
+    size_t n;
+    double* p;
+    double v;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(256); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    fill_body(a.n, a.p, a.v, bk);
+  }
+};
 void launch_flag_to_double(hipStream_t s, size_t n, const signed char* f, double* out) {
   hipLaunchKernelGGL(flag_to_double_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, f, out);
 }
@@ -1787,6 +1806,7 @@ void launch_double_to_flag(hipStream_t s, size_t n, const double* in, signed cha
   hipLaunchKernelGGL(double_to_flag_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, in, f);
 }
 void launch_fill(hipStream_t s, size_t n, double* p, double v) {
+  if (record<FillK>(dim3(pw_blocks(n)), {n, p, v})) return;
   hipLaunchKernelGGL(fill_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, p, v);
 }
 

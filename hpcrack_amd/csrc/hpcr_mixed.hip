@@ -27,6 +27,7 @@
 
 #include "hpcr_internal.h"
 #include "hpcr_tile.h"
+#include "hpcr_multi.h"
 
 namespace hpcr {
 
@@ -339,6 +340,7 @@ static void run_f32(hipStream_t st, const LevelGeom& L, const MixCoef& C, const 
   op.tz_ = (float)(wg / L.hz);
   op.R = R;
   op.N = (size_t)L.nx * L.ny * L.nz;
+  if (record<TileK<MechOpF<MODE, FINE>>>(mix_tile_grid(L), {L, op})) return;
   tile_kernel<MechOpF<MODE, FINE>><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
 }
 
@@ -389,6 +391,7 @@ static void run_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, con
   op.tz_ = wg / L.hz;
   op.R = R;
   op.N = (size_t)L.nx * L.ny * L.nz;
+  if (record<TileK<FcgOpA<LAG>>>(mix_tile_grid(L), {L, op})) return;
   tile_kernel<FcgOpA<LAG>><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
 }
 void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
@@ -441,11 +444,11 @@ __global__ __launch_bounds__(MX_TPB) void mix_start_kernel(unsigned N, const dou
 // LEAN: u and p are left alone (the next pass A, or launch_fcg_flush,
 // applies u += alpha p) and alpha moves to scal[S_APREV].
 template <bool LDSF, bool LEAN>
-__global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double* __restrict__ u,
-                                                       double* __restrict__ r, const double* __restrict__ p,
-                                                       const double* __restrict__ q, float* __restrict__ r32,
-                                                       float* __restrict__ d, float* __restrict__ x,
-                                                       const float* __restrict__ invD, float cr0, const Reduce R) {
+__device__ __forceinline__ void fcg_b_body(const LevelGeom& L, double* __restrict__ u, double* __restrict__ r,
+                                           const double* __restrict__ p, const double* __restrict__ q,
+                                           float* __restrict__ r32, float* __restrict__ d, float* __restrict__ x,
+                                           const float* __restrict__ invD, float cr0, const Reduce& R,
+                                           const Blk& bk) {
   __shared__ unsigned char sflags[LDSF ? DIRF_MAX : 1];
   const DirFlags F{sflags};
   if constexpr (LDSF) {
@@ -456,7 +459,7 @@ __global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double
   const unsigned plane = (unsigned)L.nx * (unsigned)L.ny;
   const double alpha = R.scal[S_FALPHA];
   double acc[1] = {0.0};
-  for (unsigned n = blockIdx.x * MX_TPB + threadIdx.x; n < N; n += gridDim.x * MX_TPB) {
+  for (unsigned n = bk.x * MX_TPB + threadIdx.x; n < N; n += bk.gx * MX_TPB) {
     int dm;
     if constexpr (LDSF) {
       const unsigned k = n / plane, rem = n - k * plane;
@@ -481,11 +484,41 @@ __global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double
     }
   }
   double* scal = R.scal;
-  reduce_finalize<RED_SUM, 1>(acc, R, 0.0, [=](double (&t)[1]) {
+  reduce_finalize<RED_SUM, 1>(bk, acc, R, 0.0, [=](double (&t)[1]) {
     scal[S_RR] = t[0];
     if constexpr (LEAN) scal[S_APREV] = scal[S_FALPHA];
   });
 }
+template <bool LDSF, bool LEAN>
+__global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double* __restrict__ u,
+                                                       double* __restrict__ r, const double* __restrict__ p,
+                                                       const double* __restrict__ q, float* __restrict__ r32,
+                                                       float* __restrict__ d, float* __restrict__ x,
+                                                       const float* __restrict__ invD, float cr0, const Reduce R) {
+  fcg_b_body<LDSF, LEAN>(L, u, r, p, q, r32, d, x, invD, cr0, R, Blk::hw());
+}
+struct FcgBArgs {
+  LevelGeom L;
+  double* u;
+  double* r;
+  const double* p;
+  const double* q;
+  float* r32;
+  float* d;
+  float* x;
+  const float* invD;
+  float cr0;
+  Reduce R;
+};
+template <bool LDSF, bool LEAN>
+struct FcgBK {
+  using Args = FcgBArgs;
+  static constexpr int NT = MX_TPB, MINW = 1;
+  static dim3 block() { return dim3(MX_TPB); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    fcg_b_body<LDSF, LEAN>(a.L, a.u, a.r, a.p, a.q, a.r32, a.d, a.x, a.invD, a.cr0, a.R, bk);
+  }
+};
 
 __global__ __launch_bounds__(MX_TPB) void fcg_flush_kernel(size_t n, double* __restrict__ u,
                                                            const double* __restrict__ p,
@@ -507,11 +540,8 @@ __device__ __forceinline__ void mix_ijk(const LevelGeom& L, unsigned q, int& i, 
   k = (int)kk;
 }
 
-__global__ __launch_bounds__(256) void mix_restrict_kernel(const LevelGeom Lc, const LevelGeom Lf,
-                                                           const float* __restrict__ rf, float* __restrict__ rc,
-                                                           float* __restrict__ dc, float* __restrict__ xc,
-                                                           const float* __restrict__ invD, float cr0, int start) {
-  const unsigned qn = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void mix_restrict_body(const LevelGeom& Lc, const LevelGeom& Lf, const float* __restrict__ rf, float* __restrict__ rc, float* __restrict__ dc, float* __restrict__ xc, const float* __restrict__ invD, float cr0, int start, const Blk& bk) {
+  const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lc)) return;
   int i, j, k;
   mix_ijk(Lc, qn, i, j, k);
@@ -557,13 +587,32 @@ __global__ __launch_bounds__(256) void mix_restrict_kernel(const LevelGeom Lc, c
     }
   }
 }
+__global__ __launch_bounds__(256) void mix_restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, const float* __restrict__ rf, float* __restrict__ rc, float* __restrict__ dc, float* __restrict__ xc, const float* __restrict__ invD, float cr0, int start) {
+  mix_restrict_body(Lc, Lf, rf, rc, dc, xc, invD, cr0, start, Blk::hw());
+}
+struct MixRestrictK {
+  struct Args {
+    LevelGeom Lc;
+    LevelGeom Lf;
+    const float* rf;
+    float* rc;
+    float* dc;
+    float* xc;
+    const float* invD;
+    float cr0;
+    int start;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(256); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    mix_restrict_body(a.Lc, a.Lf, a.rf, a.rc, a.dc, a.xc, a.invD, a.cr0, a.start, bk);
+  }
+};
 
 // prolongation of the coarse correction: c = P e (zero on Dirichlet DOFs),
 // x_f += c, c kept as the stencil vector of the first post-smoothing step
-__global__ __launch_bounds__(256) void mix_prolong_kernel(const LevelGeom Lf, const LevelGeom Lc,
-                                                          float* __restrict__ xf, float* __restrict__ cf,
-                                                          const float* __restrict__ e) {
-  const unsigned qn = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, float* __restrict__ xf, float* __restrict__ cf, const float* __restrict__ e, const Blk& bk) {
+  const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lf)) return;
   int x, y, z;
   mix_ijk(Lf, qn, x, y, z);
@@ -588,6 +637,23 @@ __global__ __launch_bounds__(256) void mix_prolong_kernel(const LevelGeom Lf, co
     cf[o] = corr;
   }
 }
+__global__ __launch_bounds__(256) void mix_prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, float* __restrict__ xf, float* __restrict__ cf, const float* __restrict__ e) {
+  mix_prolong_body(Lf, Lc, xf, cf, e, Blk::hw());
+}
+struct MixProlongK {
+  struct Args {
+    LevelGeom Lf;
+    LevelGeom Lc;
+    float* xf;
+    float* cf;
+    const float* e;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(256); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    mix_prolong_body(a.Lf, a.Lc, a.xf, a.cf, a.e, bk);
+  }
+};
 
 // (lambda, mu) at the Gauss points of every finest-level element (the
 // coefficients of MP_all_GAUSS_Point, computed in FP64 and stored as FP32)
@@ -626,14 +692,44 @@ void launch_mix_lame(hipStream_t st, const LevelGeom& L, const double* K, const 
   hipLaunchKernelGGL(mix_lame_kernel, g, dim3(64, 4), 0, st, L, K, G, d, flag, small_k, lm);
 }
 
-__global__ void to_f64_kernel(size_t n, const float* __restrict__ a, double* __restrict__ b) {
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void to_f64_body(size_t n, const float* __restrict__ a, double* __restrict__ b, const Blk& bk) {
+  for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
     b[q] = (double)a[q];
 }
-__global__ void to_f32_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+__global__ void to_f64_kernel(size_t n, const float* __restrict__ a, double* __restrict__ b) {
+  to_f64_body(n, a, b, Blk::hw());
+}
+struct ToF64K {
+  struct Args {
+    size_t n;
+    const float* a;
+    double* b;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(256); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    to_f64_body(a.n, a.a, a.b, bk);
+  }
+};
+__device__ __forceinline__ void to_f32_body(size_t n, const double* __restrict__ a, float* __restrict__ b, const Blk& bk) {
+  for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
     b[q] = (float)a[q];
 }
+__global__ void to_f32_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+  to_f32_body(n, a, b, Blk::hw());
+}
+struct ToF32K {
+  struct Args {
+    size_t n;
+    const double* a;
+    float* b;
+  };
+  static constexpr int NT = 256, MINW = 1;
+  static dim3 block() { return dim3(256); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    to_f32_body(a.n, a.a, a.b, bk);
+  }
+};
 
 // deterministic start vector of the power iteration, zero on Dirichlet DOFs
 __global__ void power_init_kernel(const LevelGeom L, float* __restrict__ v) {
@@ -662,6 +758,17 @@ void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, cons
                   float* r32, float* d, float* x, const float* invD, float cr0, bool lean, const Reduce& R) {
   const dim3 g(passb_blocks(Nof(L))), b(MX_TPB);
   const bool ldsf = L.nx + L.ny + L.nz <= DIRF_MAX;
+  const FcgBArgs a{L, u, r, p, q, r32, d, x, invD, cr0, R};
+  bool rec;
+  if (ldsf && lean)
+    rec = record<FcgBK<true, true>>(g, a);
+  else if (ldsf)
+    rec = record<FcgBK<true, false>>(g, a);
+  else if (lean)
+    rec = record<FcgBK<false, true>>(g, a);
+  else
+    rec = record<FcgBK<false, false>>(g, a);
+  if (rec) return;
   if (ldsf && lean)
     hipLaunchKernelGGL((fcg_b_kernel<true, true>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
   else if (ldsf)
@@ -678,18 +785,23 @@ void launch_fcg_flush(hipStream_t st, const LevelGeom& L, double* u, const doubl
 }
 void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,
                          float* dc, float* xc, const float* invD, float cr0, int start) {
+  if (record<MixRestrictK>(dim3((unsigned)((Nof(Lc) + 255) / 256)), {Lc, Lf, rf, rc, dc, xc, invD, cr0, start}))
+    return;
   hipLaunchKernelGGL(mix_restrict_kernel, dim3((unsigned)((Nof(Lc) + 255) / 256)), dim3(256), 0, st, Lc, Lf, rf, rc,
                      dc, xc, invD, cr0, start);
 }
 void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, float* xf, float* cf,
                         const float* ec) {
+  if (record<MixProlongK>(dim3((unsigned)((Nof(Lf) + 255) / 256)), {Lf, Lc, xf, cf, ec})) return;
   hipLaunchKernelGGL(mix_prolong_kernel, dim3((unsigned)((Nof(Lf) + 255) / 256)), dim3(256), 0, st, Lf, Lc, xf, cf,
                      ec);
 }
 void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b) {
+  if (record<ToF32K>(dim3(mx_blocks(n, 4096)), {n, a, b})) return;
   hipLaunchKernelGGL(to_f32_kernel, dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);
 }
 void launch_to_f64(hipStream_t st, size_t n, const float* a, double* b) {
+  if (record<ToF64K>(dim3(mx_blocks(n, 4096)), {n, a, b})) return;
   hipLaunchKernelGGL(to_f64_kernel, dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);
 }
 void launch_power_init(hipStream_t st, const LevelGeom& L, float* v) {

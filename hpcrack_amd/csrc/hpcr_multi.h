@@ -36,7 +36,9 @@ __global__ __launch_bounds__(K::NT, K::MINW) void multi_kernel(const MultiArgs<K
   const unsigned gx = m.gx[p], gy = m.gy[p], gxy = gx * gy;
   if (l >= gxy * m.gz[p]) return;  // padding block
   const Blk bk{l % gx, (l / gx) % gy, l / gxy, gx, gy, m.gz[p]};
-  K::run(m.a[p], bk);
+  // the problem's record as a value, as a kernel of its own receives it
+  const typename K::Args a = m.a[p];
+  K::run(a, bk);
 }
 
 template <class K>

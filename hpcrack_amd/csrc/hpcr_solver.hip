@@ -14,6 +14,7 @@
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -336,9 +337,9 @@ int cycle_d(hpcr_ctx* c, int l);
 // contexts come back with the same load.
 struct FuseJob {
   hpcr_ctx* c = nullptr;
-  int kind = 0;  // 0: cycle_u, 1: cycle_d
-  double Ut = 0.0;
-  int t = 0;
+  std::string sig;             // contexts with equal signatures record aligned launch sequences
+  std::string key;             // the recording's own parameters (with c: a batch graph's key)
+  std::function<int()> body;   // records the context's launches (a cycle, an outer iteration)
   int rc = 0;
   long long iters = 0;
   bool done = false;
@@ -346,19 +347,20 @@ struct FuseJob {
 
 struct hpcr_fuser {
   int device = 0;
-  hipStream_t stream = nullptr;
   int window_us = 200;
   bool graphs = true;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::vector<FuseJob*> pending;
-  int in_loop = 0;  // attached contexts inside a solve's cycle loop
+  // attached contexts inside a solve's cycle loop, with the signature of the
+  // last cycle each of them submitted (empty before the first)
+  std::map<hpcr_ctx*, std::string> in_loop;
   int attached = 0;
   bool stop = false;
   std::thread th;
   struct Plan {
     std::vector<hpcr_ctx*> members;
-    std::vector<long long> iters;
+    std::string blob;  // the recorded launches (kernel, grid, argument bytes) the graph holds
     hipGraphExec_t exec = nullptr;
     long long launches = 0, problem_launches = 0;
     int max_problems = 0;
@@ -377,21 +379,25 @@ constexpr int FUSE_GROUP_MAX = 8;     // problems per launch (MULTI_MAX of the k
 
 // Contexts with equal signatures record cycles whose positions correspond:
 // same kind, level count, cycle parameters, and one-workgroup coarse levels.
+// (kind 2: outer iteration of the mixed-precision solve, FP32 V-cycle with
+// the mechanical cycle parameters)
 std::string cycle_signature(hpcr_ctx* c, int kind) {
-  const hpcr_mg& m = kind == 0 ? c->par.mg_u : c->par.mg_d;
+  const hpcr_mg& m = kind == 1 ? c->par.mg_d : c->par.mg_u;
   std::string s = std::to_string(kind) + ":" + std::to_string(c->L) + ":" + std::to_string(m.nu0) + "," +
                   std::to_string(m.nu1) + "," + std::to_string(m.nu2) + "," + std::to_string(m.gamma) + ":";
   for (int l = 0; l < c->L; l++) s += small_level(c->lev[l].g) ? 'S' : 'T';
+  // and finest levels within a factor of two in size: members of a group run
+  // in lockstep, so a small problem is not held to a much larger one's pace
+  s += ":" + std::to_string((int)std::lround(std::log2((double)c->lev[c->L].N)));
   return s;
 }
 
 std::string plan_key(const std::vector<FuseJob*>& g) {
   std::string k;
   for (FuseJob* j : g) {
-    char b[96];
-    std::snprintf(b, sizeof b, "%p/%d/%a/%d;", (void*)j->c, j->kind, j->kind == 0 ? j->Ut : 0.0,
-                  (j->kind == 0 && j->t == 0) ? 1 : 0);
-    k += b;
+    char b[32];
+    std::snprintf(b, sizeof b, "%p/", (void*)j->c);
+    k += b + j->key + ";";
   }
   return k;
 }
@@ -403,7 +409,7 @@ void drop_plan(hpcr_fuser::Plan& p) {
 
 // Issue the recorded sequences: position by position, one launch per kernel
 // type present at that position.
-void issue(hpcr_fuser* F, std::vector<Recorder>& recs, hpcr_fuser::Plan& plan) {
+void issue(hipStream_t st, std::vector<Recorder>& recs, hpcr_fuser::Plan& plan) {
   const size_t n = recs[0].recs.size();
   std::vector<const Rec*> ptr;
   std::vector<char> used(recs.size());
@@ -418,7 +424,7 @@ void issue(hpcr_fuser* F, std::vector<Recorder>& recs, hpcr_fuser::Plan& plan) {
           used[k] = 1;
           ptr.push_back(&recs[k].recs[pos]);
         }
-      fn(F->stream, ptr.data(), (int)ptr.size());
+      fn(st, ptr.data(), (int)ptr.size());
       plan.launches += ((long long)ptr.size() + FUSE_GROUP_MAX - 1) / FUSE_GROUP_MAX;
       plan.problem_launches += (long long)ptr.size();
       plan.max_problems = std::max(plan.max_problems, std::min((int)ptr.size(), FUSE_GROUP_MAX));
@@ -427,21 +433,42 @@ void issue(hpcr_fuser* F, std::vector<Recorder>& recs, hpcr_fuser::Plan& plan) {
 }
 
 // One batch of contexts with the same cycle signature (at most FUSE_GROUP_MAX).
-void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g) {
+// Every member's launches are recorded each time (host work only); a cached
+// graph is replayed only when it was captured from the same recorded bytes,
+// so arguments passed by value that change between solves (the Chebyshev
+// coefficients of the mixed form) never replay stale.
+void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g, hipStream_t st) {
   const std::string key = plan_key(g);
   F->tick++;
+  std::vector<Recorder> recs(g.size());
+  hpcr_fuser::Plan plan;
+  for (size_t i = 0; i < g.size(); i++) {
+    hpcr_ctx* c = g[i]->c;
+    const long long it0 = c->smoother_iters;
+    t_recorder = &recs[i];
+    const int rc = g[i]->body();
+    t_recorder = nullptr;
+    g[i]->iters = c->smoother_iters - it0;
+    c->smoother_iters = it0;
+    g[i]->rc = rc;
+    plan.members.push_back(c);
+  }
   if (F->graphs) {
-    auto it = F->plans.find(key);
-    if (it != F->plans.end()) {
-      hpcr_fuser::Plan& p = it->second;
-      hipError_t e = hipGraphLaunch(p.exec, F->stream);
-      for (size_t i = 0; i < g.size(); i++) {
-        g[i]->iters = p.iters[i];
-        if (e != hipSuccess) {
-          g[i]->c->err = std::string("fused graph launch: ") + hipGetErrorString(e);
-          g[i]->rc = HPCR_ERR_DEVICE;
-        }
+    for (const Recorder& r : recs)
+      for (const Rec& q : r.recs) {
+        plan.blob.append(reinterpret_cast<const char*>(&q.fn), sizeof q.fn);
+        plan.blob.append(reinterpret_cast<const char*>(q.g), sizeof q.g);
+        plan.blob.append(reinterpret_cast<const char*>(q.a), sizeof q.a);
       }
+    auto it = F->plans.find(key);
+    if (it != F->plans.end() && it->second.blob == plan.blob) {
+      hpcr_fuser::Plan& p = it->second;
+      hipError_t e = hipGraphLaunch(p.exec, st);
+      for (FuseJob* j : g)
+        if (e != hipSuccess) {
+          j->c->err = std::string("fused graph launch: ") + hipGetErrorString(e);
+          j->rc = HPCR_ERR_DEVICE;
+        }
       p.used = F->tick;
       F->st.graph_replays++;
       F->st.launches += p.launches;
@@ -449,21 +476,10 @@ void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g) {
       F->st.max_problems = std::max(F->st.max_problems, p.max_problems);
       return;
     }
-  }
-  // record every member's cycle
-  std::vector<Recorder> recs(g.size());
-  hpcr_fuser::Plan plan;
-  for (size_t i = 0; i < g.size(); i++) {
-    hpcr_ctx* c = g[i]->c;
-    const long long it0 = c->smoother_iters;
-    t_recorder = &recs[i];
-    const int rc = g[i]->kind == 0 ? cycle_u(c, c->L, g[i]->Ut, g[i]->t) : cycle_d(c, c->L);
-    t_recorder = nullptr;
-    g[i]->iters = c->smoother_iters - it0;
-    c->smoother_iters = it0;
-    g[i]->rc = rc;
-    plan.members.push_back(c);
-    plan.iters.push_back(g[i]->iters);
+    if (it != F->plans.end()) {  // same contexts, other arguments: capture again
+      drop_plan(it->second);
+      F->plans.erase(it);
+    }
   }
   bool ok = true;
   for (size_t i = 0; i < g.size(); i++)
@@ -475,7 +491,7 @@ void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g) {
       std::vector<Recorder> one(1);
       one[0].recs.swap(recs[i].recs);
       hpcr_fuser::Plan p1;
-      issue(F, one, p1);
+      issue(st, one, p1);
       F->st.launches += p1.launches;
       F->st.problem_launches += p1.problem_launches;
     }
@@ -483,13 +499,13 @@ void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g) {
   }
   hipGraph_t graph = nullptr;
   bool capture = F->graphs;
-  if (capture && hipStreamBeginCapture(F->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) capture = false;
-  issue(F, recs, plan);
+  if (capture && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) capture = false;
+  issue(st, recs, plan);
   if (capture) {
-    hipError_t e = hipStreamEndCapture(F->stream, &graph);
+    hipError_t e = hipStreamEndCapture(st, &graph);
     if (e == hipSuccess) e = hipGraphInstantiate(&plan.exec, graph, nullptr, nullptr, 0);
     if (graph) (void)hipGraphDestroy(graph);
-    if (e == hipSuccess) e = hipGraphLaunch(plan.exec, F->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(plan.exec, st);
     if (e != hipSuccess) {
       drop_plan(plan);
       for (FuseJob* j : g) {
@@ -521,33 +537,48 @@ void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g) {
   F->st.max_problems = std::max(F->st.max_problems, plan.max_problems);
 }
 
+// mark jobs as issued and wake their submitters
+void release_jobs(hpcr_fuser* F, const std::vector<FuseJob*>& js) {
+  std::lock_guard<std::mutex> lk(F->mu);
+  for (FuseJob* j : js) j->done = true;
+  F->cv_done.notify_all();
+}
+
+// A batch: jobs grouped by signature, each group on the next of the fuser's
+// streams after its members' own work; a group's members are released as
+// soon as its launches are issued.
 void run_batch(hpcr_fuser* F, std::vector<FuseJob*>& jobs) {
   std::lock_guard<std::mutex> plk(F->plan_mu);
-  for (FuseJob* j : jobs)
-    if (hipStreamWaitEvent(F->stream, j->c->ev_in, 0) != hipSuccess) {
-      j->c->err = "fuser: cannot order after the context's stream";
-      j->rc = HPCR_ERR_DEVICE;
-    }
-  // group by cycle signature, in submission order
-  std::vector<std::string> sig(jobs.size());
   std::vector<char> taken(jobs.size(), 0);
-  for (size_t i = 0; i < jobs.size(); i++) sig[i] = cycle_signature(jobs[i]->c, jobs[i]->kind);
   for (size_t i = 0; i < jobs.size(); i++) {
-    if (taken[i] || jobs[i]->rc) continue;
+    if (taken[i]) continue;
     std::vector<FuseJob*> g;
     for (size_t k = i; k < jobs.size() && (int)g.size() < FUSE_GROUP_MAX; k++)
-      if (!taken[k] && !jobs[k]->rc && sig[k] == sig[i]) {
+      if (!taken[k] && jobs[k]->sig == jobs[i]->sig) {
         taken[k] = 1;
         g.push_back(jobs[k]);
       }
-    run_group(F, g);
-    F->st.groups++;
-  }
-  for (FuseJob* j : jobs)
-    if (hipEventRecord(j->c->ev_out, F->stream) != hipSuccess && !j->rc) {
-      j->c->err = "fuser: cannot record the batch's end";
-      j->rc = HPCR_ERR_DEVICE;
+    // the group runs on its first member's own stream (a group of one runs
+    // exactly where its context would run it)
+    hipStream_t st = g[0]->c->stream;
+    std::vector<FuseJob*> live;
+    for (FuseJob* j : g) {
+      if (j->c->stream != st && hipStreamWaitEvent(st, j->c->ev_in, 0) != hipSuccess) {
+        j->c->err = "fuser: cannot order after the context's stream";
+        j->rc = HPCR_ERR_DEVICE;
+      } else {
+        live.push_back(j);
+      }
     }
+    if (!live.empty()) run_group(F, live, st);
+    for (FuseJob* j : g)
+      if (hipEventRecord(j->c->ev_out, st) != hipSuccess && !j->rc) {
+        j->c->err = "fuser: cannot record the group's end";
+        j->rc = HPCR_ERR_DEVICE;
+      }
+    F->st.groups++;
+    release_jobs(F, g);
+  }
   F->st.batches++;
   F->st.cycles += (long long)jobs.size();
 }
@@ -559,27 +590,49 @@ void fuser_loop(hpcr_fuser* F) {
     F->cv_work.wait(lk, [&] { return F->stop || !F->pending.empty(); });
     if (F->pending.empty()) break;  // stop requested, nothing left
     const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(F->window_us);
-    F->cv_work.wait_until(lk, until, [&] { return F->stop || (int)F->pending.size() >= F->in_loop; });
+    // wait (at most the window) only for in-loop contexts whose last cycle
+    // could join a pending one's group
+    auto partners_missing = [&] {
+      for (const auto& kv : F->in_loop) {
+        bool pending = false, partner = false;
+        for (const FuseJob* j : F->pending) {
+          if (j->c == kv.first) pending = true;
+          if (j->sig == kv.second) partner = true;
+        }
+        if (partner && !pending) return true;
+      }
+      return false;
+    };
+    F->cv_work.wait_until(lk, until, [&] { return F->stop || !partners_missing(); });
     std::vector<FuseJob*> jobs;
     jobs.swap(F->pending);
     lk.unlock();
     run_batch(F, jobs);
     lk.lock();
-    for (FuseJob* j : jobs) j->done = true;
-    F->cv_done.notify_all();
   }
 }
 
-// a context's cycle through its fuser (the caller waits for the batch to be
-// issued; the context's stream then waits for the batch on the device)
-int fused_cycle(hpcr_ctx* c, int kind, double Ut, int t) {
+// whether another context inside a cycle loop of the same fuser has the
+// signature sig (only then can a fused launch hold more than this context)
+bool fuse_partner(hpcr_ctx* c, const std::string& sig) {
+  hpcr_fuser* F = c->fuser;
+  std::lock_guard<std::mutex> lk(F->mu);
+  for (const auto& kv : F->in_loop)
+    if (kv.first != c && kv.second == sig) return true;
+  return false;
+}
+
+// hand a context's launch sequence (body) to its fuser; the caller waits
+// until the batch is issued, and the context's stream then waits for the
+// batch on the device
+int fuse_submit(hpcr_ctx* c, std::string sig, std::string key, std::function<int()> body) {
   hpcr_fuser* F = c->fuser;
   HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
   FuseJob j;
   j.c = c;
-  j.kind = kind;
-  j.Ut = Ut;
-  j.t = t;
+  j.sig = std::move(sig);
+  j.key = std::move(key);
+  j.body = std::move(body);
   {
     std::unique_lock<std::mutex> lk(F->mu);
     F->pending.push_back(&j);
@@ -589,30 +642,43 @@ int fused_cycle(hpcr_ctx* c, int kind, double Ut, int t) {
   if (j.rc) return j.rc;
   HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
   c->smoother_iters += j.iters;
-  c->last_rr_level = c->L;
   return HPCR_OK;
+}
+
+int fused_cycle(hpcr_ctx* c, int kind, double Ut, int t) {
+  char key[64];
+  if (kind == 0)
+    std::snprintf(key, sizeof key, "u/%a/%d", Ut, t == 0 ? 1 : 0);
+  else
+    std::snprintf(key, sizeof key, "d");
+  int rc = fuse_submit(c, cycle_signature(c, kind), key, [=] { return kind == 0 ? cycle_u(c, c->L, Ut, t) : cycle_d(c, c->L); });
+  if (!rc) c->last_rr_level = c->L;
+  return rc;
 }
 
 // marks an attached context as inside a solve's cycle loop
 struct FuseLoop {
   hpcr_fuser* F;
-  explicit FuseLoop(hpcr_ctx* c) : F(c->fuser) {
+  hpcr_ctx* c;
+  FuseLoop(hpcr_ctx* c_, const std::string& sig) : F(c_->fuser), c(c_) {
     if (!F) return;
     std::lock_guard<std::mutex> lk(F->mu);
-    F->in_loop++;
+    F->in_loop[c] = sig;
   }
-  ~FuseLoop() {
+  ~FuseLoop() { release(); }
+  void release() {
     if (!F) return;
     std::lock_guard<std::mutex> lk(F->mu);
-    F->in_loop--;
+    F->in_loop.erase(c);
     F->cv_work.notify_all();
+    F = nullptr;
   }
 };
 
 // one mechanical cycle on the finest level, as a graph replay when enabled
 // (kernel timing records events per launch, so it runs the cycle eagerly)
 int run_cycle_u(hpcr_ctx* c, double Ut, int t) {
-  if (c->fuser) return fused_cycle(c, 0, Ut, t);
+  if (c->fuser && fuse_partner(c, cycle_signature(c, 0))) return fused_cycle(c, 0, Ut, t);
   if (!c->graphs || c->timing) return cycle_u(c, c->L, Ut, t);
   const int t0 = (t == 0) ? 1 : 0;
   if (!c->g_u || c->g_u_Ut != Ut || c->g_u_t0 != t0) {
@@ -630,7 +696,7 @@ int run_cycle_u(hpcr_ctx* c, double Ut, int t) {
   return HPCR_OK;
 }
 int run_cycle_d(hpcr_ctx* c) {
-  if (c->fuser) return fused_cycle(c, 1, 0.0, 0);
+  if (c->fuser && fuse_partner(c, cycle_signature(c, 1))) return fused_cycle(c, 1, 0.0, 0);
   if (!c->graphs || c->timing) return cycle_d(c, c->L);
   if (!c->g_d) {
     const long long it0 = c->smoother_iters;
@@ -678,7 +744,7 @@ int u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {
   if (t >= 1 && *f_norm != 0) rel = r_norm / *f_norm;
   int cycles = 0;
   {
-    FuseLoop in_loop(c);
+    FuseLoop in_loop(c, cycle_signature(c, 0));
     for (int j = 1; j <= c->par.mg_u.ncy; j++) {
       if (rel > c->par.tol_u) {
         if ((rc = run_cycle_u(c, Ut, t))) return rc;
@@ -881,14 +947,28 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
     if (lag) launch_fcg_flush(c->stream, F.g, F.u, pb[it & 1], red(c));
     lag = false;
   };
-  while (it < mp.max_iter && (rel > mp.tol || (t0 && it == 0))) {
-    if ((rc = mix_vcycle(c, m, c->par.mg_u.nu0, it > 0 ? qb[it & 1] : nullptr))) return rc;
+  // one outer iteration's launches: preconditioner V-cycle, FCG passes A and B
+  auto iteration = [&]() -> int {
+    int e = mix_vcycle(c, m, c->par.mg_u.nu0, it > 0 ? qb[it & 1] : nullptr);
+    if (e) return e;
     launch_fcg_a(c->stream, F.g, mcoef(c, L), MF.x, pb[it & 1], pb[(it + 1) & 1], qb[(it + 1) & 1], F.ru, it == 0,
                  lag ? F.u : nullptr, red(c));
     launch_fcg_b(c->stream, F.g, F.u, F.ru, pb[(it + 1) & 1], qb[(it + 1) & 1], MF.r, MF.dA, MF.x, MF.invD, MF.cr0,
                  true, red(c));
+    return check_launch(c, "flexible CG");
+  };
+  const std::string msig = c->fuser ? cycle_signature(c, 2) + "/m" + std::to_string(m) : std::string();
+  FuseLoop in_loop(c, msig);
+  while (it < mp.max_iter && (rel > mp.tol || (t0 && it == 0))) {
+    if (c->fuser && fuse_partner(c, msig)) {
+      char key[48];
+      std::snprintf(key, sizeof key, "x/%d/%d/%d", it == 0 ? 1 : 0, it & 1, lag ? 1 : 0);
+      rc = fuse_submit(c, msig, key, iteration);
+    } else {
+      rc = iteration();
+    }
+    if (rc) return rc;
     lag = true;
-    if ((rc = check_launch(c, "flexible CG"))) return rc;
     if ((rc = read_scal(c))) return rc;
     rr = c->host_scal[S_RR];
     it++;
@@ -902,6 +982,7 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
     }
     if (st && it <= HPCR_HIST_MAX) st->history[it - 1] = rr;
   }
+  in_loop.release();
   flush();
   if ((rc = check_launch(c, "flexible CG update"))) return rc;
   // externe_force_nodal (mechanical_right_hnd.cpp:18-38)
@@ -1013,7 +1094,7 @@ int phase_field(hpcr_ctx* c, int t, double* fd_norm, hpcr_stats* st) {
   double sres = 1.0;
   int cycles = 0;
   {
-    FuseLoop in_loop(c);
+    FuseLoop in_loop(c, cycle_signature(c, 1));
     for (int j = 1; j <= c->par.mg_d.ncy; j++) {
       if (r_norm > c->par.tol_d_abs) {
         if (sres > c->par.tol_d_rel) {
@@ -1197,6 +1278,9 @@ void fuser_detach(hpcr_ctx* c) {
     std::lock_guard<std::mutex> plk(F->plan_mu);
     for (auto it = F->plans.begin(); it != F->plans.end();) {
       if (std::find(it->second.members.begin(), it->second.members.end(), c) != it->second.members.end()) {
+        // (an executable graph destroyed while a replay runs is freed when
+        // the replay completes; the context's own stream waits for every
+        // fused batch it was in, and destroy() synchronises it before freeing)
         drop_plan(it->second);
         it = F->plans.erase(it);
       } else {
@@ -1399,7 +1483,7 @@ int hpcr_fuser_create(int device, int window_us, hpcr_fuser** out) {
   hpcr_fuser* F = new hpcr_fuser();
   F->device = device;
   F->window_us = window_us;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess) {
     delete F;
     return HPCR_ERR_DEVICE;
   }
@@ -1418,9 +1502,7 @@ int hpcr_fuser_destroy(hpcr_fuser* F) {
   }
   F->th.join();
   (void)hipSetDevice(F->device);
-  (void)hipStreamSynchronize(F->stream);
   for (auto& kv : F->plans) drop_plan(kv.second);
-  (void)hipStreamDestroy(F->stream);
   delete F;
   return HPCR_OK;
 }
@@ -1438,8 +1520,9 @@ int hpcr_fuser_set_graphs(hpcr_fuser* F, int on) {
 
 int hpcr_fuser_get_stats(hpcr_fuser* F, hpcr_fuser_stats* out) {
   if (!F || !out) return HPCR_ERR_ARG;
-  std::lock_guard<std::mutex> lk(F->mu);
+  // lock order plan_mu -> mu, as the fuser's thread takes them
   std::lock_guard<std::mutex> plk(F->plan_mu);
+  std::lock_guard<std::mutex> lk(F->mu);
   *out = F->st;
   return HPCR_OK;
 }

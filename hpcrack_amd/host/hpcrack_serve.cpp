@@ -28,12 +28,14 @@
 //   --mixed             mechanical solve in the mixed-precision form
 //   --no-graphs         launch the multigrid cycles kernel by kernel instead of
 //                       replaying them as HIP graphs
-//   --fuse [WINDOW_US]  run the multigrid cycles of all admitted requests
-//                       through one fusion engine (hpcr_fuser): cycles
-//                       submitted within WINDOW_US (default 200) run as
-//                       multi-problem launches, same-level sweeps of requests
-//                       with the same level structure in one launch, each batch
-//                       replayed as a HIP graph unless --no-graphs
+//   --fuse [WINDOW_US]  attach every admitted request to one fusion engine
+//                       (hpcr_fuser): the cycles of requests with the same
+//                       level structure and size class, submitted within
+//                       WINDOW_US (default 200) of each other, run as
+//                       multi-problem launches (same-level sweeps of several
+//                       requests in one launch), each batch replayed as a HIP
+//                       graph unless --no-graphs; a request with no such
+//                       partner runs its cycles on its own stream
 //   --load-factor F     u_prop = F * Ue (default 2)
 //   --tol X             squared relative residual of both solves (default 1e-16)
 //   --only ID           run request ID alone (same inputs), nothing else
@@ -206,7 +208,7 @@ Result solve(const Request& r, const Options& o, hpcr_fuser* fz, Clock::time_poi
   double f_norm = 0.0, fd_norm = 0.0, FeW = 0.0, Ue = 0.0;
   hpcr_stats su{}, sd{};
   if (!rc && fz) rc = hpcr_set_fuser(ctx, fz);
-  if (!rc && o.graphs && !fz) rc = hpcr_set_graphs(ctx, 1);
+  if (!rc && o.graphs) rc = hpcr_set_graphs(ctx, 1);
   if (!rc) rc = hpcr_set_image(ctx, r.img.data(), &tab, 1e-6);
   if (!rc) {
     if (o.mixed) {

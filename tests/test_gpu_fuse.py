@@ -6,7 +6,9 @@ the results must be identical bit for bit: u, d, the norms, the cycle counts
 and the residual histories.  The mix holds problems with the same level
 structure (they share launches: 17^3, 17x17x33, 33x17x17 and a second 17^3)
 and problems that must not be grouped with them (33^3 has another level count;
-17^3 over a 9^3 coarsest level runs its coarsest smoother as tile passes)."""
+17^3 over a 9^3 coarsest level runs its coarsest smoother as tile passes).
+In the mixed-precision form every outer iteration of the flexible CG (FP32
+V-cycle, FCG passes A and B) goes through the fuser as one unit."""
 import threading
 
 import numpy as np
@@ -45,14 +47,18 @@ def build(HA, ml, enb, seed):
     return s
 
 
-def solve(s, ml, steps=2):
-    """Two staggered increments: unit-load u_entire, Ue, strain history at
-    3 Ue, phase_field; the second step re-solves from the previous state."""
+def solve(s, ml, mixed, steps=2):
+    """Two staggered increments: unit-load u_entire (or its mixed-precision
+    form), Ue, strain history at 3 Ue, phase_field; the second step re-solves
+    from the previous state."""
     out = []
     f_norm = 0.0
     fd_norm = 0.0
     for t in range(steps):
-        f_norm, cu = s.u_entire(1.0, t, f_norm)
+        if mixed:
+            f_norm, cu = s.u_entire_mixed(1.0, t, f_norm, max_iter=200)
+        else:
+            f_norm, cu = s.u_entire(1.0, t, f_norm)
         hu = s.history().tolist()
         iu = s.last_stats.smoother_iters
         Ue = s.U_e()
@@ -66,14 +72,15 @@ def solve(s, ml, steps=2):
     return out, s.get(O.U, ml), s.get(O.D, ml), s.get(O.H, ml)
 
 
+@pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("graphs", [True, False], ids=["graphs", "eager"])
-def test_fused_solves_identical_to_solves_alone(graphs):
+def test_fused_solves_identical_to_solves_alone(graphs, mixed):
     HA = require_gpu()
-    say(f"[gpu] fused multi-problem cycles, graphs={graphs}, {len(PROBLEMS)} problems")
+    say(f"[gpu] fused multi-problem cycles, graphs={graphs}, mixed={mixed}, {len(PROBLEMS)} problems")
     alone = []
     for ml, enb, seed in PROBLEMS:
         s = build(HA, ml, enb, seed)
-        alone.append(solve(s, ml))
+        alone.append(solve(s, ml, mixed))
         s.close()
     fz = HA.Fuser(window_us=20000, graphs=graphs)
     solvers = [build(HA, ml, enb, seed) for ml, enb, seed in PROBLEMS]
@@ -84,7 +91,7 @@ def test_fused_solves_identical_to_solves_alone(graphs):
 
     def run(i):
         try:
-            fused[i] = solve(solvers[i], PROBLEMS[i][0])
+            fused[i] = solve(solvers[i], PROBLEMS[i][0], mixed)
         except Exception as e:  # reported below
             errors.append((i, repr(e)))
 
@@ -105,7 +112,11 @@ def test_fused_solves_identical_to_solves_alone(graphs):
         assert a[0] == b[0], (i, a[0], b[0])
         for x, y in zip(a[1:], b[1:]):
             assert np.array_equal(x, y), i
-    assert st["cycles"] == sum(x["cu"] + x["cd"] for r in alone for x in r[0])
+    # only problems with a partner of the same structure and size class go
+    # through the fuser (33^3 and the 9^3-coarsest 17^3 have none)
+    total = sum(x["cu"] + x["cd"] for r in alone for x in r[0])
+    solo = sum(x["cu"] + x["cd"] for i in (4, 5) for x in alone[i][0])
+    assert 0 < st["cycles"] <= total - solo
     # problems of the same level structure shared launches
     assert st["max_problems"] >= 2
     assert st["problem_launches"] > st["launches"]
