@@ -98,7 +98,10 @@ struct hpcr_ctx {
   double* q2 = nullptr;  // second FP64 q buffer of the flexible CG
   // fusion engine this context's multigrid cycles go through (optional)
   hpcr_fuser* fuser = nullptr;
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering with the fuser's stream
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering with the fused batches
+  // counts the mixed-form set-ups: the Chebyshev coefficients, passed to the
+  // kernels by value, change only there (a fused batch graph's key holds it)
+  unsigned long long mix_epoch = 0;
 };
 
 namespace {
@@ -360,7 +363,7 @@ struct hpcr_fuser {
   std::thread th;
   struct Plan {
     std::vector<hpcr_ctx*> members;
-    std::string blob;  // the recorded launches (kernel, grid, argument bytes) the graph holds
+    std::vector<long long> iters;  // smoother iterations of each member's recording
     hipGraphExec_t exec = nullptr;
     long long launches = 0, problem_launches = 0;
     int max_problems = 0;
@@ -368,6 +371,7 @@ struct hpcr_fuser {
   };
   std::mutex plan_mu;  // plans: the fuser's thread and detaching contexts
   std::map<std::string, Plan> plans;
+  std::map<std::string, int> seen;  // batch keys issued eagerly once (captured when they come back)
   unsigned long long tick = 0;
   hpcr_fuser_stats st{};
 };
@@ -433,13 +437,33 @@ void issue(hipStream_t st, std::vector<Recorder>& recs, hpcr_fuser::Plan& plan) 
 }
 
 // One batch of contexts with the same cycle signature (at most FUSE_GROUP_MAX).
-// Every member's launches are recorded each time (host work only); a cached
-// graph is replayed only when it was captured from the same recorded bytes,
-// so arguments passed by value that change between solves (the Chebyshev
-// coefficients of the mixed form) never replay stale.
+// A job's key holds every value its recording passes to the kernels by value
+// that can change between calls (the load, the first-step flag, the mixed
+// form's buffer parity and coefficient epoch), so a cached graph of the same
+// contexts and keys replays without recording again.
 void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g, hipStream_t st) {
   const std::string key = plan_key(g);
   F->tick++;
+  if (F->graphs) {
+    auto it = F->plans.find(key);
+    if (it != F->plans.end()) {
+      hpcr_fuser::Plan& p = it->second;
+      hipError_t e = hipGraphLaunch(p.exec, st);
+      for (size_t i = 0; i < g.size(); i++) {
+        g[i]->iters = p.iters[i];
+        if (e != hipSuccess) {
+          g[i]->c->err = std::string("fused graph launch: ") + hipGetErrorString(e);
+          g[i]->rc = HPCR_ERR_DEVICE;
+        }
+      }
+      p.used = F->tick;
+      F->st.graph_replays++;
+      F->st.launches += p.launches;
+      F->st.problem_launches += p.problem_launches;
+      F->st.max_problems = std::max(F->st.max_problems, p.max_problems);
+      return;
+    }
+  }
   std::vector<Recorder> recs(g.size());
   hpcr_fuser::Plan plan;
   for (size_t i = 0; i < g.size(); i++) {
@@ -452,34 +476,7 @@ void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g, hipStream_t st) {
     c->smoother_iters = it0;
     g[i]->rc = rc;
     plan.members.push_back(c);
-  }
-  if (F->graphs) {
-    for (const Recorder& r : recs)
-      for (const Rec& q : r.recs) {
-        plan.blob.append(reinterpret_cast<const char*>(&q.fn), sizeof q.fn);
-        plan.blob.append(reinterpret_cast<const char*>(q.g), sizeof q.g);
-        plan.blob.append(reinterpret_cast<const char*>(q.a), sizeof q.a);
-      }
-    auto it = F->plans.find(key);
-    if (it != F->plans.end() && it->second.blob == plan.blob) {
-      hpcr_fuser::Plan& p = it->second;
-      hipError_t e = hipGraphLaunch(p.exec, st);
-      for (FuseJob* j : g)
-        if (e != hipSuccess) {
-          j->c->err = std::string("fused graph launch: ") + hipGetErrorString(e);
-          j->rc = HPCR_ERR_DEVICE;
-        }
-      p.used = F->tick;
-      F->st.graph_replays++;
-      F->st.launches += p.launches;
-      F->st.problem_launches += p.problem_launches;
-      F->st.max_problems = std::max(F->st.max_problems, p.max_problems);
-      return;
-    }
-    if (it != F->plans.end()) {  // same contexts, other arguments: capture again
-      drop_plan(it->second);
-      F->plans.erase(it);
-    }
+    plan.iters.push_back(g[i]->iters);
   }
   bool ok = true;
   for (size_t i = 0; i < g.size(); i++)
@@ -498,7 +495,13 @@ void run_group(hpcr_fuser* F, std::vector<FuseJob*>& g, hipStream_t st) {
     return;
   }
   hipGraph_t graph = nullptr;
-  bool capture = F->graphs;
+  // capturing costs more than issuing eagerly: a batch is captured only when
+  // its key comes back (the membership of batches changes often in serving)
+  bool capture = false;
+  if (F->graphs) {
+    if (F->seen.size() > 4096) F->seen.clear();
+    capture = F->seen[key]++ > 0;
+  }
   if (capture && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) capture = false;
   issue(st, recs, plan);
   if (capture) {
@@ -808,6 +811,7 @@ MixCoef mixcoef(hpcr_ctx* c, int l) {
 // FP32 coefficients and diagonals, power iteration and Chebyshev coefficients
 int mix_setup(hpcr_ctx* c, const hpcr_mixed_params& mp) {
   int rc;
+  c->mix_epoch++;
   const int m = mp.cheb_degree;
   for (int l = 1; l <= c->L; l++) {
     Lev& v = c->lev[l];
@@ -961,8 +965,8 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
   FuseLoop in_loop(c, msig);
   while (it < mp.max_iter && (rel > mp.tol || (t0 && it == 0))) {
     if (c->fuser && fuse_partner(c, msig)) {
-      char key[48];
-      std::snprintf(key, sizeof key, "x/%d/%d/%d", it == 0 ? 1 : 0, it & 1, lag ? 1 : 0);
+      char key[64];
+      std::snprintf(key, sizeof key, "x/%d/%d/%d/%llu", it == 0 ? 1 : 0, it & 1, lag ? 1 : 0, c->mix_epoch);
       rc = fuse_submit(c, msig, key, iteration);
     } else {
       rc = iteration();
