@@ -119,7 +119,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=129, help="nodes per axis of the CPU sample (0: skip)")
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_mech_pcg_a_C2_257.txt"))
+    ap.add_argument("--traffic-file", default="",
+                    help="rocprofv3 PMC summary of pass A on this workload "
+                         "(default: profiles/traffic_mech_pcg_a_<workload>_<n>.txt; traffic is null without one)")
     ap.add_argument("--mixed", type=int, default=1,
                     help="also time the mixed-precision solve (FP64 flexible CG + FP32 V-cycle) to the same "
                          "tolerance (0: skip)")
@@ -191,7 +193,9 @@ def main():
     a = kern["mech_pcg_a"]
     avg_s = a["avg_ms"] / 1e3
     achieved = PASS_A_BYTES_PER_NODE * N / avg_s / 1e9
-    traffic = read_traffic(args.traffic_file)
+    traffic_file = args.traffic_file or os.path.join(ROOT, "profiles",
+                                                     f"traffic_mech_pcg_a_{args.workload}_{w.dims[0]}.txt")
+    traffic = read_traffic(traffic_file)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
