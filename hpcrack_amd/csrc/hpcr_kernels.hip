@@ -1239,7 +1239,9 @@ void launch_image_mp(hipStream_t s, size_t n, const unsigned char* img, const Ma
 // Transfers (multigrid_intergrid.cpp)
 // ---------------------------------------------------------------------------
 // solution_injection + Dirichlet_BD + copy to uold (multigrid_intergrid.cpp:25-121)
-__device__ __forceinline__ void inject_body(const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* __restrict__ uf, double* __restrict__ uc, double* __restrict__ ucold, int apply_bc, double Ut, const Blk& bk) {
+__device__ __forceinline__ void inject_body(const LevelGeom& Lc, const LevelGeom& Lf, int ncomp,
+                                            const double* __restrict__ uf, double* __restrict__ uc,
+                                            double* __restrict__ ucold, int apply_bc, double Ut, const Blk& bk) {
   const int i = bk.x * 64 + threadIdx.x;
   const int j = bk.y * 4 + threadIdx.y, k = bk.z;
   if (i >= Lc.nx || j >= Lc.ny) return;
@@ -1253,7 +1255,9 @@ __device__ __forceinline__ void inject_body(const LevelGeom& Lc, const LevelGeom
     if (ucold) ucold[c * Nc + qc] = v;
   }
 }
-__global__ __launch_bounds__(256) void inject_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp, const double* __restrict__ uf, double* __restrict__ uc, double* __restrict__ ucold, int apply_bc, double Ut) {
+__global__ __launch_bounds__(256) void inject_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
+                                                     const double* __restrict__ uf, double* __restrict__ uc,
+                                                     double* __restrict__ ucold, int apply_bc, double Ut) {
   inject_body(Lc, Lf, ncomp, uf, uc, ucold, apply_bc, Ut, Blk::hw());
 }
 struct InjectK {
@@ -1297,7 +1301,9 @@ __device__ __forceinline__ void node_ijk(const LevelGeom& L, unsigned q, int& i,
   k = (int)kk;
 }
 
-__device__ __forceinline__ void restrict_body(const LevelGeom& Lc, const LevelGeom& Lf, int ncomp, const double* __restrict__ rf, const double* __restrict__ Fic, double* __restrict__ fc, const Blk& bk) {
+__device__ __forceinline__ void restrict_body(const LevelGeom& Lc, const LevelGeom& Lf, int ncomp,
+                                              const double* __restrict__ rf, const double* __restrict__ Fic,
+                                              double* __restrict__ fc, const Blk& bk) {
   const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lc)) return;
   int i, j, k;
@@ -1334,7 +1340,9 @@ __device__ __forceinline__ void restrict_body(const LevelGeom& Lc, const LevelGe
     fc[c * Nc + qc] = Fic[c * Nc + qc] + acc;
   }
 }
-__global__ __launch_bounds__(256) void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp, const double* __restrict__ rf, const double* __restrict__ Fic, double* __restrict__ fc) {
+__global__ __launch_bounds__(256) void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
+                                                       const double* __restrict__ rf, const double* __restrict__ Fic,
+                                                       double* __restrict__ fc) {
   restrict_body(Lc, Lf, ncomp, rf, Fic, fc, Blk::hw());
 }
 struct RestrictK {
@@ -1360,11 +1368,13 @@ void launch_restrict(hipStream_t s, const LevelGeom& Lc, const LevelGeom& Lf, in
 }
 
 // e_c = u_c - u_c,old (the coarse-grid correction, multigrid_intergrid.cpp:484-518)
-__device__ __forceinline__ void diff_body(size_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ e, const Blk& bk) {
+__device__ __forceinline__ void diff_body(size_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                          double* __restrict__ e, const Blk& bk) {
   for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
     e[q] = a[q] - b[q];
 }
-__global__ void diff_kernel(size_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ e) {
+__global__ void diff_kernel(size_t n, const double* __restrict__ a, const double* __restrict__ b,
+                            double* __restrict__ e) {
   diff_body(n, a, b, e, Blk::hw());
 }
 struct DiffK {
@@ -1383,7 +1393,9 @@ struct DiffK {
 
 // refine_standard_routine + Dirichlet_BD (multigrid_intergrid.cpp:416-549):
 // trilinear interpolation of the correction e_c, one writer per fine node
-__device__ __forceinline__ void prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, int ncomp, double* __restrict__ uf, const double* __restrict__ e, int apply_bc, double Ut, const Blk& bk) {
+__device__ __forceinline__ void prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, int ncomp,
+                                             double* __restrict__ uf, const double* __restrict__ e, int apply_bc,
+                                             double Ut, const Blk& bk) {
   const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lf)) return;
   int x, y, z;
@@ -1415,7 +1427,9 @@ __device__ __forceinline__ void prolong_body(const LevelGeom& Lf, const LevelGeo
     uf[c * Nf + qf] = v;
   }
 }
-__global__ __launch_bounds__(256) void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp, double* __restrict__ uf, const double* __restrict__ e, int apply_bc, double Ut) {
+__global__ __launch_bounds__(256) void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp,
+                                                      double* __restrict__ uf, const double* __restrict__ e,
+                                                      int apply_bc, double Ut) {
   prolong_body(Lf, Lc, ncomp, uf, e, apply_bc, Ut, Blk::hw());
 }
 struct ProlongK {

@@ -540,7 +540,10 @@ __device__ __forceinline__ void mix_ijk(const LevelGeom& L, unsigned q, int& i, 
   k = (int)kk;
 }
 
-__device__ __forceinline__ void mix_restrict_body(const LevelGeom& Lc, const LevelGeom& Lf, const float* __restrict__ rf, float* __restrict__ rc, float* __restrict__ dc, float* __restrict__ xc, const float* __restrict__ invD, float cr0, int start, const Blk& bk) {
+__device__ __forceinline__ void mix_restrict_body(const LevelGeom& Lc, const LevelGeom& Lf,
+                                                  const float* __restrict__ rf, float* __restrict__ rc,
+                                                  float* __restrict__ dc, float* __restrict__ xc,
+                                                  const float* __restrict__ invD, float cr0, int start, const Blk& bk) {
   const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lc)) return;
   int i, j, k;
@@ -587,7 +590,10 @@ __device__ __forceinline__ void mix_restrict_body(const LevelGeom& Lc, const Lev
     }
   }
 }
-__global__ __launch_bounds__(256) void mix_restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, const float* __restrict__ rf, float* __restrict__ rc, float* __restrict__ dc, float* __restrict__ xc, const float* __restrict__ invD, float cr0, int start) {
+__global__ __launch_bounds__(256) void mix_restrict_kernel(const LevelGeom Lc, const LevelGeom Lf,
+                                                           const float* __restrict__ rf, float* __restrict__ rc,
+                                                           float* __restrict__ dc, float* __restrict__ xc,
+                                                           const float* __restrict__ invD, float cr0, int start) {
   mix_restrict_body(Lc, Lf, rf, rc, dc, xc, invD, cr0, start, Blk::hw());
 }
 struct MixRestrictK {
@@ -611,7 +617,8 @@ struct MixRestrictK {
 
 // prolongation of the coarse correction: c = P e (zero on Dirichlet DOFs),
 // x_f += c, c kept as the stencil vector of the first post-smoothing step
-__device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, float* __restrict__ xf, float* __restrict__ cf, const float* __restrict__ e, const Blk& bk) {
+__device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, float* __restrict__ xf,
+                                                 float* __restrict__ cf, const float* __restrict__ e, const Blk& bk) {
   const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lf)) return;
   int x, y, z;
@@ -637,7 +644,9 @@ __device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const Leve
     cf[o] = corr;
   }
 }
-__global__ __launch_bounds__(256) void mix_prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, float* __restrict__ xf, float* __restrict__ cf, const float* __restrict__ e) {
+__global__ __launch_bounds__(256) void mix_prolong_kernel(const LevelGeom Lf, const LevelGeom Lc,
+                                                          float* __restrict__ xf, float* __restrict__ cf,
+                                                          const float* __restrict__ e) {
   mix_prolong_body(Lf, Lc, xf, cf, e, Blk::hw());
 }
 struct MixProlongK {
@@ -692,7 +701,8 @@ void launch_mix_lame(hipStream_t st, const LevelGeom& L, const double* K, const 
   hipLaunchKernelGGL(mix_lame_kernel, g, dim3(64, 4), 0, st, L, K, G, d, flag, small_k, lm);
 }
 
-__device__ __forceinline__ void to_f64_body(size_t n, const float* __restrict__ a, double* __restrict__ b, const Blk& bk) {
+__device__ __forceinline__ void to_f64_body(size_t n, const float* __restrict__ a, double* __restrict__ b,
+                                            const Blk& bk) {
   for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
     b[q] = (double)a[q];
 }
@@ -711,7 +721,8 @@ struct ToF64K {
     to_f64_body(a.n, a.a, a.b, bk);
   }
 };
-__device__ __forceinline__ void to_f32_body(size_t n, const double* __restrict__ a, float* __restrict__ b, const Blk& bk) {
+__device__ __forceinline__ void to_f32_body(size_t n, const double* __restrict__ a, float* __restrict__ b,
+                                            const Blk& bk) {
   for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
     b[q] = (float)a[q];
 }

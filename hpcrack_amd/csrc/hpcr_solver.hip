@@ -654,7 +654,8 @@ int fused_cycle(hpcr_ctx* c, int kind, double Ut, int t) {
     std::snprintf(key, sizeof key, "u/%a/%d", Ut, t == 0 ? 1 : 0);
   else
     std::snprintf(key, sizeof key, "d");
-  int rc = fuse_submit(c, cycle_signature(c, kind), key, [=] { return kind == 0 ? cycle_u(c, c->L, Ut, t) : cycle_d(c, c->L); });
+  int rc = fuse_submit(c, cycle_signature(c, kind), key,
+                       [=] { return kind == 0 ? cycle_u(c, c->L, Ut, t) : cycle_d(c, c->L); });
   if (!rc) c->last_rr_level = c->L;
   return rc;
 }
