@@ -10,7 +10,13 @@ depend on size:
 - symmetry of the heterogeneous degraded operator: y.(A x) = x.(A y);
 - repeated solves are bit-identical;
 - the mixed-precision solve's reported residual equals an independent
-  recomputation from u and the reactions, and reaches the tolerance."""
+  recomputation from u and the reactions, and reaches the tolerance;
+- the mixed-precision solution at the tolerance (relative residual 1e-8)
+  lies within 1e-8 relative L2 of the same FP64 system converged to a
+  squared relative residual of 1e-22, modulo the zero-energy rotation mode
+  (the flexible CG's outer operator, residual and updates are FP64, so the
+  converged solution is the FP64 one whatever the preconditioner's
+  precision)."""
 import numpy as np
 import pytest
 
@@ -153,6 +159,18 @@ def test_c2_solves_repeat_bitwise_and_mixed_residual_is_true(c2):
     diff = np.linalg.norm(du) / np.linalg.norm(u)
     say(f"  |u_mixed - u_plain| / |u| modulo rotation: {diff:.3e}")
     assert diff < 3e-2
+    # against the converged FP64 solution
+    zero()
+    fr, itr = s.u_entire_mixed(1.0, 0, 0.0, tol=1e-22, max_iter=400)
+    rel_ref = s.last_stats.r_norm_final / fr
+    u_ref = s.get(O.U, L)
+    de = (u - u_ref).ravel()
+    de -= (de @ m) / (m @ m) * m
+    err = np.linalg.norm(de) / np.linalg.norm(u_ref)
+    say(f"  converged FP64 reference: {itr} iterations, rel. residual^2 {rel_ref:.3e}; "
+        f"|u_mixed - u_ref| / |u_ref| modulo rotation: {err:.3e}")
+    assert rel_ref <= 1e-22
+    assert err < 1e-8
 
 
 def W_rotation(w):
