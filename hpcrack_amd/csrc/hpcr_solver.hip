@@ -327,17 +327,21 @@ int cycle_d(hpcr_ctx* c, int l);
 }  // namespace
 
 // ---- fusion engine ------------------------------------------------------------
-// Multigrid cycles of several contexts run as multi-problem launches on one
-// stream.  A context attached to a fuser hands each cycle of its solve loop
-// to the fuser's thread and waits; the thread gathers the cycles submitted
-// within a short window (or until every attached context inside a cycle loop
-// has submitted), records each context's launch sequence (the same launch
-// functions, with a recorder set), and issues position i of the sequences of
-// contexts with the same cycle structure as one launch per kernel type.  Every
-// problem keeps its own grid, argument record, reduction buffers and device
-// scalars, so its results are bit-identical to a cycle of its own.  A batch's
-// launch sequence is captured as a HIP graph and replayed while the same
-// contexts come back with the same load.
+// Multigrid cycles (and mixed-form outer iterations) of several contexts run as
+// multi-problem launches.  A context attached to a fuser that has a partner --
+// another attached context inside a solve loop with the same cycle signature
+// -- hands each cycle to the fuser's thread and waits; without a partner it
+// runs the cycle on its own stream as an unfused context does.  The thread
+// gathers the cycles submitted within a window (cut short once no possible
+// partner is missing), records each context's launch sequence (the same
+// launch functions, with a recorder set), and issues position i of the
+// sequences of a group of same-signature contexts as one launch per kernel
+// type, on the first member's stream, ordered by events after and before the
+// other members' streams.  Every problem keeps its own grid, argument record,
+// reduction buffers and device scalars, so its results are bit-identical to a
+// cycle of its own.  A batch is issued eagerly the first time its key (the
+// members and the values they pass by value) is seen, and captured as a HIP
+// graph and replayed when the key recurs.
 struct FuseJob {
   hpcr_ctx* c = nullptr;
   std::string sig;             // contexts with equal signatures record aligned launch sequences

@@ -122,17 +122,18 @@ int hpcr_set_graphs(hpcr_ctx* ctx, int on);
 /* ---- fusion of several contexts' multigrid cycles ----------------------- */
 /*
  * No counterpart in the reference (one MPI job solves one image).  For a
- * server holding many independent problems on one GPU: contexts attached to
- * one fuser hand each multigrid cycle of hpcr_u_entire / hpcr_phase_field to
- * the fuser's thread, which runs the cycles submitted together (within
- * window_us, or once every attached context inside a cycle loop has
- * submitted) as multi-problem launches: position i of the cycles of contexts
- * with the same level structure is one launch per kernel type, every problem
- * over its own grid.  Results are bit-identical to unfused solves.  With
- * graphs on (the default) a batch's launch sequence is captured as a HIP
- * graph and replayed while the same contexts return with the same load.
- * Calls on different contexts may come from different threads; a context's
- * own calls still come from one thread.
+ * server holding many independent problems on one GPU: a context attached to
+ * a fuser hands each multigrid cycle of hpcr_u_entire / hpcr_phase_field (and
+ * each outer iteration of hpcr_u_entire_mixed) to the fuser's thread whenever
+ * another attached context with the same level structure and size class is
+ * inside a solve; otherwise it runs the cycle itself.  The fuser runs the
+ * cycles submitted together (within window_us, cut short once no possible
+ * partner is missing) as multi-problem launches: position i of the cycles of
+ * such contexts is one launch per kernel type, every problem over its own
+ * grid.  Results are bit-identical to unfused solves.  With graphs on (the
+ * default) a batch is captured as a HIP graph when it recurs (same contexts,
+ * same load) and replayed.  Calls on different contexts may come from
+ * different threads; a context's own calls still come from one thread.
  */
 typedef struct hpcr_fuser hpcr_fuser;
 typedef struct {
