@@ -117,7 +117,7 @@ def main():
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--ncy", type=int, default=30)
     ap.add_argument("--cpu-sample", type=int, default=129, help="nodes per axis of the CPU sample (0: skip)")
-    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--cpu-cycles", type=int, default=6)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--traffic-file", default="",
                     help="rocprofv3 PMC summary of pass A on this workload "
