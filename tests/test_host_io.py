@@ -49,6 +49,26 @@ def test_no_device_means_clean_error():
     assert e.value.code == -2
 
 
+def test_fuser_argument_checks():
+    """The fusion engine's entry points reject bad arguments before any
+    device call (so this runs without a GPU), and with no device visible the
+    engine cannot be created."""
+    import hpcrack_amd as HA
+    from hpcrack_amd import capi
+    L = capi.load()
+    out = ctypes.c_void_p()
+    assert L.hpcr_fuser_create(0, -1, ctypes.byref(out)) == capi.ERR_ARG
+    assert L.hpcr_fuser_create(0, 100, None) == capi.ERR_ARG
+    assert L.hpcr_fuser_destroy(None) == capi.OK
+    assert L.hpcr_fuser_set_graphs(None, 1) == capi.ERR_ARG
+    assert L.hpcr_fuser_get_stats(None, ctypes.byref(capi.FuserStats())) == capi.ERR_ARG
+    assert L.hpcr_set_fuser(None, None) == capi.ERR_ARG
+    if HA.device_count() == 0:
+        with pytest.raises(HA.HpcrError) as e:
+            HA.Fuser()
+        assert e.value.code == capi.ERR_DEVICE
+
+
 def test_reference_config_strict_drops_spaced_keys(hio):
     c = hio.read_config(os.path.join(GOLD, "reference_config.txt"), strict=True)
     seen = hio.config_seen(c)
