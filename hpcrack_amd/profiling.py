@@ -8,6 +8,10 @@
         scaled by what the same counters read for the plain device copy of B
         bytes recorded in the same run (rocclr's copyBuffer kernel)
 
+    python -m hpcrack_amd.profiling counters DB [--kernel SUBSTR]
+        per-dispatch average of every counter recorded for the largest-grid
+        dispatch of the kernel whose name contains SUBSTR
+
 The databases are the *_results.db files that rocprofv3 writes with
 --output-format rocpd (its default on ROCm 7).
 """
@@ -78,6 +82,18 @@ def traffic(fetch_db, write_db, kernel, copy_bytes):
     return "\n".join(lines)
 
 
+def counters(db, kernel):
+    cur = sqlite3.connect(db).cursor()
+    names = sorted({r[0] for r in cur.execute("select distinct counter_name from counters_collection")})
+    lines = []
+    for c in names:
+        (kn, kg), avg, n = _pick(_counter(db, c), kernel)
+        if not lines:
+            lines.append(f"kernel: {_short(kn)} grid_size={kg} (dispatches: {n})")
+        lines.append(f"{c}: {avg:.6g}")
+    return "\n".join(lines)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace_summary(argv[1], int(argv[2]) if len(argv) > 2 else 25))
@@ -92,6 +108,10 @@ def main(argv):
                 copy_bytes = int(rest[1])
             rest = rest[2:]
         print(traffic(argv[1], argv[2], kernel, copy_bytes))
+        return 0
+    if len(argv) >= 2 and argv[0] == "counters":
+        kernel = argv[3] if len(argv) >= 4 and argv[2] == "--kernel" else "MechOp<2, true>"
+        print(counters(argv[1], kernel))
         return 0
     print(__doc__)
     return 2
