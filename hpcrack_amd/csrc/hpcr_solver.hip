@@ -943,7 +943,14 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
   double rr = 0.0;
   if ((rc = true_residual(&rr))) return rc;
   if (st) st->r_norm_initial = rr;
-  double rel = (*f_norm != 0.0) ? rr / *f_norm : 1.0;
+  // relative residual as u_entire tests it (mechanical_displacement.cpp:84-102):
+  // with f_norm == 0 (e.g. a zero load) a zero residual is converged and a
+  // nonzero one never is, so the solve runs to its iteration cap like the
+  // reference's cycles do
+  auto relres = [&](double r) {
+    return (*f_norm != 0.0) ? r / *f_norm : (r == 0.0 ? 0.0 : HUGE_VAL);
+  };
+  double rel = (rr == 0.0) ? 0.0 : ((*f_norm != 0.0) ? rr / *f_norm : 1.0);
   double* pb[2] = {F.p, F.p2};
   double* qb[2] = {F.w, c->q2};
   int it = 0;
@@ -968,7 +975,9 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
   };
   const std::string msig = c->fuser ? cycle_signature(c, 2) + "/m" + std::to_string(m) : std::string();
   FuseLoop in_loop(c, msig);
-  while (it < mp.max_iter && (rel > mp.tol || (t0 && it == 0))) {
+  // at t == 0 the reference always runs a first cycle; with an exactly zero
+  // residual there is no direction to search (and p.q = 0)
+  while (it < mp.max_iter && (rel > mp.tol || (t0 && it == 0 && rr != 0.0))) {
     if (c->fuser && fuse_partner(c, msig)) {
       char key[64];
       std::snprintf(key, sizeof key, "x/%d/%d/%d/%llu", it == 0 ? 1 : 0, it & 1, lag ? 1 : 0, c->mix_epoch);
@@ -981,13 +990,13 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
     if ((rc = read_scal(c))) return rc;
     rr = c->host_scal[S_RR];
     it++;
-    rel = rr / *f_norm;
-    if (!std::isfinite(rel)) return set_err(c, HPCR_ERR_STATE, "mixed-precision iteration diverged");
+    if (!std::isfinite(rr)) return set_err(c, HPCR_ERR_STATE, "mixed-precision iteration diverged");
+    rel = relres(rr);
     const bool replace = (mp.replace_every > 0 && it % mp.replace_every == 0) || rel <= mp.tol;
     if (replace) {
       flush();
       if ((rc = true_residual(&rr))) return rc;
-      rel = rr / *f_norm;
+      rel = relres(rr);
     }
     if (st && it <= HPCR_HIST_MAX) st->history[it - 1] = rr;
   }

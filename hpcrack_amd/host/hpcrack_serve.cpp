@@ -342,6 +342,10 @@ int main(int argc, char** argv) {
   std::deque<int> waiting;  // indices into reqs, arrival order
   long long in_use = 0, peak_bytes = 0;
   int active = 0, peak_active = 0;
+  // requests that reached the front of the queue with a slot free but had
+  // to wait for memory (admission held back by the budget)
+  std::vector<char> held(reqs.size(), 0);
+  int budget_waits = 0;
   size_t arrived = 0, completed = 0;
   std::vector<Result> results(reqs.size());
   const bool alone = o.only >= 0;
@@ -367,7 +371,13 @@ int main(int argc, char** argv) {
           // the oldest waiting request, once the budget holds it
           cv.wait(lk, [&] {
             if (waiting.empty()) return arrived == reqs.size();
-            return in_use + reqs[waiting.front()].bytes <= budget;
+            const int f = waiting.front();
+            if (in_use + reqs[f].bytes <= budget) return true;
+            if (!held[f]) {
+              held[f] = 1;
+              budget_waits++;
+            }
+            return false;
           });
           if (waiting.empty()) return;
           i = waiting.front();
@@ -438,11 +448,11 @@ int main(int argc, char** argv) {
       "\"wait_ms_p50\": %.1f, \"service_ms_p50\": %.1f, \"peak_admitted_gb\": %.2f, \"peak_concurrent\": %d, "
       "\"mean_cycles_u\": %.2f, \"mean_cycles_d\": %.2f, \"makespan_s\": %.3f, \"fused\": %s, "
       "\"fused_batches\": %lld, \"fused_cycles\": %lld, \"fused_launches\": %lld, \"fused_problem_launches\": %lld, "
-      "\"fused_max_problems\": %d, \"fused_graph_replays\": %lld}\n",
+      "\"fused_max_problems\": %d, \"fused_graph_replays\": %lld, \"budget_waits\": %d}\n",
       reqs.size(), failed, o.mixed ? "mixed" : "fp64", nslots, o.budget_gb, o.rate,
       reqs.size() / std::max(1e-9, last - first), 1e3 * percentile(lat, 0.5), 1e3 * percentile(lat, 0.99),
       1e3 * percentile(lat, 1.0), 1e3 * percentile(wait, 0.5), 1e3 * percentile(service, 0.5), peak_bytes / 1e9,
       peak_active, (double)cu / reqs.size(), (double)cd / reqs.size(), last, fz ? "true" : "false", fst.batches,
-      fst.cycles, fst.launches, fst.problem_launches, fst.max_problems, fst.graph_replays);
+      fst.cycles, fst.launches, fst.problem_launches, fst.max_problems, fst.graph_replays, budget_waits);
   return failed ? 1 : 0;
 }

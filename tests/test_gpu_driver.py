@@ -9,7 +9,8 @@ import pytest
 
 import oracle as O
 from gpu_common import require_gpu, say
-from test_gpu_solves import staggered
+from gpu_common import rel_l2
+from staggered import rel_l2_mod_rotation, staggered
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -103,3 +104,36 @@ def test_driver_resume_reproduces_uninterrupted_run(tmp_path):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     resumed = open(tmp_path / "resumed.dat").read().splitlines()
     assert resumed == full
+
+
+def test_driver_history_files_match_oracle_fields(tmp_path):
+    """The history the driver writes after each step (u, v, w, H, d in the
+    write_vector_3D layout, write_read_history.cpp:18-53) against the
+    oracle's fields after the same staggered steps (2 Ue increments: H and d
+    grow from step 0; steps 0-2, before the specimen is nearly cut)."""
+    require_gpu()
+    from hpcrack_amd import io as hio
+    img = write_case(tmp_path)
+    hist = tmp_path / "hist"
+    steps = 3
+    say("[gpu] driver history files vs oracle fields")
+    r = subprocess.run(driver_args(tmp_path, steps) + ["--du-ue", "2", "--write-history", "--history-every", "1",
+                                                        "--history-dir", str(hist)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    o = O.Oracle(2, (4, 4, 4), (0, 16, 0, 16, 0, 16), small_k=1e-6, lc=2.0, mg_u=(50, 4, 2, 1, 30),
+                 mg_d=(10, 2, 1, 1, 30), tol_u=1e-16, tol_d_rel=1e-16, tol_d_abs=1e-16)
+    o.image_mp(img, [0, 1], [3e9, 70e9], [0.3, 0.2], [100.0, 1000.0], 1e-6)
+    for f in (O.K, O.G, O.GC):
+        o.coarse_vr_nodal(f)
+    ref = staggered(o, steps, 1e-6, 2.0, schedule="ue-multiple", du=2.0, snapshots=True)
+    dims = (17, 17, 17)
+    for t, e in enumerate(ref):
+        u = np.stack([hio.read_vector_3d(str(hist), n, t, 0, dims) for n in ("u", "v", "w")])
+        H = hio.read_vector_3d(str(hist), "H", t, 0, dims)
+        d = hio.read_vector_3d(str(hist), "d", t, 0, dims)
+        eu = rel_l2_mod_rotation(u, e["u"], dims, (0, 16, 0, 16, 0, 16))
+        eh, ed = rel_l2(H, e["H"]), rel_l2(d, e["d"])
+        say(f"  t={t}: u (modulo rotation) {eu:.2e}, H {eh:.2e}, d {ed:.2e}, d_max {d.max():.4f}")
+        assert eu < 1e-9 and eh < 1e-9 and ed < 1e-9
+    assert ref[-1]["d"].max() > 0.5

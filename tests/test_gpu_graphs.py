@@ -7,7 +7,7 @@ import pytest
 
 import oracle as O
 from gpu_common import make_pair, require_gpu, say
-from test_gpu_solves import staggered
+from staggered import staggered
 
 pytestmark = pytest.mark.gpu
 

@@ -165,3 +165,24 @@ def test_mixed_rejects_bad_parameters(HA):
         s.u_entire_mixed(1.0, 0, 0.0, cheb_degree=0)
     with pytest.raises(HA.HpcrError):
         s.u_entire_mixed(1.0, 0, 0.0, cheb_ratio=1.0)
+
+
+def test_zero_load_and_zero_f_norm(HA):
+    """f_norm == 0: at t == 0 with a zero load (u = 0, f = 0) the residual is
+    exactly 0, so the solve is converged with no iteration and u stays 0; at
+    t >= 1 with f_norm == 0 and a nonzero residual the relative residual is
+    never below the tolerance, so the solve runs to its iteration cap and
+    returns normally, as u_entire runs its ncy cycles
+    (mechanical_displacement.cpp:84-108)."""
+    ml, enb, dom = CASES[0]
+    o, s, _ = make_pair(HA, ml, enb, dom, seed=3, lab_incl=2, **KW)
+    L = ml
+    say("[gpu] mixed solve with f_norm == 0")
+    fn, it = s.u_entire_mixed(0.0, 0, 0.0, max_iter=20)
+    assert fn == 0.0 and it == 0
+    assert not np.any(s.get(O.U, L)) and s.last_stats.r_norm_final == 0.0
+    rng = np.random.default_rng(4)
+    s.set(O.U, L, rng.standard_normal((3,) + s.dims(L)[::-1]) * 1e-3)
+    fn, it = s.u_entire_mixed(1.0, 1, 0.0, max_iter=7)
+    assert fn == 0.0 and it == 7
+    assert np.all(np.isfinite(s.get(O.U, L)))

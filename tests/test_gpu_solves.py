@@ -2,12 +2,21 @@
 (crack_propagation.cpp:65-121) against the oracle on seeded two-phase images.
 
 Bar: u and d within 1e-9 relative L2 (FP64 path), MG cycle counts identical
-(the acceptance bar allows +-1), crack set d > 0.95 identical."""
+(the acceptance bar allows +-1), crack set d > 0.95 identical except for
+voxels within 1e-6 of the threshold.
+
+u is compared modulo the zero-energy rotation that the reference's boundary
+conditions leave free (boundary_conditions.cpp:43-58): the system is singular
+along it, so the amount of it in u is set by rounding.  A one-ulp change of
+the load moves u along it by 5e-10 and the rest of u by 1e-13
+(tests/test_oracle_trajectory.py).  The raw distance is printed and bounded
+by 1e-8."""
 import numpy as np
 import pytest
 
 import oracle as O
 from gpu_common import make_pair, rel_l2, require_gpu, say
+from staggered import rel_l2_mod_rotation, staggered
 
 pytestmark = pytest.mark.gpu
 
@@ -15,45 +24,6 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def HA():
     return require_gpu()
-
-
-def staggered(obj, steps, voxel_size, lc, schedule="reference", du=None, log=None):
-    """crack_propagation (crack_propagation.cpp:28-123) over an Oracle or a
-    Solver (same method names); returns per-step records."""
-    L = obj.maxlevel
-    f_norm, fd_norm = 0.0, 0.0
-    Ut, delta_U, u_prop = 0.0, 0.0, 1.0
-    recs = []
-    for t in range(steps):
-        f_norm, cyc_u = obj.u_entire(1.0, t, f_norm)
-        if t == 0:
-            if schedule == "reference":
-                Ue = obj.U_e()
-                Ut = Ue
-                delta_U = 1.5e-3 * Ue
-            elif schedule == "ue-multiple":
-                # fixed increments of du * Ue: the strain history exceeds the
-                # threshold from step 0 and the phase field grows
-                Ue = obj.U_e()
-                Ut, delta_U = du * Ue, du * Ue
-            else:
-                Ut, delta_U = du, du
-            u_prop = Ut
-        obj.strain_history(u_prop)
-        obj.voigt_all_level(O.H)
-        for l in range(L, -1, -1):
-            obj.fd_h(l)
-        fd_norm, cyc_d = obj.phase_field(t, fd_norm)
-        ddmax = obj.delta_d_max()
-        obj.save_d_old()
-        FeW = voxel_size * obj.externe_force_sum()
-        Ut = Ut + delta_U
-        u_prop = Ut
-        recs.append(dict(t=t, cyc_u=cyc_u, cyc_d=cyc_d, f_norm=f_norm, fd_norm=fd_norm, Ut=Ut,
-                         Fw=u_prop * FeW, ddmax=ddmax))
-        if log:
-            log(f"  t={t} cyc_u={cyc_u} cyc_d={cyc_d} Fw={u_prop * FeW:.6e}")
-    return recs
 
 
 @pytest.mark.parametrize("ml,tol_u", [(2, 1e-6), (3, 1e-6), (3, 1e-16)], ids=["17c", "33c", "33c-tight"])
@@ -69,7 +39,11 @@ def test_u_entire_and_phase_field(HA, ml, tol_u):
     say(f"  cycles oracle={co} gpu={cs} f_norm {fo:.6e} {fs:.6e}")
     assert cs == co
     assert fs == pytest.approx(fo, rel=1e-9)
-    assert rel_l2(s.get(O.U, L), o.vec(O.U, L)) < 1e-9
+    us, uo = s.get(O.U, L), o.vec(O.U, L)
+    e_rot, e_raw = rel_l2_mod_rotation(us, uo, o.dims(L), dom), rel_l2(us, uo)
+    say(f"  u rel L2 modulo rotation {e_rot:.2e}, raw {e_raw:.2e}")
+    assert e_rot < 1e-9
+    assert e_raw < 1e-8
     assert rel_l2(s.get(O.FU, L), o.vec(O.FU, L)) < 1e-9
     hs, ho = s.history(), o.history("u")
     assert len(hs) == len(ho)
@@ -96,10 +70,17 @@ def test_u_entire_and_phase_field(HA, ml, tol_u):
     assert rel_l2(s.get(O.D, L), o.field(O.D, L)) < 1e-9
 
 
-# With loads of 2 Ue per step the specimen is nearly cut through by step 4
-# (reaction down by 150x): the FAS cycle then stops at its cap of 30 cycles on
-# a nearly singular system, and rounding differences between the two
-# implementations grow to 1e-5 in the reaction; steps 0-3 are compared.
+# With loads of 2 Ue per step the crack set d > 0.95 appears at step 2 (123
+# voxels) and grows at step 3 (1003 voxels, reaction down 10x); there the
+# FAS cycle stops at its cap of 30 cycles on a nearly singular system.  The
+# bounds follow the oracle's own rounding floor on this schedule, measured by
+# running it with the unit load 1 and 1 + 2^-52
+# (tests/test_oracle_trajectory.py): u modulo rotation 1e-13 at steps 0-2
+# and 2.3e-8 at step 3, d 2.9e-11 and H 2.8e-10 at step 3.  The GPU bound is
+# 1e-9 where the floor is far below it and 50x the floor at step 3.
+FLOOR_STEP3 = dict(u=2.3e-8, d=2.9e-11, H=2.8e-10)
+
+
 @pytest.mark.parametrize("schedule,du,steps", [("fixed", 0.05, 6), ("ue-multiple", 2.0, 4)], ids=["fixed", "2Ue"])
 def test_staggered_steps_match_oracle(HA, schedule, du, steps):
     ml = 2
@@ -108,24 +89,27 @@ def test_staggered_steps_match_oracle(HA, schedule, du, steps):
     kw = dict(lc=2.0, small_k=1e-6, tol_u=1e-16, mg_u=(50, 4, 2, 1, 30), mg_d=(10, 2, 1, 1, 30),
               tol_d_rel=1e-16, tol_d_abs=1e-30)
     o, s, _ = make_pair(HA, ml, (4, 4, 4), dom, seed=21, lab_incl=3, **kw)
-    ro = staggered(o, steps, 1e-6, 2.0, schedule=schedule, du=du)
-    rs = staggered(s, steps, 1e-6, 2.0, schedule=schedule, du=du)
-    if schedule == "ue-multiple":
-        # loads of 2 Ue .. 12 Ue: the phase field is active and grows
-        assert max(r["cyc_d"] for r in ro) > 0 and o.field(O.D, ml).max() > 0.1
+    ro = staggered(o, steps, 1e-6, 2.0, schedule=schedule, du=du, snapshots=True)
+    rs = staggered(s, steps, 1e-6, 2.0, schedule=schedule, du=du, snapshots=True)
+    dims = o.dims(ml)
     for a, b in zip(ro, rs):
-        say(f"  t={a['t']} cyc_u {a['cyc_u']}/{b['cyc_u']} cyc_d {a['cyc_d']}/{b['cyc_d']} Fw {a['Fw']:.6e}/{b['Fw']:.6e}")
+        eu = rel_l2_mod_rotation(b["u"], a["u"], dims, dom)
+        ed, eh = rel_l2(b["d"], a["d"]), rel_l2(b["H"], a["H"])
+        crack_o, crack_s = a["d"] > 0.95, b["d"] > 0.95
+        say(f"  t={a['t']} cyc_u {a['cyc_u']}/{b['cyc_u']} cyc_d {a['cyc_d']}/{b['cyc_d']} Fw {a['Fw']:.6e}/"
+            f"{b['Fw']:.6e} u(mod rot) {eu:.2e} raw {rel_l2(b['u'], a['u']):.2e} d {ed:.2e} H {eh:.2e} "
+            f"crack {int(crack_o.sum())}/{int(crack_s.sum())} dmax {a['dmax']:.6f}/{b['dmax']:.6f}")
         assert abs(a["cyc_u"] - b["cyc_u"]) <= 1 and abs(a["cyc_d"] - b["cyc_d"]) <= 1
-        assert b["Fw"] == pytest.approx(a["Fw"], rel=1e-8)
-        assert b["ddmax"] == pytest.approx(a["ddmax"], rel=1e-8, abs=1e-14)
-    L = ml
-    # with damage the u solves stop at the 30-cycle cap (squared residual
-    # ~1e-10, not the 1e-16 asked) and the unconverged iterates of the two
-    # implementations agree to 1e-7; without damage (fixed 0.05) to 1e-9
-    tol = 1e-9 if schedule == "fixed" else 1e-6
-    assert rel_l2(s.get(O.U, L), o.vec(O.U, L)) < tol
-    assert rel_l2(s.get(O.D, L), o.field(O.D, L)) < tol
-    assert rel_l2(s.get(O.H, L), o.field(O.H, L)) < tol
-    do, ds = o.field(O.D, L), s.get(O.D, L)
-    near = np.abs(do - 0.95) < 1e-6
-    assert np.array_equal((do > 0.95)[~near], (ds > 0.95)[~near])
+        last_damaged = schedule == "ue-multiple" and a["t"] == 3
+        bu = 50 * FLOOR_STEP3["u"] if last_damaged else 1e-9
+        bd = max(1e-9, 50 * FLOOR_STEP3["d"]) if last_damaged else 1e-9
+        bh = 50 * FLOOR_STEP3["H"] if last_damaged else 1e-9
+        assert eu < bu and ed < bd and eh < bh
+        assert b["Fw"] == pytest.approx(a["Fw"], rel=max(1e-9, bu))
+        assert b["ddmax"] == pytest.approx(a["ddmax"], rel=1e-9, abs=1e-14)
+        assert b["dmax"] == pytest.approx(a["dmax"], rel=1e-9, abs=1e-14)
+        near = np.abs(a["d"] - 0.95) < 1e-6
+        assert np.array_equal(crack_o[~near], crack_s[~near])
+        if schedule == "ue-multiple" and a["t"] >= 2:
+            # the crack set is not empty, and not the whole specimen
+            assert 0 < crack_o.sum() < crack_o.size

@@ -195,8 +195,9 @@ def test_pcg_d_matches_textbook(nb):
     assert np.allclose(vflat(o.field(O.RD, 0)), r, atol=1e-12 * np.max(np.abs(r)))
 
 
-def textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, l):
-    """FAS gamma=1 cycle in matrix form (injection, P, P^T, Jacobi-PCG)."""
+def textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, l, hook=None):
+    """FAS gamma=1 cycle in matrix form (injection, P, P^T, Jacobi-PCG).
+    hook(l, x) sees the iterate (boundary values set) before each smoother."""
     nu0, nu1, nu2 = nu
     A = mats[l]
     vals, free = bcs[l]
@@ -205,9 +206,13 @@ def textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, l):
             x[q] = v
     if l == 0:
         setbc(u[0])
+        if hook:
+            hook(0, u[0])
         u[0], r = T.jacobi_pcg(A, f[0], u[0], nu0, free)
         return r
     setbc(u[l])
+    if hook:
+        hook(l, u[l])
     u[l], r = T.jacobi_pcg(A, f[l], u[l], nu1, free)
     dc = o.dims(l - 1)
     Inj = T.injection(dc)
@@ -220,10 +225,12 @@ def textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, l):
     u[l - 1] = uc
     rr = r.reshape(3, -1)
     f[l - 1] = mats[l - 1] @ uc + np.concatenate([P.T @ rr[c] for c in range(3)])
-    textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, l - 1)
+    textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, l - 1, hook)
     corr = (u[l - 1] - uold).reshape(3, -1)
     u[l] = u[l] + np.concatenate([P @ corr[c] for c in range(3)])
     setbc(u[l])
+    if hook:
+        hook(l, u[l])
     u[l], r = T.jacobi_pcg(A, f[l], u[l], nu2, free)
     return r
 
@@ -255,6 +262,282 @@ def test_cycle_u_matches_textbook_fas_cycle():
     scale = np.max(np.abs(u[L]))
     assert np.max(np.abs(vflat(o.vec(O.U, L)) - u[L])) <= 1e-10 * scale
     assert np.max(np.abs(vflat(o.vec(O.RU, L)) - r_tb)) <= 1e-9 * np.max(np.abs(r_tb))
+
+
+def nodal_vr_levels(fine, o, L):
+    """Coarse nodal moduli of every level by the 27-point Voigt/Reuss window
+    with edge clamping, level by level from the finest (numpy closed form)."""
+    out = {L: np.asarray(fine, dtype=np.float64)}
+    for l in range(L, 0, -1):
+        M = out[l]
+        dc = o.dims(l - 1)
+        pad = np.pad(M, 1, mode="edge")
+        V = np.zeros(dc[::-1])
+        Rs = np.zeros(dc[::-1])
+        for dz in range(3):
+            for dy in range(3):
+                for dx in range(3):
+                    win = pad[dz:dz + 2 * dc[2] - 1:2, dy:dy + 2 * dc[1] - 1:2, dx:dx + 2 * dc[0] - 1:2]
+                    V += win / 27.0
+                    Rs += 1.0 / win
+        R = 27.0 / Rs
+        out[l - 1] = 0.5 * (0.5 * (V + R) + np.sqrt(V * R))
+    return out
+
+
+def element_voigt_levels(fine, o, L):
+    """Coarse element fields as the mean of the 8 child elements (element
+    fields live in node-sized grids whose last layer is unused)."""
+    out = {L: np.asarray(fine, dtype=np.float64)}
+    for l in range(L, 0, -1):
+        dc = o.dims(l - 1)
+        E = out[l][:-1, :-1, :-1]
+        blocks = E.reshape(dc[2] - 1, 2, dc[1] - 1, 2, dc[0] - 1, 2)
+        c = np.zeros(dc[::-1])
+        c[:-1, :-1, :-1] = blocks.mean(axis=(1, 3, 5))
+        out[l - 1] = c
+    return out
+
+
+def textbook_cycle_d(o, nu, gamma, mats, d, f, l, post_calls=None):
+    """FAS cycle of the phase field in matrix form: nu1 Jacobi-PCG iterations,
+    injection, coarse right-hand side A_c d_c + P^T r, gamma coarse visits,
+    trilinear correction, then nu2 restarted Jacobi-PCG calls of nu2
+    iterations each.  post_calls overrides the number of post-smoother calls
+    (to show that the test tells the variants apart)."""
+    nu0, nu1, nu2 = nu
+    A = mats[l]
+    if l == 0:
+        d[0], r = T.jacobi_pcg(A, f[0], d[0], nu0)
+        return r
+    d[l], r = T.jacobi_pcg(A, f[l], d[l], nu1)
+    dc = o.dims(l - 1)
+    P = T.prolongation(dc)
+    d[l - 1] = T.injection(dc) @ d[l]
+    dold = d[l - 1].copy()
+    f[l - 1] = mats[l - 1] @ d[l - 1] + P.T @ r
+    for _ in range(gamma):
+        textbook_cycle_d(o, nu, gamma, mats, d, f, l - 1, post_calls)
+    d[l] = d[l] + P @ (d[l - 1] - dold)
+    for _ in range(nu2 if post_calls is None else post_calls):
+        d[l], r = T.jacobi_pcg(A, f[l], d[l], nu2)
+    return r
+
+
+def phase_problem(nu, gamma, seed, L=2, enb=(2, 2, 2), dom=DOM, lc=1.5):
+    o = make(L, enb, dom, lc=lc, mg_d=(nu[0], nu[1], nu[2], gamma, 12))
+    dims = o.dims(L)
+    rng = np.random.default_rng(seed)
+    Hf = np.zeros(dims[::-1])
+    Hf[:-1, :-1, :-1] = rng.uniform(0, 3, (dims[2] - 1, dims[1] - 1, dims[0] - 1))
+    gcf = rng.uniform(0.5, 2, dims[::-1])
+    o.set(O.H, L, Hf)
+    o.voigt_all_level(O.H)
+    o.set(O.GC, L, gcf)
+    o.coarse_vr_nodal(O.GC)
+    Hl = element_voigt_levels(Hf, o, L)
+    gcl = nodal_vr_levels(gcf, o, L)
+    mats = {}
+    for l in range(L + 1):
+        # the oracle's coarse coefficients are the closed forms
+        assert np.allclose(o.field(O.H, l), Hl[l], rtol=1e-14, atol=0)
+        assert np.allclose(o.field(O.GC, l), gcl[l], rtol=1e-14, atol=0)
+        mats[l] = T.phase_matrix(o.dims(l), o.h(l), Hl[l], gcl[l], lc)
+    d0 = rng.uniform(0, 0.4, dims[::-1])
+    fd = rng.standard_normal(dims[::-1])
+    o.set(O.D, L, d0)
+    o.set(O.FD, L, fd)
+    return o, mats, vflat(d0).copy(), vflat(fd).copy()
+
+
+@pytest.mark.parametrize("nu,gamma", [((10, 2, 1), 1), ((10, 2, 3), 1), ((6, 1, 2), 2)],
+                         ids=["as-shipped", "nu2=3", "W-cycle"])
+def test_cycle_d_matches_textbook_fas_cycle(nu, gamma):
+    """cycle_d (phase_field_multigrid.cpp:18-44) with coarsen_d, coarsen_f_d
+    and refine_d (multigrid_intergrid.cpp:92-103, 345-371, 523-533) against
+    the matrix-form FAS cycle on 3 levels: the coarse operators use the
+    Voigt mean of H and the Voigt/Reuss nodal gc, and the post-smoother is
+    nu2 calls of nu2 iterations."""
+    L = 2
+    o, mats, d0, fd = phase_problem(nu, gamma, 51)
+    d = {L: d0.copy()}
+    f = {L: fd.copy()}
+    r_tb = textbook_cycle_d(o, nu, gamma, mats, d, f, L)
+    fdn = o.cycle_d(L, 0.0)
+    got = vflat(o.field(O.D, L))
+    scale = np.max(np.abs(d[L]))
+    err = np.max(np.abs(got - d[L])) / scale
+    assert err <= 1e-12
+    assert np.max(np.abs(vflat(o.field(O.RD, L)) - r_tb)) <= 1e-10 * np.max(np.abs(r_tb))
+    # the last pcg_d call (finest post-smoother) sets fd_norm = |fd|^2
+    assert fdn == pytest.approx(float(fd @ fd), rel=1e-14)
+    # the coarse right-hand side is the FAS one
+    assert np.allclose(vflat(o.field(O.FD, L - 1)), f[L - 1], rtol=0, atol=1e-12 * np.max(np.abs(f[L - 1])))
+    if nu[2] > 1:
+        # one post-smoother call instead of nu2 calls is 1e3 x the bound away
+        d1 = {L: d0.copy()}
+        f1 = {L: fd.copy()}
+        textbook_cycle_d(o, nu, gamma, mats, d1, f1, L, post_calls=1)
+        assert np.max(np.abs(got - d1[L])) / scale > 1e3 * 1e-12
+
+
+def textbook_u_entire(o, L, Ut, mats, bcs, u0, f0, tol, ncy, nu, f_norm, t):
+    """u_entire's loop (mechanical_displacement.cpp:72-108) over the matrix
+    form: initial masked residual; at t >= 1 with f_norm != 0 the relative
+    residual is r.r/f_norm, else 1; up to ncy cycles while it exceeds tol,
+    each followed by the masked residual of the finest level.  At t == 0
+    every finest-level smoother call stores the reactions A x of its
+    starting iterate on the Dirichlet DOFs of f and sets f_norm to their
+    squared norm, so the last call's value is the one the test uses."""
+    vals, free = bcs[L]
+    u = {L: u0.copy()}
+    f = {L: f0.copy()}
+    A = mats[L]
+    fixed = np.setdiff1d(np.arange(len(u0)), free)
+    r = f0 - A @ u0
+    mask = np.zeros_like(r)
+    mask[free] = 1.0
+    r_norm = float(np.sum((r * mask) ** 2))
+    hist = [r_norm]
+    state = {"f_norm": f_norm}
+
+    def hook(l, x):
+        if t == 0 and l == L:
+            react = (A @ x)[fixed]
+            f[L][fixed] = react
+            state["f_norm"] = float(np.sum(react ** 2))
+
+    rel = r_norm / f_norm if (t >= 1 and f_norm != 0) else 1.0
+    cycles = 0
+    for _ in range(ncy):
+        if rel > tol:
+            r = textbook_cycle_u(o, Ut, nu, mats, bcs, u, f, L, hook)
+            f_norm = state["f_norm"]
+            cycles += 1
+            r_norm = float(np.sum(r ** 2))
+            rel = r_norm / f_norm
+            hist.append(r_norm)
+        if rel <= tol:
+            break
+    return u[L], f_norm, cycles, np.array(hist)
+
+
+@pytest.mark.parametrize("tol,t,f_in", [(1e-6, 0, 0.0), (1e-12, 0, 0.0), (1e-12, 1, 3.7), ("edge", 0, 0.0),
+                                        (1e-6, 0, 1e30)],
+                         ids=["shipped-tol", "tight", "t1", "tol-edge", "t0-stale-fnorm"])
+def test_u_entire_stopping_logic_matches_textbook(tol, t, f_in):
+    """Cycle count, residual history and f_norm of u_entire against the
+    matrix-form loop: pins the squared relative test (tol-edge puts the
+    tolerance just above the 5th cycle's relative residual, so the solve
+    stops there and not one cycle later), the f_norm of t == 0 (whatever
+    value comes in), the t >= 1 first-residual rule and the ncy cap."""
+    ncy = 8
+    edge = tol == "edge"
+    o, dims, K, G, d, fl = fine_mech(61, maxlevel=2, enb=(2, 2, 2), tol_u=1e-30 if edge else tol,
+                                     mg_u=(50, 4, 2, 1, ncy))
+    L = 2
+    Ut = 1.0
+    u0 = np.zeros((3,) + dims[::-1])
+    if t == 1:
+        u0 = np.random.default_rng(62).standard_normal((3,) + dims[::-1]) * 1e-3
+    o.set(O.U, L, u0)
+    o.set(O.FU, L, np.zeros((3,) + dims[::-1]))
+    o.mech_setup()
+    flags = o.field(O.FLAG, L).copy()
+    mats, bcs = {}, {}
+    for l in range(L + 1):
+        dl = o.dims(l)
+        coef = (T.fine_coefficients(dl, o.h(l), K, G, d, flags, 1e-5) if l == L
+                else T.coarse_coefficients(dl, o.field(O.KE, l), o.field(O.GE, l)))
+        mats[l] = T.elastic_matrix(dl, o.h(l), coef)
+        vals, fixed, free = bc_info(o, l, Ut)
+        bcs[l] = (vals, free)
+    f0 = np.zeros(3 * int(np.prod(dims)))
+    if edge:
+        _, fn1, _, h1 = textbook_u_entire(o, L, Ut, mats, bcs, vflat(u0), f0, 1e-30, ncy, (50, 4, 2), f_in, t)
+        tol = 1.25 * h1[5] / fn1
+        o = make(L, (2, 2, 2), DOM, tol_u=tol, mg_u=(50, 4, 2, 1, ncy))
+        for fld, a in ((O.K, K), (O.G, G), (O.D, d)):
+            o.set(fld, L, a)
+        o.set(O.U, L, u0)
+        o.mech_setup()
+    u_tb, fn_tb, cyc_tb, hist_tb = textbook_u_entire(o, L, Ut, mats, bcs, vflat(u0), f0, tol, ncy, (50, 4, 2), f_in, t)
+    o.set(O.U, L, u0)
+    o.set(O.FU, L, np.zeros((3,) + dims[::-1]))
+    fn, cyc = o.u_entire(Ut, t, f_in)
+    hist = o.history("u")
+    assert cyc == cyc_tb
+    assert fn == pytest.approx(fn_tb, rel=1e-12)
+    assert len(hist) == len(hist_tb)
+    # the two formulations round differently and the 50-iteration coarsest
+    # CG amplifies that from cycle to cycle: a few 1e-6 after 8 cycles
+    assert np.allclose(hist, hist_tb, rtol=1e-4, atol=1e-30)
+    if edge:
+        assert cyc == 5
+    elif tol == 1e-6:
+        assert cyc < ncy  # stopped by the test, not the cap
+    else:
+        assert cyc == ncy  # stopped by the cap
+    m = rotation_mode(o, L).reshape(-1)
+    e = vflat(o.vec(O.U, L)) - u_tb
+    e -= (e @ m) / (m @ m) * m
+    assert np.linalg.norm(e) <= 1e-9 * np.linalg.norm(u_tb)
+
+
+def textbook_phase_field(o, L, mats, d0, fd, nu, gamma, ncy, tol_rel, tol_abs):
+    """phase_field's loop (phase_field.cpp:19-56): absolute test on the
+    squared residual, relative test against fd_norm (= |fd|^2 after the
+    last smoother call), residual read from the smoothers' updated residual."""
+    d = {L: d0.copy()}
+    f = {L: fd.copy()}
+    r = fd - mats[L] @ d0
+    r_norm = float(r @ r)
+    hist = [r_norm]
+    sres = 1.0
+    cycles = 0
+    fd_norm = 0.0
+    for _ in range(ncy):
+        if r_norm > tol_abs:
+            if sres > tol_rel:
+                r = textbook_cycle_d(o, nu, gamma, mats, d, f, L)
+                fd_norm = float(fd @ fd)
+                cycles += 1
+            r_norm = float(r @ r)
+            sres = r_norm / fd_norm
+            hist.append(r_norm)
+            if sres <= tol_rel:
+                break
+        else:
+            break
+    return d[L], cycles, np.array(hist)
+
+
+@pytest.mark.parametrize("tol_rel,tol_abs", [(1e-12, 1e-12), (1e-24, 1e-40), (1e-12, 1e3)],
+                         ids=["shipped", "cap", "abs-stop"])
+def test_phase_field_stopping_logic_matches_textbook(tol_rel, tol_abs):
+    L = 2
+    nu = (10, 2, 1)
+    ncy = 6
+    o, mats, d0, fd = phase_problem(nu, 1, 71)
+    o2 = make(L, (2, 2, 2), DOM, lc=1.5, mg_d=(10, 2, 1, 1, ncy), tol_d_rel=tol_rel, tol_d_abs=tol_abs)
+    for l in range(L + 1):
+        o2.set(O.H, l, o.field(O.H, l))
+        o2.set(O.GC, l, o.field(O.GC, l))
+    o2.set(O.D, L, d0.reshape(o.dims(L)[::-1]))
+    o2.set(O.FD, L, fd.reshape(o.dims(L)[::-1]))
+    d_tb, cyc_tb, hist_tb = textbook_phase_field(o, L, mats, d0, fd, nu, 1, ncy, tol_rel, tol_abs)
+    fdn, cyc = o2.phase_field(0, 0.0)
+    hist = o2.history("d")
+    assert cyc == cyc_tb
+    assert len(hist) == len(hist_tb)
+    assert np.allclose(hist, hist_tb, rtol=1e-6, atol=0)
+    assert np.linalg.norm(vflat(o2.field(O.D, L)) - d_tb) <= 1e-11 * np.linalg.norm(d_tb)
+    if tol_abs == 1e3:
+        assert cyc == 0 and len(hist) == 1
+    elif tol_rel == 1e-24:
+        assert cyc == ncy
+    else:
+        assert 0 < cyc < ncy
 
 
 def test_u_entire_converges_to_direct_solve():
