@@ -1337,6 +1337,49 @@ void destroy(hpcr_ctx* c) {
 
 }  // namespace
 
+// -----------------------------------------------------------------------------
+// Device scratch of hpcr_set_image_noalloc: the material table, the
+// out-of-table flag and a staging area through which the image streams in
+// chunks of NOALLOC_STAGE bytes.  One copy per device (each at its own
+// address), held until the process ends.  Every call on a device shares that
+// device's copy, so calls that overlap on one device race.
+// -----------------------------------------------------------------------------
+namespace {
+constexpr size_t NOALLOC_STAGE = (size_t)4 << 20;
+constexpr size_t NOALLOC_TAB_OFF = 0;
+constexpr size_t NOALLOC_BAD_OFF = (sizeof(MatTable) + 255) / 256 * 256;
+constexpr size_t NOALLOC_IMG_OFF = NOALLOC_BAD_OFF + 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_IMG_OFF + NOALLOC_STAGE;
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of this device's scratch, looked up once per device
+// with that device current.  The host-side name `scratch` is a stand-in, not
+// a device address: host code uses only the pointer returned here.
+int scratch_of(hpcr_ctx* c, unsigned char** out) {
+  if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES) return set_err(c, HPCR_ERR_DEVICE, "device index too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(c, HPCR_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[c->device] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[c->device];
+  return HPCR_OK;
+}
+
+__global__ void zero_flag_kernel(int* f) {
+  if (threadIdx.x == 0) f[0] = 0;
+}
+}  // namespace
+
 // =============================================================================
 // C-ABI
 // =============================================================================
@@ -1606,6 +1649,57 @@ int hpcr_set_image(hpcr_ctx* c, const unsigned char* img, const hpcr_material_ta
   if (bad) return set_err(c, HPCR_ERR_ARG, "gray level outside the material table");
   for (int f : {HPCR_K, HPCR_G, HPCR_GC}) {
     int rc = hpcr_coarse_vr(c, f);
+    if (rc) return rc;
+  }
+  return HPCR_OK;
+}
+
+int hpcr_set_image_noalloc(hpcr_ctx* c, const unsigned char* img, const hpcr_material_table* tab,
+                           double voxel_size) {
+  if (!c || !img || !tab) return HPCR_ERR_ARG;
+  if (tab->n < 2 || tab->n > HPCR_MAX_PHASES)
+    return set_err(c, HPCR_ERR_ARG, "set_image_noalloc: the material table must have 2 to 256 rows "
+                                    "(its device copy lives in a fixed scratch area)");
+  if (hipSetDevice(c->device) != hipSuccess) return set_err(c, HPCR_ERR_DEVICE, "hipSetDevice");
+  unsigned char* scr = nullptr;
+  int rc = scratch_of(c, &scr);
+  if (rc) return rc;
+  Lev& F = c->lev[c->L];
+  MatTable h{};
+  h.n = tab->n;
+  for (int i = 0; i < tab->n; i++) {
+    h.gray[i] = tab->gray[i];
+    h.E[i] = tab->E[i];
+    h.nu[i] = tab->nu[i];
+    h.gc[i] = tab->gc[i];
+  }
+  MatTable* dtab = (MatTable*)(scr + NOALLOC_TAB_OFF);
+  int* dbad = (int*)(scr + NOALLOC_BAD_OFF);
+  unsigned char* dimg = scr + NOALLOC_IMG_OFF;
+  // the scratch keeps the previous call's contents: the flag starts at zero
+  // (a kernel, not a memset, so that no memset node can end up in a graph)
+  hipError_t e = hipMemcpyAsync(dtab, &h, sizeof(MatTable), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    zero_flag_kernel<<<dim3(1), dim3(64), 0, c->stream>>>(dbad);
+    e = hipGetLastError();
+  }
+  // the image streams through the staging area; stream order keeps each
+  // chunk's copy behind the previous chunk's kernel
+  for (size_t off = 0; off < F.N && e == hipSuccess; off += NOALLOC_STAGE) {
+    const size_t len = std::min(NOALLOC_STAGE, F.N - off);
+    e = hipMemcpyAsync(dimg, img + off, len, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+      launch_image_mp(c->stream, len, dimg, dtab, voxel_size, F.K + off, F.G + off, F.gc + off, dbad);
+      e = hipGetLastError();
+    }
+  }
+  int bad = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_err(c, HPCR_ERR_DEVICE, std::string("set_image_noalloc: ") + hipGetErrorString(e));
+  if (bad) return set_err(c, HPCR_ERR_ARG, "gray level outside the material table");
+  for (int f : {HPCR_K, HPCR_G, HPCR_GC}) {
+    rc = hpcr_coarse_vr(c, f);
     if (rc) return rc;
   }
   return HPCR_OK;

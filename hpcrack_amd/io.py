@@ -74,8 +74,9 @@ def _lib():
         L.hpcr_write_step_state.argtypes = [ctypes.c_char_p, ctypes.POINTER(StepState)]
         L.hpcr_read_step_state.restype = ctypes.c_int
         L.hpcr_read_step_state.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(StepState)]
-        L.hpcr_set_image.restype = ctypes.c_int
-        L.hpcr_set_image.argtypes = [ctypes.c_void_p, ub, ctypes.POINTER(MaterialTable), ctypes.c_double]
+        for fn in (L.hpcr_set_image, L.hpcr_set_image_noalloc):
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_void_p, ub, ctypes.POINTER(MaterialTable), ctypes.c_double]
         L.hpcr_synth_image.restype = ctypes.c_int
         L.hpcr_synth_image.argtypes = [ctypes.POINTER(SynthSpec), ub, dp]
         _proto_done = True
@@ -210,11 +211,12 @@ def read_step_state(dir, t):
     return {k: getattr(s, k) for k, _ in StepState._fields_}
 
 
-def set_image(solver, img, table, voxel_size):
+def set_image(solver, img, table, voxel_size, noalloc=False):
     """input_total on the device: upload the finest level's uint8 image,
-    image_MP into K, G, gc and coarse_VR of the three fields."""
+    image_MP into K, G, gc and coarse_VR of the three fields.  noalloc:
+    through hpcr_set_image_noalloc (no device allocation; fixed scratch)."""
     img = np.ascontiguousarray(img, dtype=np.uint8)
-    rc = _lib().hpcr_set_image(solver._h, img.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), ctypes.byref(table),
-                               voxel_size)
+    fn = _lib().hpcr_set_image_noalloc if noalloc else _lib().hpcr_set_image
+    rc = fn(solver._h, img.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), ctypes.byref(table), voxel_size)
     if rc != 0:
         raise IoError(f"set_image failed ({rc})")

@@ -100,6 +100,19 @@ int hpcr_image_mp(const hpcr_material_table* tab, const unsigned char* img, size
  * level.  HPCR_ERR_ARG for a gray level outside the table. */
 int hpcr_set_image(hpcr_ctx* ctx, const unsigned char* img, const hpcr_material_table* tab, double voxel_size);
 
+/* Allocation-free variant of hpcr_set_image, with identical results.  It has
+ * no counterpart in the reference: it is for callers that cannot allocate
+ * device memory (serving loops that reuse pooled contexts, harnesses that
+ * forbid hipMalloc).  It calls no hipMalloc/hipFree: the material table, the
+ * out-of-table flag and a 4 MiB staging area through which the image streams
+ * in chunks live in a fixed device scratch of about 4.01 MiB per device, held
+ * until the process ends.  Limits: at most HPCR_MAX_PHASES (256) table rows,
+ * else HPCR_ERR_ARG; any image size.  All calls on one device share that
+ * scratch, so calls that overlap on one device (from different host threads)
+ * race; calls one after another do not. */
+int hpcr_set_image_noalloc(hpcr_ctx* ctx, const unsigned char* img, const hpcr_material_table* tab,
+                           double voxel_size);
+
 /* Seeded synthetic voxel image: non-overlapping spheres on integer node
  * centres placed by rejection (xoshiro256** seeded through splitmix64).
  * Inclusions first (count n_incl, radii U(incl_rmin, incl_rmax)), then pores
