@@ -15,10 +15,21 @@ The JSON line also carries the roofline of the dominant kernel (the fused
 p-update + degraded-elasticity apply + p.Ap pass on the finest level, timed
 with HIP events on the solver's stream) and a CPU baseline: the oracle run on
 the host cores over a bounded sample (see cpu_baseline.sample).
+
+Outside the timed region it also measures the metric's other parts:
+- `mixed`: the FP64 flexible CG with the FP32 V-cycle preconditioner, timed to
+  the same 1e-8 tolerance, against the same flexible CG with the identical
+  V-cycle in FP64 (the like-for-like FP32 vs FP64 comparison at one
+  tolerance), and against the plain FAS solve at the residual that solve
+  reaches (it stalls above the tolerance at its 30-cycle cap);
+- `serve`: p50/p99 latency and solves/s of the request server on the
+  256-request stream (Poisson 20/s, node dims per axis from {33, 65, 129,
+  257}), mixed-precision and FP64 forms.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -78,6 +89,15 @@ def copy_calibration(mib):
     log(f"[bench] copy calibration: {mib} MiB device-to-device copy")
 
 
+def host_cores():
+    """Logical CPUs of the host and those this process may run on."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    return os.cpu_count() or 1, aff
+
+
 def cpu_baseline(workload, n_sample, cycles, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -98,15 +118,34 @@ def cpu_baseline(workload, n_sample, cycles, threads):
     _, cyc = o.u_entire(1.0, 0, 0.0)
     dt = time.perf_counter() - t0
     dof = 3 * int(np.prod(w.dims))
+    nproc, aff = host_cores()
     return {
         "value": dof * cyc / dt / 1e6,
         "unit": "MDOF·iter/s",
         "cores": O.lib().ora_get_threads(),
         "kind": "port",
-        "sample": f"oracle u_entire on the same generator scaled to {n_sample}^3 nodes "
-                  f"({w.n_incl} inclusions, {100 * frac[1]:.1f}% pores), t=0 from u=0, "
-                  f"{cyc} V-cycles in {dt:.1f} s",
+        "nproc": nproc,
+        "affinity_cpus": aff,
+        "sample": f"oracle u_entire (OpenMP, {O.lib().ora_get_threads()} threads) on the same generator at "
+                  f"{n_sample}^3 nodes ({w.n_incl} inclusions, {100 * frac[1]:.1f}% pores), t=0 from u=0, "
+                  f"{cyc} V-cycles in {dt:.1f} s; the oracle evaluates each element's Gauss-point "
+                  f"coefficients once where the reference does it 8 times and scatters without critical "
+                  f"sections, so it is faster than the reference it restates",
     }
+
+
+def serve_stream(mixed, requests, rate, timeout):
+    """The request server on the serving stream; its JSON summary line."""
+    exe = os.path.join(ROOT, "hpcrack_amd", "hpcrack_serve")
+    cmd = [exe, "--requests", str(requests), "--rate", str(rate)] + (["--mixed"] if mixed else [])
+    t = time.perf_counter()
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"hpcrack_serve failed ({r.returncode}): {r.stderr[-2000:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    log(f"[bench] serve {'mixed' if mixed else 'fp64'}: {out['solves_per_s']} solves/s, p50 "
+        f"{out['latency_ms_p50']} ms, p99 {out['latency_ms_p99']} ms ({time.perf_counter() - t:.1f} s)")
+    return out
 
 
 def main():
@@ -116,8 +155,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--ncy", type=int, default=30)
-    ap.add_argument("--cpu-sample", type=int, default=129, help="nodes per axis of the CPU sample (0: skip)")
-    ap.add_argument("--cpu-cycles", type=int, default=6)
+    ap.add_argument("--cpu-sample", type=int, default=257, help="nodes per axis of the CPU sample (0: skip)")
+    ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--traffic-file", default="",
                     help="rocprofv3 PMC summary of pass A on this workload "
@@ -130,6 +169,9 @@ def main():
     ap.add_argument("--mixed-ref-tol", type=float, default=1e-22,
                     help="squared relative residual of the untimed FP64 reference solve that the mixed and plain "
                          "solutions are compared with (0: skip)")
+    ap.add_argument("--serve", type=int, default=1, help="also run the request server on the serving stream (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -228,23 +270,7 @@ def main():
     mixed = None
     if args.mixed:
         u_plain = s.get(U, L)
-        for i in range(args.warmup):
-            zero_state()
-            fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, cheb_degree=args.mixed_cheb,
-                                      cheb_ratio=args.mixed_ratio)
-            log(f"[bench] mixed warmup {i}: {it} iterations, rel. residual^2 {s.last_stats.r_norm_final / fm:.3e}")
-        s.synchronize()
-        t0 = time.perf_counter()
-        its, mrels = 0, []
-        for i in range(args.steps):
-            zero_state()
-            fm, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, cheb_degree=args.mixed_cheb,
-                                      cheb_ratio=args.mixed_ratio)
-            its += it
-            mrels.append(s.last_stats.r_norm_final / fm)
-        s.synchronize()
-        mel = time.perf_counter() - t0
-        u_mixed = s.get(U, L)
+        plain_rel = max(rels)
         mode = W.rotation_mode(w, L)
 
         def err(u, ref):
@@ -253,6 +279,39 @@ def main():
             du = du - (du @ mode) / (mode @ mode) * mode
             return float(np.linalg.norm(du) / np.linalg.norm(ref))
 
+        def timed_fcg(tol, p64, label, mfma=False):
+            """warmup + timed flexible-CG solves from u = 0: (solves/s, ms, iterations per solve, worst rel)"""
+            kw = dict(tol=tol, max_iter=200, cheb_degree=args.mixed_cheb, cheb_ratio=args.mixed_ratio,
+                      precond_fp64=p64, element_mfma=mfma)
+            for i in range(args.warmup):
+                zero_state()
+                fm, it = s.u_entire_mixed(1.0, 0, 0.0, **kw)
+                log(f"[bench] {label} warmup {i}: {it} iterations, rel. residual^2 "
+                    f"{s.last_stats.r_norm_final / fm:.3e}")
+            s.synchronize()
+            t0 = time.perf_counter()
+            its, worst = 0, 0.0
+            for i in range(args.steps):
+                zero_state()
+                fm, it = s.u_entire_mixed(1.0, 0, 0.0, **kw)
+                its += it
+                worst = max(worst, s.last_stats.r_norm_final / fm)
+            s.synchronize()
+            el = time.perf_counter() - t0
+            log(f"[bench] {label}: {its} iterations in {el:.3f} s")
+            return args.steps / el, el / args.steps * 1e3, its / args.steps, worst
+
+        # FP32 V-cycle preconditioner (the mixed-precision form) to the tolerance
+        sps32, ms32, it32, rel32 = timed_fcg(w.tol_u, False, "mixed FP32 V-cycle")
+        u_mixed = s.get(U, L)
+        # the identical V-cycle in FP64, same flexible CG, same tolerance
+        sps64, ms64, it64, rel64 = timed_fcg(w.tol_u, True, "FP64 V-cycle")
+        u_p64 = s.get(U, L)
+        # the FP32 V-cycle with the finest level's element contraction on the matrix cores
+        spsm, msm, itm, relm = timed_fcg(w.tol_u, False, "mixed FP32 V-cycle, matrix-core element", mfma=True)
+        u_mfma = s.get(U, L)
+        # the mixed form timed to the residual the plain FAS solve reaches
+        spsq, msq, itq, relq = timed_fcg(plain_rel, False, "mixed FP32 V-cycle to the plain solve's residual")
         # untimed reference: the same FP64 system converged far past the tolerance (the flexible
         # CG's outer operator, residual and updates are FP64; the preconditioner's precision does
         # not change the converged solution)
@@ -260,28 +319,60 @@ def main():
         if args.mixed_ref_tol > 0:
             zero_state()
             fr, itr = s.u_entire_mixed(1.0, 0, 0.0, tol=args.mixed_ref_tol, max_iter=400,
-                                       cheb_degree=args.mixed_cheb, cheb_ratio=args.mixed_ratio)
-            ref = {"rel_residual_squared": s.last_stats.r_norm_final / fr, "fcg_iterations": itr}
+                                       cheb_degree=args.mixed_cheb, cheb_ratio=args.mixed_ratio, precond_fp64=True)
+            ref = {"rel_residual_squared": s.last_stats.r_norm_final / fr, "fcg_iterations": itr,
+                   "preconditioner": "f64 V-cycle"}
             u_ref = s.get(U, L)
             log(f"[bench] FP64 reference: {itr} iterations, rel. residual^2 {ref['rel_residual_squared']:.3e}")
+        plain_sps = args.steps / elapsed
         mixed = {
-            "solves_per_s": round(args.steps / mel, 3),
-            "ms_per_solve": round(mel / args.steps * 1e3, 2),
-            "fcg_iterations_per_solve": its / args.steps,
-            "mdof_iter_s": round(3 * N * its / mel / 1e6, 2),
-            "final_rel_residual_squared": max(mrels),
-            "reached_tol": bool(max(mrels) <= w.tol_u),
-            "speedup_solves_vs_plain": round((args.steps / mel) / (args.steps / elapsed), 3),
-            "u_rel_l2_vs_plain_fp64": err(u_mixed, u_plain),
+            "solves_per_s": round(sps32, 3),
+            "ms_per_solve": round(ms32, 2),
+            "fcg_iterations_per_solve": it32,
+            "mdof_iter_s": round(3 * N * it32 * sps32 / 1e6, 2),
+            "final_rel_residual_squared": rel32,
+            "reached_tol": bool(rel32 <= w.tol_u),
             "u_rel_l2_vs_fp64_converged": err(u_mixed, u_ref) if ref else None,
+            # like for like: the same flexible CG and V-cycle with the preconditioner in FP64,
+            # to the same tolerance
+            "fp64_vcycle": {
+                "solves_per_s": round(sps64, 3),
+                "ms_per_solve": round(ms64, 2),
+                "fcg_iterations_per_solve": it64,
+                "final_rel_residual_squared": rel64,
+                "reached_tol": bool(rel64 <= w.tol_u),
+                "u_rel_l2_vs_fp64_converged": err(u_p64, u_ref) if ref else None,
+            },
+            "speedup_fp32_vs_fp64_vcycle_same_tol": round(sps32 / sps64, 3),
+            # the same FP32 V-cycle with the finest level's element contraction as batched FP32
+            # matrix-core products (v_mfma_f32_16x16x4_f32), FP32 accumulation
+            "mfma_element": {
+                "solves_per_s": round(spsm, 3),
+                "ms_per_solve": round(msm, 2),
+                "fcg_iterations_per_solve": itm,
+                "final_rel_residual_squared": relm,
+                "reached_tol": bool(relm <= w.tol_u),
+                "u_rel_l2_vs_fp64_converged": err(u_mfma, u_ref) if ref else None,
+            },
+            # against the plain FAS solve, which stops at its cycle cap above the tolerance: the
+            # mixed form timed to the residual the plain solve reaches, and the plain solve's state
+            "plain_fp64_final_rel_residual_squared": plain_rel,
+            "plain_fp64_reached_tol": bool(plain_rel <= w.tol_u),
             "plain_u_rel_l2_vs_fp64_converged": err(u_plain, u_ref) if ref else None,
+            "at_plain_residual": {
+                "tol": plain_rel,
+                "solves_per_s": round(spsq, 3),
+                "fcg_iterations_per_solve": itq,
+                "speedup_vs_plain": round(spsq / plain_sps, 3),
+            },
+            "solves_ratio_vs_plain_capped_unconverged": round(sps32 / plain_sps, 3),
             "fp64_converged_reference": ref,
-            "plain_fp64_final_rel_residual_squared": max(rels),
             "cheb_degree": args.mixed_cheb,
             "cheb_ratio": args.mixed_ratio,
             "dtype": "f32 V-cycle (Chebyshev-Jacobi, transfers, operators), f64 outer flexible CG, f64 coarsest",
         }
-        log(f"[bench] mixed: {its} iterations in {mel:.3f} s, {mixed['solves_per_s']} solves/s")
+        log(f"[bench] mixed: {sps32:.3f} solves/s; FP64 V-cycle {sps64:.3f} solves/s "
+            f"(x{sps32 / sps64:.2f}); at the plain residual {spsq:.3f} solves/s")
 
     cpu = None
     if args.cpu_sample > 0:
@@ -289,6 +380,14 @@ def main():
             f"{args.cpu_threads} threads")
         cpu = cpu_baseline(w, args.cpu_sample, args.cpu_cycles, args.cpu_threads)
         log(f"[bench] cpu baseline {cpu['value']:.3f} MDOF-iter/s")
+
+    serve = None
+    if args.serve:
+        serve = {"stream": f"{args.serve_requests} requests, Poisson {args.serve_rate}/s, node dims per axis from "
+                           "{33,65,129,257}, inclusion fraction U(0.10,0.30), one load step each, 8 slots, "
+                           "HBM budget 200 GB",
+                 "mixed": serve_stream(True, args.serve_requests, args.serve_rate, 600),
+                 "fp64": serve_stream(False, args.serve_requests, args.serve_rate, 600)}
 
     out = {
         "metric": METRIC,
@@ -318,6 +417,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "serve": serve,
         "kernels_finest_level": kern,
     }
     print(json.dumps(out), flush=True)

@@ -54,7 +54,8 @@ _d = ctypes.c_double
 class MixedParams(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int), ("tol", ctypes.c_double), ("cheb_degree", ctypes.c_int),
                 ("power_iters", ctypes.c_int), ("cheb_ratio", ctypes.c_double), ("cheb_upper", ctypes.c_double),
-                ("replace_every", ctypes.c_int)]
+                ("replace_every", ctypes.c_int), ("precond_fp64", ctypes.c_int),
+                ("element_mfma", ctypes.c_int)]
 
 
 class FuserStats(ctypes.Structure):
@@ -82,6 +83,7 @@ PROTOTYPES = {
     "hpcr_u_entire_mixed": (_i, [_vp, _d, _i, _dp, ctypes.POINTER(MixedParams), ctypes.POINTER(Stats)]),
     "hpcr_estimate_bytes": (_i, [ctypes.POINTER(Geometry), _i, ctypes.POINTER(ctypes.c_longlong)]),
     "hpcr_set_graphs": (_i, [_vp, _i]),
+    "hpcr_set_reduction_fence": (_i, [_vp, _i]),
     "hpcr_fuser_create": (_i, [_i, _i, ctypes.POINTER(_vp)]),
     "hpcr_fuser_destroy": (_i, [_vp]),
     "hpcr_fuser_set_graphs": (_i, [_vp, _i]),
@@ -97,6 +99,7 @@ PROTOTYPES = {
     "hpcr_d_max": (_i, [_vp, _dp]),
     "hpcr_save_d_old": (_i, [_vp]),
     "hpcr_internal_force": (_i, [_vp, _i, _i, _i]),
+    "hpcr_internal_force_f32": (_i, [_vp, _i, _i, _i]),
     "hpcr_prem_u": (_i, [_vp, _i, _i]),
     "hpcr_fld": (_i, [_vp, _i, _i, _i]),
     "hpcr_precondition_m": (_i, [_vp, _i, _i]),

@@ -85,10 +85,19 @@ struct LevelGeom {
   const unsigned char* fz;  // bit0 z == z_a,   bit1 z == z_b
 };
 
+// most nodes of one level: the kernels address a node of one field
+// component by a 32-bit byte offset (8 bytes per node)
+constexpr long long MAX_LEVEL_NODES = 1LL << 29;
+
+// equal spacing in x, y and z: the mechanical tile passes fold the grid
+// scale into the moduli
+inline bool cubic_grid(const LevelGeom& L) { return L.hx == L.hy && L.hy == L.hz; }
+
 struct Reduce {
   double* partials;     // >= blocks * 4 doubles
   unsigned int* counter; // zero between kernels
   double* scal;          // S_COUNT doubles
+  int fence;             // order the partials before the ticket by a release fence (see reduce_finalize)
 };
 
 // Multi-problem launches.  While a recorder is set on the calling thread the
@@ -122,13 +131,19 @@ struct MechCoef {
   double small_k;
 };
 
-// FP32 copies of the mechanical coefficients of one level (preconditioner of
-// the mixed-precision solve).
-struct MixCoef {
+// Coefficients of one level of the mixed-precision solve's preconditioner,
+// whose arithmetic type P is float (the mixed-precision form) or double (the
+// same V-cycle in FP64).  Finest level: P = float reads the precomputed
+// Gauss-point moduli `lm`, P = double the nodal K, G, d and split flag of `m`
+// as the FAS passes do; coarse levels: element moduli Ke, Ge of type P.
+template <class P>
+struct MixCoefT {
   int fine;
-  const float* lm;  // finest level: (lambda, mu) at the 8 Gauss points, 16 floats per element (element-major)
-  const float* Ke;  // coarse levels: element moduli
-  const float* Ge;
+  int mfma;         // FP32 finest level: element contraction on the matrix cores
+  const float* lm;  // FP32 finest level: (lambda, mu) at the 8 Gauss points, 16 floats per element (element-major)
+  const P* Ke;      // coarse levels: element moduli
+  const P* Ge;
+  MechCoef m;       // FP64 finest level
 };
 
 // Material table in device memory (layout of hpcr_material_table)
@@ -207,38 +222,49 @@ void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, cons
                         double* scal);
 
 // ---- mixed-precision solve (hpcr_mixed.hip) ----
+// The preconditioner's launchers are templated on its arithmetic type P
+// (instantiated for float and double).
 // Chebyshev step: r -= A s (masked), d = cd s + cr D^-1 r, x += d; with qprev,
 // also x.qprev -> scal[S_ZQ]
-void launch_mix_cheb(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout, float* r,
-                     float* x, const float* invD, float cd, float cr, const double* qprev, const Reduce& R);
+template <class P>
+void launch_mix_cheb(hipStream_t st, const LevelGeom& L, const MixCoefT<P>& C, const P* s, P* dout, P* r, P* x,
+                     const P* invD, P cd, P cr, const double* qprev, const Reduce& R);
 // r -= A s (masked)
-void launch_mix_res(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* r);
+template <class P>
+void launch_mix_res(hipStream_t st, const LevelGeom& L, const MixCoefT<P>& C, const P* s, P* r);
 // w = D^-1 A v (masked); Rayleigh quotient (v.Av)/(v.Dv) -> scal[S_LAM]
-void launch_mix_power(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* v, float* w,
-                      const float* invD, const Reduce& R);
-void launch_power_init(hipStream_t st, const LevelGeom& L, float* v);
+template <class P>
+void launch_mix_power(hipStream_t st, const LevelGeom& L, const MixCoefT<P>& C, const P* v, P* w, const P* invD,
+                      const Reduce& R);
+template <class P>
+void launch_power_init(hipStream_t st, const LevelGeom& L, P* v);
 // flexible CG pass A (FP64): p = z + beta p_old, q = A p, p.q, p.r -> alpha.
 // With u_lag (not first): also u += alpha_prev p_old, the update a lean
 // pass B left out.
-void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
+template <class P>
+void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const P* z, const double* p_old,
                   double* p_new, double* q, const double* r, int first, double* u_lag, const Reduce& R);
 // the update a lean pass B left out, when no pass A follows: u += alpha_prev p
 void launch_fcg_flush(hipStream_t st, const LevelGeom& L, double* u, const double* p, const Reduce& R);
 // flexible CG pass B (FP64) + start of the next preconditioner application;
 // lean: u and p are left alone and alpha is kept in scal[S_APREV]
+template <class P>
 void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, const double* p, const double* q,
-                  float* r32, float* d, float* x, const float* invD, float cr0, bool lean, const Reduce& R);
-void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, float* r32, float* d, float* x,
-                      const float* invD, float cr0);
-void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,
-                         float* dc, float* xc, const float* invD, float cr0, int start);
-void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, float* xf, float* cf,
-                        const float* ec);
-void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b);
+                  P* r32, P* d, P* x, const P* invD, P cr0, bool lean, const Reduce& R);
+template <class P>
+void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, P* r32, P* d, P* x, const P* invD,
+                      P cr0);
+template <class P>
+void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const P* rf, P* rc, P* dc, P* xc,
+                         const P* invD, P cr0, int start);
+template <class P>
+void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, P* xf, P* cf, const P* ec);
+// b = a converted to B (double -> float, float -> double, double -> double)
+template <class A, class B>
+void launch_convert(hipStream_t st, size_t n, const A* a, B* b);
 // FP32 Gauss-point (lambda, mu) of the finest level's elements
 void launch_mix_lame(hipStream_t st, const LevelGeom& L, const double* K, const double* G, const double* d,
                      const signed char* flag, double small_k, float* lm);
-void launch_to_f64(hipStream_t st, size_t n, const float* a, double* b);
 void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke, const double* Ge, float* x,
                             const float* b, const double* invD, int NB, double* scal);
 

@@ -47,7 +47,11 @@ thread_local Recorder* t_recorder = nullptr;
 // ---- mechanical tile ops --------------------------------------------------
 enum MechMode { M_APPLY = 0, M_RESID = 1, M_PCGA = 2, M_DIAG = 3 };
 
-template <int MODE, bool FINE>
+// FOLD: the grid is cubic (hx = hy = hz = h), so the gradient's 1/h and the
+// transposed contraction's w_g/h fold into the moduli (times w_g/h^2, once
+// per node plane on the finest level, once per element on coarse levels);
+// with a power-of-two h this gives the same bits as the scaled form.
+template <int MODE, bool FINE, bool FOLD = false>
 struct MechOp {
   using T = double;
   using TR = double;
@@ -78,8 +82,10 @@ struct MechOp {
   double* out3;         // RESID: z
   int flag;             // RESID: t0fine; PCGA: first
   double sx, sy, sz, tx_, ty_, tz_, wg, ihx2, ihy2, ihz2;
+  double fs;            // FOLD: w_g / h^2
   Reduce R;
   size_t N;
+  static_assert(!(FOLD && MODE == M_DIAG), "the diagonal keeps its own scaling");
 
   __device__ __forceinline__ State state() const {
     State s;
@@ -92,23 +98,23 @@ struct MechOp {
 #pragma unroll
     for (int k = 0; k < NR; k++) r[k] = 0.0;
     if (!ok) return;
-    const size_t q = nidx(L, gx, gy, gz);
+    const unsigned ob = nid32(L, gx, gy, gz) * 8u;
     if constexpr (MODE == M_APPLY || MODE == M_RESID) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) r[c] = x[c * N + q];
+      for (int c = 0; c < 3; c++) r[c] = ldo(x + c * N, ob);
     } else if constexpr (MODE == M_PCGA) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) r[c] = x[c * N + q];
+      for (int c = 0; c < 3; c++) r[c] = ldo(x + c * N, ob);
       if (!flag) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) r[3 + c] = p_old[c * N + q];
+        for (int c = 0; c < 3; c++) r[3 + c] = ldo(p_old + c * N, ob);
       }
     }
     if constexpr (FINE) {
       constexpr int o = NR - 3;
-      r[o + 0] = C.K[q];
-      r[o + 1] = C.G[q];
-      r[o + 2] = C.d[q];
+      r[o + 0] = ldo(C.K, ob);
+      r[o + 1] = ldo(C.G, ob);
+      r[o + 2] = ldo(C.d, ob);
     }
   }
 
@@ -116,12 +122,12 @@ struct MechOp {
 #pragma unroll
     for (int k = 0; k < NE; k++) e[k] = TE(0);
     if (!ok) return;
-    const size_t q = nidx(L, ex, ey, ez);
+    const unsigned q = nid32(L, ex, ey, ez);
     if constexpr (FINE) {
-      e[0] = C.flag[q];
+      e[0] = ldo(C.flag, q);
     } else {
-      e[0] = C.Ke[q];
-      e[1] = C.Ge[q];
+      e[0] = ldo(C.Ke, 8u * q);
+      e[1] = ldo(C.Ge, 8u * q);
     }
   }
 
@@ -134,6 +140,11 @@ struct MechOp {
     } else {
 #pragma unroll
       for (int k = 0; k < NV; k++) v[k] = r[k];
+    }
+    if constexpr (FINE && FOLD) {
+      // K and G times w_g/h^2 (zero stays zero: the zero rule is unchanged)
+      v[NU + 0] *= fs;
+      v[NU + 1] *= fs;
     }
   }
 
@@ -150,6 +161,10 @@ struct MechOp {
     } else {
       lam0 = e[0] - (2.0 / 3.0) * e[1];
       mu0 = e[1];
+      if constexpr (FOLD) {
+        lam0 *= fs;
+        mu0 *= fs;
+      }
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -174,9 +189,9 @@ struct MechOp {
 #pragma unroll
       for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
       if constexpr (FINE)
-        mech_element<false>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
+        mech_element<false, !FOLD>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
       else
-        mech_element<true>(u, lam, mu, lam0, mu0, sx, sy, sz, tx_, ty_, tz_, f);
+        mech_element<true, !FOLD>(u, lam, mu, lam0, mu0, sx, sy, sz, tx_, ty_, tz_, f);
     }
   }
 
@@ -187,11 +202,11 @@ struct MechOp {
   };
   __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
     if constexpr (MODE == M_RESID) {
-      const size_t q = nidx(L, i, j, k);
+      const unsigned ob = nid32(L, i, j, k) * 8u;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        n.f[c] = out2[c * N + q];
-        n.iD[c] = invD[c * N + q];
+        n.f[c] = ldo(out2 + c * N, ob);
+        n.iD[c] = ldo(invD + c * N, ob);
       }
     }
   }
@@ -199,10 +214,10 @@ struct MechOp {
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k, int dm,
                                            const double (&v)[3], const State& st, const NData& nd,
                                            Acc& acc) const {
-    const size_t q = nidx(L, i, j, k);
+    const unsigned ob = nid32(L, i, j, k) * 8u;
     if constexpr (MODE == M_APPLY) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) out[c * N + q] = v[c];
+      for (int c = 0; c < 3; c++) sto(out + c * N, ob, v[c]);
     } else if constexpr (MODE == M_RESID) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -210,12 +225,12 @@ struct MechOp {
         const bool dir = (dm >> c) & 1;
         if (flag && dir) {
           fv = v[c];
-          out2[c * N + q] = fv;
+          sto(out2 + c * N, ob, fv);
         }
         const double r = dir ? 0.0 : fv - v[c];
         const double z = r * nd.iD[c];
-        out[c * N + q] = r;
-        out3[c * N + q] = z;
+        sto(out + c * N, ob, r);
+        sto(out3 + c * N, ob, z);
         acc.a[0] += r * z;
         acc.a[1] += r * r;
         acc.a[2] += fv * fv;
@@ -224,15 +239,15 @@ struct MechOp {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const double p = own[c];
-        out2[c * N + q] = p;
-        out[c * N + q] = v[c];
+        sto(out2 + c * N, ob, p);
+        sto(out + c * N, ob, v[c]);
         acc.a[0] += p * v[c];
       }
     } else {  // M_DIAG
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        if (out2) out2[c * N + q] = v[c];
-        out[c * N + q] = 1.0 / v[c];
+        if (out2) sto(out2 + c * N, ob, v[c]);
+        sto(out + c * N, ob, 1.0 / v[c]);
         acc.a[0] += (v[c] == 0.0) ? 1.0 : 0.0;
       }
     }
@@ -405,9 +420,9 @@ struct PfOp {
 
 static dim3 tile_grid(const LevelGeom& L) { return dim3((unsigned)tile_blocks(L.nx, L.ny, L.nz, L.zc)); }
 
-template <int MODE, bool FINE>
-static MechOp<MODE, FINE> mech_op(const LevelGeom& L, const MechCoef& C) {
-  MechOp<MODE, FINE> op{};
+template <int MODE, bool FINE, bool FOLD>
+static MechOp<MODE, FINE, FOLD> mech_op(const LevelGeom& L, const MechCoef& C) {
+  MechOp<MODE, FINE, FOLD> op{};
   op.C = C;
   op.N = (size_t)L.nx * L.ny * L.nz;
   op.sx = 1.0 / L.hx;
@@ -420,24 +435,36 @@ static MechOp<MODE, FINE> mech_op(const LevelGeom& L, const MechCoef& C) {
   op.ihx2 = 1.0 / (L.hx * L.hx);
   op.ihy2 = 1.0 / (L.hy * L.hy);
   op.ihz2 = 1.0 / (L.hz * L.hz);
+  op.fs = op.tx_ * op.sx;
   return op;
+}
+
+template <int MODE, bool FINE, bool FOLD>
+static void run_mech_op(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, const double* invD,
+                        const double* p_old, double* out, double* out2, double* out3, int flag, const Reduce& R) {
+  auto op = mech_op<MODE, FINE, FOLD>(L, C);
+  op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
+  op.flag = flag; op.R = R;
+  using Op = MechOp<MODE, FINE, FOLD>;
+  if (record<TileK<Op>>(tile_grid(L), {L, op})) return;
+  tile_kernel<Op><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
 }
 
 template <int MODE>
 static void run_mech(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, const double* invD,
                      const double* p_old, double* out, double* out2, double* out3, int flag, const Reduce& R) {
+  constexpr bool CAN_FOLD = (MODE != M_DIAG);
+  const bool fold = CAN_FOLD && cubic_grid(L);
   if (C.fine) {
-    auto op = mech_op<MODE, true>(L, C);
-    op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
-    op.flag = flag; op.R = R;
-    if (record<TileK<MechOp<MODE, true>>>(tile_grid(L), {L, op})) return;
-    tile_kernel<MechOp<MODE, true>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
+    if (fold)
+      run_mech_op<MODE, true, CAN_FOLD>(s, L, C, x, invD, p_old, out, out2, out3, flag, R);
+    else
+      run_mech_op<MODE, true, false>(s, L, C, x, invD, p_old, out, out2, out3, flag, R);
   } else {
-    auto op = mech_op<MODE, false>(L, C);
-    op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
-    op.flag = flag; op.R = R;
-    if (record<TileK<MechOp<MODE, false>>>(tile_grid(L), {L, op})) return;
-    tile_kernel<MechOp<MODE, false>><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
+    if (fold)
+      run_mech_op<MODE, false, CAN_FOLD>(s, L, C, x, invD, p_old, out, out2, out3, flag, R);
+    else
+      run_mech_op<MODE, false, false>(s, L, C, x, invD, p_old, out, out2, out3, flag, R);
   }
 }
 

@@ -1,6 +1,9 @@
 // Mixed-precision form of the mechanical solve: an FP64 flexible conjugate
 // gradient (FCG(1), one-term orthogonalisation) on the finest level whose
-// preconditioner is one FP32 multigrid V-cycle.
+// preconditioner is one FP32 multigrid V-cycle.  Every preconditioner kernel
+// is templated on its arithmetic type P: P = float is the mixed-precision
+// form, P = double the same V-cycle in FP64 (the like-for-like comparison of
+// the two precisions at the same tolerance).
 //
 // Preconditioner.  Correction scheme on the same level hierarchy and
 // rediscretised operators as the FAS solve (finest: nodal K, G, d with the
@@ -34,57 +37,72 @@ namespace hpcr {
 // F_CHEBQ: a Chebyshev step that also accumulates x.q_prev into S_ZQ
 enum FMode { F_CHEB = 0, F_RES = 1, F_POWER = 2, F_CHEBQ = 3 };
 
-template <int MODE, bool FINE>
-struct MechOpF {
-  using T = float;
-  using TR = float;
+// FOLD: cubic grid, the grid scale w_g/h^2 folded into the moduli (the FP32
+// finest level's Gauss-point moduli carry it from the set-up, the FP64
+// finest level scales nodal K and G as its planes are committed, coarse
+// levels scale K_e, G_e per element); see mech_element.
+template <class P, int MODE, bool FINE, bool FOLD = false>
+struct MechOpP {
+  using T = P;
+  using TR = P;
+  // the FP64 finest level reads nodal K, G, d and the split flag, as the FAS
+  // passes do; the FP32 one the precomputed Gauss-point moduli
+  static constexpr bool F64 = std::is_same<P, double>::value;
+  static constexpr bool NODAL = F64 && FINE;
   static constexpr int NC = 3;
   static constexpr int NU = 3;
-  static constexpr int NV = NU;
+  static constexpr int NV = NODAL ? 6 : NU;
   static constexpr int NR = NV;
   static constexpr bool NEEDS_OWN = (MODE != F_RES);
-  static constexpr int NSL = HPCR_MIXF_SLOTS, MIN_WAVES = HPCR_MIXF_WAVES;
-  using Raw = float[NR];
+  static constexpr int NSL = F64 ? NSLOT : HPCR_MIXF_SLOTS, MIN_WAVES = F64 ? 1 : HPCR_MIXF_WAVES;
+  using Raw = P[NR];
   struct Acc {
     double a[2];
     __device__ Acc() { a[0] = a[1] = 0.0; }
   };
   struct State {};
-  // finest level: (lambda, mu) at the 8 Gauss points, precomputed once per
-  // solve; coarse levels: K_e, G_e
-  // finest level: four float4 per element, (lambda, mu) of Gauss points
-  // 2k and 2k+1 in float4 k (16-byte loads and LDS accesses)
-  using TE = std::conditional_t<FINE, float4, float>;
-  static constexpr int NE = FINE ? 4 : 2;
-  MixCoef C;
-  const float* s;        // stencil vector
-  float* dout;           // CHEB: new direction; POWER: D^-1 A s
-  float* r;              // CHEB, RES: residual, updated in place
-  float* x;              // CHEB: iterate, updated in place
-  const float* invD;
-  float cd, cr;          // CHEB: d = cd s + cr D^-1 r
+  // finest level, FP32: four float4 per element, (lambda, mu) of Gauss
+  // points 2k and 2k+1 in float4 k (16-byte loads and LDS accesses);
+  // finest level, FP64: the split flag; coarse levels: K_e, G_e
+  using TE = std::conditional_t<FINE, std::conditional_t<F64, int, float4>, P>;
+  static constexpr int NE = FINE ? (F64 ? 1 : 4) : 2;
+  MixCoefT<P> C;
+  const P* s;            // stencil vector
+  P* dout;               // CHEB: new direction; POWER: D^-1 A s
+  P* r;                  // CHEB, RES: residual, updated in place
+  P* x;                  // CHEB: iterate, updated in place
+  const P* invD;
+  P cd, cr;              // CHEB: d = cd s + cr D^-1 r
   const double* qprev;   // CHEB (optional): accumulate x.qprev into S_ZQ
-  float sx, sy, sz, tx_, ty_, tz_;
+  P sx, sy, sz, tx_, ty_, tz_;
+  P fs;                  // FOLD: w_g / h^2
   Reduce R;
   size_t N;
 
   __device__ __forceinline__ State state() const { return State{}; }
 
   __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State&,
-                                        float (&v)[NR]) const {
+                                        P (&v)[NR]) const {
 #pragma unroll
-    for (int k = 0; k < NR; k++) v[k] = 0.0f;
+    for (int k = 0; k < NR; k++) v[k] = P(0);
     if (!ok) return;
     const size_t q = nidx(L, gx, gy, gz);
 #pragma unroll
     for (int c = 0; c < 3; c++) v[c] = s[c * N + q];
+    if constexpr (NODAL) {
+      v[3] = C.m.K[q];
+      v[4] = C.m.G[q];
+      v[5] = C.m.d[q];
+    }
   }
   __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, TE (&e)[NE]) const {
 #pragma unroll
     for (int k = 0; k < NE; k++) e[k] = TE{};
     if (!ok) return;
     const size_t q = nidx(L, ex, ey, ez);
-    if constexpr (FINE) {
+    if constexpr (NODAL) {
+      e[0] = (int)C.m.flag[q];
+    } else if constexpr (FINE) {
       const float4* p = reinterpret_cast<const float4*>(C.lm) + 4 * q;
 #pragma unroll
       for (int k = 0; k < 4; k++) e[k] = p[k];
@@ -93,18 +111,29 @@ struct MechOpF {
       e[1] = C.Ge[q];
     }
   }
-  __device__ __forceinline__ void commit(const float (&v)[NR], const State&, float (&o)[NV]) const {
+  __device__ __forceinline__ void commit(const P (&v)[NR], const State&, P (&o)[NV]) const {
 #pragma unroll
     for (int k = 0; k < NV; k++) o[k] = v[k];
+    if constexpr (NODAL && FOLD) {
+      o[3] *= fs;
+      o[4] *= fs;
+    }
   }
 
-  __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[NSL][NV][PLANE], int s0, int tx,
+  __device__ __forceinline__ void element(const LevelGeom& L, const P (&sN)[NSL][NV][PLANE], int s0, int tx,
                                           int ty, int ex, int ey, int ez, const State&, const TE (&e)[NE],
-                                          float (&f)[3][2][2][2]) const {
-    float u[3][2][2][2];
+                                          P (&f)[3][2][2][2]) const {
+    P u[3][2][2][2];
 #pragma unroll
     for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
-    if constexpr (FINE) {
+    if constexpr (NODAL) {
+      double Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
+      gather8(sN, s0, tx, ty, 3, Kn);
+      gather8(sN, s0, tx, ty, 4, Gn);
+      gather8(sN, s0, tx, ty, 5, dn);
+      fine_lame(Kn, Gn, dn, e[0], C.m.small_k, lam, mu);
+      mech_element<false, !FOLD>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
+    } else if constexpr (FINE) {
       float lam[2][2][2], mu[2][2][2];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -114,18 +143,22 @@ struct MechOpF {
         lam[g1 >> 2][(g1 >> 1) & 1][g1 & 1] = e[k].z;
         mu[g1 >> 2][(g1 >> 1) & 1][g1 & 1] = e[k].w;
       }
-      mech_element<false>(u, lam, mu, 0.0f, 0.0f, sx, sy, sz, tx_, ty_, tz_, f);
+      mech_element<false, !FOLD>(u, lam, mu, 0.0f, 0.0f, sx, sy, sz, tx_, ty_, tz_, f);
     } else {
-      const float lam0 = e[0] - (2.0f / 3.0f) * e[1];
-      float dl[2][2][2], dm[2][2][2];
-      mech_element<true>(u, dl, dm, lam0, e[1], sx, sy, sz, tx_, ty_, tz_, f);
+      P lam0 = e[0] - (P(2) / P(3)) * e[1], mu0 = e[1];
+      if constexpr (FOLD) {
+        lam0 *= fs;
+        mu0 *= fs;
+      }
+      P dl[2][2][2], dm[2][2][2];
+      mech_element<true, !FOLD>(u, dl, dm, lam0, mu0, sx, sy, sz, tx_, ty_, tz_, f);
     }
   }
 
   // the epilogue's inputs at the owner's node, read one plane ahead
   static constexpr bool CHEB = (MODE == F_CHEB || MODE == F_CHEBQ);
   struct NData {
-    float r[3], x[3], iD[3];
+    P r[3], x[3], iD[3];
     double qp[MODE == F_CHEBQ ? 3 : 1];
   };
   __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
@@ -139,17 +172,17 @@ struct MechOpF {
     }
   }
 
-  __device__ __forceinline__ void epilogue(const LevelGeom& L, const float (&own)[3], int i, int j, int k, int dm,
-                                           const float (&v)[3], const State&, const NData& nd,
+  __device__ __forceinline__ void epilogue(const LevelGeom& L, const P (&own)[3], int i, int j, int k, int dm,
+                                           const P (&v)[3], const State&, const NData& nd,
                                            Acc& acc) const {
     const size_t q = nidx(L, i, j, k);
     if constexpr (CHEB) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t o = c * N + q;
-        const float rn = ((dm >> c) & 1) ? 0.0f : nd.r[c] - v[c];
-        const float dn = cd * own[c] + cr * nd.iD[c] * rn;
-        const float xn = nd.x[c] + dn;
+        const P rn = ((dm >> c) & 1) ? P(0) : nd.r[c] - v[c];
+        const P dn = cd * own[c] + cr * nd.iD[c] * rn;
+        const P xn = nd.x[c] + dn;
         r[o] = rn;
         dout[o] = dn;
         x[o] = xn;
@@ -159,15 +192,15 @@ struct MechOpF {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t o = c * N + q;
-        r[o] = ((dm >> c) & 1) ? 0.0f : nd.r[c] - v[c];
+        r[o] = ((dm >> c) & 1) ? P(0) : nd.r[c] - v[c];
       }
     } else {  // F_POWER
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t o = c * N + q;
         const bool dir = (dm >> c) & 1;
-        const float di = nd.iD[c];
-        dout[o] = dir ? 0.0f : di * v[c];
+        const P di = nd.iD[c];
+        dout[o] = dir ? P(0) : di * v[c];
         if (!dir) {
           acc.a[0] += (double)own[c] * (double)v[c];
           acc.a[1] += (double)own[c] * (double)own[c] / (double)di;
@@ -187,11 +220,204 @@ struct MechOpF {
   }
 };
 
-// FCG pass A (FP64, finest level): p = z + beta p_prev with z the FP32
-// preconditioner output; q = A p; p.q, p.r -> alpha.  LAG: also
+// ---------------------------------------------------------------------------
+// The FP32 finest-level element contraction on the matrix cores
+// (v_mfma_f32_16x16x4_f32), the reference's B^T D B element product
+// (mechanical_pcg.cpp:114-172) as batched GEMMs.  A wave holds 64 elements
+// (tx = lane % 32, ty = 2 w + lane / 32), in four groups of 16.
+//   1. gradient: for every component c and group, D (16 x 16) = A (16 x 8) U
+//      (8 x 16) in two K-steps; row 4q + r of A is the derivative along axis q
+//      at Gauss-point pair r (its sum-factorised B-matrix row: an edge
+//      difference times two 1-D weights), rows 12..15 are zero, column n is
+//      the group's element n, U holds the element's 8 nodal values (read from
+//      the LDS node planes in the operand layout);
+//   2. each lane reads its own element's 36 derivatives from a per-wave LDS
+//      buffer, forms the stresses at the 8 Gauss points with its (lambda, mu)
+//      and writes back the 36 transposed-contraction inputs (Sx, Sy, Sz);
+//      the (lambda, mu) loads are issued from global memory at the start, so
+//      the gradient products cover their latency, and they take no LDS (the
+//      LDS goes to the wave buffers: three blocks per CU);
+//   3. transposed contraction: F (16 x 16) = A^T (16 x 12) S (12 x 16) in three
+//      K-steps; rows 0..7 are the element's 8 nodal forces;
+//   4. each lane reads its own element's 24 forces.
+// FP32 products with FP32 accumulation.  Operand layout of the instruction
+// (one wave): A[i][k] in lane 16 k + i, B[k][j] in lane 16 k + j, D[4 q + r][j]
+// in register r of lane 16 q + j.  The LDS buffer is private to the wave; its
+// lanes meet only through in-order LDS accesses of that wave.
+// ---------------------------------------------------------------------------
+typedef float mf4 __attribute__((ext_vector_type(4)));
+constexpr int MF_STRIDE = 36;  // floats per element in the wave's LDS buffer
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// row m = 4 q + r of the gradient matrix at node k (bits iz, iy, ix), with the
+// axis scales s[q]
+__device__ __forceinline__ float mf_grad_coef(int m, int k, const float (&s)[3]) {
+  const int q = m >> 2, r = m & 3;
+  if (q > 2) return 0.0f;
+  const int iz = k >> 2, iy = (k >> 1) & 1, ix = k & 1;
+  const int a = r >> 1, b = r & 1;
+  auto Lf = [](int i, int g) { return (i == g) ? (float)GA : (float)GB; };
+  float v;
+  if (q == 0)
+    v = (ix ? 1.0f : -1.0f) * Lf(iy, b) * Lf(iz, a);  // d/dx at (gz, gy) = (a, b)
+  else if (q == 1)
+    v = (iy ? 1.0f : -1.0f) * Lf(ix, b) * Lf(iz, a);  // d/dy at (gz, gx)
+  else
+    v = (iz ? 1.0f : -1.0f) * Lf(ix, b) * Lf(iy, a);  // d/dz at (gy, gx)
+  return v * s[q];
+}
+
+template <int MODE, bool FOLD>
+struct MechOpMfma : MechOpP<float, MODE, true, FOLD> {
+  using Base = MechOpP<float, MODE, true, FOLD>;
+  // no per-element data through the LDS plane pipeline (element() loads it)
+  using TE = float;
+  static constexpr int NE = 1;
+  static constexpr int NSL = Base::NSL, NV = Base::NV;
+
+  __device__ __forceinline__ void efetch(const LevelGeom&, int, int, int, bool, TE (&e)[NE]) const { e[0] = 0.0f; }
+
+  __device__ __forceinline__ void element(const LevelGeom& L, const float (&sN)[NSL][NV][PLANE], int s0, int tx,
+                                          int ty, int ex, int ey, int ez, const typename Base::State&,
+                                          const TE (&)[NE], float (&f)[3][2][2][2]) const {
+    __shared__ float sM[TPB / 64][64 * MF_STRIDE];
+    // this lane's Gauss-point moduli, zero for an element outside the grid
+    float4 lmv[4];
+    {
+      const bool ok = ex >= 0 && ey >= 0 && ex <= L.nx - 2 && ey <= L.ny - 2 && ez >= 0 && ez <= L.nz - 2;
+      const float4* p = reinterpret_cast<const float4*>(this->C.lm) + 4 * nidx(L, ok ? ex : 0, ok ? ey : 0, ok ? ez : 0);
+#pragma unroll
+      for (int k = 0; k < 4; k++) lmv[k] = ok ? p[k] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    float* M = sM[w];
+    const int i16 = lane & 15, k4 = lane >> 4;
+    const int s1 = (NSL == 3) ? (s0 == 2 ? 0 : s0 + 1) : (s0 ^ 1);
+    const float sg[3] = {FOLD ? 1.0f : this->sx, FOLD ? 1.0f : this->sy, FOLD ? 1.0f : this->sz};
+    const float st[3] = {FOLD ? 1.0f : this->tx_, FOLD ? 1.0f : this->ty_, FOLD ? 1.0f : this->tz_};
+    // operands of A (gradient, K-steps 0..1) and A^T (transposed, K-steps 0..2)
+    const float ag0 = mf_grad_coef(i16, k4, sg), ag1 = mf_grad_coef(i16, 4 + k4, sg);
+    float at[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) at[q] = (i16 < 8) ? mf_grad_coef(4 * q + k4, i16, st) : 0.0f;
+    // 1. gradients of the wave's 64 elements
+    const int iy = k4 >> 1, ix = k4 & 1;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int el = 16 * g + i16;                        // element of this operand column
+      const int qe = (el & 31) + PX * (2 * w + (el >> 5)) + ix + PX * iy;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        mf4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(ag0, sN[s0][c][qe], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(ag1, sN[s1][c][qe], d, 0, 0, 0);
+        if (k4 < 3) *reinterpret_cast<mf4*>(&M[el * MF_STRIDE + c * 12 + 4 * k4]) = d;
+      }
+    }
+    wave_lds_sync();
+    // 2. stresses at the Gauss points of this lane's element
+    {
+      float G[3][3][4];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const mf4 v = *reinterpret_cast<const mf4*>(&M[lane * MF_STRIDE + c * 12 + 4 * q]);
+          G[c][q][0] = v[0];
+          G[c][q][1] = v[1];
+          G[c][q][2] = v[2];
+          G[c][q][3] = v[3];
+        }
+      float lam[2][2][2], mu[2][2][2];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int g0 = 2 * k, g1 = 2 * k + 1;
+        lam[g0 >> 2][(g0 >> 1) & 1][g0 & 1] = lmv[k].x;
+        mu[g0 >> 2][(g0 >> 1) & 1][g0 & 1] = lmv[k].y;
+        lam[g1 >> 2][(g1 >> 1) & 1][g1 & 1] = lmv[k].z;
+        mu[g1 >> 2][(g1 >> 1) & 1][g1 & 1] = lmv[k].w;
+      }
+      float Sx[3][4], Sy[3][4], Sz[3][4];
+#pragma unroll
+      for (int gz = 0; gz < 2; gz++)
+#pragma unroll
+        for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+          for (int gx = 0; gx < 2; gx++) {
+            const int px = 2 * gz + gy, py = 2 * gz + gx, pz = 2 * gy + gx;
+            const float exx = G[0][0][px], eyy = G[1][1][py], ezz = G[2][2][pz];
+            const float gxy = G[1][0][px] + G[0][1][py];
+            const float gxz = G[2][0][px] + G[0][2][pz];
+            const float gyz = G[2][1][py] + G[1][2][pz];
+            const float l = lam[gz][gy][gx], m = mu[gz][gy][gx];
+            const float lt = l * (exx + eyy + ezz), m2 = 2.0f * m;
+            const float sxx = lt + m2 * exx, syy = lt + m2 * eyy, szz = lt + m2 * ezz;
+            const float sxy = m * gxy, sxz = m * gxz, syz = m * gyz;
+            auto put = [](float& a, float v, bool first) {
+              if (first) a = v; else a += v;
+            };
+            put(Sx[0][px], sxx, gx == 0); put(Sx[1][px], sxy, gx == 0); put(Sx[2][px], sxz, gx == 0);
+            put(Sy[0][py], sxy, gy == 0); put(Sy[1][py], syy, gy == 0); put(Sy[2][py], syz, gy == 0);
+            put(Sz[0][pz], sxz, gz == 0); put(Sz[1][pz], syz, gz == 0); put(Sz[2][pz], szz, gz == 0);
+          }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        float* o = &M[lane * MF_STRIDE + c * 12];
+        *reinterpret_cast<mf4*>(o + 0) = mf4{Sx[c][0], Sx[c][1], Sx[c][2], Sx[c][3]};
+        *reinterpret_cast<mf4*>(o + 4) = mf4{Sy[c][0], Sy[c][1], Sy[c][2], Sy[c][3]};
+        *reinterpret_cast<mf4*>(o + 8) = mf4{Sz[c][0], Sz[c][1], Sz[c][2], Sz[c][3]};
+      }
+    }
+    wave_lds_sync();
+    // 3. nodal forces: component c's inputs are read for all four groups
+    //    before its forces overwrite them
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      mf4 d[4];
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const int el = 16 * g + i16;
+        d[g] = mf4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+          d[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(at[q], M[el * MF_STRIDE + c * 12 + 4 * q + k4], d[g], 0, 0, 0);
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int g = 0; g < 4; g++)
+        if (k4 < 2) *reinterpret_cast<mf4*>(&M[(16 * g + i16) * MF_STRIDE + c * 12 + 4 * k4]) = d[g];
+    }
+    wave_lds_sync();
+    // 4. this lane's element: f[c][iz][iy][ix] = F[c][4 iz + 2 iy + ix]
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const mf4 a = *reinterpret_cast<const mf4*>(&M[lane * MF_STRIDE + c * 12 + 0]);
+      const mf4 b = *reinterpret_cast<const mf4*>(&M[lane * MF_STRIDE + c * 12 + 4]);
+      f[c][0][0][0] = a[0];
+      f[c][0][0][1] = a[1];
+      f[c][0][1][0] = a[2];
+      f[c][0][1][1] = a[3];
+      f[c][1][0][0] = b[0];
+      f[c][1][0][1] = b[1];
+      f[c][1][1][0] = b[2];
+      f[c][1][1][1] = b[3];
+    }
+    // the buffer is rewritten by the next layer's gradients only after
+    // every lane has read its forces
+    wave_lds_sync();
+  }
+};
+
+// FCG pass A (FP64, finest level): p = z + beta p_prev with z the
+// preconditioner output (type P); q = A p; p.q, p.r -> alpha.  LAG: also
 // u += alpha_prev p_prev at the own nodes, the update of the previous
 // iteration that its lean pass B left out (the same fma, one kernel later).
-template <bool LAG>
+template <class P, bool LAG, bool FOLD = false>
 struct FcgOpA {
   using T = double;
   using TR = double;
@@ -201,10 +427,10 @@ struct FcgOpA {
   static constexpr int NR = 9;
   static constexpr bool NEEDS_OWN = true;
   static constexpr int NSL = NSLOT, MIN_WAVES = 1;
-  // raw node values in flight: z stays FP32 until the commit, so that no
-  // conversion consumes a load before the layer that needs it
+  // raw node values in flight: z stays in its own type until the commit, so
+  // that no conversion consumes a load before the layer that needs it
   struct Raw {
-    float z[3];
+    P z[3];
     double p[3], K, G, d;
   };
   struct Acc {
@@ -218,7 +444,7 @@ struct FcgOpA {
   using TE = int;
   static constexpr int NE = 1;
   MechCoef C;
-  const float* z;
+  const P* z;
   const double* p_old;
   double* p_new;
   double* q;
@@ -226,6 +452,7 @@ struct FcgOpA {
   double* u;  // LAG
   int first;
   double sx, sy, sz, tx_, ty_, tz_;
+  double fs;  // FOLD: w_g / h^2
   Reduce R;
   size_t N;
 
@@ -239,7 +466,7 @@ struct FcgOpA {
                                         Raw& v) const {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      v.z[c] = 0.0f;
+      v.z[c] = P(0);
       v.p[c] = 0.0;
     }
     v.K = v.G = v.d = 0.0;
@@ -261,8 +488,8 @@ struct FcgOpA {
   __device__ __forceinline__ void commit(const Raw& v, const State& st, double (&w)[NV]) const {
 #pragma unroll
     for (int c = 0; c < 3; c++) w[c] = (double)v.z[c] + st.beta * v.p[c];
-    w[3] = v.K;
-    w[4] = v.G;
+    w[3] = FOLD ? v.K * fs : v.K;
+    w[4] = FOLD ? v.G * fs : v.G;
     w[5] = v.d;
   }
   __device__ __forceinline__ void element(const LevelGeom& L, const double (&sN)[NSL][NV][PLANE], int s0, int tx,
@@ -275,7 +502,7 @@ struct FcgOpA {
     gather8(sN, s0, tx, ty, 4, Gn);
     gather8(sN, s0, tx, ty, 5, dn);
     fine_lame(Kn, Gn, dn, e[0], C.small_k, lam, mu);
-    mech_element<false>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
+    mech_element<false, !FOLD>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
   }
   struct NData {
     double r[3];
@@ -318,10 +545,11 @@ struct FcgOpA {
 
 static dim3 mix_tile_grid(const LevelGeom& L) { return dim3((unsigned)tile_blocks(L.nx, L.ny, L.nz, L.zc)); }
 
-template <int MODE, bool FINE>
-static void run_f32(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout, float* r,
-                    float* x, const float* invD, float cd, float cr, const double* qprev, const Reduce& R) {
-  MechOpF<MODE, FINE> op{};
+template <class P, int MODE, bool FINE, bool FOLD, bool MFMA = false>
+static void run_mixp(hipStream_t st, const LevelGeom& L, const MixCoefT<P>& C, const P* s, P* dout, P* r, P* x,
+                     const P* invD, P cd, P cr, const double* qprev, const Reduce& R) {
+  using Op = std::conditional_t<MFMA, MechOpMfma<MODE, FOLD>, MechOpP<P, MODE, FINE, FOLD>>;
+  Op op{};
   op.C = C;
   op.s = s;
   op.dout = dout;
@@ -332,48 +560,70 @@ static void run_f32(hipStream_t st, const LevelGeom& L, const MixCoef& C, const 
   op.cr = cr;
   op.qprev = qprev;
   const double wg = 0.125 * L.hx * L.hy * L.hz;
-  op.sx = (float)(1.0 / L.hx);
-  op.sy = (float)(1.0 / L.hy);
-  op.sz = (float)(1.0 / L.hz);
-  op.tx_ = (float)(wg / L.hx);
-  op.ty_ = (float)(wg / L.hy);
-  op.tz_ = (float)(wg / L.hz);
+  op.sx = (P)(1.0 / L.hx);
+  op.sy = (P)(1.0 / L.hy);
+  op.sz = (P)(1.0 / L.hz);
+  op.tx_ = (P)(wg / L.hx);
+  op.ty_ = (P)(wg / L.hy);
+  op.tz_ = (P)(wg / L.hz);
+  op.fs = (P)(wg / (L.hx * L.hx));
   op.R = R;
   op.N = (size_t)L.nx * L.ny * L.nz;
-  if (record<TileK<MechOpF<MODE, FINE>>>(mix_tile_grid(L), {L, op})) return;
-  tile_kernel<MechOpF<MODE, FINE>><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
+  if (record<TileK<Op>>(mix_tile_grid(L), {L, op})) return;
+  tile_kernel<Op><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
 }
 
-template <int MODE>
-static void run_f32_level(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout,
-                          float* r, float* x, const float* invD, float cd, float cr, const double* qprev,
-                          const Reduce& R) {
-  if (C.fine)
-    run_f32<MODE, true>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
-  else
-    run_f32<MODE, false>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+template <class P, int MODE>
+static void run_mixp_level(hipStream_t st, const LevelGeom& L, const MixCoefT<P>& C, const P* s, P* dout, P* r,
+                           P* x, const P* invD, P cd, P cr, const double* qprev, const Reduce& R) {
+  // the FP32 finest level's moduli carry the grid scale when the set-up
+  // folded it (mix_lame with the cubic grid's w_g/h^2)
+  const bool fold = cubic_grid(L);
+  if (C.fine) {
+    if constexpr (std::is_same<P, float>::value) {
+      if (C.mfma) {
+        if (fold)
+          run_mixp<P, MODE, true, true, true>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+        else
+          run_mixp<P, MODE, true, false, true>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+        return;
+      }
+    }
+    if (fold)
+      run_mixp<P, MODE, true, true>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+    else
+      run_mixp<P, MODE, true, false>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+  } else {
+    if (fold)
+      run_mixp<P, MODE, false, true>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+    else
+      run_mixp<P, MODE, false, false>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+  }
 }
 
-void launch_mix_cheb(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* dout, float* r,
-                     float* x, const float* invD, float cd, float cr, const double* qprev, const Reduce& R) {
+template <class P>
+void launch_mix_cheb(hipStream_t st, const LevelGeom& L, const MixCoefT<P>& C, const P* s, P* dout, P* r, P* x,
+                     const P* invD, P cd, P cr, const double* qprev, const Reduce& R) {
   if (qprev)
-    run_f32_level<F_CHEBQ>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
+    run_mixp_level<P, F_CHEBQ>(st, L, C, s, dout, r, x, invD, cd, cr, qprev, R);
   else
-    run_f32_level<F_CHEB>(st, L, C, s, dout, r, x, invD, cd, cr, nullptr, R);
+    run_mixp_level<P, F_CHEB>(st, L, C, s, dout, r, x, invD, cd, cr, nullptr, R);
 }
-void launch_mix_res(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* s, float* r) {
+template <class P>
+void launch_mix_res(hipStream_t st, const LevelGeom& L, const MixCoefT<P>& C, const P* s, P* r) {
   Reduce R{};
-  run_f32_level<F_RES>(st, L, C, s, nullptr, r, nullptr, nullptr, 0.0f, 0.0f, nullptr, R);
+  run_mixp_level<P, F_RES>(st, L, C, s, nullptr, r, nullptr, nullptr, P(0), P(0), nullptr, R);
 }
-void launch_mix_power(hipStream_t st, const LevelGeom& L, const MixCoef& C, const float* v, float* w,
-                      const float* invD, const Reduce& R) {
-  run_f32_level<F_POWER>(st, L, C, v, w, nullptr, nullptr, invD, 0.0f, 0.0f, nullptr, R);
+template <class P>
+void launch_mix_power(hipStream_t st, const LevelGeom& L, const MixCoefT<P>& C, const P* v, P* w, const P* invD,
+                      const Reduce& R) {
+  run_mixp_level<P, F_POWER>(st, L, C, v, w, nullptr, nullptr, invD, P(0), P(0), nullptr, R);
 }
 
-template <bool LAG>
-static void run_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
+template <class P, bool LAG, bool FOLD>
+static void run_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const P* z, const double* p_old,
                       double* p_new, double* q, const double* r, int first, double* u, const Reduce& R) {
-  FcgOpA<LAG> op{};
+  FcgOpA<P, LAG, FOLD> op{};
   op.C = C;
   op.u = u;
   op.z = z;
@@ -389,17 +639,28 @@ static void run_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, con
   op.tx_ = wg / L.hx;
   op.ty_ = wg / L.hy;
   op.tz_ = wg / L.hz;
+  op.fs = wg / (L.hx * L.hx);
   op.R = R;
   op.N = (size_t)L.nx * L.ny * L.nz;
-  if (record<TileK<FcgOpA<LAG>>>(mix_tile_grid(L), {L, op})) return;
-  tile_kernel<FcgOpA<LAG>><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
+  using Op = FcgOpA<P, LAG, FOLD>;
+  if (record<TileK<Op>>(mix_tile_grid(L), {L, op})) return;
+  tile_kernel<Op><<<mix_tile_grid(L), dim3(TPB), 0, st>>>(L, op);
 }
-void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const float* z, const double* p_old,
+template <class P>
+void launch_fcg_a(hipStream_t st, const LevelGeom& L, const MechCoef& C, const P* z, const double* p_old,
                   double* p_new, double* q, const double* r, int first, double* u_lag, const Reduce& R) {
-  if (u_lag && !first)
-    run_fcg_a<true>(st, L, C, z, p_old, p_new, q, r, 0, u_lag, R);
-  else
-    run_fcg_a<false>(st, L, C, z, p_old, p_new, q, r, first, nullptr, R);
+  const bool fold = cubic_grid(L);
+  if (u_lag && !first) {
+    if (fold)
+      run_fcg_a<P, true, true>(st, L, C, z, p_old, p_new, q, r, 0, u_lag, R);
+    else
+      run_fcg_a<P, true, false>(st, L, C, z, p_old, p_new, q, r, 0, u_lag, R);
+  } else {
+    if (fold)
+      run_fcg_a<P, false, true>(st, L, C, z, p_old, p_new, q, r, first, nullptr, R);
+    else
+      run_fcg_a<P, false, false>(st, L, C, z, p_old, p_new, q, r, first, nullptr, R);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -421,16 +682,16 @@ __device__ __forceinline__ int node_mask(const LevelGeom& L, unsigned q) {
 }
 
 // start of the finest-level smoother from r (FP64): r32 = r, d = cr0 D^-1 r, x = d
+template <class P>
 __global__ __launch_bounds__(MX_TPB) void mix_start_kernel(unsigned N, const double* __restrict__ r64,
-                                                           float* __restrict__ r32, float* __restrict__ d,
-                                                           float* __restrict__ x, const float* __restrict__ invD,
-                                                           float cr0) {
+                                                           P* __restrict__ r32, P* __restrict__ d,
+                                                           P* __restrict__ x, const P* __restrict__ invD, P cr0) {
   for (unsigned q = blockIdx.x * MX_TPB + threadIdx.x; q < N; q += gridDim.x * MX_TPB) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t o = (size_t)c * N + q;
-      const float rv = (float)r64[o];
-      const float dv = cr0 * invD[o] * rv;
+      const P rv = (P)r64[o];
+      const P dv = cr0 * invD[o] * rv;
       r32[o] = rv;
       d[o] = dv;
       x[o] = dv;
@@ -443,12 +704,11 @@ __global__ __launch_bounds__(MX_TPB) void mix_start_kernel(unsigned N, const dou
 // Dirichlet axis flags are staged in LDS (nx + ny + nz <= DIRF_MAX).
 // LEAN: u and p are left alone (the next pass A, or launch_fcg_flush,
 // applies u += alpha p) and alpha moves to scal[S_APREV].
-template <bool LDSF, bool LEAN>
+template <class P, bool LDSF, bool LEAN>
 __device__ __forceinline__ void fcg_b_body(const LevelGeom& L, double* __restrict__ u, double* __restrict__ r,
                                            const double* __restrict__ p, const double* __restrict__ q,
-                                           float* __restrict__ r32, float* __restrict__ d, float* __restrict__ x,
-                                           const float* __restrict__ invD, float cr0, const Reduce& R,
-                                           const Blk& bk) {
+                                           P* __restrict__ r32, P* __restrict__ d, P* __restrict__ x,
+                                           const P* __restrict__ invD, P cr0, const Reduce& R, const Blk& bk) {
   __shared__ unsigned char sflags[LDSF ? DIRF_MAX : 1];
   const DirFlags F{sflags};
   if constexpr (LDSF) {
@@ -476,8 +736,8 @@ __device__ __forceinline__ void fcg_b_body(const LevelGeom& L, double* __restric
       if ((dm >> c) & 1) rv = 0.0;
       r[o] = rv;
       acc[0] += rv * rv;
-      const float rf = (float)rv;
-      const float dv = cr0 * invD[o] * rf;
+      const P rf = (P)rv;
+      const P dv = cr0 * invD[o] * rf;
       r32[o] = rf;
       d[o] = dv;
       x[o] = dv;
@@ -489,34 +749,35 @@ __device__ __forceinline__ void fcg_b_body(const LevelGeom& L, double* __restric
     if constexpr (LEAN) scal[S_APREV] = scal[S_FALPHA];
   });
 }
-template <bool LDSF, bool LEAN>
+template <class P, bool LDSF, bool LEAN>
 __global__ __launch_bounds__(MX_TPB) void fcg_b_kernel(const LevelGeom L, double* __restrict__ u,
                                                        double* __restrict__ r, const double* __restrict__ p,
-                                                       const double* __restrict__ q, float* __restrict__ r32,
-                                                       float* __restrict__ d, float* __restrict__ x,
-                                                       const float* __restrict__ invD, float cr0, const Reduce R) {
-  fcg_b_body<LDSF, LEAN>(L, u, r, p, q, r32, d, x, invD, cr0, R, Blk::hw());
+                                                       const double* __restrict__ q, P* __restrict__ r32,
+                                                       P* __restrict__ d, P* __restrict__ x,
+                                                       const P* __restrict__ invD, P cr0, const Reduce R) {
+  fcg_b_body<P, LDSF, LEAN>(L, u, r, p, q, r32, d, x, invD, cr0, R, Blk::hw());
 }
+template <class P>
 struct FcgBArgs {
   LevelGeom L;
   double* u;
   double* r;
   const double* p;
   const double* q;
-  float* r32;
-  float* d;
-  float* x;
-  const float* invD;
-  float cr0;
+  P* r32;
+  P* d;
+  P* x;
+  const P* invD;
+  P cr0;
   Reduce R;
 };
-template <bool LDSF, bool LEAN>
+template <class P, bool LDSF, bool LEAN>
 struct FcgBK {
-  using Args = FcgBArgs;
+  using Args = FcgBArgs<P>;
   static constexpr int NT = MX_TPB, MINW = 1;
   static dim3 block() { return dim3(MX_TPB); }
   __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
-    fcg_b_body<LDSF, LEAN>(a.L, a.u, a.r, a.p, a.q, a.r32, a.d, a.x, a.invD, a.cr0, a.R, bk);
+    fcg_b_body<P, LDSF, LEAN>(a.L, a.u, a.r, a.p, a.q, a.r32, a.d, a.x, a.invD, a.cr0, a.R, bk);
   }
 };
 
@@ -528,7 +789,8 @@ __global__ __launch_bounds__(MX_TPB) void fcg_flush_kernel(size_t n, double* __r
     u[o] = fma(alpha, p[o], u[o]);
 }
 
-// restriction of the FP32 residual (full weighting, as the FP64 transfer);
+// restriction of the preconditioner's residual (full weighting, as the FP64
+// transfer of the FAS path);
 // starts the coarse smoother (d = cr0 D^-1 r, x = d), or, on the coarsest
 // level (start == 0), only clears x for the coarsest solve
 __device__ __forceinline__ void mix_ijk(const LevelGeom& L, unsigned q, int& i, int& j, int& k) {
@@ -540,85 +802,89 @@ __device__ __forceinline__ void mix_ijk(const LevelGeom& L, unsigned q, int& i, 
   k = (int)kk;
 }
 
+template <class P>
 __device__ __forceinline__ void mix_restrict_body(const LevelGeom& Lc, const LevelGeom& Lf,
-                                                  const float* __restrict__ rf, float* __restrict__ rc,
-                                                  float* __restrict__ dc, float* __restrict__ xc,
-                                                  const float* __restrict__ invD, float cr0, int start, const Blk& bk) {
+                                                  const P* __restrict__ rf, P* __restrict__ rc, P* __restrict__ dc,
+                                                  P* __restrict__ xc, const P* __restrict__ invD, P cr0, int start,
+                                                  const Blk& bk) {
   const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lc)) return;
   int i, j, k;
   mix_ijk(Lc, qn, i, j, k);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const int x = 2 * i, y = 2 * j, z = 2 * k;
-  float wx[3], wy[3], wz[3];
+  P wx[3], wy[3], wz[3];
 #pragma unroll
   for (int t = 0; t < 3; t++) {
-    wx[t] = (x + t - 1 < 0 || x + t - 1 > Lf.nx - 1) ? 0.0f : (t == 1 ? 1.0f : 0.5f);
-    wy[t] = (y + t - 1 < 0 || y + t - 1 > Lf.ny - 1) ? 0.0f : (t == 1 ? 1.0f : 0.5f);
-    wz[t] = (z + t - 1 < 0 || z + t - 1 > Lf.nz - 1) ? 0.0f : (t == 1 ? 1.0f : 0.5f);
+    wx[t] = (x + t - 1 < 0 || x + t - 1 > Lf.nx - 1) ? P(0) : (t == 1 ? P(1) : P(0.5));
+    wy[t] = (y + t - 1 < 0 || y + t - 1 > Lf.ny - 1) ? P(0) : (t == 1 ? P(1) : P(0.5));
+    wz[t] = (z + t - 1 < 0 || z + t - 1 > Lf.nz - 1) ? P(0) : (t == 1 ? P(1) : P(0.5));
   }
   const size_t qc = nidx(Lc, i, j, k);
   const int dm = dir_mask(Lc, i, j, k);
   for (int c = 0; c < 3; c++) {
-    const float* r = rf + c * Nf;
-    float acc = 0.0f;
+    const P* r = rf + c * Nf;
+    P acc = P(0);
 #pragma unroll
     for (int dz = 0; dz < 3; dz++) {
-      if (wz[dz] == 0.0f) continue;
-      float accy = 0.0f;
+      if (wz[dz] == P(0)) continue;
+      P accy = P(0);
 #pragma unroll
       for (int dy = 0; dy < 3; dy++) {
-        if (wy[dy] == 0.0f) continue;
+        if (wy[dy] == P(0)) continue;
         const size_t row = nidx(Lf, 0, y + dy - 1, z + dz - 1);
-        float accx = 0.0f;
+        P accx = P(0);
 #pragma unroll
         for (int dx = 0; dx < 3; dx++)
-          if (wx[dx] != 0.0f) accx += wx[dx] * r[row + x + dx - 1];
+          if (wx[dx] != P(0)) accx += wx[dx] * r[row + x + dx - 1];
         accy += wy[dy] * accx;
       }
       acc += wz[dz] * accy;
     }
-    if ((dm >> c) & 1) acc = 0.0f;
+    if ((dm >> c) & 1) acc = P(0);
     const size_t o = c * Nc + qc;
     rc[o] = acc;
     if (start) {
-      const float dv = cr0 * invD[o] * acc;
+      const P dv = cr0 * invD[o] * acc;
       dc[o] = dv;
       xc[o] = dv;
     } else {
-      xc[o] = 0.0f;
+      xc[o] = P(0);
     }
   }
 }
+template <class P>
 __global__ __launch_bounds__(256) void mix_restrict_kernel(const LevelGeom Lc, const LevelGeom Lf,
-                                                           const float* __restrict__ rf, float* __restrict__ rc,
-                                                           float* __restrict__ dc, float* __restrict__ xc,
-                                                           const float* __restrict__ invD, float cr0, int start) {
-  mix_restrict_body(Lc, Lf, rf, rc, dc, xc, invD, cr0, start, Blk::hw());
+                                                           const P* __restrict__ rf, P* __restrict__ rc,
+                                                           P* __restrict__ dc, P* __restrict__ xc,
+                                                           const P* __restrict__ invD, P cr0, int start) {
+  mix_restrict_body<P>(Lc, Lf, rf, rc, dc, xc, invD, cr0, start, Blk::hw());
 }
+template <class P>
 struct MixRestrictK {
   struct Args {
     LevelGeom Lc;
     LevelGeom Lf;
-    const float* rf;
-    float* rc;
-    float* dc;
-    float* xc;
-    const float* invD;
-    float cr0;
+    const P* rf;
+    P* rc;
+    P* dc;
+    P* xc;
+    const P* invD;
+    P cr0;
     int start;
   };
   static constexpr int NT = 256, MINW = 1;
   static dim3 block() { return dim3(256); }
   __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
-    mix_restrict_body(a.Lc, a.Lf, a.rf, a.rc, a.dc, a.xc, a.invD, a.cr0, a.start, bk);
+    mix_restrict_body<P>(a.Lc, a.Lf, a.rf, a.rc, a.dc, a.xc, a.invD, a.cr0, a.start, bk);
   }
 };
 
 // prolongation of the coarse correction: c = P e (zero on Dirichlet DOFs),
 // x_f += c, c kept as the stencil vector of the first post-smoothing step
-__device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, float* __restrict__ xf,
-                                                 float* __restrict__ cf, const float* __restrict__ e, const Blk& bk) {
+template <class P>
+__device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, P* __restrict__ xf,
+                                                 P* __restrict__ cf, const P* __restrict__ e, const Blk& bk) {
   const unsigned qn = bk.x * 256 + threadIdx.x;
   if (qn >= (unsigned)Nof(Lf)) return;
   int x, y, z;
@@ -626,50 +892,53 @@ __device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const Leve
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
   const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
-  const float w = ((x & 1) ? 0.5f : 1.0f) * ((y & 1) ? 0.5f : 1.0f) * ((z & 1) ? 0.5f : 1.0f);
+  const P w = ((x & 1) ? P(0.5) : P(1)) * ((y & 1) ? P(0.5) : P(1)) * ((z & 1) ? P(0.5) : P(1));
   const size_t qf = nidx(Lf, x, y, z);
   const int dm = dir_mask(Lf, x, y, z);
   const size_t q00 = nidx(Lc, 0, j0, k0), q10 = nidx(Lc, 0, j1, k0), q01 = nidx(Lc, 0, j0, k1),
                q11 = nidx(Lc, 0, j1, k1);
   for (int c = 0; c < 3; c++) {
-    const float* ec = e + c * Nc;
-    const float a00 = ec[q00 + i0] + ((x & 1) ? ec[q00 + i1] : 0.0f);
-    const float a10 = (y & 1) ? ec[q10 + i0] + ((x & 1) ? ec[q10 + i1] : 0.0f) : 0.0f;
-    const float a01 = (z & 1) ? ec[q01 + i0] + ((x & 1) ? ec[q01 + i1] : 0.0f) : 0.0f;
-    const float a11 = ((y & 1) && (z & 1)) ? ec[q11 + i0] + ((x & 1) ? ec[q11 + i1] : 0.0f) : 0.0f;
-    float corr = w * (a00 + a10 + a01 + a11);
-    if ((dm >> c) & 1) corr = 0.0f;
+    const P* ec = e + c * Nc;
+    const P a00 = ec[q00 + i0] + ((x & 1) ? ec[q00 + i1] : P(0));
+    const P a10 = (y & 1) ? ec[q10 + i0] + ((x & 1) ? ec[q10 + i1] : P(0)) : P(0);
+    const P a01 = (z & 1) ? ec[q01 + i0] + ((x & 1) ? ec[q01 + i1] : P(0)) : P(0);
+    const P a11 = ((y & 1) && (z & 1)) ? ec[q11 + i0] + ((x & 1) ? ec[q11 + i1] : P(0)) : P(0);
+    P corr = w * (a00 + a10 + a01 + a11);
+    if ((dm >> c) & 1) corr = P(0);
     const size_t o = c * Nf + qf;
     xf[o] += corr;
     cf[o] = corr;
   }
 }
+template <class P>
 __global__ __launch_bounds__(256) void mix_prolong_kernel(const LevelGeom Lf, const LevelGeom Lc,
-                                                          float* __restrict__ xf, float* __restrict__ cf,
-                                                          const float* __restrict__ e) {
-  mix_prolong_body(Lf, Lc, xf, cf, e, Blk::hw());
+                                                          P* __restrict__ xf, P* __restrict__ cf,
+                                                          const P* __restrict__ e) {
+  mix_prolong_body<P>(Lf, Lc, xf, cf, e, Blk::hw());
 }
+template <class P>
 struct MixProlongK {
   struct Args {
     LevelGeom Lf;
     LevelGeom Lc;
-    float* xf;
-    float* cf;
-    const float* e;
+    P* xf;
+    P* cf;
+    const P* e;
   };
   static constexpr int NT = 256, MINW = 1;
   static dim3 block() { return dim3(256); }
   __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
-    mix_prolong_body(a.Lf, a.Lc, a.xf, a.cf, a.e, bk);
+    mix_prolong_body<P>(a.Lf, a.Lc, a.xf, a.cf, a.e, bk);
   }
 };
 
 // (lambda, mu) at the Gauss points of every finest-level element (the
 // coefficients of MP_all_GAUSS_Point, computed in FP64 and stored as FP32)
+// (times w_g/h^2 on a cubic grid, the scale the FP32 passes then leave out)
 __global__ __launch_bounds__(256) void mix_lame_kernel(const LevelGeom L, const double* __restrict__ K,
                                                        const double* __restrict__ G, const double* __restrict__ d,
                                                        const signed char* __restrict__ flag, double small_k,
-                                                       float* __restrict__ lm) {
+                                                       double scale, float* __restrict__ lm) {
   const int ex = blockIdx.x * 64 + threadIdx.x, ey = blockIdx.y * 4 + threadIdx.y, ez = blockIdx.z;
   if (ex > L.nx - 2 || ey > L.ny - 2) return;
   double Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
@@ -686,6 +955,15 @@ __global__ __launch_bounds__(256) void mix_lame_kernel(const LevelGeom L, const 
       }
   const size_t e = nidx(L, ex, ey, ez);
   fine_lame(Kn, Gn, dn, (int)flag[e], small_k, lam, mu);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        lam[a][b][c] *= scale;
+        mu[a][b][c] *= scale;
+      }
   // element-major: 16 floats per element, Gauss points 2k and 2k+1 in float4 k
   float4* o = reinterpret_cast<float4*>(lm) + 4 * e;
 #pragma unroll
@@ -698,52 +976,35 @@ __global__ __launch_bounds__(256) void mix_lame_kernel(const LevelGeom L, const 
 void launch_mix_lame(hipStream_t st, const LevelGeom& L, const double* K, const double* G, const double* d,
                      const signed char* flag, double small_k, float* lm) {
   const dim3 g((L.nx - 1 + 63) / 64, (L.ny - 1 + 3) / 4, L.nz - 1);
-  hipLaunchKernelGGL(mix_lame_kernel, g, dim3(64, 4), 0, st, L, K, G, d, flag, small_k, lm);
+  const double scale = cubic_grid(L) ? 0.125 * L.hx * L.hy * L.hz / (L.hx * L.hx) : 1.0;
+  hipLaunchKernelGGL(mix_lame_kernel, g, dim3(64, 4), 0, st, L, K, G, d, flag, small_k, scale, lm);
 }
 
-__device__ __forceinline__ void to_f64_body(size_t n, const float* __restrict__ a, double* __restrict__ b,
-                                            const Blk& bk) {
+// b = a, converted to b's type (a copy when the types agree)
+template <class A, class B>
+__device__ __forceinline__ void cvt_body(size_t n, const A* __restrict__ a, B* __restrict__ b, const Blk& bk) {
   for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
-    b[q] = (double)a[q];
+    b[q] = (B)a[q];
 }
-__global__ void to_f64_kernel(size_t n, const float* __restrict__ a, double* __restrict__ b) {
-  to_f64_body(n, a, b, Blk::hw());
+template <class A, class B>
+__global__ void cvt_kernel(size_t n, const A* __restrict__ a, B* __restrict__ b) {
+  cvt_body<A, B>(n, a, b, Blk::hw());
 }
-struct ToF64K {
+template <class A, class B>
+struct CvtK {
   struct Args {
     size_t n;
-    const float* a;
-    double* b;
+    const A* a;
+    B* b;
   };
   static constexpr int NT = 256, MINW = 1;
   static dim3 block() { return dim3(256); }
-  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
-    to_f64_body(a.n, a.a, a.b, bk);
-  }
-};
-__device__ __forceinline__ void to_f32_body(size_t n, const double* __restrict__ a, float* __restrict__ b,
-                                            const Blk& bk) {
-  for (size_t q = (size_t)bk.x * blockDim.x + threadIdx.x; q < n; q += (size_t)bk.gx * blockDim.x)
-    b[q] = (float)a[q];
-}
-__global__ void to_f32_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
-  to_f32_body(n, a, b, Blk::hw());
-}
-struct ToF32K {
-  struct Args {
-    size_t n;
-    const double* a;
-    float* b;
-  };
-  static constexpr int NT = 256, MINW = 1;
-  static dim3 block() { return dim3(256); }
-  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
-    to_f32_body(a.n, a.a, a.b, bk);
-  }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) { cvt_body<A, B>(a.n, a.a, a.b, bk); }
 };
 
 // deterministic start vector of the power iteration, zero on Dirichlet DOFs
-__global__ void power_init_kernel(const LevelGeom L, float* __restrict__ v) {
+template <class P>
+__global__ void power_init_kernel(const LevelGeom L, P* __restrict__ v) {
   const unsigned N = (unsigned)Nof(L);
   for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
     const int dm = node_mask(L, n);
@@ -754,69 +1015,92 @@ __global__ void power_init_kernel(const LevelGeom L, float* __restrict__ v) {
       h *= 2246822519u;
       h ^= h >> 13;
       const float val = 0.5f + (float)(h & 0xffffu) * (1.0f / 65536.0f);
-      v[(size_t)c * N + n] = ((dm >> c) & 1) ? 0.0f : val;
+      v[(size_t)c * N + n] = ((dm >> c) & 1) ? P(0) : (P)val;
     }
   }
 }
 
-void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, float* r32, float* d, float* x,
-                      const float* invD, float cr0) {
+template <class P>
+void launch_mix_start(hipStream_t st, const LevelGeom& L, const double* r64, P* r32, P* d, P* x, const P* invD,
+                      P cr0) {
   const size_t N = Nof(L);
-  hipLaunchKernelGGL(mix_start_kernel, dim3(mx_blocks(N, 4096)), dim3(MX_TPB), 0, st, (unsigned)N, r64, r32, d, x,
+  hipLaunchKernelGGL(mix_start_kernel<P>, dim3(mx_blocks(N, 4096)), dim3(MX_TPB), 0, st, (unsigned)N, r64, r32, d, x,
                      invD, cr0);
 }
+template <class P>
 void launch_fcg_b(hipStream_t st, const LevelGeom& L, double* u, double* r, const double* p, const double* q,
-                  float* r32, float* d, float* x, const float* invD, float cr0, bool lean, const Reduce& R) {
+                  P* r32, P* d, P* x, const P* invD, P cr0, bool lean, const Reduce& R) {
   const dim3 g(passb_blocks(Nof(L))), b(MX_TPB);
   const bool ldsf = L.nx + L.ny + L.nz <= DIRF_MAX;
-  const FcgBArgs a{L, u, r, p, q, r32, d, x, invD, cr0, R};
+  const FcgBArgs<P> a{L, u, r, p, q, r32, d, x, invD, cr0, R};
   bool rec;
   if (ldsf && lean)
-    rec = record<FcgBK<true, true>>(g, a);
+    rec = record<FcgBK<P, true, true>>(g, a);
   else if (ldsf)
-    rec = record<FcgBK<true, false>>(g, a);
+    rec = record<FcgBK<P, true, false>>(g, a);
   else if (lean)
-    rec = record<FcgBK<false, true>>(g, a);
+    rec = record<FcgBK<P, false, true>>(g, a);
   else
-    rec = record<FcgBK<false, false>>(g, a);
+    rec = record<FcgBK<P, false, false>>(g, a);
   if (rec) return;
   if (ldsf && lean)
-    hipLaunchKernelGGL((fcg_b_kernel<true, true>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
+    hipLaunchKernelGGL((fcg_b_kernel<P, true, true>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
   else if (ldsf)
-    hipLaunchKernelGGL((fcg_b_kernel<true, false>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
+    hipLaunchKernelGGL((fcg_b_kernel<P, true, false>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
   else if (lean)
-    hipLaunchKernelGGL((fcg_b_kernel<false, true>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
+    hipLaunchKernelGGL((fcg_b_kernel<P, false, true>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
   else
-    hipLaunchKernelGGL((fcg_b_kernel<false, false>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
+    hipLaunchKernelGGL((fcg_b_kernel<P, false, false>), g, b, 0, st, L, u, r, p, q, r32, d, x, invD, cr0, R);
 }
 void launch_fcg_flush(hipStream_t st, const LevelGeom& L, double* u, const double* p, const Reduce& R) {
   const size_t n = 3 * Nof(L);
   hipLaunchKernelGGL(fcg_flush_kernel, dim3(mx_blocks(n, 4096)), dim3(MX_TPB), 0, st, n, u, p,
                      static_cast<const double*>(R.scal));
 }
-void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const float* rf, float* rc,
-                         float* dc, float* xc, const float* invD, float cr0, int start) {
-  if (record<MixRestrictK>(dim3((unsigned)((Nof(Lc) + 255) / 256)), {Lc, Lf, rf, rc, dc, xc, invD, cr0, start}))
-    return;
-  hipLaunchKernelGGL(mix_restrict_kernel, dim3((unsigned)((Nof(Lc) + 255) / 256)), dim3(256), 0, st, Lc, Lf, rf, rc,
-                     dc, xc, invD, cr0, start);
+template <class P>
+void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& Lf, const P* rf, P* rc, P* dc, P* xc,
+                         const P* invD, P cr0, int start) {
+  const dim3 g((unsigned)((Nof(Lc) + 255) / 256));
+  if (record<MixRestrictK<P>>(g, {Lc, Lf, rf, rc, dc, xc, invD, cr0, start})) return;
+  hipLaunchKernelGGL(mix_restrict_kernel<P>, g, dim3(256), 0, st, Lc, Lf, rf, rc, dc, xc, invD, cr0, start);
 }
-void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, float* xf, float* cf,
-                        const float* ec) {
-  if (record<MixProlongK>(dim3((unsigned)((Nof(Lf) + 255) / 256)), {Lf, Lc, xf, cf, ec})) return;
-  hipLaunchKernelGGL(mix_prolong_kernel, dim3((unsigned)((Nof(Lf) + 255) / 256)), dim3(256), 0, st, Lf, Lc, xf, cf,
-                     ec);
+template <class P>
+void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, P* xf, P* cf, const P* ec) {
+  const dim3 g((unsigned)((Nof(Lf) + 255) / 256));
+  if (record<MixProlongK<P>>(g, {Lf, Lc, xf, cf, ec})) return;
+  hipLaunchKernelGGL(mix_prolong_kernel<P>, g, dim3(256), 0, st, Lf, Lc, xf, cf, ec);
 }
-void launch_to_f32(hipStream_t st, size_t n, const double* a, float* b) {
-  if (record<ToF32K>(dim3(mx_blocks(n, 4096)), {n, a, b})) return;
-  hipLaunchKernelGGL(to_f32_kernel, dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);
+template <class A, class B>
+void launch_convert(hipStream_t st, size_t n, const A* a, B* b) {
+  if (record<CvtK<A, B>>(dim3(mx_blocks(n, 4096)), {n, a, b})) return;
+  hipLaunchKernelGGL((cvt_kernel<A, B>), dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);
 }
-void launch_to_f64(hipStream_t st, size_t n, const float* a, double* b) {
-  if (record<ToF64K>(dim3(mx_blocks(n, 4096)), {n, a, b})) return;
-  hipLaunchKernelGGL(to_f64_kernel, dim3(mx_blocks(n, 4096)), dim3(256), 0, st, n, a, b);
+template <class P>
+void launch_power_init(hipStream_t st, const LevelGeom& L, P* v) {
+  hipLaunchKernelGGL(power_init_kernel<P>, dim3(mx_blocks(Nof(L), 4096)), dim3(256), 0, st, L, v);
 }
-void launch_power_init(hipStream_t st, const LevelGeom& L, float* v) {
-  hipLaunchKernelGGL(power_init_kernel, dim3(mx_blocks(Nof(L), 4096)), dim3(256), 0, st, L, v);
-}
+
+// the two preconditioner precisions
+#define HPCR_MIX_INSTANTIATE(P)                                                                                   \
+  template void launch_mix_cheb<P>(hipStream_t, const LevelGeom&, const MixCoefT<P>&, const P*, P*, P*, P*,       \
+                                   const P*, P, P, const double*, const Reduce&);                                \
+  template void launch_mix_res<P>(hipStream_t, const LevelGeom&, const MixCoefT<P>&, const P*, P*);               \
+  template void launch_mix_power<P>(hipStream_t, const LevelGeom&, const MixCoefT<P>&, const P*, P*, const P*,    \
+                                    const Reduce&);                                                               \
+  template void launch_power_init<P>(hipStream_t, const LevelGeom&, P*);                                          \
+  template void launch_fcg_a<P>(hipStream_t, const LevelGeom&, const MechCoef&, const P*, const double*, double*, \
+                                double*, const double*, int, double*, const Reduce&);                             \
+  template void launch_fcg_b<P>(hipStream_t, const LevelGeom&, double*, double*, const double*, const double*,    \
+                                P*, P*, P*, const P*, P, bool, const Reduce&);                                    \
+  template void launch_mix_start<P>(hipStream_t, const LevelGeom&, const double*, P*, P*, P*, const P*, P);       \
+  template void launch_mix_restrict<P>(hipStream_t, const LevelGeom&, const LevelGeom&, const P*, P*, P*, P*,     \
+                                       const P*, P, int);                                                         \
+  template void launch_mix_prolong<P>(hipStream_t, const LevelGeom&, const LevelGeom&, P*, P*, const P*);
+HPCR_MIX_INSTANTIATE(float)
+HPCR_MIX_INSTANTIATE(double)
+#undef HPCR_MIX_INSTANTIATE
+template void launch_convert<double, float>(hipStream_t, size_t, const double*, float*);
+template void launch_convert<float, double>(hipStream_t, size_t, const float*, double*);
+template void launch_convert<double, double>(hipStream_t, size_t, const double*, double*);
 
 }  // namespace hpcr

@@ -50,13 +50,17 @@ struct Lev {
   std::vector<int> kplanes;
 };
 
-// FP32 state of one level of the mixed-precision preconditioner
+// State of one level of the mixed-precision solve's preconditioner, in its
+// arithmetic type P (float; or double for the FP64 V-cycle).  With P = double
+// invD, Ke and Ge point at the FAS path's own FP64 buffers.
+template <class P>
 struct MixLev {
-  float *x = nullptr, *r = nullptr, *dA = nullptr, *dB = nullptr, *invD = nullptr;
-  float *lm = nullptr, *Ke = nullptr, *Ge = nullptr;
+  P *x = nullptr, *r = nullptr, *dA = nullptr, *dB = nullptr, *invD = nullptr;
+  float* lm = nullptr;
+  P *Ke = nullptr, *Ge = nullptr;
   double lam = 0.0;
-  float cr0 = 0.0f;
-  std::vector<float> cd, cr;  // Chebyshev steps 1..m-1
+  P cr0 = P(0);
+  std::vector<P> cd, cr;  // Chebyshev steps 1..m-1
 };
 
 struct TimingRec {
@@ -93,8 +97,10 @@ struct hpcr_ctx {
   double g_u_Ut = 0.0;
   int g_u_t0 = -1;
   long long g_u_iters = 0, g_d_iters = 0;  // smoother iterations per captured cycle
-  // mixed-precision solve (allocated on first use)
-  std::vector<MixLev> mix;
+  // mixed-precision solve (allocated on first use): FP32 preconditioner, and
+  // the FP64 one of the like-for-like comparison
+  std::vector<MixLev<float>> mix;
+  std::vector<MixLev<double>> mix64;
   double* q2 = nullptr;  // second FP64 q buffer of the flexible CG
   // fusion engine this context's multigrid cycles go through (optional)
   hpcr_fuser* fuser = nullptr;
@@ -102,6 +108,8 @@ struct hpcr_ctx {
   // counts the mixed-form set-ups: the Chebyshev coefficients, passed to the
   // kernels by value, change only there (a fused batch graph's key holds it)
   unsigned long long mix_epoch = 0;
+  int mix_mfma = 0;  // the FP32 finest level's element contraction on the matrix cores
+  int ticket_fence = 0;  // reductions order their partials by a release fence (hpcr_set_reduction_fence)
 };
 
 namespace {
@@ -152,7 +160,7 @@ int alloc_n(hpcr_ctx* c, T** p, size_t n) {
   return dev_alloc(c, (void**)p, n * sizeof(T));
 }
 
-Reduce red(hpcr_ctx* c) { return Reduce{c->partials, c->counter, c->scal}; }
+Reduce red(hpcr_ctx* c) { return Reduce{c->partials, c->counter, c->scal, c->ticket_fence}; }
 
 MechCoef mcoef(hpcr_ctx* c, int l) {
   Lev& v = c->lev[l];
@@ -781,59 +789,87 @@ int u_entire(hpcr_ctx* c, double Ut, int t, double* f_norm, hpcr_stats* st) {
 }
 
 // ---- mixed-precision form of u_entire ---------------------------------------
+template <class P>
+std::vector<MixLev<P>>& mixv(hpcr_ctx* c) {
+  if constexpr (std::is_same<P, double>::value)
+    return c->mix64;
+  else
+    return c->mix;
+}
+
+template <class P>
 int mix_alloc(hpcr_ctx* c) {
-  if (!c->mix.empty()) return HPCR_OK;
-  std::vector<MixLev> mix(c->L + 1);
+  constexpr bool F64 = std::is_same<P, double>::value;
+  if (!mixv<P>(c).empty()) return HPCR_OK;
+  std::vector<MixLev<P>> mix(c->L + 1);
   int rc;
   for (int l = 0; l <= c->L; l++) {
-    const size_t N = c->lev[l].N;
-    MixLev& m = mix[l];
+    Lev& v = c->lev[l];
+    const size_t N = v.N;
+    MixLev<P>& m = mix[l];
     if ((rc = alloc_n(c, &m.x, 3 * N)) || (rc = alloc_n(c, &m.r, 3 * N))) return rc;
     if (l == 0) continue;
-    if ((rc = alloc_n(c, &m.dA, 3 * N)) || (rc = alloc_n(c, &m.dB, 3 * N)) || (rc = alloc_n(c, &m.invD, 3 * N)))
-      return rc;
-    if (l == c->L) {
-      if ((rc = alloc_n(c, &m.lm, 16 * N))) return rc;
+    if ((rc = alloc_n(c, &m.dA, 3 * N)) || (rc = alloc_n(c, &m.dB, 3 * N))) return rc;
+    if constexpr (F64) {
+      m.invD = v.invDu;
+      if (l < c->L) {
+        m.Ke = v.Ke;
+        m.Ge = v.Ge;
+      }
     } else {
-      if ((rc = alloc_n(c, &m.Ke, N)) || (rc = alloc_n(c, &m.Ge, N))) return rc;
+      if ((rc = alloc_n(c, &m.invD, 3 * N))) return rc;
+      if (l == c->L) {
+        if ((rc = alloc_n(c, &m.lm, 16 * N))) return rc;
+      } else {
+        if ((rc = alloc_n(c, &m.Ke, N)) || (rc = alloc_n(c, &m.Ge, N))) return rc;
+      }
     }
   }
-  if ((rc = alloc_n(c, &c->q2, 3 * c->lev[c->L].N))) return rc;
-  c->mix = std::move(mix);
+  if (!c->q2 && (rc = alloc_n(c, &c->q2, 3 * c->lev[c->L].N))) return rc;
+  mixv<P>(c) = std::move(mix);
   return HPCR_OK;
 }
 
-MixCoef mixcoef(hpcr_ctx* c, int l) {
-  const MixLev& m = c->mix[l];
-  MixCoef k{};
+template <class P>
+MixCoefT<P> mixcoef(hpcr_ctx* c, int l) {
+  const MixLev<P>& m = mixv<P>(c)[l];
+  MixCoefT<P> k{};
   k.fine = (l == c->L) ? 1 : 0;
+  k.mfma = c->mix_mfma;
   k.lm = m.lm;
   k.Ke = m.Ke;
   k.Ge = m.Ge;
+  k.m = mcoef(c, l);
   return k;
 }
 
-// FP32 coefficients and diagonals, power iteration and Chebyshev coefficients
+// preconditioner coefficients and diagonals (FP32 copies; the FP64 form
+// reads the FAS path's own), power iteration and Chebyshev coefficients
+template <class P>
 int mix_setup(hpcr_ctx* c, const hpcr_mixed_params& mp) {
+  constexpr bool F64 = std::is_same<P, double>::value;
   int rc;
   c->mix_epoch++;
+  c->mix_mfma = mp.element_mfma;
   const int m = mp.cheb_degree;
   for (int l = 1; l <= c->L; l++) {
     Lev& v = c->lev[l];
-    MixLev& M = c->mix[l];
-    launch_to_f32(c->stream, 3 * v.N, v.invDu, M.invD);
-    if (l == c->L) {
-      launch_mix_lame(c->stream, v.g, v.K, v.G, v.d, v.flag, c->par.small_k, M.lm);
-    } else {
-      launch_to_f32(c->stream, v.N, v.Ke, M.Ke);
-      launch_to_f32(c->stream, v.N, v.Ge, M.Ge);
+    MixLev<P>& M = mixv<P>(c)[l];
+    if constexpr (!F64) {
+      launch_convert(c->stream, 3 * v.N, v.invDu, M.invD);
+      if (l == c->L) {
+        launch_mix_lame(c->stream, v.g, v.K, v.G, v.d, v.flag, c->par.small_k, M.lm);
+      } else {
+        launch_convert(c->stream, v.N, v.Ke, M.Ke);
+        launch_convert(c->stream, v.N, v.Ge, M.Ge);
+      }
     }
     // power iteration on D^-1 A: Rayleigh quotient of the last iterate
-    float* a = M.dA;
-    float* b = M.dB;
+    P* a = M.dA;
+    P* b = M.dB;
     launch_power_init(c->stream, v.g, a);
     for (int it = 0; it < mp.power_iters; it++) {
-      launch_mix_power(c->stream, v.g, mixcoef(c, l), a, b, M.invD, red(c));
+      launch_mix_power(c->stream, v.g, mixcoef<P>(c, l), (const P*)a, b, (const P*)M.invD, red(c));
       std::swap(a, b);
     }
     if ((rc = check_launch(c, "mixed set-up"))) return rc;
@@ -843,68 +879,74 @@ int mix_setup(hpcr_ctx* c, const hpcr_mixed_params& mp) {
       return set_err(c, HPCR_ERR_STATE, "power iteration gave no positive eigenvalue estimate");
     const double hi = mp.cheb_upper * M.lam, lo = hi / mp.cheb_ratio;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-    M.cr0 = (float)(1.0 / theta);
-    M.cd.assign(m, 0.0f);
-    M.cr.assign(m, 0.0f);
+    M.cr0 = (P)(1.0 / theta);
+    M.cd.assign(m, P(0));
+    M.cr.assign(m, P(0));
     double rho_prev = 1.0 / sigma;
     for (int j = 1; j < m; j++) {
       const double rho = 1.0 / (2.0 * sigma - rho_prev);
-      M.cd[j] = (float)(rho * rho_prev);
-      M.cr[j] = (float)(2.0 * rho / delta);
+      M.cd[j] = (P)(rho * rho_prev);
+      M.cr[j] = (P)(2.0 * rho / delta);
       rho_prev = rho;
     }
   }
   return HPCR_OK;
 }
 
-// one FP32 V-cycle on the residual held in mix[L].r (smoother started);
-// the result is mix[L].x.  With qprev, x.qprev goes to scal[S_ZQ].
+// one preconditioner V-cycle on the residual held in mix[L].r (smoother
+// started); the result is mix[L].x.  With qprev, x.qprev goes to scal[S_ZQ].
+template <class P>
 int mix_vcycle(hpcr_ctx* c, int m, int nu0, const double* qprev) {
+  constexpr bool F64 = std::is_same<P, double>::value;
   const int L = c->L;
+  std::vector<MixLev<P>>& mix = mixv<P>(c);
   for (int l = L; l >= 1; l--) {
-    MixLev& M = c->mix[l];
+    MixLev<P>& M = mix[l];
     const LevelGeom& g = c->lev[l].g;
-    const MixCoef C = mixcoef(c, l);
-    float* sv = M.dA;
-    float* ov = M.dB;
+    const MixCoefT<P> C = mixcoef<P>(c, l);
+    P* sv = M.dA;
+    P* ov = M.dB;
     for (int j = 1; j < m; j++) {
-      launch_mix_cheb(c->stream, g, C, sv, ov, M.r, M.x, M.invD, M.cd[j], M.cr[j], nullptr, red(c));
+      launch_mix_cheb(c->stream, g, C, (const P*)sv, ov, M.r, M.x, (const P*)M.invD, M.cd[j], M.cr[j], nullptr,
+                      red(c));
       std::swap(sv, ov);
     }
-    launch_mix_res(c->stream, g, C, sv, M.r);
-    MixLev& K = c->mix[l - 1];
+    launch_mix_res(c->stream, g, C, (const P*)sv, M.r);
+    MixLev<P>& K = mix[l - 1];
     if (l - 1 >= 1)
-      launch_mix_restrict(c->stream, c->lev[l - 1].g, g, M.r, K.r, K.dA, K.x, K.invD, K.cr0, 1);
+      launch_mix_restrict(c->stream, c->lev[l - 1].g, g, (const P*)M.r, K.r, K.dA, K.x, (const P*)K.invD, K.cr0, 1);
     else
-      launch_mix_restrict(c->stream, c->lev[0].g, g, M.r, K.r, nullptr, K.x, nullptr, 0.0f, 0);
+      launch_mix_restrict(c->stream, c->lev[0].g, g, (const P*)M.r, K.r, (P*)nullptr, K.x, (const P*)nullptr, P(0),
+                          0);
   }
   {
     Lev& v0 = c->lev[0];
-    if (small_level(v0.g)) {
-      launch_small_pcg_u_f32(c->stream, v0.g, v0.Ke, v0.Ge, c->mix[0].x, c->mix[0].r, v0.invDu, nu0,
-                             c->scal + S_SCRATCH);
+    if (!F64 && small_level(v0.g)) {
+      if constexpr (!F64)
+        launch_small_pcg_u_f32(c->stream, v0.g, v0.Ke, v0.Ge, mix[0].x, mix[0].r, v0.invDu, nu0,
+                               c->scal + S_SCRATCH);
     } else {
-      // larger coarsest level: the FP64 smoother of the FAS path on the
-      // level's own buffers (zero start, homogeneous boundary values)
-      launch_to_f64(c->stream, 3 * v0.N, c->mix[0].r, v0.fu);
+      // the FP64 smoother of the FAS path on the coarsest level's own buffers
+      // (zero start, homogeneous boundary values)
+      launch_convert(c->stream, 3 * v0.N, (const P*)mix[0].r, v0.fu);
       launch_fill(c->stream, 3 * v0.N, v0.u, 0.0);
       int rc = pcg_u(c, 0, nu0, 0.0, 1);
       if (rc) return rc;
-      launch_to_f32(c->stream, 3 * v0.N, v0.u, c->mix[0].x);
+      launch_convert(c->stream, 3 * v0.N, (const double*)v0.u, mix[0].x);
     }
   }
   for (int l = 1; l <= L; l++) {
-    MixLev& M = c->mix[l];
+    MixLev<P>& M = mix[l];
     const LevelGeom& g = c->lev[l].g;
-    const MixCoef C = mixcoef(c, l);
-    launch_mix_prolong(c->stream, g, c->lev[l - 1].g, M.x, M.dA, c->mix[l - 1].x);
-    float* sv = M.dA;
-    float* ov = M.dB;
-    launch_mix_cheb(c->stream, g, C, sv, ov, M.r, M.x, M.invD, 0.0f, M.cr0, (l == L && m == 1) ? qprev : nullptr,
-                    red(c));
+    const MixCoefT<P> C = mixcoef<P>(c, l);
+    launch_mix_prolong(c->stream, g, c->lev[l - 1].g, M.x, M.dA, (const P*)mix[l - 1].x);
+    P* sv = M.dA;
+    P* ov = M.dB;
+    launch_mix_cheb(c->stream, g, C, (const P*)sv, ov, M.r, M.x, (const P*)M.invD, P(0), M.cr0,
+                    (l == L && m == 1) ? qprev : nullptr, red(c));
     std::swap(sv, ov);
     for (int j = 1; j < m; j++) {
-      launch_mix_cheb(c->stream, g, C, sv, ov, M.r, M.x, M.invD, M.cd[j], M.cr[j],
+      launch_mix_cheb(c->stream, g, C, (const P*)sv, ov, M.r, M.x, (const P*)M.invD, M.cd[j], M.cr[j],
                       (l == L && j == m - 1) ? qprev : nullptr, red(c));
       std::swap(sv, ov);
     }
@@ -912,7 +954,8 @@ int mix_vcycle(hpcr_ctx* c, int m, int nu0, const double* qprev) {
   return check_launch(c, "mixed V-cycle");
 }
 
-int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mixed_params& mp, hpcr_stats* st) {
+template <class P>
+int u_entire_mixed_p(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mixed_params& mp, hpcr_stats* st) {
   int rc;
   const int L = c->L;
   Lev& F = c->lev[L];
@@ -920,12 +963,12 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
   if (mp.cheb_degree < 1 || mp.cheb_degree > 8 || mp.power_iters < 1 || !(mp.cheb_ratio > 1.0) ||
       !(mp.cheb_upper > 0.0) || mp.max_iter < 0)
     return set_err(c, HPCR_ERR_ARG, "bad mixed-precision parameters");
-  if ((rc = mix_alloc(c))) return rc;
+  if ((rc = mix_alloc<P>(c))) return rc;
   c->smoother_iters = 0;
   if ((rc = mech_setup(c))) return rc;
   if ((rc = mech_diag_all(c))) return rc;
-  if ((rc = mix_setup(c, mp))) return rc;
-  MixLev& MF = c->mix[L];
+  if ((rc = mix_setup<P>(c, mp))) return rc;
+  MixLev<P>& MF = mixv<P>(c)[L];
   const int t0 = (t == 0) ? 1 : 0;
   // residual from u with its boundary values; at t == 0 also f := A u on
   // Dirichlet DOFs and f_norm = |f|^2, as the finest smoother does
@@ -937,7 +980,7 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
     if ((e = read_scal(c))) return e;
     *rr = c->host_scal[S_RR];
     if (t0) *f_norm = c->host_scal[S_FF];
-    launch_mix_start(c->stream, F.g, F.ru, MF.r, MF.dA, MF.x, MF.invD, MF.cr0);
+    launch_mix_start(c->stream, F.g, F.ru, MF.r, MF.dA, MF.x, (const P*)MF.invD, MF.cr0);
     return check_launch(c, "mixed start");
   };
   double rr = 0.0;
@@ -965,15 +1008,17 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
   };
   // one outer iteration's launches: preconditioner V-cycle, FCG passes A and B
   auto iteration = [&]() -> int {
-    int e = mix_vcycle(c, m, c->par.mg_u.nu0, it > 0 ? qb[it & 1] : nullptr);
+    int e = mix_vcycle<P>(c, m, c->par.mg_u.nu0, it > 0 ? qb[it & 1] : nullptr);
     if (e) return e;
-    launch_fcg_a(c->stream, F.g, mcoef(c, L), MF.x, pb[it & 1], pb[(it + 1) & 1], qb[(it + 1) & 1], F.ru, it == 0,
-                 lag ? F.u : nullptr, red(c));
-    launch_fcg_b(c->stream, F.g, F.u, F.ru, pb[(it + 1) & 1], qb[(it + 1) & 1], MF.r, MF.dA, MF.x, MF.invD, MF.cr0,
-                 true, red(c));
+    launch_fcg_a(c->stream, F.g, mcoef(c, L), (const P*)MF.x, pb[it & 1], pb[(it + 1) & 1], qb[(it + 1) & 1], F.ru,
+                 it == 0, lag ? F.u : nullptr, red(c));
+    launch_fcg_b(c->stream, F.g, F.u, F.ru, pb[(it + 1) & 1], qb[(it + 1) & 1], MF.r, MF.dA, MF.x, (const P*)MF.invD,
+                 MF.cr0, true, red(c));
     return check_launch(c, "flexible CG");
   };
-  const std::string msig = c->fuser ? cycle_signature(c, 2) + "/m" + std::to_string(m) : std::string();
+  const std::string msig =
+      c->fuser ? cycle_signature(c, 2) + "/m" + std::to_string(m) + (std::is_same<P, double>::value ? "/p64" : "")
+               : std::string();
   FuseLoop in_loop(c, msig);
   // at t == 0 the reference always runs a first cycle; with an exactly zero
   // residual there is no direction to search (and p.q = 0)
@@ -1017,6 +1062,15 @@ int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mix
     st->smoother_iters = 0;
   }
   return HPCR_OK;
+}
+
+int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mixed_params& mp, hpcr_stats* st) {
+  if (mp.precond_fp64 != 0 && mp.precond_fp64 != 1) return set_err(c, HPCR_ERR_ARG, "precond_fp64 must be 0 or 1");
+  if (mp.element_mfma != 0 && mp.element_mfma != 1) return set_err(c, HPCR_ERR_ARG, "element_mfma must be 0 or 1");
+  if (mp.element_mfma && mp.precond_fp64)
+    return set_err(c, HPCR_ERR_ARG, "the matrix-core element contraction is an FP32 V-cycle form");
+  return mp.precond_fp64 ? u_entire_mixed_p<double>(c, Ut, t, f_norm, mp, st)
+                         : u_entire_mixed_p<float>(c, Ut, t, f_norm, mp, st);
 }
 
 // ---- phase field ------------------------------------------------------------
@@ -1263,7 +1317,8 @@ int build_levels(hpcr_ctx* c) {
 }
 
 // device bytes that build_levels (and, with mixed, mix_alloc) allocate
-long long estimate_bytes(const hpcr_geometry& G, bool mixed) {
+// mixed: 0 none, 1 with the FP32 preconditioner's buffers, 2 with the FP64 one's
+long long estimate_bytes(const hpcr_geometry& G, int mixed) {
   const int L = G.maxlevel;
   long long total = 0, pb = 4096;
   long ii = G.element_nb[0], jj = G.element_nb[1], kk = G.element_nb[2];
@@ -1275,9 +1330,13 @@ long long estimate_bytes(const hpcr_geometry& G, bool mixed) {
     const long tb = tile_blocks((int)nx, (int)ny, (int)nz, choose_zchunk((int)nx, (int)ny, (int)nz));
     if (tb > pb) pb = tb;
     if (passb_blocks((size_t)N) > pb) pb = passb_blocks((size_t)N);
-    if (mixed) {
+    if (mixed == 1) {
       total += 2 * 3 * N * 4;
       if (l > 0) total += 3 * 3 * N * 4 + (l == L ? 16 * N * 4 : 2 * N * 4);
+      if (l == L) total += 3 * N * 8;
+    } else if (mixed == 2) {
+      total += 2 * 3 * N * 8;
+      if (l > 0) total += 2 * 3 * N * 8;
       if (l == L) total += 3 * N * 8;
     }
     ii *= 2;
@@ -1530,6 +1589,12 @@ int hpcr_coarse_vr(hpcr_ctx* c, int f) {
   return check_launch(c, "coarse_vr");
 }
 
+int hpcr_set_reduction_fence(hpcr_ctx* c, int on) {
+  if (!c) return HPCR_ERR_ARG;
+  c->ticket_fence = on ? 1 : 0;
+  return HPCR_OK;
+}
+
 int hpcr_set_graphs(hpcr_ctx* c, int on) {
   if (!c) return HPCR_ERR_ARG;
   c->graphs = on != 0;
@@ -1608,7 +1673,8 @@ int hpcr_estimate_bytes(const hpcr_geometry* geo, int mixed, long long* bytes) {
   if (!geo || !bytes || geo->maxlevel < 0 || geo->maxlevel > 12 || geo->element_nb[0] < 1 ||
       geo->element_nb[1] < 1 || geo->element_nb[2] < 1)
     return HPCR_ERR_ARG;
-  *bytes = estimate_bytes(*geo, mixed != 0);
+  if (mixed < 0 || mixed > 2) return HPCR_ERR_ARG;
+  *bytes = estimate_bytes(*geo, mixed);
   return HPCR_OK;
 }
 
@@ -1827,6 +1893,31 @@ int hpcr_internal_force(hpcr_ctx* c, int l, int src, int dst) {
   launch_mech_apply(c->stream, c->lev[l].g, mcoef(c, l), field_ptr(c, src, l), field_ptr(c, dst, l));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c, "internal_force");
+}
+
+int hpcr_internal_force_f32(hpcr_ctx* c, int src, int dst, int mfma) {
+  if (!c || !field_vec(src) || !field_vec(dst) || src == dst || (mfma != 0 && mfma != 1)) return HPCR_ERR_ARG;
+  const int L = c->L;
+  if (L < 1) return set_err(c, HPCR_ERR_ARG, "the FP32 preconditioner needs at least two levels");
+  if (!field_ptr(c, src, L) || !field_ptr(c, dst, L)) return set_err(c, HPCR_ERR_ARG, "field not held on level");
+  (void)hipSetDevice(c->device);
+  int rc = mix_alloc<float>(c);
+  if (rc) return rc;
+  Lev& F = c->lev[L];
+  MixLev<float>& M = c->mix[L];
+  const size_t n = 3 * F.N;
+  // the Gauss-point moduli as the mixed solve's set-up forms them (split flag
+  // and d as they stand), s = src in FP32, r = 0, then r -= A s (masked)
+  launch_mix_lame(c->stream, F.g, F.K, F.G, F.d, F.flag, c->par.small_k, M.lm);
+  launch_convert(c->stream, n, (const double*)field_ptr(c, src, L), M.dA);
+  HIPCHK(c, hipMemsetAsync(M.r, 0, n * sizeof(float), c->stream));
+  MixCoefT<float> C = mixcoef<float>(c, L);
+  C.mfma = mfma;
+  launch_mix_res(c->stream, F.g, C, (const float*)M.dA, M.r);
+  launch_convert(c->stream, n, (const float*)M.r, field_ptr(c, dst, L));
+  c->mix_epoch++;  // lm was rewritten: a mixed-form batch graph must not replay
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c, "internal_force_f32");
 }
 
 int hpcr_prem_u(hpcr_ctx* c, int l, int dst) {

@@ -23,6 +23,24 @@ __device__ __forceinline__ size_t nidx(const LevelGeom& L, int i, int j, int k) 
 }
 __host__ __device__ __forceinline__ size_t Nof(const LevelGeom& L) { return (size_t)L.nx * L.ny * L.nz; }
 
+// Node index as a 32-bit count, and loads/stores at a 32-bit byte offset from
+// a kernel-uniform base pointer: the instruction takes the base from scalar
+// registers and the offset from one vector register, so an access costs no
+// 64-bit address arithmetic.  A level holds at most MAX_LEVEL_NODES nodes
+// (checked when a context is created), so the byte offset of a node within
+// one component of a field fits in 32 bits.
+__device__ __forceinline__ unsigned nid32(const LevelGeom& L, int i, int j, int k) {
+  return (unsigned)i + (unsigned)L.nx * ((unsigned)j + (unsigned)L.ny * (unsigned)k);
+}
+template <class T>
+__device__ __forceinline__ T ldo(const T* base, unsigned boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <class T>
+__device__ __forceinline__ void sto(T* base, unsigned boff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
 // ---------------------------------------------------------------------------
 // Reductions
 // ---------------------------------------------------------------------------
@@ -71,10 +89,18 @@ struct Blk {
   __device__ __forceinline__ unsigned count() const { return gx * gy * gz; }
 };
 
-#ifndef HPCR_TICKET_FENCE
-#define HPCR_TICKET_FENCE 0
-#endif
 // Block partials -> last block -> fin(result[NV]) run by thread 0 of the last block.
+//
+// Ordering of a block's partials before its ticket.  The partials are
+// agent-scope atomic stores, which write through the XCD's L2 (the `sc1`
+// stores of gfx950); the block waits for their completion (s_waitcnt
+// vmcnt(0)) and only then takes the ticket, and the last block reads the
+// partials with agent-scope atomic loads.  This relies on the instruction
+// order store -> wait -> ticket in the generated code, which a CPU test of
+// the built assembly checks.  R.fence = 1 orders them with a release fence
+// instead (and an acquire fence in the last block), as the memory model
+// prescribes; on this GPU that fence writes back the XCD's whole L2, once
+// per block.
 template <int OP, int NV, class Fin>
 __device__ __forceinline__ void reduce_finalize(const Blk& bk, double (&v)[NV], const Reduce& R, double init,
                                                 Fin fin) {
@@ -86,23 +112,16 @@ __device__ __forceinline__ void reduce_finalize(const Blk& bk, double (&v)[NV], 
     const unsigned b = bk.lin();
     for (int k = 0; k < NV; k++)
       __hip_atomic_store(&R.partials[(size_t)b * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#if HPCR_TICKET_FENCE
-    __threadfence();
-#else
-    // The partials are agent-scope atomic stores, which write through the
-    // XCD's L2; waiting for their completion orders them before the ticket.
-    // A release fence here would also write back every dirty line of the L2
-    // (all of the pass's streaming stores), once per block.
-    __builtin_amdgcn_s_waitcnt(0);
-#endif
+    if (R.fence)
+      __threadfence();
+    else
+      __builtin_amdgcn_s_waitcnt(0);
     unsigned t = atomicAdd(R.counter, 1u);
     is_last = (t == nb - 1);
   }
   __syncthreads();
   if (!is_last) return;
-#if HPCR_TICKET_FENCE
-  __threadfence();
-#endif
+  if (R.fence) __threadfence();
   // the partial loads below are agent-scope atomic loads, issued after the
   // last ticket: every block's partials completed before its ticket
   double acc[NV];
@@ -127,13 +146,11 @@ __device__ __forceinline__ void reduce_finalize(double (&v)[NV], const Reduce& R
 // ---------------------------------------------------------------------------
 // Interpolation of a node pair (a, b) to the two Gauss points:
 // (GA a + GB b, GB a + GA b).  With GA + GB = 1 this is b + GA (a - b) and
-// a - GA (a - b): three operations instead of four (the "delta form").  The
-// FP32 preconditioner uses it; the FP64 path keeps the two-product form by
-// default, whose rounding the oracle comparisons were pinned with
-// (HPCR_DELTA_INTERP_F64=1 selects the delta form there too: about 5 % off the
-// fine-level pass A, at a different rounding).
+// a - GA (a - b): three operations instead of four (the "delta form"), used
+// by both precisions.  HPCR_DELTA_INTERP_F64=0 keeps the two-product form in
+// FP64 (about 7 % slower fine-level passes, the same operator up to rounding).
 #ifndef HPCR_DELTA_INTERP_F64
-#define HPCR_DELTA_INTERP_F64 0
+#define HPCR_DELTA_INTERP_F64 1
 #endif
 template <class T>
 constexpr bool kDelta = !std::is_same<T, double>::value || HPCR_DELTA_INTERP_F64;
@@ -197,8 +214,9 @@ __device__ __forceinline__ void interp8_t_add(const T (&q)[2][2][2], T (&f)[2][2
 }
 // bilinear interpolation of 4 edge values e[a][b] (a: outer axis, b: inner
 // axis) to Gauss pairs o[ga][gb], scaled by s (the delta form scales the
-// result, the two-product form the inner stage)
-template <class T>
+// result, the two-product form the inner stage).  SC false: no scaling (the
+// caller has folded the grid scale into the moduli).
+template <bool SC = true, class T>
 __device__ __forceinline__ void interp4(const T (&e)[2][2], T s, T (&o)[2][2]) {
   if constexpr (kDelta<T>) {
     T t0, t1, u0, u1;
@@ -206,16 +224,19 @@ __device__ __forceinline__ void interp4(const T (&e)[2][2], T s, T (&o)[2][2]) {
     lerp2(e[1][0], e[1][1], u0, u1);
     lerp2(t0, u0, o[0][0], o[1][0]);
     lerp2(t1, u1, o[0][1], o[1][1]);
+    if constexpr (SC) {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+      for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int b = 0; b < 2; b++) o[a][b] *= s;
+        for (int b = 0; b < 2; b++) o[a][b] *= s;
+    }
   } else {
     const T ga = T(GA), gb = T(GB);
-    const T t0 = s * (ga * e[0][0] + gb * e[0][1]);
-    const T t1 = s * (gb * e[0][0] + ga * e[0][1]);
-    const T u0 = s * (ga * e[1][0] + gb * e[1][1]);
-    const T u1 = s * (gb * e[1][0] + ga * e[1][1]);
+    auto sc = [s](T v) { return SC ? s * v : v; };
+    const T t0 = sc(ga * e[0][0] + gb * e[0][1]);
+    const T t1 = sc(gb * e[0][0] + ga * e[0][1]);
+    const T u0 = sc(ga * e[1][0] + gb * e[1][1]);
+    const T u1 = sc(gb * e[1][0] + ga * e[1][1]);
     o[0][0] = ga * t0 + gb * u0;
     o[0][1] = ga * t1 + gb * u1;
     o[1][0] = gb * t0 + ga * u0;
@@ -223,7 +244,7 @@ __device__ __forceinline__ void interp4(const T (&e)[2][2], T s, T (&o)[2][2]) {
   }
 }
 // transpose of interp4
-template <class T>
+template <bool SC = true, class T>
 __device__ __forceinline__ void interp4_t(const T (&S)[2][2], T s, T (&R)[2][2]) {
   if constexpr (kDelta<T>) {
     T q00, q01, q10, q11;
@@ -231,26 +252,29 @@ __device__ __forceinline__ void interp4_t(const T (&S)[2][2], T s, T (&R)[2][2])
     lerp2(S[0][1], S[1][1], q01, q11);
     lerp2(q00, q01, R[0][0], R[0][1]);
     lerp2(q10, q11, R[1][0], R[1][1]);
+    if constexpr (SC) {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+      for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int b = 0; b < 2; b++) R[a][b] *= s;
+        for (int b = 0; b < 2; b++) R[a][b] *= s;
+    }
   } else {
     const T ga = T(GA), gb = T(GB);
+    auto sc = [s](T v) { return SC ? s * v : v; };
     const T q00 = ga * S[0][0] + gb * S[1][0];  // a=0, gb=0
     const T q01 = ga * S[0][1] + gb * S[1][1];
     const T q10 = gb * S[0][0] + ga * S[1][0];  // a=1
     const T q11 = gb * S[0][1] + ga * S[1][1];
-    R[0][0] = s * (ga * q00 + gb * q01);
-    R[0][1] = s * (gb * q00 + ga * q01);
-    R[1][0] = s * (ga * q10 + gb * q11);
-    R[1][1] = s * (gb * q10 + ga * q11);
+    R[0][0] = sc(ga * q00 + gb * q01);
+    R[0][1] = sc(gb * q00 + ga * q01);
+    R[1][0] = sc(ga * q10 + gb * q11);
+    R[1][1] = sc(gb * q10 + ga * q11);
   }
 }
 
 // Gradient of a nodal scalar v[iz][iy][ix] at the Gauss points:
 // gxv[gz][gy] (d/dx), gyv[gz][gx] (d/dy), gzv[gy][gx] (d/dz)
-template <class T>
+template <bool SC = true, class T>
 __device__ __forceinline__ void grad8(const T (&v)[2][2][2], T sx, T sy, T sz, T (&gxv)[2][2], T (&gyv)[2][2],
                                       T (&gzv)[2][2]) {
   T ex[2][2], ey[2][2], ez[2][2];
@@ -262,19 +286,19 @@ __device__ __forceinline__ void grad8(const T (&v)[2][2][2], T sx, T sy, T sz, T
       ey[a][b] = v[a][1][b] - v[a][0][b];  // [iz][ix]
       ez[a][b] = v[1][a][b] - v[0][a][b];  // [iy][ix]
     }
-  interp4(ex, sx, gxv);
-  interp4(ey, sy, gyv);
-  interp4(ez, sz, gzv);
+  interp4<SC>(ex, sx, gxv);
+  interp4<SC>(ey, sy, gyv);
+  interp4<SC>(ez, sz, gzv);
 }
 // Transposed gradient: f[iz][iy][ix] = sum_g dN/dx_j * S_j (SET) or
 // f += (SET false).  Assigning the x part instead of adding it to 0 saves
 // the additions of zero, which the compiler keeps (0 + x is +0 for x = -0);
 // every other value is the same.
-template <bool SET, class T>
+template <bool SET, bool SC = true, class T>
 __device__ __forceinline__ void grad8_t(const T (&Sx)[2][2], const T (&Sy)[2][2], const T (&Sz)[2][2], T sx, T sy,
                                         T sz, T (&f)[2][2][2]) {
   T R[2][2];
-  interp4_t(Sx, sx, R);  // R[iz][iy]
+  interp4_t<SC>(Sx, sx, R);  // R[iz][iy]
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -287,7 +311,7 @@ __device__ __forceinline__ void grad8_t(const T (&Sx)[2][2], const T (&Sy)[2][2]
         f[a][b][0] -= R[a][b];
       }
     }
-  interp4_t(Sy, sy, R);  // R[iz][ix]
+  interp4_t<SC>(Sy, sy, R);  // R[iz][ix]
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -295,7 +319,7 @@ __device__ __forceinline__ void grad8_t(const T (&Sx)[2][2], const T (&Sy)[2][2]
       f[a][1][b] += R[a][b];
       f[a][0][b] -= R[a][b];
     }
-  interp4_t(Sz, sz, R);  // R[iy][ix]
+  interp4_t<SC>(Sz, sz, R);  // R[iy][ix]
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -397,14 +421,18 @@ __device__ __forceinline__ void fine_lame(const T (&Kn)[2][2][2], const T (&Gn)[
 }
 
 // y_e = K_e u_e for one element: u[c][iz][iy][ix] -> f[c][iz][iy][ix]
-// (Internal_Force_element, mechanical_pcg.cpp:114-172)
-template <bool CONST_COEF, class T>
+// (Internal_Force_element, mechanical_pcg.cpp:114-172).  SC false: the
+// gradient and its transpose are left unscaled (sx..tz unused), for callers
+// that have folded the grid scale into the moduli: on a grid with
+// hx = hy = hz = h, every term of f carries the factor (1/h)(w_g/h) =
+// w_g/h^2, and lambda, mu enter f linearly.
+template <bool CONST_COEF, bool SC = true, class T>
 __device__ __forceinline__ void mech_element(const T (&u)[3][2][2][2], const T (&lam)[2][2][2],
                                              const T (&mu)[2][2][2], T lam0, T mu0, T sx, T sy, T sz, T tx, T ty,
                                              T tz, T (&f)[3][2][2][2]) {
   T dx[3][2][2], dy[3][2][2], dz[3][2][2];
 #pragma unroll
-  for (int c = 0; c < 3; c++) grad8(u[c], sx, sy, sz, dx[c], dy[c], dz[c]);
+  for (int c = 0; c < 3; c++) grad8<SC>(u[c], sx, sy, sz, dx[c], dy[c], dz[c]);
   // Sx sums over gx, Sy over gy, Sz over gz: the first Gauss point of each
   // sum assigns, the second adds (no additions of zero)
   T Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
@@ -432,7 +460,7 @@ __device__ __forceinline__ void mech_element(const T (&u)[3][2][2][2], const T (
         put(Sz[0][gy][gx], sxz, gz == 0); put(Sz[1][gy][gx], syz, gz == 0); put(Sz[2][gy][gx], szz, gz == 0);
       }
 #pragma unroll
-  for (int c = 0; c < 3; c++) grad8_t<true>(Sx[c], Sy[c], Sz[c], tx, ty, tz, f[c]);
+  for (int c = 0; c < 3; c++) grad8_t<true, SC>(Sx[c], Sy[c], Sz[c], tx, ty, tz, f[c]);
 }
 
 // Jacobi diagonal of one element (PreM_u_element, mechanical_pcg.cpp:63-100)

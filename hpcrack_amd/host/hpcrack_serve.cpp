@@ -26,6 +26,9 @@
 //   --slots K           concurrent requests (default 8)
 //   --budget-gb B       HBM admission budget (default 200)
 //   --mixed             mechanical solve in the mixed-precision form
+//   --cheb D[,R[,U]]    mixed form: Chebyshev degree D, smoothing range R and
+//                       safety factor U on the largest eigenvalue
+//                       (default 2,6,1.1)
 //   --no-graphs         launch the multigrid cycles kernel by kernel instead of
 //                       replaying them as HIP graphs
 //   --fuse [WINDOW_US]  attach every admitted request to one fusion engine
@@ -40,6 +43,11 @@
 //   --tol X             squared relative residual of both solves (default 1e-16)
 //   --only ID           run request ID alone (same inputs), nothing else
 //   --out PATH          per-request results (default: none)
+//   --image-threads T   host threads that build the request images before the
+//                       clock starts (default: up to 16)
+//   --hw-queues Q       hardware queues the HIP runtime spreads the slots'
+//                       streams over (GPU_MAX_HW_QUEUES, 1..32; default: the
+//                       runtime's own, 4)
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -92,6 +100,10 @@ struct Options {
   int slots = 8;
   double budget_gb = 200.0;
   bool mixed = false;
+  int cheb_degree = 2;
+  double cheb_ratio = 6.0, cheb_upper = 1.1;
+  int image_threads = 0;
+  int hw_queues = 0;
   bool graphs = true;
   int fuse_window_us = -1;  // < 0: no fusion
   double load_factor = 2.0;
@@ -212,7 +224,7 @@ Result solve(const Request& r, const Options& o, hpcr_fuser* fz, Clock::time_poi
   if (!rc) rc = hpcr_set_image(ctx, r.img.data(), &tab, 1e-6);
   if (!rc) {
     if (o.mixed) {
-      hpcr_mixed_params mp{200, o.tol, 2, 10, 6.0, 1.1, 10};
+      hpcr_mixed_params mp{200, o.tol, o.cheb_degree, 10, o.cheb_ratio, o.cheb_upper, 10, 0, 0};
       rc = hpcr_u_entire_mixed(ctx, 1.0, 0, &f_norm, &mp, &su);
     } else {
       rc = hpcr_u_entire(ctx, 1.0, 0, &f_norm, &su);
@@ -275,6 +287,17 @@ int main(int argc, char** argv) {
     } else if (s == "--slots") o.slots = std::atoi(next());
     else if (s == "--budget-gb") o.budget_gb = std::atof(next());
     else if (s == "--mixed") o.mixed = true;
+    else if (s == "--cheb") {
+      const char* v = next();
+      o.cheb_degree = std::atoi(v);
+      const char* c = std::strchr(v, ',');
+      if (c) {
+        o.cheb_ratio = std::atof(c + 1);
+        c = std::strchr(c + 1, ',');
+        if (c) o.cheb_upper = std::atof(c + 1);
+      }
+    } else if (s == "--image-threads") o.image_threads = std::atoi(next());
+    else if (s == "--hw-queues") o.hw_queues = std::atoi(next());
     else if (s == "--no-graphs") o.graphs = false;
     else if (s == "--fuse") {
       o.fuse_window_us = 200;
@@ -289,9 +312,19 @@ int main(int argc, char** argv) {
       return 2;
     }
   }
-  if (o.requests < 1 || o.rate <= 0 || o.sizes.empty() || o.slots < 1 || o.budget_gb <= 0) {
+  if (o.requests < 1 || o.rate <= 0 || o.sizes.empty() || o.slots < 1 || o.budget_gb <= 0 || o.cheb_degree < 1 ||
+      o.cheb_degree > 8 || !(o.cheb_ratio > 1.0) || !(o.cheb_upper > 0.0)) {
     std::fprintf(stderr, "bad options\n");
     return 2;
+  }
+  if (o.hw_queues < 0 || o.hw_queues > 32) {
+    std::fprintf(stderr, "--hw-queues takes 1..32\n");
+    return 2;
+  }
+  if (o.hw_queues > 0) {
+    // read by the HIP runtime when it initialises (the first HIP call below)
+    const std::string q = std::to_string(o.hw_queues);
+    setenv("GPU_MAX_HW_QUEUES", q.c_str(), 1);
   }
   for (int n : o.sizes)
     if (n < 9 || ((n - 1) & 7) != 0) {
@@ -309,16 +342,33 @@ int main(int argc, char** argv) {
     if (o.only >= o.requests) return 2;
     reqs = {reqs[o.only]};
   }
-  for (size_t i = 0; i < reqs.size(); i++) {
-    Request& r = reqs[i];
-    if (build_image(r)) {
-      std::fprintf(stderr, "image of request %d failed\n", r.id);
+  {
+    // the request images (the payload), built on host threads before the
+    // clock starts; each depends only on its own seed
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = o.image_threads > 0 ? o.image_threads : (int)std::min(16u, hw);
+    std::atomic<size_t> next_img{0}, ready{0};
+    std::atomic<int> bad{-1};
+    std::vector<std::thread> builders;
+    for (int t = 0; t < nt; t++)
+      builders.emplace_back([&] {
+        for (size_t i; (i = next_img.fetch_add(1)) < reqs.size();) {
+          Request& r = reqs[i];
+          if (build_image(r) || hpcr_estimate_bytes(&r.geo, o.mixed ? 1 : 0, &r.bytes)) {
+            bad = r.id;
+            continue;
+          }
+          const size_t k = ++ready;
+          if (k % 32 == 0 || k == reqs.size()) {
+            std::fprintf(stderr, "[serve] %zu/%zu request images ready\n", k, reqs.size());
+            std::fflush(stderr);
+          }
+        }
+      });
+    for (auto& b : builders) b.join();
+    if (bad >= 0) {
+      std::fprintf(stderr, "image of request %d failed\n", (int)bad);
       return 1;
-    }
-    if (hpcr_estimate_bytes(&r.geo, o.mixed ? 1 : 0, &r.bytes)) return 1;
-    if ((i + 1) % 32 == 0 || i + 1 == reqs.size()) {
-      std::fprintf(stderr, "[serve] %zu/%zu request images ready\n", i + 1, reqs.size());
-      std::fflush(stderr);
     }
   }
   const long long budget = (long long)(o.budget_gb * 1e9);
@@ -448,11 +498,13 @@ int main(int argc, char** argv) {
       "\"wait_ms_p50\": %.1f, \"service_ms_p50\": %.1f, \"peak_admitted_gb\": %.2f, \"peak_concurrent\": %d, "
       "\"mean_cycles_u\": %.2f, \"mean_cycles_d\": %.2f, \"makespan_s\": %.3f, \"fused\": %s, "
       "\"fused_batches\": %lld, \"fused_cycles\": %lld, \"fused_launches\": %lld, \"fused_problem_launches\": %lld, "
-      "\"fused_max_problems\": %d, \"fused_graph_replays\": %lld, \"budget_waits\": %d}\n",
+      "\"fused_max_problems\": %d, \"fused_graph_replays\": %lld, \"budget_waits\": %d, \"cheb_degree\": %d, "
+      "\"hw_queues\": %d}\n",
       reqs.size(), failed, o.mixed ? "mixed" : "fp64", nslots, o.budget_gb, o.rate,
       reqs.size() / std::max(1e-9, last - first), 1e3 * percentile(lat, 0.5), 1e3 * percentile(lat, 0.99),
       1e3 * percentile(lat, 1.0), 1e3 * percentile(wait, 0.5), 1e3 * percentile(service, 0.5), peak_bytes / 1e9,
       peak_active, (double)cu / reqs.size(), (double)cd / reqs.size(), last, fz ? "true" : "false", fst.batches,
-      fst.cycles, fst.launches, fst.problem_launches, fst.max_problems, fst.graph_replays, budget_waits);
+      fst.cycles, fst.launches, fst.problem_launches, fst.max_problems, fst.graph_replays, budget_waits,
+      o.mixed ? o.cheb_degree : 0, o.hw_queues);
   return failed ? 1 : 0;
 }

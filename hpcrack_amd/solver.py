@@ -29,7 +29,9 @@ def device_count():
 
 
 def estimate_bytes(maxlevel, element_nb, domain, mixed=False):
-    """Device bytes a Solver of this geometry holds (no device needed)."""
+    """Device bytes a Solver of this geometry holds (no device needed);
+    mixed: True (or 1) adds the mixed-precision solve's buffers, 2 those of
+    its FP64-preconditioner form."""
     L = capi.load()
     geo = capi.Geometry()
     geo.maxlevel = maxlevel
@@ -37,7 +39,7 @@ def estimate_bytes(maxlevel, element_nb, domain, mixed=False):
         geo.element_nb[i] = int(element_nb[i])
     geo.xa, geo.xb, geo.ya, geo.yb, geo.za, geo.zb = [float(v) for v in domain]
     out = ctypes.c_longlong(0)
-    rc = L.hpcr_estimate_bytes(ctypes.byref(geo), 1 if mixed else 0, ctypes.byref(out))
+    rc = L.hpcr_estimate_bytes(ctypes.byref(geo), int(mixed), ctypes.byref(out))
     if rc != 0:
         raise HpcrError(rc, "bad geometry")
     return out.value
@@ -167,6 +169,11 @@ class Solver:
     def internal_force(self, l, src=P, dst=W):
         self._chk(self._L.hpcr_internal_force(self._h, l, src, dst))
 
+    def internal_force_f32(self, src=P, dst=W, mfma=False):
+        """The mixed solve's FP32 finest-level operator: dst = -(A src),
+        zero on Dirichlet DOFs (matrix-core element contraction with mfma)."""
+        self._chk(self._L.hpcr_internal_force_f32(self._h, src, dst, 1 if mfma else 0))
+
     def prem_u(self, l, dst=W):
         self._chk(self._L.hpcr_prem_u(self._h, l, dst))
 
@@ -239,19 +246,28 @@ class Solver:
         """Run this Solver's multigrid cycles through `fuser` (None detaches)."""
         self._chk(self._L.hpcr_set_fuser(self._h, fuser._h if fuser is not None else None))
 
+    def set_reduction_fence(self, on):
+        """Order reduction partials by a release/acquire fence pair instead
+        of a completion wait (identical results)."""
+        self._chk(self._L.hpcr_set_reduction_fence(self._h, 1 if on else 0))
+
     def set_graphs(self, on):
         """Replay whole multigrid cycles as HIP graphs (identical results)."""
         self._chk(self._L.hpcr_set_graphs(self._h, 1 if on else 0))
 
     def u_entire_mixed(self, Ut, t, f_norm, tol=None, max_iter=100, cheb_degree=2, power_iters=10,
-                       cheb_ratio=6.0, cheb_upper=1.1, replace_every=10):
+                       cheb_ratio=6.0, cheb_upper=1.1, replace_every=10, precond_fp64=False,
+                       element_mfma=False):
         """Mixed-precision form of u_entire (FP64 flexible CG, FP32 V-cycle);
         returns (f_norm, outer iterations).  tol defaults to the context's
-        tol_u (squared relative residual)."""
+        tol_u (squared relative residual).  precond_fp64: the same V-cycle in
+        FP64 (the like-for-like comparison of the two precisions).
+        element_mfma: the FP32 finest-level element contraction on the
+        matrix cores."""
         fn = ctypes.c_double(f_norm)
         st = capi.Stats()
         mp = capi.MixedParams(max_iter, self.tol_u if tol is None else tol, cheb_degree, power_iters, cheb_ratio,
-                              cheb_upper, replace_every)
+                              cheb_upper, replace_every, 1 if precond_fp64 else 0, 1 if element_mfma else 0)
         self._chk(self._L.hpcr_u_entire_mixed(self._h, Ut, t, ctypes.byref(fn), ctypes.byref(mp), ctypes.byref(st)))
         self.last_stats = st
         return fn.value, st.cycles

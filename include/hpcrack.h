@@ -118,6 +118,12 @@ long long hpcr_device_bytes(const hpcr_ctx* ctx); /* device memory held */
  * of launching their kernels one by one; results are identical.  Off by
  * default; kernel timing (hpcr_timing_enable) runs cycles eagerly. */
 int hpcr_set_graphs(hpcr_ctx* ctx, int on);
+/* The deterministic reductions (dot products, norms) order each block's
+ * partial sums before its completion ticket by waiting for the stores, which
+ * write through to the device-coherent L2 (the default, 0), or, with on = 1,
+ * by a release/acquire fence pair -- the memory model's form, slower on this
+ * GPU (it writes back the L2 once per block).  Results are identical. */
+int hpcr_set_reduction_fence(hpcr_ctx* ctx, int on);
 
 /* ---- fusion of several contexts' multigrid cycles ----------------------- */
 /*
@@ -150,9 +156,10 @@ int hpcr_fuser_destroy(hpcr_fuser* fuser); /* HPCR_ERR_STATE while contexts are 
 int hpcr_fuser_set_graphs(hpcr_fuser* fuser, int on);
 int hpcr_fuser_get_stats(hpcr_fuser* fuser, hpcr_fuser_stats* out);
 int hpcr_set_fuser(hpcr_ctx* ctx, hpcr_fuser* fuser); /* NULL detaches */
-/* device bytes a context of this geometry holds (with mixed != 0: plus the
- * buffers of the mixed-precision solve), without allocating; for admission
- * control of several contexts on one device */
+/* device bytes a context of this geometry holds (mixed == 1: plus the
+ * buffers of the mixed-precision solve; mixed == 2: plus those of its FP64
+ * preconditioner form), without allocating; for admission control of several
+ * contexts on one device */
 int hpcr_estimate_bytes(const hpcr_geometry* geo, int mixed, long long* bytes);
 int hpcr_synchronize(hpcr_ctx* ctx);
 
@@ -195,6 +202,13 @@ typedef struct hpcr_mixed_params {
   double cheb_ratio;
   double cheb_upper;
   int replace_every;
+  int precond_fp64; /* 0: the FP32 V-cycle (the mixed-precision form); 1: the
+                       same V-cycle in FP64 -- the like-for-like comparison of
+                       the two precisions at one tolerance */
+  int element_mfma; /* FP32 V-cycle: 1 runs the finest level's element
+                       contraction (B^T D B, src/mechanical_pcg.cpp:114-172) as
+                       batched FP32 matrix-core products with FP32
+                       accumulation; 0 (default) on the vector ALUs */
 } hpcr_mixed_params;
 int hpcr_u_entire_mixed(hpcr_ctx* ctx, double Ut, int t, double* f_norm, const hpcr_mixed_params* mp,
                         hpcr_stats* stats);
@@ -220,6 +234,11 @@ int hpcr_save_d_old(hpcr_ctx* ctx); /* d_old = d (src/crack_propagation.cpp:93) 
 /* ---- finer-grained entry points (operator and smoother level) ---------- */
 /* Internal_Force (src/mechanical_pcg.cpp:174-191): dst = A(src), 3 comps */
 int hpcr_internal_force(hpcr_ctx* ctx, int level, int src_field, int dst_field);
+/* Internal_Force of the finest level in the mixed solve's FP32 preconditioner
+ * (src/mechanical_pcg.cpp:174-191 at FP32): dst = -(A src), 0 on Dirichlet
+ * DOFs, with the Gauss-point moduli from the current K, G, d and split flag;
+ * mfma = 1 takes the matrix-core element contraction, 0 the vector one. */
+int hpcr_internal_force_f32(hpcr_ctx* ctx, int src_field, int dst_field, int mfma);
 /* PreM_u (src/mechanical_pcg.cpp:102-112): dst = diag(A) */
 int hpcr_prem_u(hpcr_ctx* ctx, int level, int dst_field);
 /* Fld (src/phase_field_pcg.cpp:93-103) and Precondition_M (:49-59) */

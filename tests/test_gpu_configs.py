@@ -162,7 +162,10 @@ def test_c3_mixed_solve_reaches_tolerance_and_fp64_solution(HA, W):
     say(f"  mixed: {it} iterations, rel. residual^2 {rel:.3e}; FP64 reference {itr} iterations, {rel_ref:.3e}; "
         f"error {e:.3e}")
     assert rel <= 1e-16 and rel_ref <= 1e-22
-    assert fm == pytest.approx(fr, rel=1e-12)
+    # f_norm at t == 0 is the squared norm of the reactions A u at the
+    # Dirichlet DOFs of the final iterate, so it agrees to about the
+    # solutions' distance (1e-10 here), not to rounding
+    assert fm == pytest.approx(fr, rel=1e-8)
     assert e < 1e-8
     s.close()
 
@@ -212,19 +215,20 @@ def serve(args, out):
 @pytest.mark.parametrize("mixed", [True, False], ids=["mixed", "fp64"])
 def test_c5_short_stream_budget_bound(tmp_path, HA, mixed):
     """24 requests of the C5 stream (Poisson 20/s, dims per axis from
-    {33, 65, 129, 257}, non-cubic included) under a 12 GB budget: a 257^3
-    request alone needs ~6-8 GB, so admissions wait on memory.  Each
+    {33, 65, 129, 257}, non-cubic included) under a 6 GB budget: the largest
+    request of this stream needs 3.7 GB (FP64) or 5.1 GB (mixed) and eight
+    slots would hold up to 10 GB, so admissions wait on memory.  Each
     request's result is identical when it runs alone."""
-    base = ["--requests", "24", "--rate", "20", "--sizes", "33,65,129,257", "--seed", "3000", "--budget-gb", "12"]
+    base = ["--requests", "24", "--rate", "20", "--sizes", "33,65,129,257", "--seed", "3000", "--budget-gb", "6"]
     base += ["--mixed"] if mixed else []
-    say(f"[gpu] C5 short stream, {'mixed' if mixed else 'fp64'}, budget 12 GB")
+    say(f"[gpu] C5 short stream, {'mixed' if mixed else 'fp64'}, budget 6 GB")
     mix_sum, mix = serve(base + ["--slots", "8"], tmp_path / "mix.txt")
     say(f"  mix: {mix_sum}")
     alone_sum, alone = serve(base + ["--slots", "1"], tmp_path / "alone.txt")
     say(f"  alone: {alone_sum}")
     assert mix_sum["failed"] == 0 and alone_sum["failed"] == 0
     assert mix_sum["budget_waits"] > 0
-    assert mix_sum["peak_admitted_gb"] <= 12.0
+    assert mix_sum["peak_admitted_gb"] <= 6.0
     assert mix_sum["peak_concurrent"] > 1 and alone_sum["peak_concurrent"] == 1
     dims = {v["dims"] for v in mix.values()}
     assert any(len(set(d.split("x"))) > 1 for d in dims)  # non-cubic requests

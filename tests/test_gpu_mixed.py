@@ -13,7 +13,9 @@ Checks: the requested squared relative residual is reached (recomputed in
 FP64 from u); the error against the direct solve follows the residual; the
 Dirichlet reactions and f_norm match; a damaged state and a t >= 1 step
 (f_norm frozen, split flags from the previous u) behave as in u_entire; two
-runs are bit-identical; bad parameters are refused."""
+runs are bit-identical; bad parameters are refused.  The same checks run
+with the V-cycle in FP64 (precond_fp64), the like-for-like comparison of the
+two preconditioner precisions."""
 import numpy as np
 import pytest
 import scipy.sparse.linalg as spla
@@ -32,6 +34,11 @@ CASES = [
     (2, (2, 4, 3), (-1.0, 7.0, 0.0, 16.0, 0.5, 12.5)),
 ]
 IDS = ["17c", "9x17x13"]
+PREC = pytest.mark.parametrize("p64", [False, True], ids=["f32", "f64"])
+# preconditioner forms: FP32 (element contraction on the vector ALUs), FP64,
+# FP32 with the element contraction on the matrix cores
+FORMS = pytest.mark.parametrize("form", [dict(), dict(precond_fp64=True), dict(element_mfma=True)],
+                                ids=["f32", "f64", "f32-mfma"])
 
 
 @pytest.fixture(scope="module")
@@ -82,10 +89,11 @@ def zero_u(s, L):
         s.fill(O.FU, L, 0.0, c)
 
 
+@FORMS
 @pytest.mark.parametrize("case", CASES, ids=IDS)
-def test_mixed_matches_direct_solve(HA, case):
+def test_mixed_matches_direct_solve(HA, case, form):
     ml, enb, dom = case
-    say(f"[gpu] mixed vs direct solve {case}")
+    say(f"[gpu] mixed vs direct solve {case}, preconditioner {form}")
     o, s, lab = make_pair(HA, ml, enb, dom, seed=40, lab_incl=4, **KW)
     K, G, _ = material_from_labels(lab)
     L = ml
@@ -99,7 +107,7 @@ def test_mixed_matches_direct_solve(HA, case):
     errs = []
     for tol in (1e-16, 1e-22):
         zero_u(s, L)
-        fs, it = s.u_entire_mixed(1.0, 0, 0.0, tol=tol, max_iter=200)
+        fs, it = s.u_entire_mixed(1.0, 0, 0.0, tol=tol, max_iter=200, **form)
         rel = s.last_stats.r_norm_final / fs
         err = err_modulo_rotation(s.get(O.U, L), ud, m)
         errs.append(err)
@@ -114,11 +122,12 @@ def test_mixed_matches_direct_solve(HA, case):
     assert np.linalg.norm(fu[fixed] - Au[fixed]) <= 1e-9 * np.linalg.norm(Au[fixed])
 
 
-def test_mixed_damaged_state_and_later_step(HA):
+@PREC
+def test_mixed_damaged_state_and_later_step(HA, p64):
     """Damage d in [-0.05, 1.05] on 30 % of the nodes (zeros elsewhere), then
     a t = 1 step: split flags from the step-0 solution, f_norm frozen."""
     ml, enb, dom = CASES[0]
-    say("[gpu] mixed damaged state + step 1, 17^3")
+    say(f"[gpu] mixed damaged state + step 1, 17^3, FP64 V-cycle {p64}")
     o, s, lab = make_pair(HA, ml, enb, dom, seed=41, lab_incl=5, **KW)
     K, G, _ = material_from_labels(lab)
     L = ml
@@ -128,10 +137,10 @@ def test_mixed_damaged_state_and_later_step(HA):
     s.set(O.D, L, d)
     m = rotation_mode(dims, h, dom)
     zero_u(s, L)
-    f0, it0 = s.u_entire_mixed(1.0, 0, 0.0, tol=1e-22, max_iter=200)
+    f0, it0 = s.u_entire_mixed(1.0, 0, 0.0, tol=1e-22, max_iter=200, precond_fp64=p64)
     ud0, _, _ = direct_solution(dims, h, dom, K, G, d, np.zeros(dims[::-1]), 1.0, KW["small_k"])
     e0 = err_modulo_rotation(s.get(O.U, L), ud0, m)
-    f1, it1 = s.u_entire_mixed(1.7, 1, f0, tol=1e-22, max_iter=200)
+    f1, it1 = s.u_entire_mixed(1.7, 1, f0, tol=1e-22, max_iter=200, precond_fp64=p64)
     flags = s.get(O.FLAG, L)
     ud1, _, _ = direct_solution(dims, h, dom, K, G, d, flags, 1.7, KW["small_k"])
     e1 = err_modulo_rotation(s.get(O.U, L), ud1, m)
@@ -142,20 +151,59 @@ def test_mixed_damaged_state_and_later_step(HA):
     assert e0 < 1e-9 and e1 < 1e-9
 
 
-def test_mixed_reaches_tolerance_at_33_and_is_deterministic(HA):
+@PREC
+def test_mixed_reaches_tolerance_at_33_and_is_deterministic(HA, p64):
     ml, enb, dom = 3, (4, 4, 4), (-16.0, 16.0, -16.0, 16.0, -16.0, 16.0)
-    say("[gpu] mixed 33^3: tolerance, iterations, determinism")
+    say(f"[gpu] mixed 33^3: tolerance, iterations, determinism, FP64 V-cycle {p64}")
     _, s, _ = make_pair(HA, ml, enb, dom, seed=43, lab_incl=4, **KW)
     L = ml
     out = []
     for _ in range(2):
         zero_u(s, L)
-        fs, it = s.u_entire_mixed(1.0, 0, 0.0, tol=1e-16)
+        fs, it = s.u_entire_mixed(1.0, 0, 0.0, tol=1e-16, precond_fp64=p64)
         out.append((fs, it, s.get(O.U, L).copy(), list(s.last_stats.history[:it])))
         assert s.last_stats.r_norm_final / fs <= 1e-16 and it <= 40
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2])
     assert out[0][3] == out[1][3]
+
+
+@pytest.mark.parametrize("geo", [
+    (2, (4, 4, 4), (0.0, 16.0, 0.0, 16.0, 0.0, 16.0)),
+    (2, (2, 4, 3), (-1.0, 7.0, 0.0, 12.0, 0.5, 12.5)),
+    (3, (4, 4, 4), (-16.0, 16.0, -16.0, 16.0, -16.0, 16.0)),
+], ids=["17c", "aniso-9x17x13", "33c"])
+def test_fp32_operator_on_vector_alus_and_matrix_cores_matches_fp64(HA, geo):
+    """The preconditioner's FP32 finest-level operator, with the element
+    contraction on the vector ALUs and on the matrix cores, against the FP64
+    operator (itself pinned to the oracle) on a damaged state with all three
+    split flags and a random vector: FP32 rounding only.  The anisotropic
+    case (hx != hy) runs the scaled form, the cubic ones the folded scale."""
+    ml, enb, dom = geo
+    say(f"[gpu] FP32 operator (vector / matrix cores) vs FP64, {geo}")
+    o, s, _ = make_pair(HA, ml, enb, dom, seed=45, lab_incl=4, **KW)
+    L = ml
+    dims, h = o.dims(L), o.h(L)
+    rng = np.random.default_rng(46)
+    shape = dims[::-1]
+    s.set(O.D, L, rng.uniform(-0.05, 1.05, shape) * (rng.random(shape) < 0.4))
+    s.set(O.FLAG, L, rng.integers(0, 3, shape).astype(np.float64))
+    s.set(O.P, L, rng.standard_normal((3,) + shape))
+    s.internal_force(L, O.P, O.W)
+    y64 = s.get(O.W, L).ravel()
+    s.internal_force_f32(O.P, O.W, mfma=False)
+    yv = -s.get(O.W, L).ravel()
+    s.internal_force_f32(O.P, O.W, mfma=True)
+    ym = -s.get(O.W, L).ravel()
+    fixed = np.zeros(y64.size, dtype=bool)
+    fixed[list(T.dirichlet(dims, h, (dom[0], dom[2], dom[4]), (dom[1], dom[3], dom[5]), 0.0))] = True
+    ev = np.linalg.norm(yv[~fixed] - y64[~fixed]) / np.linalg.norm(y64[~fixed])
+    em = np.linalg.norm(ym[~fixed] - y64[~fixed]) / np.linalg.norm(y64[~fixed])
+    mv = np.max(np.abs(yv[~fixed] - y64[~fixed])) / np.max(np.abs(y64[~fixed]))
+    mm = np.max(np.abs(ym[~fixed] - y64[~fixed])) / np.max(np.abs(y64[~fixed]))
+    say(f"  vector ALUs: rel L2 {ev:.2e}, max {mv:.2e}; matrix cores: rel L2 {em:.2e}, max {mm:.2e}")
+    assert fixed.any() and not np.any(yv[fixed]) and not np.any(ym[fixed])
+    assert ev < 1e-5 and em < 1e-5 and mv < 1e-5 and mm < 1e-5
 
 
 def test_mixed_rejects_bad_parameters(HA):
@@ -165,6 +213,13 @@ def test_mixed_rejects_bad_parameters(HA):
         s.u_entire_mixed(1.0, 0, 0.0, cheb_degree=0)
     with pytest.raises(HA.HpcrError):
         s.u_entire_mixed(1.0, 0, 0.0, cheb_ratio=1.0)
+    for bad in ((2, 0), (0, 2), (1, 1)):  # (precond_fp64, element_mfma)
+        mp = HA.capi.MixedParams(10, 1e-16, 2, 10, 6.0, 1.1, 10, *bad)
+        fn = HA.capi.ctypes.c_double(0.0)
+        st = HA.capi.Stats()
+        rc = s._L.hpcr_u_entire_mixed(s._h, 1.0, 0, HA.capi.ctypes.byref(fn), HA.capi.ctypes.byref(mp),
+                                      HA.capi.ctypes.byref(st))
+        assert rc == HA.capi.ERR_ARG
 
 
 def test_zero_load_and_zero_f_norm(HA):
