@@ -137,7 +137,7 @@ def cpu_baseline(workload, n_sample, cycles, threads):
 def serve_stream(mixed, requests, rate, timeout):
     """The request server on the serving stream; its JSON summary line."""
     exe = os.path.join(ROOT, "hpcrack_amd", "hpcrack_serve")
-    cmd = [exe, "--requests", str(requests), "--rate", str(rate)] + (["--mixed"] if mixed else [])
+    cmd = [exe, "--requests", str(requests), "--rate", repr(float(rate))] + (["--mixed"] if mixed else [])
     t = time.perf_counter()
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
@@ -166,6 +166,8 @@ def main():
                          "tolerance (0: skip)")
     ap.add_argument("--mixed-cheb", type=int, default=2, help="Chebyshev degree of the FP32 smoother")
     ap.add_argument("--mixed-ratio", type=float, default=6.0, help="Chebyshev smoothing range lambda_max/lambda_min")
+    ap.add_argument("--mixed-coarse-iters", type=int, default=0,
+                    help="iterations of the preconditioner's coarsest solve (0: nu0_u)")
     ap.add_argument("--mixed-ref-tol", type=float, default=1e-22,
                     help="squared relative residual of the untimed FP64 reference solve that the mixed and plain "
                          "solutions are compared with (0: skip)")
@@ -282,7 +284,7 @@ def main():
         def timed_fcg(tol, p64, label, mfma=False):
             """warmup + timed flexible-CG solves from u = 0: (solves/s, ms, iterations per solve, worst rel)"""
             kw = dict(tol=tol, max_iter=200, cheb_degree=args.mixed_cheb, cheb_ratio=args.mixed_ratio,
-                      precond_fp64=p64, element_mfma=mfma)
+                      precond_fp64=p64, element_mfma=mfma, coarse_iters=args.mixed_coarse_iters)
             for i in range(args.warmup):
                 zero_state()
                 fm, it = s.u_entire_mixed(1.0, 0, 0.0, **kw)
@@ -369,6 +371,7 @@ def main():
             "fp64_converged_reference": ref,
             "cheb_degree": args.mixed_cheb,
             "cheb_ratio": args.mixed_ratio,
+            "coarse_iters": args.mixed_coarse_iters or w.mg_u[0],
             "dtype": "f32 V-cycle (Chebyshev-Jacobi, transfers, operators), f64 outer flexible CG, f64 coarsest",
         }
         log(f"[bench] mixed: {sps32:.3f} solves/s; FP64 V-cycle {sps64:.3f} solves/s "
@@ -383,11 +386,15 @@ def main():
 
     serve = None
     if args.serve:
+        # the stream at its arrival rate (latency) and as one burst (capacity: every request
+        # waiting at t = 0, so solves/s is the server's throughput, not the arrival rate)
         serve = {"stream": f"{args.serve_requests} requests, Poisson {args.serve_rate}/s, node dims per axis from "
                            "{33,65,129,257}, inclusion fraction U(0.10,0.30), one load step each, 8 slots, "
                            "HBM budget 200 GB",
                  "mixed": serve_stream(True, args.serve_requests, args.serve_rate, 600),
-                 "fp64": serve_stream(False, args.serve_requests, args.serve_rate, 600)}
+                 "mixed_capacity": serve_stream(True, args.serve_requests, 1e5, 600),
+                 "fp64": serve_stream(False, args.serve_requests, args.serve_rate, 600),
+                 "fp64_capacity": serve_stream(False, args.serve_requests, 1e5, 600)}
 
     out = {
         "metric": METRIC,

@@ -55,7 +55,7 @@ class MixedParams(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int), ("tol", ctypes.c_double), ("cheb_degree", ctypes.c_int),
                 ("power_iters", ctypes.c_int), ("cheb_ratio", ctypes.c_double), ("cheb_upper", ctypes.c_double),
                 ("replace_every", ctypes.c_int), ("precond_fp64", ctypes.c_int),
-                ("element_mfma", ctypes.c_int)]
+                ("element_mfma", ctypes.c_int), ("coarse_iters", ctypes.c_int)]
 
 
 class FuserStats(ctypes.Structure):

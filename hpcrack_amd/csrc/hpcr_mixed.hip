@@ -86,24 +86,26 @@ struct MechOpP {
 #pragma unroll
     for (int k = 0; k < NR; k++) v[k] = P(0);
     if (!ok) return;
-    const size_t q = nidx(L, gx, gy, gz);
+    const unsigned q = nid32(L, gx, gy, gz);
 #pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = s[c * N + q];
+    for (int c = 0; c < 3; c++) v[c] = ldo(s + c * N, q * (unsigned)sizeof(P));
     if constexpr (NODAL) {
-      v[3] = C.m.K[q];
-      v[4] = C.m.G[q];
-      v[5] = C.m.d[q];
+      v[3] = ldo(C.m.K, 8u * q);
+      v[4] = ldo(C.m.G, 8u * q);
+      v[5] = ldo(C.m.d, 8u * q);
     }
   }
   __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, TE (&e)[NE]) const {
 #pragma unroll
     for (int k = 0; k < NE; k++) e[k] = TE{};
     if (!ok) return;
-    const size_t q = nidx(L, ex, ey, ez);
+    const unsigned q = nid32(L, ex, ey, ez);
     if constexpr (NODAL) {
-      e[0] = (int)C.m.flag[q];
+      e[0] = (int)ldo(C.m.flag, q);
     } else if constexpr (FINE) {
-      const float4* p = reinterpret_cast<const float4*>(C.lm) + 4 * q;
+      // 64 bytes per element: the byte offset of a level's elements fits in
+      // 32 bits up to 2^26 elements; beyond, index in 64 bits
+      const float4* p = reinterpret_cast<const float4*>(C.lm) + 4 * (size_t)q;
 #pragma unroll
       for (int k = 0; k < 4; k++) e[k] = p[k];
     } else {
@@ -162,45 +164,40 @@ struct MechOpP {
     double qp[MODE == F_CHEBQ ? 3 : 1];
   };
   __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
-    const size_t q = nidx(L, i, j, k);
+    const unsigned q = nid32(L, i, j, k), ob = q * (unsigned)sizeof(P);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      if constexpr (MODE != F_POWER) n.r[c] = r[c * N + q];
-      if constexpr (CHEB) n.x[c] = x[c * N + q];
-      if constexpr (MODE != F_RES) n.iD[c] = invD[c * N + q];
-      if constexpr (MODE == F_CHEBQ) n.qp[c] = qprev[c * N + q];
+      if constexpr (MODE != F_POWER) n.r[c] = ldo(r + c * N, ob);
+      if constexpr (CHEB) n.x[c] = ldo(x + c * N, ob);
+      if constexpr (MODE != F_RES) n.iD[c] = ldo(invD + c * N, ob);
+      if constexpr (MODE == F_CHEBQ) n.qp[c] = ldo(qprev + c * N, 8u * q);
     }
   }
 
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const P (&own)[3], int i, int j, int k, int dm,
                                            const P (&v)[3], const State&, const NData& nd,
                                            Acc& acc) const {
-    const size_t q = nidx(L, i, j, k);
+    const unsigned ob = nid32(L, i, j, k) * (unsigned)sizeof(P);
     if constexpr (CHEB) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const size_t o = c * N + q;
         const P rn = ((dm >> c) & 1) ? P(0) : nd.r[c] - v[c];
         const P dn = cd * own[c] + cr * nd.iD[c] * rn;
         const P xn = nd.x[c] + dn;
-        r[o] = rn;
-        dout[o] = dn;
-        x[o] = xn;
+        sto(r + c * N, ob, rn);
+        sto(dout + c * N, ob, dn);
+        sto(x + c * N, ob, xn);
         if constexpr (MODE == F_CHEBQ) acc.a[0] += (double)xn * nd.qp[c];
       }
     } else if constexpr (MODE == F_RES) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const size_t o = c * N + q;
-        r[o] = ((dm >> c) & 1) ? P(0) : nd.r[c] - v[c];
-      }
+      for (int c = 0; c < 3; c++) sto(r + c * N, ob, ((dm >> c) & 1) ? P(0) : nd.r[c] - v[c]);
     } else {  // F_POWER
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const size_t o = c * N + q;
         const bool dir = (dm >> c) & 1;
         const P di = nd.iD[c];
-        dout[o] = dir ? P(0) : di * v[c];
+        sto(dout + c * N, ob, dir ? P(0) : di * v[c]);
         if (!dir) {
           acc.a[0] += (double)own[c] * (double)v[c];
           acc.a[1] += (double)own[c] * (double)own[c] / (double)di;
@@ -471,19 +468,19 @@ struct FcgOpA {
     }
     v.K = v.G = v.d = 0.0;
     if (!ok) return;
-    const size_t o = nidx(L, gx, gy, gz);
+    const unsigned q = nid32(L, gx, gy, gz), ob = 8u * q;
 #pragma unroll
-    for (int c = 0; c < 3; c++) v.z[c] = z[c * N + o];
+    for (int c = 0; c < 3; c++) v.z[c] = ldo(z + c * N, q * (unsigned)sizeof(P));
     if (!first) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) v.p[c] = p_old[c * N + o];
+      for (int c = 0; c < 3; c++) v.p[c] = ldo(p_old + c * N, ob);
     }
-    v.K = C.K[o];
-    v.G = C.G[o];
-    v.d = C.d[o];
+    v.K = ldo(C.K, ob);
+    v.G = ldo(C.G, ob);
+    v.d = ldo(C.d, ob);
   }
   __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, int (&e)[1]) const {
-    e[0] = ok ? (int)C.flag[nidx(L, ex, ey, ez)] : 0;
+    e[0] = ok ? (int)ldo(C.flag, nid32(L, ex, ey, ez)) : 0;
   }
   __device__ __forceinline__ void commit(const Raw& v, const State& st, double (&w)[NV]) const {
 #pragma unroll
@@ -509,28 +506,28 @@ struct FcgOpA {
     double u[LAG ? 3 : 1], p[LAG ? 3 : 1];
   };
   __device__ __forceinline__ void nfetch(const LevelGeom& L, int i, int j, int k, NData& n) const {
-    const size_t o = nidx(L, i, j, k);
+    const unsigned ob = 8u * nid32(L, i, j, k);
 #pragma unroll
-    for (int c = 0; c < 3; c++) n.r[c] = r[c * N + o];
+    for (int c = 0; c < 3; c++) n.r[c] = ldo(r + c * N, ob);
     if constexpr (LAG) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        n.u[c] = u[c * N + o];
-        n.p[c] = p_old[c * N + o];
+        n.u[c] = ldo(u + c * N, ob);
+        n.p[c] = ldo(p_old + c * N, ob);
       }
     }
   }
   __device__ __forceinline__ void epilogue(const LevelGeom& L, const double (&own)[3], int i, int j, int k, int dm,
                                            const double (&v)[3], const State& st, const NData& nd,
                                            Acc& acc) const {
-    const size_t o = nidx(L, i, j, k);
+    const unsigned ob = 8u * nid32(L, i, j, k);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      p_new[c * N + o] = own[c];
-      q[c * N + o] = v[c];
+      sto(p_new + c * N, ob, own[c]);
+      sto(q + c * N, ob, v[c]);
       acc.a[0] += own[c] * v[c];
       acc.a[1] += own[c] * nd.r[c];
-      if constexpr (LAG) u[c * N + o] = fma(st.aprev, nd.p[c], nd.u[c]);
+      if constexpr (LAG) sto(u + c * N, ob, fma(st.aprev, nd.p[c], nd.u[c]));
     }
   }
   __device__ __forceinline__ void finish(Acc& acc, const Blk& bk) const {

@@ -960,6 +960,7 @@ int u_entire_mixed_p(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_m
   const int L = c->L;
   Lev& F = c->lev[L];
   if (L < 1) return set_err(c, HPCR_ERR_ARG, "mixed precision needs at least two levels");
+  if (mp.coarse_iters < 0) return set_err(c, HPCR_ERR_ARG, "coarse_iters must be >= 0");
   if (mp.cheb_degree < 1 || mp.cheb_degree > 8 || mp.power_iters < 1 || !(mp.cheb_ratio > 1.0) ||
       !(mp.cheb_upper > 0.0) || mp.max_iter < 0)
     return set_err(c, HPCR_ERR_ARG, "bad mixed-precision parameters");
@@ -998,6 +999,7 @@ int u_entire_mixed_p(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_m
   double* qb[2] = {F.w, c->q2};
   int it = 0;
   const int m = mp.cheb_degree;
+  const int nu0 = mp.coarse_iters > 0 ? mp.coarse_iters : c->par.mg_u.nu0;
   // Pass B leaves u += alpha p to the next pass A; `lag` says that the update
   // of the last iteration (direction pb[it & 1]) is still to be applied,
   // which `flush` does where u is read before another pass A runs.
@@ -1008,7 +1010,7 @@ int u_entire_mixed_p(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_m
   };
   // one outer iteration's launches: preconditioner V-cycle, FCG passes A and B
   auto iteration = [&]() -> int {
-    int e = mix_vcycle<P>(c, m, c->par.mg_u.nu0, it > 0 ? qb[it & 1] : nullptr);
+    int e = mix_vcycle<P>(c, m, nu0, it > 0 ? qb[it & 1] : nullptr);
     if (e) return e;
     launch_fcg_a(c->stream, F.g, mcoef(c, L), (const P*)MF.x, pb[it & 1], pb[(it + 1) & 1], qb[(it + 1) & 1], F.ru,
                  it == 0, lag ? F.u : nullptr, red(c));
@@ -1017,7 +1019,8 @@ int u_entire_mixed_p(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_m
     return check_launch(c, "flexible CG");
   };
   const std::string msig =
-      c->fuser ? cycle_signature(c, 2) + "/m" + std::to_string(m) + (std::is_same<P, double>::value ? "/p64" : "")
+      c->fuser ? cycle_signature(c, 2) + "/m" + std::to_string(m) + "/c" + std::to_string(nu0) +
+                     (std::is_same<P, double>::value ? "/p64" : "")
                : std::string();
   FuseLoop in_loop(c, msig);
   // at t == 0 the reference always runs a first cycle; with an exactly zero
@@ -1472,6 +1475,15 @@ int hpcr_ctx_create(const hpcr_geometry* geo, const hpcr_params* par, int device
   if (par->mg_u.ncy < 0 || par->mg_d.ncy < 0 || par->mg_u.gamma < 1 || par->mg_d.gamma < 1) {
     g_create_error = "bad multigrid parameters";
     return HPCR_ERR_ARG;
+  }
+  {
+    // the kernels address a node of one field component by a 32-bit byte offset
+    long long n = 1;
+    for (int d = 0; d < 3; d++) n *= (long long)geo->element_nb[d] * (1LL << geo->maxlevel) + 1;
+    if (n > MAX_LEVEL_NODES) {
+      g_create_error = "finest level above 2^29 nodes";
+      return HPCR_ERR_ARG;
+    }
   }
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {

@@ -29,6 +29,8 @@
 //   --cheb D[,R[,U]]    mixed form: Chebyshev degree D, smoothing range R and
 //                       safety factor U on the largest eigenvalue
 //                       (default 2,6,1.1)
+//   --coarse-iters C    mixed form: iterations of the preconditioner's
+//                       coarsest solve (default: nu0_u = 50)
 //   --no-graphs         launch the multigrid cycles kernel by kernel instead of
 //                       replaying them as HIP graphs
 //   --fuse [WINDOW_US]  attach every admitted request to one fusion engine
@@ -104,6 +106,7 @@ struct Options {
   double cheb_ratio = 6.0, cheb_upper = 1.1;
   int image_threads = 0;
   int hw_queues = 0;
+  int coarse_iters = 0;
   bool graphs = true;
   int fuse_window_us = -1;  // < 0: no fusion
   double load_factor = 2.0;
@@ -224,7 +227,7 @@ Result solve(const Request& r, const Options& o, hpcr_fuser* fz, Clock::time_poi
   if (!rc) rc = hpcr_set_image(ctx, r.img.data(), &tab, 1e-6);
   if (!rc) {
     if (o.mixed) {
-      hpcr_mixed_params mp{200, o.tol, o.cheb_degree, 10, o.cheb_ratio, o.cheb_upper, 10, 0, 0};
+      hpcr_mixed_params mp{200, o.tol, o.cheb_degree, 10, o.cheb_ratio, o.cheb_upper, 10, 0, 0, o.coarse_iters};
       rc = hpcr_u_entire_mixed(ctx, 1.0, 0, &f_norm, &mp, &su);
     } else {
       rc = hpcr_u_entire(ctx, 1.0, 0, &f_norm, &su);
@@ -298,6 +301,7 @@ int main(int argc, char** argv) {
       }
     } else if (s == "--image-threads") o.image_threads = std::atoi(next());
     else if (s == "--hw-queues") o.hw_queues = std::atoi(next());
+    else if (s == "--coarse-iters") o.coarse_iters = std::atoi(next());
     else if (s == "--no-graphs") o.graphs = false;
     else if (s == "--fuse") {
       o.fuse_window_us = 200;
@@ -313,7 +317,7 @@ int main(int argc, char** argv) {
     }
   }
   if (o.requests < 1 || o.rate <= 0 || o.sizes.empty() || o.slots < 1 || o.budget_gb <= 0 || o.cheb_degree < 1 ||
-      o.cheb_degree > 8 || !(o.cheb_ratio > 1.0) || !(o.cheb_upper > 0.0)) {
+      o.cheb_degree > 8 || !(o.cheb_ratio > 1.0) || !(o.cheb_upper > 0.0) || o.coarse_iters < 0) {
     std::fprintf(stderr, "bad options\n");
     return 2;
   }

@@ -12,6 +12,13 @@
         per-dispatch average of every counter recorded for the largest-grid
         dispatch of the kernel whose name contains SUBSTR
 
+    python -m hpcrack_amd.profiling solve WORKLOAD FORM [SOLVES]
+        the program to profile: SOLVES (default 1) mechanical solves of a
+        benchmark workload (C1..C4) from u = 0, FORM one of plain (FAS,
+        1 cycle), f32 (mixed, FP32 V-cycle), f64 (FP64 V-cycle), mfma (FP32
+        V-cycle with the matrix-core element contraction), each to the
+        workload's tolerance
+
 The databases are the *_results.db files that rocprofv3 writes with
 --output-format rocpd (its default on ROCm 7).
 """
@@ -94,12 +101,40 @@ def counters(db, kernel):
     return "\n".join(lines)
 
 
+def solve(workload, form, solves=1):
+    import time
+    import hpcrack_amd as HA
+    import hpcrack_amd.workloads as W
+    from hpcrack_amd.capi import FU, U
+    w = W.WORKLOADS[workload]
+    if form == "plain":
+        w.mg_u = (w.mg_u[0], w.mg_u[1], w.mg_u[2], w.mg_u[3], 1)
+    img, frac, K, G, gc = w.materials()
+    s = HA.Solver(w.maxlevel, w.element_nb, w.domain, **w.solver_kwargs())
+    W.upload(s, w, K, G, gc)
+    L = w.maxlevel
+    kw = {"f32": {}, "f64": {"precond_fp64": True}, "mfma": {"element_mfma": True}}
+    for i in range(solves):
+        for c in range(3):
+            s.fill(U, L, 0.0, c)
+            s.fill(FU, L, 0.0, c)
+        t = time.perf_counter()
+        if form == "plain":
+            fn, it = s.u_entire(1.0, 0, 0.0)
+        else:
+            fn, it = s.u_entire_mixed(1.0, 0, 0.0, tol=w.tol_u, max_iter=200, **kw[form])
+        s.synchronize()
+        print(f"{w.name} {form}: {it} iterations, rel. residual^2 {s.last_stats.r_norm_final / fn:.3e}, "
+              f"{time.perf_counter() - t:.3f} s", flush=True)
+    return 0
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace_summary(argv[1], int(argv[2]) if len(argv) > 2 else 25))
         return 0
     if len(argv) >= 3 and argv[0] == "traffic":
-        kernel, copy_bytes = "MechOp<2, true>", 512 * 1024 * 1024
+        kernel, copy_bytes = "MechOp<2, true, true>", 512 * 1024 * 1024
         rest = argv[3:]
         while rest:
             if rest[0] == "--kernel":
@@ -109,8 +144,10 @@ def main(argv):
             rest = rest[2:]
         print(traffic(argv[1], argv[2], kernel, copy_bytes))
         return 0
+    if len(argv) >= 3 and argv[0] == "solve":
+        return solve(argv[1], argv[2], int(argv[3]) if len(argv) > 3 else 1)
     if len(argv) >= 2 and argv[0] == "counters":
-        kernel = argv[3] if len(argv) >= 4 and argv[2] == "--kernel" else "MechOp<2, true>"
+        kernel = argv[3] if len(argv) >= 4 and argv[2] == "--kernel" else "MechOp<2, true, true>"
         print(counters(argv[1], kernel))
         return 0
     print(__doc__)

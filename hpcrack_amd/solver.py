@@ -257,17 +257,19 @@ class Solver:
 
     def u_entire_mixed(self, Ut, t, f_norm, tol=None, max_iter=100, cheb_degree=2, power_iters=10,
                        cheb_ratio=6.0, cheb_upper=1.1, replace_every=10, precond_fp64=False,
-                       element_mfma=False):
+                       element_mfma=False, coarse_iters=0):
         """Mixed-precision form of u_entire (FP64 flexible CG, FP32 V-cycle);
         returns (f_norm, outer iterations).  tol defaults to the context's
         tol_u (squared relative residual).  precond_fp64: the same V-cycle in
         FP64 (the like-for-like comparison of the two precisions).
         element_mfma: the FP32 finest-level element contraction on the
-        matrix cores."""
+        matrix cores.  coarse_iters: iterations of the preconditioner's
+        coarsest solve (0: the context's nu0_u)."""
         fn = ctypes.c_double(f_norm)
         st = capi.Stats()
         mp = capi.MixedParams(max_iter, self.tol_u if tol is None else tol, cheb_degree, power_iters, cheb_ratio,
-                              cheb_upper, replace_every, 1 if precond_fp64 else 0, 1 if element_mfma else 0)
+                              cheb_upper, replace_every, 1 if precond_fp64 else 0, 1 if element_mfma else 0,
+                              coarse_iters)
         self._chk(self._L.hpcr_u_entire_mixed(self._h, Ut, t, ctypes.byref(fn), ctypes.byref(mp), ctypes.byref(st)))
         self.last_stats = st
         return fn.value, st.cycles

@@ -189,7 +189,7 @@ int hpcr_phase_field(hpcr_ctx* ctx, int t, double* fd_norm, hpcr_stats* stats);
  * Chebyshev-Jacobi smoothing (degree cheb_degree before and after the coarse
  * correction, bounds [cheb_upper*lam/cheb_ratio, cheb_upper*lam] from
  * power_iters power iterations per level) and an FP64 coarsest solve of
- * nu0_u Jacobi-PCG iterations.  Stops when sum(r^2)/f_norm <= tol with the residual
+ * coarse_iters (default nu0_u) Jacobi-PCG iterations.  Stops when sum(r^2)/f_norm <= tol with the residual
  * recomputed from u, or after max_iter iterations; the residual is also
  * recomputed every replace_every iterations (0: only at the end).  stats
  * counts outer iterations in `cycles`.  Allocates its FP32 buffers (about
@@ -209,6 +209,8 @@ typedef struct hpcr_mixed_params {
                        contraction (B^T D B, src/mechanical_pcg.cpp:114-172) as
                        batched FP32 matrix-core products with FP32
                        accumulation; 0 (default) on the vector ALUs */
+  int coarse_iters; /* Jacobi-PCG iterations of the preconditioner's coarsest
+                       solve; 0 (default): the context's nu0_u */
 } hpcr_mixed_params;
 int hpcr_u_entire_mixed(hpcr_ctx* ctx, double Ut, int t, double* f_norm, const hpcr_mixed_params* mp,
                         hpcr_stats* stats);

@@ -37,8 +37,9 @@ IDS = ["17c", "9x17x13"]
 PREC = pytest.mark.parametrize("p64", [False, True], ids=["f32", "f64"])
 # preconditioner forms: FP32 (element contraction on the vector ALUs), FP64,
 # FP32 with the element contraction on the matrix cores
-FORMS = pytest.mark.parametrize("form", [dict(), dict(precond_fp64=True), dict(element_mfma=True)],
-                                ids=["f32", "f64", "f32-mfma"])
+FORMS = pytest.mark.parametrize("form", [dict(), dict(precond_fp64=True), dict(element_mfma=True),
+                                         dict(coarse_iters=12)],
+                                ids=["f32", "f64", "f32-mfma", "f32-coarse12"])
 
 
 @pytest.fixture(scope="module")
@@ -213,7 +214,7 @@ def test_mixed_rejects_bad_parameters(HA):
         s.u_entire_mixed(1.0, 0, 0.0, cheb_degree=0)
     with pytest.raises(HA.HpcrError):
         s.u_entire_mixed(1.0, 0, 0.0, cheb_ratio=1.0)
-    for bad in ((2, 0), (0, 2), (1, 1)):  # (precond_fp64, element_mfma)
+    for bad in ((2, 0, 0), (0, 2, 0), (1, 1, 0), (0, 0, -1)):  # (precond_fp64, element_mfma, coarse_iters)
         mp = HA.capi.MixedParams(10, 1e-16, 2, 10, 6.0, 1.1, 10, *bad)
         fn = HA.capi.ctypes.c_double(0.0)
         st = HA.capi.Stats()
