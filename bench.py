@@ -98,7 +98,7 @@ def host_cores():
     return os.cpu_count() or 1, aff
 
 
-def cpu_baseline(workload, n_sample, cycles, threads):
+def cpu_baseline(workload, n_sample, cycles, threads, ref_loops=False):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -108,17 +108,26 @@ def cpu_baseline(workload, n_sample, cycles, threads):
     w.mg_u = (w.mg_u[0], w.mg_u[1], w.mg_u[2], w.mg_u[3], cycles)
     img, frac, K, G, gc = w.materials()
     O.set_threads(threads)
-    o = O.Oracle(w.maxlevel, w.element_nb, w.domain, **w.solver_kwargs())
-    L = w.maxlevel
-    for f, a in ((O.K, K), (O.G, G), (O.GC, gc)):
-        o.set(f, L, a)
-    for f in (O.K, O.G, O.GC):
-        o.coarse_vr_nodal(f)
-    t0 = time.perf_counter()
-    _, cyc = o.u_entire(1.0, 0, 0.0)
-    dt = time.perf_counter() - t0
+    O.set_ref_loops(ref_loops)
+    try:
+        o = O.Oracle(w.maxlevel, w.element_nb, w.domain, **w.solver_kwargs())
+        L = w.maxlevel
+        for f, a in ((O.K, K), (O.G, G), (O.GC, gc)):
+            o.set(f, L, a)
+        for f in (O.K, O.G, O.GC):
+            o.coarse_vr_nodal(f)
+        t0 = time.perf_counter()
+        _, cyc = o.u_entire(1.0, 0, 0.0)
+        dt = time.perf_counter() - t0
+    finally:
+        O.set_ref_loops(False)
     dof = 3 * int(np.prod(w.dims))
     nproc, aff = host_cores()
+    form = ("in the reference's loop form (Gauss-point coefficients re-evaluated 8 times per element, "
+            "critical-section scatter-adds)" if ref_loops else
+            "in its own loop form, which evaluates each element's Gauss-point coefficients once where the "
+            "reference does it 8 times and scatters without critical sections, so it is faster than the "
+            "reference it restates")
     return {
         "value": dof * cyc / dt / 1e6,
         "unit": "MDOF·iter/s",
@@ -128,9 +137,7 @@ def cpu_baseline(workload, n_sample, cycles, threads):
         "affinity_cpus": aff,
         "sample": f"oracle u_entire (OpenMP, {O.lib().ora_get_threads()} threads) on the same generator at "
                   f"{n_sample}^3 nodes ({w.n_incl} inclusions, {100 * frac[1]:.1f}% pores), t=0 from u=0, "
-                  f"{cyc} V-cycles in {dt:.1f} s; the oracle evaluates each element's Gauss-point "
-                  f"coefficients once where the reference does it 8 times and scatters without critical "
-                  f"sections, so it is faster than the reference it restates",
+                  f"{cyc} V-cycles in {dt:.1f} s, {form}",
     }
 
 
@@ -158,13 +165,19 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=257, help="nodes per axis of the CPU sample (0: skip)")
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-ref-sample", type=int, default=65,
+                    help="nodes per axis of the second CPU sample, in the reference's loop form (0: skip)")
+    ap.add_argument("--cpu-sweep", default="",
+                    help="only time CPU samples, one JSON line each, no GPU: comma-separated "
+                         "THREADS:CYCLES:FORM:NODES with FORM o (the oracle's loops) or r (the reference's)")
     ap.add_argument("--traffic-file", default="",
                     help="rocprofv3 PMC summary of pass A on this workload "
                          "(default: profiles/traffic_mech_pcg_a_<workload>_<n>.txt; traffic is null without one)")
     ap.add_argument("--mixed", type=int, default=1,
                     help="also time the mixed-precision solve (FP64 flexible CG + FP32 V-cycle) to the same "
                          "tolerance (0: skip)")
-    ap.add_argument("--mixed-cheb", type=int, default=2, help="Chebyshev degree of the FP32 smoother")
+    ap.add_argument("--mixed-cheb", type=int, default=0,
+                    help="Chebyshev degree of the smoother (0: chosen from the modulus contrast)")
     ap.add_argument("--mixed-ratio", type=float, default=6.0, help="Chebyshev smoothing range lambda_max/lambda_min")
     ap.add_argument("--mixed-coarse-iters", type=int, default=0,
                     help="iterations of the preconditioner's coarsest solve (0: nu0_u)")
@@ -177,6 +190,18 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device copy of this size (counter calibration)")
     args = ap.parse_args()
+
+    if args.cpu_sweep:
+        import hpcrack_amd.workloads as W
+        w = W.WORKLOADS[args.workload]
+        for spec in args.cpu_sweep.split(","):
+            th, cyc, form, n = spec.split(":")
+            t = time.perf_counter()
+            r = cpu_baseline(w, int(n), int(cyc), int(th), ref_loops=(form == "r"))
+            r["wall_s"] = round(time.perf_counter() - t, 1)
+            r["omp_proc_bind"] = os.environ.get("OMP_PROC_BIND", "")
+            print(json.dumps(r), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -283,7 +308,7 @@ def main():
 
         def timed_fcg(tol, p64, label, mfma=False):
             """warmup + timed flexible-CG solves from u = 0: (solves/s, ms, iterations per solve, worst rel)"""
-            kw = dict(tol=tol, max_iter=200, cheb_degree=args.mixed_cheb, cheb_ratio=args.mixed_ratio,
+            kw = dict(tol=tol, max_iter=200, cheb_degree=cheb[0], cheb_ratio=cheb[1], cheb_upper=cheb[2],
                       precond_fp64=p64, element_mfma=mfma, coarse_iters=args.mixed_coarse_iters)
             for i in range(args.warmup):
                 zero_state()
@@ -303,6 +328,9 @@ def main():
             log(f"[bench] {label}: {its} iterations in {el:.3f} s")
             return args.steps / el, el / args.steps * 1e3, its / args.steps, worst
 
+        # the Chebyshev setting every form below uses (the contrast-chosen one by default)
+        cheb = s.mixed_resolve(args.mixed_cheb, args.mixed_ratio)
+        log(f"[bench] mixed smoother: degree {cheb[0]}, range {cheb[1]}, safety {cheb[2]} (contrast {cheb[3]:.3g})")
         # FP32 V-cycle preconditioner (the mixed-precision form) to the tolerance
         sps32, ms32, it32, rel32 = timed_fcg(w.tol_u, False, "mixed FP32 V-cycle")
         u_mixed = s.get(U, L)
@@ -321,7 +349,8 @@ def main():
         if args.mixed_ref_tol > 0:
             zero_state()
             fr, itr = s.u_entire_mixed(1.0, 0, 0.0, tol=args.mixed_ref_tol, max_iter=400,
-                                       cheb_degree=args.mixed_cheb, cheb_ratio=args.mixed_ratio, precond_fp64=True)
+                                       cheb_degree=cheb[0], cheb_ratio=cheb[1], cheb_upper=cheb[2],
+                                       precond_fp64=True)
             ref = {"rel_residual_squared": s.last_stats.r_norm_final / fr, "fcg_iterations": itr,
                    "preconditioner": "f64 V-cycle"}
             u_ref = s.get(U, L)
@@ -369,8 +398,10 @@ def main():
             },
             "solves_ratio_vs_plain_capped_unconverged": round(sps32 / plain_sps, 3),
             "fp64_converged_reference": ref,
-            "cheb_degree": args.mixed_cheb,
-            "cheb_ratio": args.mixed_ratio,
+            "cheb_degree": cheb[0],
+            "cheb_ratio": cheb[1],
+            "cheb_upper": cheb[2],
+            "cheb_chosen_by": "modulus contrast %.3g" % cheb[3] if args.mixed_cheb == 0 else "argument",
             "coarse_iters": args.mixed_coarse_iters or w.mg_u[0],
             "dtype": "f32 V-cycle (Chebyshev-Jacobi, transfers, operators), f64 outer flexible CG, f64 coarsest",
         }
@@ -383,6 +414,11 @@ def main():
             f"{args.cpu_threads} threads")
         cpu = cpu_baseline(w, args.cpu_sample, args.cpu_cycles, args.cpu_threads)
         log(f"[bench] cpu baseline {cpu['value']:.3f} MDOF-iter/s")
+        if args.cpu_ref_sample > 0:
+            # the same algorithm with the reference's own loop costs, on a smaller sample
+            r = cpu_baseline(w, args.cpu_ref_sample, 1, args.cpu_threads, ref_loops=True)
+            cpu["reference_loop_form"] = {"value": r["value"], "sample": r["sample"]}
+            log(f"[bench] cpu baseline, reference loop form: {r['value']:.3f} MDOF-iter/s")
 
     serve = None
     if args.serve:

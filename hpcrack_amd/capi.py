@@ -81,6 +81,7 @@ PROTOTYPES = {
     "hpcr_u_entire": (_i, [_vp, _d, _i, _dp, ctypes.POINTER(Stats)]),
     "hpcr_phase_field": (_i, [_vp, _i, _dp, ctypes.POINTER(Stats)]),
     "hpcr_u_entire_mixed": (_i, [_vp, _d, _i, _dp, ctypes.POINTER(MixedParams), ctypes.POINTER(Stats)]),
+    "hpcr_mixed_resolve": (_i, [_vp, ctypes.POINTER(MixedParams), _dp]),
     "hpcr_estimate_bytes": (_i, [ctypes.POINTER(Geometry), _i, ctypes.POINTER(ctypes.c_longlong)]),
     "hpcr_set_graphs": (_i, [_vp, _i]),
     "hpcr_set_reduction_fence": (_i, [_vp, _i]),

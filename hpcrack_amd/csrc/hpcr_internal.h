@@ -86,8 +86,9 @@ struct LevelGeom {
 };
 
 // most nodes of one level: the kernels address a node of one field
-// component by a 32-bit byte offset (8 bytes per node)
-constexpr long long MAX_LEVEL_NODES = 1LL << 29;
+// component by a 32-bit byte offset (8 bytes per node), and a buffer
+// resource's size (8 bytes per node) stays below the out-of-range offset
+constexpr long long MAX_LEVEL_NODES = (1LL << 29) - 1;
 
 // equal spacing in x, y and z: the mechanical tile passes fold the grid
 // scale into the moduli
@@ -163,15 +164,21 @@ struct PfCoef {
 };
 
 // z-chunk of the tile kernels: the longest chunk that still gives >= 1024
-// blocks, so that small levels are not serialised along z
+// blocks, so that small levels are not serialised along z.  A level that
+// gives fewer blocks than HPCR_ZC_ONE_BELOW (the GPU's compute units) at two
+// node planes per block takes one: its blocks then march over two element
+// layers instead of three, which is what bounds such a pass.
 #ifndef HPCR_ZC_MIN_BLOCKS
 #define HPCR_ZC_MIN_BLOCKS 1024
+#endif
+#ifndef HPCR_ZC_ONE_BELOW
+#define HPCR_ZC_ONE_BELOW 256
 #endif
 inline int choose_zchunk(int nx, int ny, int nz) {
   const long xy = (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY);
   for (int zc = ZCHUNK_MAX; zc > 2; zc /= 2)
     if (xy * ((nz + zc - 1) / zc) >= HPCR_ZC_MIN_BLOCKS) return zc;
-  return 2;
+  return xy * ((nz + 1) / 2) < HPCR_ZC_ONE_BELOW ? 1 : 2;
 }
 inline long tile_blocks(int nx, int ny, int nz, int zc) {
   return (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY) * ((nz + zc - 1) / zc);
@@ -306,6 +313,7 @@ void launch_elastic_limit(hipStream_t s, const LevelGeom& L, const double* u, co
 // reductions into scal[S_RED]
 void launch_top_sum(hipStream_t s, const LevelGeom& L, const double* fz, const Reduce& R);
 void launch_max_diff(hipStream_t s, size_t n, const double* a, const double* b_or_null, const Reduce& R);
+void launch_contrast(hipStream_t s, size_t n, const double* a, const Reduce& R);
 
 // squared norm into scal[S_RR]
 void launch_norm2(hipStream_t s, size_t n, const double* a, const Reduce& R);

@@ -91,6 +91,14 @@ struct Sphere {
 
 }  // namespace
 
+// file name of a history field (write_read_history.cpp:18-53): NAME_t_T_Proc_P.dat
+// under DIR, or NAME itself when it already names a .dat file
+static std::string history_path(const char* dir, const char* name, int t, int proc) {
+  if (std::strstr(name, ".dat")) return name;
+  return std::string(dir ? dir : "Time_step_data") + "/" + name + "_t_" + std::to_string(t) + "_Proc_" +
+         std::to_string(proc) + ".dat";
+}
+
 extern "C" {
 
 int hpcr_read_config(const char* path, int strict, hpcr_config* out) {
@@ -264,14 +272,6 @@ int hpcr_write_raw(const char* path, const unsigned char* img, size_t n) {
   fclose(f);
   return w == n ? HPCR_OK : HPCR_ERR_IO;
 }
-
-namespace {
-std::string history_path(const char* dir, const char* name, int t, int proc) {
-  if (std::strstr(name, ".dat")) return name;
-  return std::string(dir ? dir : "Time_step_data") + "/" + name + "_t_" + std::to_string(t) + "_Proc_" +
-         std::to_string(proc) + ".dat";
-}
-}  // namespace
 
 int hpcr_write_vector_3d(const char* dir, const char* name, int t, int proc, const double* v, size_t n) {
   if (!name || !v) return HPCR_ERR_ARG;

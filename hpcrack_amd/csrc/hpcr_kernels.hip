@@ -93,41 +93,40 @@ struct MechOp {
     return s;
   }
 
+  // node values through buffer resources: a node outside the grid (or past
+  // the chunk) reads zeros without a branch
   __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State& st,
                                         double (&r)[NR]) const {
 #pragma unroll
     for (int k = 0; k < NR; k++) r[k] = 0.0;
-    if (!ok) return;
-    const unsigned ob = nid32(L, gx, gy, gz) * 8u;
-    if constexpr (MODE == M_APPLY || MODE == M_RESID) {
+    const unsigned ob = ok ? nid32(L, gx, gy, gz) * 8u : BOFF_NONE;
+    const size_t nb = N * 8;
+    if constexpr (MODE == M_APPLY || MODE == M_RESID || MODE == M_PCGA) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) r[c] = ldo(x + c * N, ob);
-    } else if constexpr (MODE == M_PCGA) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) r[c] = ldo(x + c * N, ob);
+      for (int c = 0; c < 3; c++) r[c] = bld(brsrc(x + c * N, nb), ob);
+    }
+    if constexpr (MODE == M_PCGA) {
       if (!flag) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) r[3 + c] = ldo(p_old + c * N, ob);
+        for (int c = 0; c < 3; c++) r[3 + c] = bld(brsrc(p_old + c * N, nb), ob);
       }
     }
     if constexpr (FINE) {
       constexpr int o = NR - 3;
-      r[o + 0] = ldo(C.K, ob);
-      r[o + 1] = ldo(C.G, ob);
-      r[o + 2] = ldo(C.d, ob);
+      r[o + 0] = bld(brsrc(C.K, nb), ob);
+      r[o + 1] = bld(brsrc(C.G, nb), ob);
+      r[o + 2] = bld(brsrc(C.d, nb), ob);
     }
   }
 
   __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, TE (&e)[NE]) const {
-#pragma unroll
-    for (int k = 0; k < NE; k++) e[k] = TE(0);
-    if (!ok) return;
     const unsigned q = nid32(L, ex, ey, ez);
     if constexpr (FINE) {
-      e[0] = ldo(C.flag, q);
+      e[0] = bld_i8(brsrc(C.flag, N), ok ? q : BOFF_NONE);
     } else {
-      e[0] = ldo(C.Ke, 8u * q);
-      e[1] = ldo(C.Ge, 8u * q);
+      const unsigned ob = ok ? 8u * q : BOFF_NONE;
+      e[0] = bld(brsrc(C.Ke, N * 8), ob);
+      e[1] = bld(brsrc(C.Ge, N * 8), ob);
     }
   }
 
@@ -193,6 +192,26 @@ struct MechOp {
       else
         mech_element<true, !FOLD>(u, lam, mu, lam0, mu0, sx, sy, sz, tx_, ty_, tz_, f);
     }
+  }
+
+  // finest-level operator passes: each element takes its bottom node plane's
+  // interpolated moduli and in-plane gradient parts from the element below
+  using Carry = std::conditional_t<FINE && MODE != M_DIAG && kDelta<double>, FineCarry<double>, void>;
+  template <class C_>
+  __device__ __forceinline__ void element_planes(const LevelGeom& L, const double (&sN)[NSL][NV][PLANE], int s0,
+                                                 int tx, int ty, int ex, int ey, int ez, const State& st,
+                                                 const TE (&e)[NE], double (&f)[3][2][2][2], C_& cr,
+                                                 bool first) const {
+    double Kn[2][2][2], Gn[2][2][2], dn[2][2][2];
+    gather8(sN, s0, tx, ty, NU + 0, Kn);
+    gather8(sN, s0, tx, ty, NU + 1, Gn);
+    gather8(sN, s0, tx, ty, NU + 2, dn);
+    double lam[2][2][2], mu[2][2][2];
+    fine_lame_planes(Kn, Gn, dn, e[0], C.small_k, cr, first, lam, mu);
+    double u[3][2][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
+    mech_element_planes<!FOLD>(u, lam, mu, cr, first, sx, sy, sz, tx_, ty_, tz_, f);
   }
 
   static constexpr bool NEEDS_OWN = (MODE == M_PCGA);
@@ -876,12 +895,33 @@ void launch_dirichlet_copy(hipStream_t s, const LevelGeom& L, double* f, const d
 // the contributions of its (up to) 8 elements in a fixed order.  Reductions
 // are block-wide with a fixed shuffle tree.  The recurrence is that of the
 // two-pass path: p = z + beta p, alpha = r.z / p.Ap, beta = r.z_new / r.z.
+//
+// A mechanical level of at most SMALL_TPB / 3 elements (the 4x4x4-element
+// coarsest level) spreads each element over three threads, one per
+// displacement component: thread c*E + e takes the gradient of component c,
+// the gradients meet in LDS, and the thread forms row c of the stress and its
+// transposed contraction.  The arithmetic is that of the one-thread element
+// (mech_element), split three ways, so that the apply, which one wavefront
+// would otherwise run alone, runs on three.
 constexpr int SMALL_TPB = 256;
 constexpr int SMALL_NPT = SMALL_NODES / SMALL_TPB;
 static_assert(SMALL_NODES % SMALL_TPB == 0 && SMALL_ELEMS == SMALL_TPB, "small-level layout");
 
+// Block-wide sums, every thread receiving the total: wavefront partials to LDS,
+// one barrier, then every thread adds the partials in wavefront order (the
+// order of block_red, so the total is the same in every thread and equal to
+// block_red's).  Consecutive calls alternate between the two halves of sh, so
+// a half is written again only after the barrier of the call in between,
+// which every thread reaches after its reads of that half.
+#ifndef HPCR_SMALL_RED1
+#define HPCR_SMALL_RED1 1
+#endif
+#ifndef HPCR_SMALL_SPLIT
+#define HPCR_SMALL_SPLIT 1
+#endif
 template <int NV>
-__device__ __forceinline__ void block_sum_all(double (&v)[NV], double* sh) {
+__device__ __forceinline__ void block_sum_all(double (&v)[NV], double* sh, int& half) {
+#if !HPCR_SMALL_RED1
   block_red<RED_SUM, NV>(v, sh);
   __shared__ double bc[NV];
   if (threadIdx.x == 0)
@@ -890,6 +930,23 @@ __device__ __forceinline__ void block_sum_all(double (&v)[NV], double* sh) {
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = bc[k];
   __syncthreads();
+  return;
+#endif
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double* h = sh + half * (4 * 16);
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_red<RED_SUM>(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) h[k * 16 + wid] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double a = h[k * 16];
+    for (int w = 1; w < nw; w++) a = a + h[k * 16 + w];
+    v[k] = a;
+  }
+  half ^= 1;
 }
 
 template <class TIO>
@@ -920,21 +977,27 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
   const LevelGeom& L = a.L;
   __shared__ double sP[NC][SMALL_NODES];
   __shared__ double sE[NC * 8][SMALL_ELEMS];
-  __shared__ double sh[4 * 16];
+  __shared__ double sh[2 * 4 * 16];
+  int half = 0;
   const int tid = threadIdx.x;
   const int N = L.nx * L.ny * L.nz;
   const int ex_n = L.nx - 1, ey_n = L.ny - 1;
   const int E = ex_n * ey_n * (L.nz - 1);
   const double sx = 1.0 / L.hx, sy = 1.0 / L.hy, sz = 1.0 / L.hz;
   const double wg = 0.125 * L.hx * L.hy * L.hz;
+  // one element over three threads (component ct of element et), or one thread
+  constexpr int SPLIT_E = SMALL_TPB / 3;
+  const bool split = HPCR_SMALL_SPLIT && NC == 3 && E <= SPLIT_E;
+  const int et = split ? tid % E : tid;
+  const int ct = split ? tid / E : 0;
 
   // element of this thread and its constant coefficients
-  const bool has_e = tid < E;
+  const bool has_e = split ? tid < 3 * E : tid < E;
   int ei = 0, ej = 0, ek = 0;
   if (has_e) {
-    ei = tid % ex_n;
-    ej = (tid / ex_n) % ey_n;
-    ek = tid / (ex_n * ey_n);
+    ei = et % ex_n;
+    ej = (et / ex_n) % ey_n;
+    ek = et / (ex_n * ey_n);
   }
   double lam0 = 0.0, mu0 = 0.0;
   double pm[2][2][2], pk[2][2][2];
@@ -986,9 +1049,102 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
   }
 
   // y = A v for v in sP: element contributions, then node sums
+  // gradients of the split element: component c's d/dx, d/dy, d/dz at the
+  // Gauss points (4 values each) of element e
+  __shared__ double sG[3][12][SPLIT_E];
   auto apply = [&](double (&y)[SMALL_NPT][NC]) {
     __syncthreads();
-    if (has_e) {
+    if constexpr (NC == 3) {
+      if (split) {
+        // the steps of mech_element<true> for row ct of the element
+        double dxc[2][2], dyc[2][2], dzc[2][2];
+        if (has_e) {
+          double u[2][2][2];
+#pragma unroll
+          for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+              for (int ix = 0; ix < 2; ix++)
+                u[iz][iy][ix] = sP[ct][(ei + ix) + L.nx * ((ej + iy) + L.ny * (ek + iz))];
+          grad8<true>(u, sx, sy, sz, dxc, dyc, dzc);
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+              sG[ct][0 + 2 * a + b][et] = dxc[a][b];
+              sG[ct][4 + 2 * a + b][et] = dyc[a][b];
+              sG[ct][8 + 2 * a + b][et] = dzc[a][b];
+            }
+        }
+        __syncthreads();
+        if (has_e) {
+          double dx[3][2][2], dy[3][2][2], dz[3][2][2];
+#pragma unroll
+          for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) {
+                dx[c][a][b] = sG[c][0 + 2 * a + b][et];
+                dy[c][a][b] = sG[c][4 + 2 * a + b][et];
+                dz[c][a][b] = sG[c][8 + 2 * a + b][et];
+              }
+          double Sx[2][2], Sy[2][2], Sz[2][2];
+          // row ct of the stress (a wavefront's threads share ct when E is 64)
+          auto rows = [&](auto row) {
+            constexpr int C = decltype(row)::value;
+            auto put = [](double& s, double v, bool first) {
+              if (first) s = v; else s += v;
+            };
+#pragma unroll
+            for (int gz = 0; gz < 2; gz++)
+#pragma unroll
+              for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+                for (int gx = 0; gx < 2; gx++) {
+                  const double exx = dx[0][gz][gy], eyy = dy[1][gz][gx], ezz = dz[2][gy][gx];
+                  const double tr = exx + eyy + ezz;
+                  const double lt = lam0 * tr, m2 = 2.0 * mu0;
+                  if constexpr (C == 0) {
+                    const double gxy = dx[1][gz][gy] + dy[0][gz][gx];
+                    const double gxz = dx[2][gz][gy] + dz[0][gy][gx];
+                    put(Sx[gz][gy], lt + m2 * exx, gx == 0);
+                    put(Sy[gz][gx], mu0 * gxy, gy == 0);
+                    put(Sz[gy][gx], mu0 * gxz, gz == 0);
+                  } else if constexpr (C == 1) {
+                    const double gxy = dx[1][gz][gy] + dy[0][gz][gx];
+                    const double gyz = dy[2][gz][gx] + dz[1][gy][gx];
+                    put(Sx[gz][gy], mu0 * gxy, gx == 0);
+                    put(Sy[gz][gx], lt + m2 * eyy, gy == 0);
+                    put(Sz[gy][gx], mu0 * gyz, gz == 0);
+                  } else {
+                    const double gxz = dx[2][gz][gy] + dz[0][gy][gx];
+                    const double gyz = dy[2][gz][gx] + dz[1][gy][gx];
+                    put(Sx[gz][gy], mu0 * gxz, gx == 0);
+                    put(Sy[gz][gx], mu0 * gyz, gy == 0);
+                    put(Sz[gy][gx], lt + m2 * ezz, gz == 0);
+                  }
+                }
+          };
+          if (ct == 0)
+            rows(std::integral_constant<int, 0>());
+          else if (ct == 1)
+            rows(std::integral_constant<int, 1>());
+          else
+            rows(std::integral_constant<int, 2>());
+          double f[2][2][2];
+          grad8_t<true, true>(Sx, Sy, Sz, wg / L.hx, wg / L.hy, wg / L.hz, f);
+#pragma unroll
+          for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+              for (int ix = 0; ix < 2; ix++) sE[ct * 8 + iz * 4 + iy * 2 + ix][et] = f[iz][iy][ix];
+        }
+      }
+    }
+    if (!split && has_e) {
       double f[NC][2][2][2];
       if constexpr (NC == 3) {
         double u[3][2][2][2];
@@ -1066,7 +1222,7 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
       acc[1] += r[s][c] * r[s][c];
       acc[2] += fv[s][c] * fv[s][c];
     }
-  block_sum_all<3>(acc, sh);
+  block_sum_all<3>(acc, sh, half);
   // the phase-field smoother resets fd_norm on every call (phase_field_pcg.cpp:163)
   if (NC == 1 && tid == 0) a.scal[S_FF] = acc[2];
   double rz = acc[0], rr = acc[1], beta = 0.0, pw = 0.0, alpha = 0.0;
@@ -1086,7 +1242,7 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
     for (int s = 0; s < SMALL_NPT; s++)
 #pragma unroll
       for (int c = 0; c < NC; c++) pa[0] += p[s][c] * w[s][c];
-    block_sum_all<1>(pa, sh);
+    block_sum_all<1>(pa, sh, half);
     pw = pa[0];
     alpha = rz / pw;
     // pass B: x += alpha p; r -= alpha w; mask; z = r / D; r.z, r.r
@@ -1103,7 +1259,7 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
         ab[0] += rv * z[s][c];
         ab[1] += rv * rv;
       }
-    block_sum_all<2>(ab, sh);
+    block_sum_all<2>(ab, sh, half);
     beta = ab[0] / rz;
     rz = ab[0];
     rr = ab[1];
@@ -1336,36 +1492,20 @@ __device__ __forceinline__ void restrict_body(const LevelGeom& Lc, const LevelGe
   int i, j, k;
   node_ijk(Lc, qn, i, j, k);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
-  const int x = 2 * i, y = 2 * j, z = 2 * k;
-  double wx[3], wy[3], wz[3];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    wx[d] = (x + d - 1 < 0 || x + d - 1 > Lf.nx - 1) ? 0.0 : (d == 1 ? 1.0 : 0.5);
-    wy[d] = (y + d - 1 < 0 || y + d - 1 > Lf.ny - 1) ? 0.0 : (d == 1 ? 1.0 : 0.5);
-    wz[d] = (z + d - 1 < 0 || z + d - 1 > Lf.nz - 1) ? 0.0 : (d == 1 ? 1.0 : 0.5);
-  }
+  const Restrict27<double> R(Lf, i, j, k);
   const size_t qc = nidx(Lc, i, j, k);
-  for (int c = 0; c < ncomp; c++) {
-    const double* r = rf + c * Nf;
-    double acc = 0.0;
+  auto comps = [&](auto nc) {
+    constexpr int NC = decltype(nc)::value;
+    double acc[NC];
 #pragma unroll
-    for (int dz = 0; dz < 3; dz++) {
-      if (wz[dz] == 0.0) continue;
-      double accy = 0.0;
+    for (int c = 0; c < NC; c++) acc[c] = R(rf + c * Nf);
 #pragma unroll
-      for (int dy = 0; dy < 3; dy++) {
-        if (wy[dy] == 0.0) continue;
-        const size_t row = nidx(Lf, 0, y + dy - 1, z + dz - 1);
-        double accx = 0.0;
-#pragma unroll
-        for (int dx = 0; dx < 3; dx++)
-          if (wx[dx] != 0.0) accx += wx[dx] * r[row + x + dx - 1];
-        accy += wy[dy] * accx;
-      }
-      acc += wz[dz] * accy;
-    }
-    fc[c * Nc + qc] = Fic[c * Nc + qc] + acc;
-  }
+    for (int c = 0; c < NC; c++) fc[c * Nc + qc] = Fic[c * Nc + qc] + acc[c];
+  };
+  if (ncomp == 3)
+    comps(std::integral_constant<int, 3>());
+  else
+    comps(std::integral_constant<int, 1>());
 }
 __global__ __launch_bounds__(256) void restrict_kernel(const LevelGeom Lc, const LevelGeom Lf, int ncomp,
                                                        const double* __restrict__ rf, const double* __restrict__ Fic,
@@ -1799,6 +1939,26 @@ __global__ __launch_bounds__(256) void max_diff_kernel(size_t n, const double* _
 }
 void launch_max_diff(hipStream_t s, size_t n, const double* a, const double* b_or_null, const Reduce& R) {
   hipLaunchKernelGGL(max_diff_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, a, b_or_null, R);
+}
+
+// stiffness contrast of a nodal modulus field: max(a) into scal[S_RED] and
+// max(1/a) over a > 0 into scal[S_SCRATCH] (a = 0 nodes are the zero-node
+// rule's, not a phase)
+__global__ __launch_bounds__(256) void contrast_kernel(size_t n, const double* __restrict__ a, const Reduce R) {
+  double m[2] = {0.0, 0.0};
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const double v = a[q];
+    if (m[0] < v) m[0] = v;
+    if (v > 0.0 && m[1] < 1.0 / v) m[1] = 1.0 / v;
+  }
+  double* scal = R.scal;
+  reduce_finalize<RED_MAX, 2>(m, R, 0.0, [=](double (&s)[2]) {
+    scal[S_RED] = s[0];
+    scal[S_SCRATCH] = s[1];
+  });
+}
+void launch_contrast(hipStream_t s, size_t n, const double* a, const Reduce& R) {
+  hipLaunchKernelGGL(contrast_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, a, R);
 }
 
 // squared norm of ncomp*N values into scal[S_RR] (grid_norm, pcg_utilities.cpp:35-58)

@@ -809,35 +809,12 @@ __device__ __forceinline__ void mix_restrict_body(const LevelGeom& Lc, const Lev
   int i, j, k;
   mix_ijk(Lc, qn, i, j, k);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
-  const int x = 2 * i, y = 2 * j, z = 2 * k;
-  P wx[3], wy[3], wz[3];
-#pragma unroll
-  for (int t = 0; t < 3; t++) {
-    wx[t] = (x + t - 1 < 0 || x + t - 1 > Lf.nx - 1) ? P(0) : (t == 1 ? P(1) : P(0.5));
-    wy[t] = (y + t - 1 < 0 || y + t - 1 > Lf.ny - 1) ? P(0) : (t == 1 ? P(1) : P(0.5));
-    wz[t] = (z + t - 1 < 0 || z + t - 1 > Lf.nz - 1) ? P(0) : (t == 1 ? P(1) : P(0.5));
-  }
+  const Restrict27<P> R(Lf, i, j, k);
   const size_t qc = nidx(Lc, i, j, k);
   const int dm = dir_mask(Lc, i, j, k);
+#pragma unroll
   for (int c = 0; c < 3; c++) {
-    const P* r = rf + c * Nf;
-    P acc = P(0);
-#pragma unroll
-    for (int dz = 0; dz < 3; dz++) {
-      if (wz[dz] == P(0)) continue;
-      P accy = P(0);
-#pragma unroll
-      for (int dy = 0; dy < 3; dy++) {
-        if (wy[dy] == P(0)) continue;
-        const size_t row = nidx(Lf, 0, y + dy - 1, z + dz - 1);
-        P accx = P(0);
-#pragma unroll
-        for (int dx = 0; dx < 3; dx++)
-          if (wx[dx] != P(0)) accx += wx[dx] * r[row + x + dx - 1];
-        accy += wy[dy] * accx;
-      }
-      acc += wz[dz] * accy;
-    }
+    P acc = R(rf + c * Nf);
     if ((dm >> c) & 1) acc = P(0);
     const size_t o = c * Nc + qc;
     rc[o] = acc;

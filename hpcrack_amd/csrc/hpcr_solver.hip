@@ -1067,11 +1067,40 @@ int u_entire_mixed_p(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_m
   return HPCR_OK;
 }
 
-int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mixed_params& mp, hpcr_stats* st) {
+// cheb_degree 0: the smoother setting from the finest level's bulk-modulus
+// contrast max K / min K (K > 0).  Near-void phases (contrast above
+// MIX_AUTO_CONTRAST, e.g. pores at 1e-6 E) make the largest eigenvalues of
+// D^-1 A spread, and a wider, higher-degree polynomial pays for its extra
+// passes; moderate contrast keeps the cheap degree-2 setting.
+constexpr double MIX_AUTO_CONTRAST = 1e3;
+int mixed_resolve(hpcr_ctx* c, hpcr_mixed_params& mp, double* contrast) {
+  if (mp.cheb_degree != 0) return HPCR_OK;
+  Lev& F = c->lev[c->L];
+  launch_contrast(c->stream, (size_t)F.g.nx * F.g.ny * F.g.nz, F.K, red(c));
+  int rc = check_launch(c, "modulus contrast");
+  if (rc || (rc = read_scal(c))) return rc;
+  const double k = c->host_scal[S_RED] * c->host_scal[S_SCRATCH];
+  if (contrast) *contrast = k;
+  if (k > MIX_AUTO_CONTRAST) {
+    mp.cheb_degree = 4;
+    mp.cheb_ratio = 12.0;
+    mp.cheb_upper = 1.3;
+  } else {
+    mp.cheb_degree = 2;
+    mp.cheb_ratio = 6.0;
+    mp.cheb_upper = 1.1;
+  }
+  return HPCR_OK;
+}
+
+int u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpcr_mixed_params& mp_in, hpcr_stats* st) {
+  hpcr_mixed_params mp = mp_in;
   if (mp.precond_fp64 != 0 && mp.precond_fp64 != 1) return set_err(c, HPCR_ERR_ARG, "precond_fp64 must be 0 or 1");
   if (mp.element_mfma != 0 && mp.element_mfma != 1) return set_err(c, HPCR_ERR_ARG, "element_mfma must be 0 or 1");
   if (mp.element_mfma && mp.precond_fp64)
     return set_err(c, HPCR_ERR_ARG, "the matrix-core element contraction is an FP32 V-cycle form");
+  int rc = mixed_resolve(c, mp, nullptr);
+  if (rc) return rc;
   return mp.precond_fp64 ? u_entire_mixed_p<double>(c, Ut, t, f_norm, mp, st)
                          : u_entire_mixed_p<float>(c, Ut, t, f_norm, mp, st);
 }
@@ -1796,6 +1825,13 @@ int hpcr_u_entire_mixed(hpcr_ctx* c, double Ut, int t, double* f_norm, const hpc
   (void)hipSetDevice(c->device);
   if (st) memset(st, 0, sizeof(*st));
   return u_entire_mixed(c, Ut, t, f_norm, *mp, st);
+}
+
+int hpcr_mixed_resolve(hpcr_ctx* c, hpcr_mixed_params* mp, double* contrast) {
+  if (!c || !mp) return HPCR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (contrast) *contrast = 0.0;
+  return mixed_resolve(c, *mp, contrast);
 }
 
 int hpcr_phase_field(hpcr_ctx* c, int t, double* fd_norm, hpcr_stats* st) {

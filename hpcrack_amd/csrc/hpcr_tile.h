@@ -23,6 +23,65 @@ __device__ __forceinline__ size_t nidx(const LevelGeom& L, int i, int j, int k) 
 }
 __host__ __device__ __forceinline__ size_t Nof(const LevelGeom& L) { return (size_t)L.nx * L.ny * L.nz; }
 
+// Full-weighting restriction stencil at one coarse node (i, j, k)
+// (coarsen_f_standard_routine, multigrid_intergrid.cpp:124-342) in its
+// separable form: 1-D weights 1/2, 1, 1/2 per axis around fine node
+// (2i, 2j, 2k), out-of-range fine neighbours weighted 0.  The 27 loads of a
+// component are unconditional, from indices clamped to the grid, so they are
+// all in flight together; an out-of-range term then adds 0 * r, which leaves
+// each partial sum as it is (up to the sign of an exact zero), so the value is
+// that of the form that skips those terms.
+template <class P>
+struct Restrict27 {
+  size_t row[3][3];  // fine node index of (0, y, z) for each (dz, dy)
+  int xs[3];
+  P wx[3], wy[3], wz[3];
+  __device__ __forceinline__ Restrict27(const LevelGeom& Lf, int i, int j, int k) {
+    const int x = 2 * i, y = 2 * j, z = 2 * k;
+    int ys[3], zs[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      const bool ox = x + t - 1 < 0 || x + t - 1 > Lf.nx - 1;
+      const bool oy = y + t - 1 < 0 || y + t - 1 > Lf.ny - 1;
+      const bool oz = z + t - 1 < 0 || z + t - 1 > Lf.nz - 1;
+      wx[t] = ox ? P(0) : (t == 1 ? P(1) : P(0.5));
+      wy[t] = oy ? P(0) : (t == 1 ? P(1) : P(0.5));
+      wz[t] = oz ? P(0) : (t == 1 ? P(1) : P(0.5));
+      xs[t] = min(max(x + t - 1, 0), Lf.nx - 1);
+      ys[t] = min(max(y + t - 1, 0), Lf.ny - 1);
+      zs[t] = min(max(z + t - 1, 0), Lf.nz - 1);
+    }
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) row[dz][dy] = nidx(Lf, 0, ys[dy], zs[dz]);
+  }
+  // sum_z wz (sum_y wy (sum_x wx r)) of one component
+  __device__ __forceinline__ P operator()(const P* __restrict__ r) const {
+    P v[3][3][3];
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) v[dz][dy][dx] = r[row[dz][dy] + xs[dx]];
+    P acc = P(0);
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++) {
+      P accy = P(0);
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) {
+        P accx = P(0);
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) accx += wx[dx] * v[dz][dy][dx];
+        accy += wy[dy] * accx;
+      }
+      acc += wz[dz] * accy;
+    }
+    return acc;
+  }
+};
+
 // Node index as a 32-bit count, and loads/stores at a 32-bit byte offset from
 // a kernel-uniform base pointer: the instruction takes the base from scalar
 // registers and the offset from one vector register, so an access costs no
@@ -39,6 +98,29 @@ __device__ __forceinline__ T ldo(const T* base, unsigned boff) {
 template <class T>
 __device__ __forceinline__ void sto(T* base, unsigned boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// Loads and stores through buffer resources.  The instruction takes the
+// base from the resource in scalar registers and a 32-bit byte offset from one
+// vector register, so no per-lane 64-bit address is formed, and the hardware's
+// range check makes a load at or past the resource's size return 0: a fetch of
+// a node outside the grid (offset BOFF_NONE) needs no branch and no zeroing.
+// A resource covers one component of one field of one level, at most
+// MAX_LEVEL_NODES * 8 <= BOFF_NONE bytes.
+constexpr unsigned BOFF_NONE = 0xFFFFFFF8u;
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void* base, size_t bytes) {
+  // dword 3: 32-bit data format, the raw-buffer setting of gfx9 parts
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(unsigned)bytes, 0x00020000);
+}
+__device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, unsigned boff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
+}
+__device__ __forceinline__ signed char bld_i8(__amdgpu_buffer_rsrc_t r, unsigned boff) {
+  return (signed char)__builtin_amdgcn_raw_buffer_load_b8(r, boff, 0, 0);
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned boff, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, boff, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -184,6 +266,26 @@ __device__ __forceinline__ void interp8(const T (&v)[2][2][2], T (&o)[2][2][2]) 
 #pragma unroll
     for (int gx = 0; gx < 2; gx++) lerp2(s[0][gy][gx], s[1][gy][gx], o[0][gy][gx], o[1][gy][gx]);
 }
+// interp8 split at the node planes, for values carried up a z-march: the x-
+// and y-passes of one node plane p[iy][ix] (s[gy][gx]), and the z-pass
+// between the bottom and top planes' results.  interp8 runs the same
+// operations in the same order.
+template <class T>
+__device__ __forceinline__ void interp_plane(const T (&p)[2][2], T (&s)[2][2]) {
+  T t[2][2];
+#pragma unroll
+  for (int iy = 0; iy < 2; iy++) lerp2(p[iy][0], p[iy][1], t[iy][0], t[iy][1]);
+#pragma unroll
+  for (int gx = 0; gx < 2; gx++) lerp2(t[0][gx], t[1][gx], s[0][gx], s[1][gx]);
+}
+template <class T>
+__device__ __forceinline__ void interp_z(const T (&s0)[2][2], const T (&s1)[2][2], T (&o)[2][2][2]) {
+#pragma unroll
+  for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+    for (int gx = 0; gx < 2; gx++) lerp2(s0[gy][gx], s1[gy][gx], o[0][gy][gx], o[1][gy][gx]);
+}
+
 // transpose of interp8: nodal f[iz][iy][ix] (+)= sum_g L L L q[gz][gy][gx]
 // (SET assigns instead of adding to f)
 template <bool SET = false, class T>
@@ -289,6 +391,40 @@ __device__ __forceinline__ void grad8(const T (&v)[2][2][2], T sx, T sy, T sz, T
   interp4<SC>(ex, sx, gxv);
   interp4<SC>(ey, sy, gyv);
   interp4<SC>(ez, sz, gzv);
+}
+// grad8 split at the node planes: the in-plane parts of d/dx and d/dy of one
+// plane p[iy][ix] (its x-edge differences interpolated over y, its y-edge
+// differences over x), and the gradient from the bottom and top planes' parts
+// and the element's z-edges.  grad8 runs the same operations in the same
+// order.
+template <class T>
+__device__ __forceinline__ void grad_plane(const T (&p)[2][2], T (&xp)[2], T (&yp)[2]) {
+  lerp2(p[0][1] - p[0][0], p[1][1] - p[1][0], xp[0], xp[1]);
+  lerp2(p[1][0] - p[0][0], p[1][1] - p[0][1], yp[0], yp[1]);
+}
+template <bool SC = true, class T>
+__device__ __forceinline__ void grad8_planes(const T (&v)[2][2][2], const T (&xb)[2], const T (&yb)[2],
+                                             const T (&xt)[2], const T (&yt)[2], T sx, T sy, T sz,
+                                             T (&gxv)[2][2], T (&gyv)[2][2], T (&gzv)[2][2]) {
+  lerp2(xb[0], xt[0], gxv[0][0], gxv[1][0]);
+  lerp2(xb[1], xt[1], gxv[0][1], gxv[1][1]);
+  lerp2(yb[0], yt[0], gyv[0][0], gyv[1][0]);
+  lerp2(yb[1], yt[1], gyv[0][1], gyv[1][1]);
+  T ez[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) ez[a][b] = v[1][a][b] - v[0][a][b];
+  interp4<SC>(ez, sz, gzv);
+  if constexpr (SC) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        gxv[a][b] *= sx;
+        gyv[a][b] *= sy;
+      }
+  }
 }
 // Transposed gradient: f[iz][iy][ix] = sum_g dN/dx_j * S_j (SET) or
 // f += (SET false).  Assigning the x part instead of adding it to 0 saves
@@ -420,19 +556,96 @@ __device__ __forceinline__ void fine_lame(const T (&Kn)[2][2][2], const T (&Gn)[
   }
 }
 
-// y_e = K_e u_e for one element: u[c][iz][iy][ix] -> f[c][iz][iy][ix]
-// (Internal_Force_element, mechanical_pcg.cpp:114-172).  SC false: the
-// gradient and its transpose are left unscaled (sx..tz unused), for callers
-// that have folded the grid scale into the moduli: on a grid with
-// hx = hy = hz = h, every term of f carries the factor (1/h)(w_g/h) =
-// w_g/h^2, and lambda, mu enter f linearly.
-template <bool CONST_COEF, bool SC = true, class T>
-__device__ __forceinline__ void mech_element(const T (&u)[3][2][2][2], const T (&lam)[2][2][2],
-                                             const T (&mu)[2][2][2], T lam0, T mu0, T sx, T sy, T sz, T tx, T ty,
-                                             T tz, T (&f)[3][2][2][2]) {
-  T dx[3][2][2], dy[3][2][2], dz[3][2][2];
+// What a fine element passes up the z-march to the element above it: its top
+// node plane's x/y-interpolated K and G, whether that plane has a zero K, G or
+// d node, and its in-plane gradient parts (grad_plane) of u.
+template <class T>
+struct FineCarry {
+  T Ks[2][2], Gs[2][2];
+  bool zK, zG, zd;
+  T gx[3][2], gy[3][2];
+};
+template <class T>
+__device__ __forceinline__ bool any_zero4(const T (&p)[2][2]) {
+  return (p[0][0] == T(0)) | (p[0][1] == T(0)) | (p[1][0] == T(0)) | (p[1][1] == T(0));
+}
+
+// fine_lame with the bottom plane's part of the work taken from the element
+// below (first: the lowest element of a block's march computes both planes);
+// the same values as fine_lame
+template <class T>
+__device__ __forceinline__ void fine_lame_planes(const T (&Kn)[2][2][2], const T (&Gn)[2][2][2],
+                                                 const T (&dn)[2][2][2], int flag, T small_k, FineCarry<T>& cr,
+                                                 bool first, T (&lam)[2][2][2], T (&mu)[2][2][2]) {
+  T Kb[2][2], Gb[2][2];
+  bool zKb, zGb, zdb;
+  if (first) {
+    interp_plane(Kn[0], Kb);
+    interp_plane(Gn[0], Gb);
+    zKb = any_zero4(Kn[0]);
+    zGb = any_zero4(Gn[0]);
+    zdb = any_zero4(dn[0]);
+  } else {
 #pragma unroll
-  for (int c = 0; c < 3; c++) grad8<SC>(u[c], sx, sy, sz, dx[c], dy[c], dz[c]);
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        Kb[a][b] = cr.Ks[a][b];
+        Gb[a][b] = cr.Gs[a][b];
+      }
+    zKb = cr.zK;
+    zGb = cr.zG;
+    zdb = cr.zd;
+  }
+  interp_plane(Kn[1], cr.Ks);
+  interp_plane(Gn[1], cr.Gs);
+  cr.zK = any_zero4(Kn[1]);
+  cr.zG = any_zero4(Gn[1]);
+  cr.zd = any_zero4(dn[1]);
+  const bool zK = zKb | cr.zK, zG = zGb | cr.zG;
+  const T bK = (!zK && flag == 0) ? T(1) : T(0), aK = (!zK && flag != 0) ? T(1) : T(0);
+  const T bG = (!zG && flag != 2) ? T(1) : T(0), aG = (!zG && flag == 2) ? T(1) : T(0);
+  T Kg[2][2][2], Gg[2][2][2];
+  interp_z(Kb, cr.Ks, Kg);
+  interp_z(Gb, cr.Gs, Gg);
+  if (zdb | cr.zd) {
+    const T g0 = degr(T(0), small_k);
+    const T gK = aK + bK * g0, gG = aG + bG * g0;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const T Gd = mul_nc(gG, Gg[a][b][c]);
+          lam[a][b][c] = mul_nc(gK, Kg[a][b][c]) - (T(2) / T(3)) * Gd;
+          mu[a][b][c] = Gd;
+        }
+  } else {
+    T dg[2][2][2];
+    interp8(dn, dg);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const T g = degr(dg[a][b][c], small_k);
+          const T Gd = mul_nc(aG + bG * g, Gg[a][b][c]);
+          lam[a][b][c] = mul_nc(aK + bK * g, Kg[a][b][c]) - (T(2) / T(3)) * Gd;
+          mu[a][b][c] = Gd;
+        }
+  }
+}
+
+// The element's forces from its displacement gradients at the Gauss points:
+// stresses, then the transposed contraction (the second half of
+// mech_element).
+template <bool CONST_COEF, bool SC = true, class T>
+__device__ __forceinline__ void mech_element_from_grad(const T (&dx)[3][2][2], const T (&dy)[3][2][2],
+                                                       const T (&dz)[3][2][2], const T (&lam)[2][2][2],
+                                                       const T (&mu)[2][2][2], T lam0, T mu0, T tx, T ty, T tz,
+                                                       T (&f)[3][2][2][2]) {
   // Sx sums over gx, Sy over gy, Sz over gz: the first Gauss point of each
   // sum assigns, the second adds (no additions of zero)
   T Sx[3][2][2], Sy[3][2][2], Sz[3][2][2];
@@ -461,6 +674,45 @@ __device__ __forceinline__ void mech_element(const T (&u)[3][2][2][2], const T (
       }
 #pragma unroll
   for (int c = 0; c < 3; c++) grad8_t<true, SC>(Sx[c], Sy[c], Sz[c], tx, ty, tz, f[c]);
+}
+
+// y_e = K_e u_e for one element: u[c][iz][iy][ix] -> f[c][iz][iy][ix]
+// (Internal_Force_element, mechanical_pcg.cpp:114-172).  SC false: the
+// gradient and its transpose are left unscaled (sx..tz unused), for callers
+// that have folded the grid scale into the moduli: on a grid with
+// hx = hy = hz = h, every term of f carries the factor (1/h)(w_g/h) =
+// w_g/h^2, and lambda, mu enter f linearly.
+template <bool CONST_COEF, bool SC = true, class T>
+__device__ __forceinline__ void mech_element(const T (&u)[3][2][2][2], const T (&lam)[2][2][2],
+                                             const T (&mu)[2][2][2], T lam0, T mu0, T sx, T sy, T sz, T tx, T ty,
+                                             T tz, T (&f)[3][2][2][2]) {
+  T dx[3][2][2], dy[3][2][2], dz[3][2][2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) grad8<SC>(u[c], sx, sy, sz, dx[c], dy[c], dz[c]);
+  mech_element_from_grad<CONST_COEF, SC>(dx, dy, dz, lam, mu, lam0, mu0, tx, ty, tz, f);
+}
+
+// mech_element<false, SC> with the gradient's bottom-plane parts taken from the
+// element below (grad8_planes); the same values as mech_element
+template <bool SC = true, class T>
+__device__ __forceinline__ void mech_element_planes(const T (&u)[3][2][2][2], const T (&lam)[2][2][2],
+                                                    const T (&mu)[2][2][2], FineCarry<T>& cr, bool first, T sx,
+                                                    T sy, T sz, T tx, T ty, T tz, T (&f)[3][2][2][2]) {
+  static_assert(kDelta<T>, "the plane-split gradient follows the delta-form interpolation");
+  T dx[3][2][2], dy[3][2][2], dz[3][2][2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T xb[2], yb[2];
+    if (first) {
+      grad_plane(u[c][0], xb, yb);
+    } else {
+      xb[0] = cr.gx[c][0]; xb[1] = cr.gx[c][1];
+      yb[0] = cr.gy[c][0]; yb[1] = cr.gy[c][1];
+    }
+    grad_plane(u[c][1], cr.gx[c], cr.gy[c]);
+    grad8_planes<SC>(u[c], xb, yb, cr.gx[c], cr.gy[c], sx, sy, sz, dx[c], dy[c], dz[c]);
+  }
+  mech_element_from_grad<false, SC>(dx, dy, dz, lam, mu, T(0), T(0), tx, ty, tz, f);
 }
 
 // Jacobi diagonal of one element (PreM_u_element, mechanical_pcg.cpp:63-100)
@@ -639,6 +891,19 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 #define HPCR_TILE_ATTR
 #endif
 
+// An op whose element passes values up the z-march declares `Carry` (not
+// void) and element_planes(..., Carry&, bool first).
+template <class Op, class = void>
+struct CarryOf {
+  using type = char;
+  static constexpr bool value = false;
+};
+template <class Op>
+struct CarryOf<Op, std::void_t<typename Op::Carry>> {
+  using type = std::conditional_t<std::is_void<typename Op::Carry>::value, char, typename Op::Carry>;
+  static constexpr bool value = !std::is_void<typename Op::Carry>::value;
+};
+
 template <class Op>
 __device__ __forceinline__ void tile_body(const LevelGeom& L, const Op& op, const Blk& bk) {
   constexpr int NC = Op::NC;  // output components
@@ -717,6 +982,7 @@ __device__ __forceinline__ void tile_body(const LevelGeom& L, const Op& op, cons
   const int exc = min(max(ex, 0), L.nx - 1), eyc = min(max(ey, 0), L.ny - 1);
 
   typename Op::Acc acc;
+  typename CarryOf<Op>::type up{};  // what each element passes to the one above
   T carry[NC];
 #pragma unroll
   for (int c = 0; c < NC; c++) carry[c] = T(0);
@@ -732,7 +998,10 @@ __device__ __forceinline__ void tile_body(const LevelGeom& L, const Op& op, cons
     fetch(z0 + 2);
   }
   __syncthreads();
-  for (int kz = z0 - 1; kz < zend; ++kz) {
+  // One layer of the march.  The layer below the chunk (kz = z0 - 1) runs
+  // first and on its own, so that an op with a carry gets `first` as a
+  // compile-time constant.
+  auto layer = [&](int kz, auto first) {
     const int s0 = slot(kz);
     const int xb = (NXB == 2) ? (kz & 1) : 0;
     // Every thread computes its element.  One outside the grid (a node off
@@ -745,7 +1014,10 @@ __device__ __forceinline__ void tile_body(const LevelGeom& L, const Op& op, cons
       TE e[NE];
 #pragma unroll
       for (int k = 0; k < NE; k++) e[k] = sE[s0][k][tid];
-      op.element(L, sN, s0, tx, ty, ex, ey, kz, st, e, f);
+      if constexpr (CarryOf<Op>::value)
+        op.element_planes(L, sN, s0, tx, ty, ex, ey, kz, st, e, f, up, decltype(first)::value);
+      else
+        op.element(L, sN, s0, tx, ty, ex, ey, kz, st, e, f);
     }
     // x-direction: node (tx, .) takes corner 0 of its own element and corner 1
     // of element tx-1 (adjacent lane); g[c][iz][iy]
@@ -789,7 +1061,9 @@ __device__ __forceinline__ void tile_body(const LevelGeom& L, const Op& op, cons
     }
     op.nfetch(L, exc, eyc, min(kz + 1, L.nz - 1), nd);
     if constexpr (NS == 2) __syncthreads();
-  }
+    };
+  layer(z0 - 1, std::true_type());
+  for (int kz = z0; kz < zend; ++kz) layer(kz, std::false_type());
   op.finish(acc, bk);
 }
 

@@ -27,8 +27,8 @@
 //   --budget-gb B       HBM admission budget (default 200)
 //   --mixed             mechanical solve in the mixed-precision form
 //   --cheb D[,R[,U]]    mixed form: Chebyshev degree D, smoothing range R and
-//                       safety factor U on the largest eigenvalue
-//                       (default 2,6,1.1)
+//                       safety factor U on the largest eigenvalue (default 0:
+//                       chosen per request from its modulus contrast)
 //   --coarse-iters C    mixed form: iterations of the preconditioner's
 //                       coarsest solve (default: nu0_u = 50)
 //   --no-graphs         launch the multigrid cycles kernel by kernel instead of
@@ -102,7 +102,7 @@ struct Options {
   int slots = 8;
   double budget_gb = 200.0;
   bool mixed = false;
-  int cheb_degree = 2;
+  int cheb_degree = 0;
   double cheb_ratio = 6.0, cheb_upper = 1.1;
   int image_threads = 0;
   int hw_queues = 0;
@@ -316,7 +316,7 @@ int main(int argc, char** argv) {
       return 2;
     }
   }
-  if (o.requests < 1 || o.rate <= 0 || o.sizes.empty() || o.slots < 1 || o.budget_gb <= 0 || o.cheb_degree < 1 ||
+  if (o.requests < 1 || o.rate <= 0 || o.sizes.empty() || o.slots < 1 || o.budget_gb <= 0 || o.cheb_degree < 0 ||
       o.cheb_degree > 8 || !(o.cheb_ratio > 1.0) || !(o.cheb_upper > 0.0) || o.coarse_iters < 0) {
     std::fprintf(stderr, "bad options\n");
     return 2;

@@ -255,16 +255,18 @@ class Solver:
         """Replay whole multigrid cycles as HIP graphs (identical results)."""
         self._chk(self._L.hpcr_set_graphs(self._h, 1 if on else 0))
 
-    def u_entire_mixed(self, Ut, t, f_norm, tol=None, max_iter=100, cheb_degree=2, power_iters=10,
+    def u_entire_mixed(self, Ut, t, f_norm, tol=None, max_iter=100, cheb_degree=0, power_iters=10,
                        cheb_ratio=6.0, cheb_upper=1.1, replace_every=10, precond_fp64=False,
                        element_mfma=False, coarse_iters=0):
         """Mixed-precision form of u_entire (FP64 flexible CG, FP32 V-cycle);
         returns (f_norm, outer iterations).  tol defaults to the context's
-        tol_u (squared relative residual).  precond_fp64: the same V-cycle in
-        FP64 (the like-for-like comparison of the two precisions).
-        element_mfma: the FP32 finest-level element contraction on the
-        matrix cores.  coarse_iters: iterations of the preconditioner's
-        coarsest solve (0: the context's nu0_u)."""
+        tol_u (squared relative residual).  cheb_degree 0 chooses the
+        Chebyshev degree, range and safety factor from the modulus contrast
+        (mixed_resolve); otherwise cheb_ratio and cheb_upper apply.
+        precond_fp64: the same V-cycle in FP64 (the like-for-like comparison
+        of the two precisions).  element_mfma: the FP32 finest-level element
+        contraction on the matrix cores.  coarse_iters: iterations of the
+        preconditioner's coarsest solve (0: the context's nu0_u)."""
         fn = ctypes.c_double(f_norm)
         st = capi.Stats()
         mp = capi.MixedParams(max_iter, self.tol_u if tol is None else tol, cheb_degree, power_iters, cheb_ratio,
@@ -273,6 +275,14 @@ class Solver:
         self._chk(self._L.hpcr_u_entire_mixed(self._h, Ut, t, ctypes.byref(fn), ctypes.byref(mp), ctypes.byref(st)))
         self.last_stats = st
         return fn.value, st.cycles
+
+    def mixed_resolve(self, cheb_degree=0, cheb_ratio=6.0, cheb_upper=1.1):
+        """(cheb_degree, cheb_ratio, cheb_upper, contrast) that u_entire_mixed
+        uses on the current fields; contrast is 0 unless cheb_degree is 0."""
+        mp = capi.MixedParams(100, self.tol_u, cheb_degree, 10, cheb_ratio, cheb_upper, 10, 0, 0, 0)
+        k = ctypes.c_double(0.0)
+        self._chk(self._L.hpcr_mixed_resolve(self._h, ctypes.byref(mp), ctypes.byref(k)))
+        return mp.cheb_degree, mp.cheb_ratio, mp.cheb_upper, k.value
 
     def phase_field(self, t, fd_norm):
         """Phase-field solve; returns (fd_norm, cycles)."""

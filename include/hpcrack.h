@@ -197,7 +197,11 @@ int hpcr_phase_field(hpcr_ctx* ctx, int t, double* fd_norm, hpcr_stats* stats);
 typedef struct hpcr_mixed_params {
   int max_iter;
   double tol;
-  int cheb_degree;
+  int cheb_degree; /* 1..8, or 0: chosen with cheb_ratio and cheb_upper from
+                      the finest level's bulk-modulus contrast max K / min K
+                      (K > 0): above 1e3 (near-void phases) degree 4, range
+                      12, safety 1.3; otherwise degree 2, range 6, safety
+                      1.1 (see hpcr_mixed_resolve) */
   int power_iters;
   double cheb_ratio;
   double cheb_upper;
@@ -214,6 +218,11 @@ typedef struct hpcr_mixed_params {
 } hpcr_mixed_params;
 int hpcr_u_entire_mixed(hpcr_ctx* ctx, double Ut, int t, double* f_norm, const hpcr_mixed_params* mp,
                         hpcr_stats* stats);
+/* The smoother setting hpcr_u_entire_mixed would use on the context's
+ * current fields: with mp->cheb_degree == 0 fills cheb_degree, cheb_ratio and
+ * cheb_upper from the modulus contrast (written to *contrast when not NULL);
+ * otherwise leaves mp unchanged.  No counterpart in the reference. */
+int hpcr_mixed_resolve(hpcr_ctx* ctx, hpcr_mixed_params* mp, double* contrast);
 
 /* ---- between-solve updates --------------------------------------------- */
 /* strain_history (src/strain_history.cpp:162-210), finest level */

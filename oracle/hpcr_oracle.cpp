@@ -25,6 +25,15 @@
 //  * MP_choice loops use the correct x/y extents (the reference swaps them,
 //    material_selection.cpp:143-146, identical when nx == ny).
 //
+// Reference-cost loops (ora_set_ref_loops(1), for the timed CPU baseline only):
+// Internal_Force and PreM_u then run the reference's loop form instead --
+// a static OpenMP loop over element layers, per-element heap-allocated work
+// arrays, MP_all_GAUSS_Point re-evaluated inside the Gauss-point loop
+// (mechanical_pcg.cpp:132-134) and scatter-adds inside "omp critical"
+// (mechanical_pcg.cpp:172-177, 99-104).  Values are the same; the scatter
+// order, and with it the rounding, depends on thread timing, so this mode is
+// never used as the checker.
+//
 // Parity pins for each function are listed in tests/test_oracle_*.py.
 
 #include <cmath>
@@ -176,6 +185,7 @@ static inline double W_energy(const double s[3][3], const double e[3][3]) {
 }
 
 static int n_threads_override = 0;
+static int ref_loops = 0;
 static inline int nthreads() {
 #ifdef _OPENMP
   return n_threads_override > 0 ? n_threads_override : omp_get_max_threads();
@@ -345,6 +355,39 @@ static void internal_force(Ctx& C, int l, double* Fi[3], const double* const u[3
   dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
   const double wg = 0.125 * C.hx[l] * C.hy[l] * C.hz[l];
   for (int c = 0; c < 3; c++) std::fill(Fi[c], Fi[c] + D.n(), 0.0);
+  if (ref_loops) {
+    // the reference's loop form (mechanical_pcg.cpp:114-191), for timing
+    element_loop(D, [&](int i, int j, int k) {
+      std::vector<double> Kd(8, 0.0), Gd(8, 0.0), sig(72, 0.0);
+      for (int g = 0; g < 8; g++) {
+        mp_all_gauss_point(C, l, S, i, j, k, Kd.data(), Gd.data());
+        double eps[3][3], s[3][3];
+        epsilon_routine(eps, u, D, S, i, j, k, g);
+        double tr = trace_epsilon(eps);
+        sigma_g(s, eps, Kd[g], Gd[g], tr);
+        for (int c = 0; c < 3; c++)
+          for (int m = 0; m < 3; m++) sig[g * 9 + c * 3 + m] = s[c][m];
+      }
+      for (int sf = 0; sf < 8; sf++) {
+        double fu = 0.0, fv = 0.0, fw = 0.0;
+        for (int g = 0; g < 8; g++) {
+          const double* sg = &sig[g * 9];
+          fu += wg * (sg[0] * S.DN[0][g][sf] + sg[3] * S.DN[1][g][sf] + sg[6] * S.DN[2][g][sf]);
+          fv += wg * (sg[1] * S.DN[0][g][sf] + sg[4] * S.DN[1][g][sf] + sg[7] * S.DN[2][g][sf]);
+          fw += wg * (sg[2] * S.DN[0][g][sf] + sg[5] * S.DN[1][g][sf] + sg[8] * S.DN[2][g][sf]);
+        }
+        size_t q = D.at(Cx[sf] + i, Cy[sf] + j, Cz[sf] + k);
+#pragma omp critical
+        {
+          Fi[0][q] += fu;
+          Fi[1][q] += fv;
+          Fi[2][q] += fw;
+        }
+      }
+    });
+    C.applies_u++;
+    return;
+  }
   element_loop_coloured(D, [&](int i, int j, int k) {
     double Kd[8], Gd[8];
     mp_all_gauss_point(C, l, S, i, j, k, Kd, Gd);
@@ -380,6 +423,30 @@ static void prem_u(Ctx& C, int l, double* preM[3]) {
   dshape_function(S, C.hx[l], C.hy[l], C.hz[l]);
   const double wg = 0.125 * C.hx[l] * C.hy[l] * C.hz[l];
   for (int c = 0; c < 3; c++) std::fill(preM[c], preM[c] + D.n(), 0.0);
+  if (ref_loops) {
+    // the reference's loop form (mechanical_pcg.cpp:63-112), for timing
+    element_loop(D, [&](int i, int j, int k) {
+      std::vector<double> Kd(8, 0.0), Gd(8, 0.0);
+      mp_all_gauss_point(C, l, S, i, j, k, Kd.data(), Gd.data());
+      for (int sf = 0; sf < 8; sf++) {
+        double su = 0.0, sv = 0.0, sw = 0.0;
+        for (int g = 0; g < 8; g++) {
+          const double d0 = S.DN[0][g][sf], d1 = S.DN[1][g][sf], d2 = S.DN[2][g][sf];
+          su += wg * ((Kd[g] - 2. / 3. * Gd[g]) * d0 * d0 + 2. * Gd[g] * d0 * d0 + Gd[g] * (d1 * d1 + d2 * d2));
+          sv += wg * ((Kd[g] - 2. / 3. * Gd[g]) * d1 * d1 + 2. * Gd[g] * d1 * d1 + Gd[g] * (d0 * d0 + d2 * d2));
+          sw += wg * ((Kd[g] - 2. / 3. * Gd[g]) * d2 * d2 + 2. * Gd[g] * d2 * d2 + Gd[g] * (d0 * d0 + d1 * d1));
+        }
+        size_t q = D.at(Cx[sf] + i, Cy[sf] + j, Cz[sf] + k);
+#pragma omp critical
+        {
+          preM[0][q] += su;
+          preM[1][q] += sv;
+          preM[2][q] += sw;
+        }
+      }
+    });
+    return;
+  }
   element_loop_coloured(D, [&](int i, int j, int k) {
     double Kd[8], Gd[8];
     mp_all_gauss_point(C, l, S, i, j, k, Kd, Gd);
@@ -1031,6 +1098,7 @@ static double U_e(Ctx& C) {
 extern "C" {
 
 void ora_set_threads(int n) { n_threads_override = n; }
+void ora_set_ref_loops(int on) { ref_loops = on ? 1 : 0; }
 int ora_get_threads() { return nthreads(); }
 
 void ora_shape_functions(double hx, double hy, double hz, double* N64, double* DN192) {

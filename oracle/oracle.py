@@ -81,6 +81,7 @@ def lib():
         L.ora_image_mp.argtypes = [vp, ctypes.POINTER(ctypes.c_ubyte), ip, ctypes.c_int, dp, dp, dp,
                                    ctypes.c_double]
         L.ora_set_threads.argtypes = [ctypes.c_int]
+        L.ora_set_ref_loops.argtypes = [ctypes.c_int]
         L.ora_get_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -103,6 +104,14 @@ def shape_functions(hx, hy, hz):
 
 def set_threads(n):
     lib().ora_set_threads(int(n))
+
+
+def set_ref_loops(on):
+    """Internal_Force and PreM_u in the reference's loop form (redundant
+    Gauss-point coefficients, critical-section scatter): for timing the CPU
+    baseline only; the scatter order, and so the rounding, then depends on
+    thread timing."""
+    lib().ora_set_ref_loops(1 if on else 0)
 
 
 class Oracle:

@@ -210,10 +210,12 @@ def test_fp32_operator_on_vector_alus_and_matrix_cores_matches_fp64(HA, geo):
 def test_mixed_rejects_bad_parameters(HA):
     ml, enb, dom = CASES[0]
     _, s, _ = make_pair(HA, ml, enb, dom, seed=44, **KW)
+    # cheb_degree 0 is the contrast-chosen setting; negative and > 8 are refused
+    for deg in (-1, 9):
+        with pytest.raises(HA.HpcrError):
+            s.u_entire_mixed(1.0, 0, 0.0, cheb_degree=deg)
     with pytest.raises(HA.HpcrError):
-        s.u_entire_mixed(1.0, 0, 0.0, cheb_degree=0)
-    with pytest.raises(HA.HpcrError):
-        s.u_entire_mixed(1.0, 0, 0.0, cheb_ratio=1.0)
+        s.u_entire_mixed(1.0, 0, 0.0, cheb_degree=2, cheb_ratio=1.0)
     for bad in ((2, 0, 0), (0, 2, 0), (1, 1, 0), (0, 0, -1)):  # (precond_fp64, element_mfma, coarse_iters)
         mp = HA.capi.MixedParams(10, 1e-16, 2, 10, 6.0, 1.1, 10, *bad)
         fn = HA.capi.ctypes.c_double(0.0)
@@ -242,3 +244,45 @@ def test_zero_load_and_zero_f_norm(HA):
     fn, it = s.u_entire_mixed(1.0, 1, 0.0, max_iter=7)
     assert fn == 0.0 and it == 7
     assert np.all(np.isfinite(s.get(O.U, L)))
+
+
+def test_auto_chebyshev_setting_follows_modulus_contrast(HA):
+    """cheb_degree 0 resolves from max K / min K: the two-phase image (contrast
+    K_incl / K_matrix = 38.9e9 / 2.5e9) keeps degree 2, range 6, safety 1.1;
+    with near-void nodes (E = 1e-6 E_matrix) it takes degree 4, range 12,
+    safety 1.3.  An auto solve is bit-identical to the explicit call."""
+    ml, enb, dom = CASES[0]
+    say("[gpu] mixed smoother setting from the modulus contrast")
+    o, s, lab = make_pair(HA, ml, enb, dom, seed=41, lab_incl=4, **KW)
+    L = ml
+    K, G, gc = material_from_labels(lab)
+    kinc, kmat = 70e9 / 1.8 * 1e-6, 3e9 / 1.2 * 1e-6
+    deg, ratio, up, k = s.mixed_resolve()
+    assert (deg, ratio, up) == (2, 6.0, 1.1)
+    assert k == pytest.approx(kinc / kmat, rel=1e-12)
+    assert s.mixed_resolve(3, 8.0, 1.2) == (3, 8.0, 1.2, 0.0)
+
+    def solve(**kw):
+        zero_u(s, L)
+        fs, it = s.u_entire_mixed(1.0, 0, 0.0, tol=1e-16, max_iter=200, **kw)
+        return s.get(O.U, L), it, s.last_stats.r_norm_final / fs
+
+    ua, ita, ra = solve()
+    ue, ite, re = solve(cheb_degree=2, cheb_ratio=6.0, cheb_upper=1.1)
+    assert ita == ite and ra <= 1e-16 and np.array_equal(ua, ue)
+    # a pore: a ball of nodes at 1e-6 of the matrix modulus
+    dims = o.dims(L)
+    zz, yy, xx = np.meshgrid(*(np.arange(n) for n in dims[::-1]), indexing="ij")
+    pore = (xx - 5) ** 2 + (yy - 11) ** 2 + (zz - 8) ** 2 <= 6
+    K2, G2 = np.where(pore, K * 1e-6, K), np.where(pore, G * 1e-6, G)
+    for f, a in ((O.K, K2), (O.G, G2), (O.GC, gc)):
+        s.set(f, L, a)
+    for f in (O.K, O.G, O.GC):
+        s.coarse_vr(f)
+    deg, ratio, up, k = s.mixed_resolve()
+    assert (deg, ratio, up) == (4, 12.0, 1.3)
+    assert k == pytest.approx(K.max() / (K[pore].min() * 1e-6), rel=1e-12)
+    ua, ita, ra = solve()
+    ue, ite, re = solve(cheb_degree=4, cheb_ratio=12.0, cheb_upper=1.3)
+    say(f"  pore image: {ita} iterations, squared rel residual {ra:.3e}")
+    assert ita == ite and ra <= 1e-16 and np.array_equal(ua, ue)

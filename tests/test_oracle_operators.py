@@ -260,3 +260,37 @@ def test_reference_pointwise_build_agrees_with_oracle():
         assert np.allclose(eps[g * 9:(g + 1) * 9].reshape(3, 3), S, atol=1e-14)
         sref = 2.0 * np.trace(S) * np.eye(3) + 2 * 0.7 * (S - np.trace(S) / 3 * np.eye(3))
         assert np.allclose(sig[g * 9:(g + 1) * 9].reshape(3, 3), sref, atol=1e-14)
+
+
+@pytest.mark.parametrize("level", [1, 0], ids=["fine", "coarse"])
+def test_reference_loop_form_gives_the_same_operators(level):
+    """The reference-cost loop form that times the CPU baseline
+    (re-evaluated Gauss-point coefficients, critical-section scatter) applies
+    the same operator and diagonal, on the finest level (nodal moduli,
+    damage, split flags) and on a coarse one (element constants): equal up
+    to the scatter order's rounding."""
+    o = O.Oracle(1, ENB, DOMAIN, small_k=1e-5, lc=1.7)
+    K, G, d, fl = T.mesh(o.dims(1), 7, zero_frac=0.1)
+    for f, a in ((O.K, K), (O.G, G), (O.D, d), (O.FLAG, fl)):
+        o.set(f, 1, a)
+    for f in (O.K, O.G):
+        o.coarse_vr_nodal(f)
+    o.mech_setup()
+    dims = o.dims(level)
+    x = np.random.default_rng(8).standard_normal((3,) + dims[::-1])
+    o.set(O.P, level, x)
+    out = {}
+    O.set_threads(4)
+    try:
+        for ref in (False, True):
+            O.set_ref_loops(ref)
+            o.internal_force(level, O.P, O.W)
+            y = o.vec(O.W, level).copy()
+            o.prem_u(level, O.W)
+            out[ref] = (y, o.vec(O.W, level).copy())
+    finally:
+        O.set_ref_loops(False)
+        O.set_threads(0)
+    for a, b in zip(out[False], out[True]):
+        assert np.max(np.abs(a)) > 0
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a))
