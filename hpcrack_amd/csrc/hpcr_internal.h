@@ -134,17 +134,22 @@ struct MechCoef {
 
 // Coefficients of one level of the mixed-precision solve's preconditioner,
 // whose arithmetic type P is float (the mixed-precision form) or double (the
-// same V-cycle in FP64).  Finest level: P = float reads the precomputed
-// Gauss-point moduli `lm`, P = double the nodal K, G, d and split flag of `m`
-// as the FAS passes do; coarse levels: element moduli Ke, Ge of type P.
+// same V-cycle in FP64).  Finest level: the nodal K, G, d of type P (for P =
+// double the FAS path's own fields) and the split flag of `m`, from which the
+// passes form the Gauss-point moduli as the FAS passes do; with the
+// matrix-core contraction, the precomputed FP32 Gauss-point moduli `lm`.
+// Coarse levels: element moduli Ke, Ge of type P.
 template <class P>
 struct MixCoefT {
   int fine;
   int mfma;         // FP32 finest level: element contraction on the matrix cores
-  const float* lm;  // FP32 finest level: (lambda, mu) at the 8 Gauss points, 16 floats per element (element-major)
+  const float* lm;  // FP32 finest level, matrix cores: (lambda, mu) at the 8 Gauss points, 16 floats per element
+  const P* Kn;      // finest level: nodal K, G, d
+  const P* Gn;
+  const P* dn;
   const P* Ke;      // coarse levels: element moduli
   const P* Ge;
-  MechCoef m;       // FP64 finest level
+  MechCoef m;       // finest level: split flag, small_k
 };
 
 // Material table in device memory (layout of hpcr_material_table)

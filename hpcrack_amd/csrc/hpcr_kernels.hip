@@ -895,14 +895,7 @@ void launch_dirichlet_copy(hipStream_t s, const LevelGeom& L, double* f, const d
 // the contributions of its (up to) 8 elements in a fixed order.  Reductions
 // are block-wide with a fixed shuffle tree.  The recurrence is that of the
 // two-pass path: p = z + beta p, alpha = r.z / p.Ap, beta = r.z_new / r.z.
-//
-// A mechanical level of at most SMALL_TPB / 3 elements (the 4x4x4-element
-// coarsest level) spreads each element over three threads, one per
-// displacement component: thread c*E + e takes the gradient of component c,
-// the gradients meet in LDS, and the thread forms row c of the stress and its
-// transposed contraction.  The arithmetic is that of the one-thread element
-// (mech_element), split three ways, so that the apply, which one wavefront
-// would otherwise run alone, runs on three.
+
 constexpr int SMALL_TPB = 256;
 constexpr int SMALL_NPT = SMALL_NODES / SMALL_TPB;
 static_assert(SMALL_NODES % SMALL_TPB == 0 && SMALL_ELEMS == SMALL_TPB, "small-level layout");
@@ -913,25 +906,8 @@ static_assert(SMALL_NODES % SMALL_TPB == 0 && SMALL_ELEMS == SMALL_TPB, "small-l
 // block_red's).  Consecutive calls alternate between the two halves of sh, so
 // a half is written again only after the barrier of the call in between,
 // which every thread reaches after its reads of that half.
-#ifndef HPCR_SMALL_RED1
-#define HPCR_SMALL_RED1 1
-#endif
-#ifndef HPCR_SMALL_SPLIT
-#define HPCR_SMALL_SPLIT 1
-#endif
 template <int NV>
 __device__ __forceinline__ void block_sum_all(double (&v)[NV], double* sh, int& half) {
-#if !HPCR_SMALL_RED1
-  block_red<RED_SUM, NV>(v, sh);
-  __shared__ double bc[NV];
-  if (threadIdx.x == 0)
-    for (int k = 0; k < NV; k++) bc[k] = v[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; k++) v[k] = bc[k];
-  __syncthreads();
-  return;
-#endif
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double* h = sh + half * (4 * 16);
 #pragma unroll
@@ -985,19 +961,14 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
   const int E = ex_n * ey_n * (L.nz - 1);
   const double sx = 1.0 / L.hx, sy = 1.0 / L.hy, sz = 1.0 / L.hz;
   const double wg = 0.125 * L.hx * L.hy * L.hz;
-  // one element over three threads (component ct of element et), or one thread
-  constexpr int SPLIT_E = SMALL_TPB / 3;
-  const bool split = HPCR_SMALL_SPLIT && NC == 3 && E <= SPLIT_E;
-  const int et = split ? tid % E : tid;
-  const int ct = split ? tid / E : 0;
 
   // element of this thread and its constant coefficients
-  const bool has_e = split ? tid < 3 * E : tid < E;
+  const bool has_e = tid < E;
   int ei = 0, ej = 0, ek = 0;
   if (has_e) {
-    ei = et % ex_n;
-    ej = (et / ex_n) % ey_n;
-    ek = et / (ex_n * ey_n);
+    ei = tid % ex_n;
+    ej = (tid / ex_n) % ey_n;
+    ek = tid / (ex_n * ey_n);
   }
   double lam0 = 0.0, mu0 = 0.0;
   double pm[2][2][2], pk[2][2][2];
@@ -1049,102 +1020,9 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
   }
 
   // y = A v for v in sP: element contributions, then node sums
-  // gradients of the split element: component c's d/dx, d/dy, d/dz at the
-  // Gauss points (4 values each) of element e
-  __shared__ double sG[3][12][SPLIT_E];
   auto apply = [&](double (&y)[SMALL_NPT][NC]) {
     __syncthreads();
-    if constexpr (NC == 3) {
-      if (split) {
-        // the steps of mech_element<true> for row ct of the element
-        double dxc[2][2], dyc[2][2], dzc[2][2];
-        if (has_e) {
-          double u[2][2][2];
-#pragma unroll
-          for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-            for (int iy = 0; iy < 2; iy++)
-#pragma unroll
-              for (int ix = 0; ix < 2; ix++)
-                u[iz][iy][ix] = sP[ct][(ei + ix) + L.nx * ((ej + iy) + L.ny * (ek + iz))];
-          grad8<true>(u, sx, sy, sz, dxc, dyc, dzc);
-#pragma unroll
-          for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int b = 0; b < 2; b++) {
-              sG[ct][0 + 2 * a + b][et] = dxc[a][b];
-              sG[ct][4 + 2 * a + b][et] = dyc[a][b];
-              sG[ct][8 + 2 * a + b][et] = dzc[a][b];
-            }
-        }
-        __syncthreads();
-        if (has_e) {
-          double dx[3][2][2], dy[3][2][2], dz[3][2][2];
-#pragma unroll
-          for (int c = 0; c < 3; c++)
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-              for (int b = 0; b < 2; b++) {
-                dx[c][a][b] = sG[c][0 + 2 * a + b][et];
-                dy[c][a][b] = sG[c][4 + 2 * a + b][et];
-                dz[c][a][b] = sG[c][8 + 2 * a + b][et];
-              }
-          double Sx[2][2], Sy[2][2], Sz[2][2];
-          // row ct of the stress (a wavefront's threads share ct when E is 64)
-          auto rows = [&](auto row) {
-            constexpr int C = decltype(row)::value;
-            auto put = [](double& s, double v, bool first) {
-              if (first) s = v; else s += v;
-            };
-#pragma unroll
-            for (int gz = 0; gz < 2; gz++)
-#pragma unroll
-              for (int gy = 0; gy < 2; gy++)
-#pragma unroll
-                for (int gx = 0; gx < 2; gx++) {
-                  const double exx = dx[0][gz][gy], eyy = dy[1][gz][gx], ezz = dz[2][gy][gx];
-                  const double tr = exx + eyy + ezz;
-                  const double lt = lam0 * tr, m2 = 2.0 * mu0;
-                  if constexpr (C == 0) {
-                    const double gxy = dx[1][gz][gy] + dy[0][gz][gx];
-                    const double gxz = dx[2][gz][gy] + dz[0][gy][gx];
-                    put(Sx[gz][gy], lt + m2 * exx, gx == 0);
-                    put(Sy[gz][gx], mu0 * gxy, gy == 0);
-                    put(Sz[gy][gx], mu0 * gxz, gz == 0);
-                  } else if constexpr (C == 1) {
-                    const double gxy = dx[1][gz][gy] + dy[0][gz][gx];
-                    const double gyz = dy[2][gz][gx] + dz[1][gy][gx];
-                    put(Sx[gz][gy], mu0 * gxy, gx == 0);
-                    put(Sy[gz][gx], lt + m2 * eyy, gy == 0);
-                    put(Sz[gy][gx], mu0 * gyz, gz == 0);
-                  } else {
-                    const double gxz = dx[2][gz][gy] + dz[0][gy][gx];
-                    const double gyz = dy[2][gz][gx] + dz[1][gy][gx];
-                    put(Sx[gz][gy], mu0 * gxz, gx == 0);
-                    put(Sy[gz][gx], mu0 * gyz, gy == 0);
-                    put(Sz[gy][gx], lt + m2 * ezz, gz == 0);
-                  }
-                }
-          };
-          if (ct == 0)
-            rows(std::integral_constant<int, 0>());
-          else if (ct == 1)
-            rows(std::integral_constant<int, 1>());
-          else
-            rows(std::integral_constant<int, 2>());
-          double f[2][2][2];
-          grad8_t<true, true>(Sx, Sy, Sz, wg / L.hx, wg / L.hy, wg / L.hz, f);
-#pragma unroll
-          for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-            for (int iy = 0; iy < 2; iy++)
-#pragma unroll
-              for (int ix = 0; ix < 2; ix++) sE[ct * 8 + iz * 4 + iy * 2 + ix][et] = f[iz][iy][ix];
-        }
-      }
-    }
-    if (!split && has_e) {
+    if (has_e) {
       double f[NC][2][2][2];
       if constexpr (NC == 3) {
         double u[3][2][2][2];

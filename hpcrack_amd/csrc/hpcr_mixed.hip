@@ -41,14 +41,15 @@ enum FMode { F_CHEB = 0, F_RES = 1, F_POWER = 2, F_CHEBQ = 3 };
 // finest level's Gauss-point moduli carry it from the set-up, the FP64
 // finest level scales nodal K and G as its planes are committed, coarse
 // levels scale K_e, G_e per element); see mech_element.
-template <class P, int MODE, bool FINE, bool FOLD = false>
+template <class P, int MODE, bool FINE, bool FOLD = false, bool LM = false>
 struct MechOpP {
   using T = P;
   using TR = P;
-  // the FP64 finest level reads nodal K, G, d and the split flag, as the FAS
-  // passes do; the FP32 one the precomputed Gauss-point moduli
+  // the finest level reads nodal K, G, d and the split flag, as the FAS
+  // passes do (LM: the precomputed FP32 Gauss-point moduli instead)
   static constexpr bool F64 = std::is_same<P, double>::value;
-  static constexpr bool NODAL = F64 && FINE;
+  static_assert(!(LM && F64), "the precomputed moduli are FP32");
+  static constexpr bool NODAL = FINE && !LM;
   static constexpr int NC = 3;
   static constexpr int NU = 3;
   static constexpr int NV = NODAL ? 6 : NU;
@@ -64,8 +65,8 @@ struct MechOpP {
   // finest level, FP32: four float4 per element, (lambda, mu) of Gauss
   // points 2k and 2k+1 in float4 k (16-byte loads and LDS accesses);
   // finest level, FP64: the split flag; coarse levels: K_e, G_e
-  using TE = std::conditional_t<FINE, std::conditional_t<F64, int, float4>, P>;
-  static constexpr int NE = FINE ? (F64 ? 1 : 4) : 2;
+  using TE = std::conditional_t<FINE, std::conditional_t<NODAL, int, float4>, P>;
+  static constexpr int NE = FINE ? (NODAL ? 1 : 4) : 2;
   MixCoefT<P> C;
   const P* s;            // stencil vector
   P* dout;               // CHEB: new direction; POWER: D^-1 A s
@@ -81,28 +82,34 @@ struct MechOpP {
 
   __device__ __forceinline__ State state() const { return State{}; }
 
+  // node values through buffer resources: a node outside the grid (or past
+  // the chunk) reads zeros without a branch
   __device__ __forceinline__ void fetch(const LevelGeom& L, int gx, int gy, int gz, bool ok, const State&,
                                         P (&v)[NR]) const {
+    const unsigned ob = ok ? nid32(L, gx, gy, gz) * (unsigned)sizeof(P) : BOFF_NONE;
+    const size_t nb = N * sizeof(P);
 #pragma unroll
-    for (int k = 0; k < NR; k++) v[k] = P(0);
-    if (!ok) return;
-    const unsigned q = nid32(L, gx, gy, gz);
-#pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = ldo(s + c * N, q * (unsigned)sizeof(P));
+    for (int c = 0; c < 3; c++) v[c] = bldp<P>(brsrc(s + c * N, nb), ob);
     if constexpr (NODAL) {
-      v[3] = ldo(C.m.K, 8u * q);
-      v[4] = ldo(C.m.G, 8u * q);
-      v[5] = ldo(C.m.d, 8u * q);
+      v[3] = bldp<P>(brsrc(C.Kn, nb), ob);
+      v[4] = bldp<P>(brsrc(C.Gn, nb), ob);
+      v[5] = bldp<P>(brsrc(C.dn, nb), ob);
     }
   }
   __device__ __forceinline__ void efetch(const LevelGeom& L, int ex, int ey, int ez, bool ok, TE (&e)[NE]) const {
+    if constexpr (NODAL) {
+      e[0] = (int)bld_i8(brsrc(C.m.flag, N), ok ? nid32(L, ex, ey, ez) : BOFF_NONE);
+    } else {
+      efetch_lm_or_coarse(L, ex, ey, ez, ok, e);
+    }
+  }
+  __device__ __forceinline__ void efetch_lm_or_coarse(const LevelGeom& L, int ex, int ey, int ez, bool ok,
+                                                      TE (&e)[NE]) const {
 #pragma unroll
     for (int k = 0; k < NE; k++) e[k] = TE{};
     if (!ok) return;
     const unsigned q = nid32(L, ex, ey, ez);
-    if constexpr (NODAL) {
-      e[0] = (int)ldo(C.m.flag, q);
-    } else if constexpr (FINE) {
+    if constexpr (FINE) {
       // 64 bytes per element: the byte offset of a level's elements fits in
       // 32 bits up to 2^26 elements; beyond, index in 64 bits
       const float4* p = reinterpret_cast<const float4*>(C.lm) + 4 * (size_t)q;
@@ -129,12 +136,12 @@ struct MechOpP {
 #pragma unroll
     for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
     if constexpr (NODAL) {
-      double Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
+      P Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
       gather8(sN, s0, tx, ty, 3, Kn);
       gather8(sN, s0, tx, ty, 4, Gn);
       gather8(sN, s0, tx, ty, 5, dn);
-      fine_lame(Kn, Gn, dn, e[0], C.m.small_k, lam, mu);
-      mech_element<false, !FOLD>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
+      fine_lame(Kn, Gn, dn, e[0], (P)C.m.small_k, lam, mu);
+      mech_element<false, !FOLD>(u, lam, mu, P(0), P(0), sx, sy, sz, tx_, ty_, tz_, f);
     } else if constexpr (FINE) {
       float lam[2][2][2], mu[2][2][2];
 #pragma unroll
@@ -155,6 +162,23 @@ struct MechOpP {
       P dl[2][2][2], dm[2][2][2];
       mech_element<true, !FOLD>(u, dl, dm, lam0, mu0, sx, sy, sz, tx_, ty_, tz_, f);
     }
+  }
+
+  // finest level: each element takes its bottom node plane's interpolated
+  // moduli and in-plane gradient parts from the element below
+  using Carry = std::conditional_t<NODAL && kDelta<P>, FineCarry<P>, void>;
+  template <class C_>
+  __device__ __forceinline__ void element_planes(const LevelGeom& L, const P (&sN)[NSL][NV][PLANE], int s0, int tx,
+                                                 int ty, int ex, int ey, int ez, const State&, const TE (&e)[NE],
+                                                 P (&f)[3][2][2][2], C_& cr, bool first) const {
+    P u[3][2][2][2], Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
+    gather8(sN, s0, tx, ty, 3, Kn);
+    gather8(sN, s0, tx, ty, 4, Gn);
+    gather8(sN, s0, tx, ty, 5, dn);
+    fine_lame_planes(Kn, Gn, dn, e[0], (P)C.m.small_k, cr, first, lam, mu);
+    mech_element_planes<!FOLD>(u, lam, mu, cr, first, sx, sy, sz, tx_, ty_, tz_, f);
   }
 
   // the epilogue's inputs at the owner's node, read one plane ahead
@@ -270,8 +294,8 @@ __device__ __forceinline__ float mf_grad_coef(int m, int k, const float (&s)[3])
 }
 
 template <int MODE, bool FOLD>
-struct MechOpMfma : MechOpP<float, MODE, true, FOLD> {
-  using Base = MechOpP<float, MODE, true, FOLD>;
+struct MechOpMfma : MechOpP<float, MODE, true, FOLD, true> {
+  using Base = MechOpP<float, MODE, true, FOLD, true>;
   // no per-element data through the LDS plane pipeline (element() loads it)
   using TE = float;
   static constexpr int NE = 1;

@@ -56,7 +56,8 @@ struct Lev {
 template <class P>
 struct MixLev {
   P *x = nullptr, *r = nullptr, *dA = nullptr, *dB = nullptr, *invD = nullptr;
-  float* lm = nullptr;
+  float* lm = nullptr;  // finest level, matrix-core form: Gauss-point moduli (allocated on first use)
+  P *Kn = nullptr, *Gn = nullptr, *dn = nullptr;  // finest level, FP32: nodal K, G, d
   P *Ke = nullptr, *Ge = nullptr;
   double lam = 0.0;
   P cr0 = P(0);
@@ -819,7 +820,7 @@ int mix_alloc(hpcr_ctx* c) {
     } else {
       if ((rc = alloc_n(c, &m.invD, 3 * N))) return rc;
       if (l == c->L) {
-        if ((rc = alloc_n(c, &m.lm, 16 * N))) return rc;
+        if ((rc = alloc_n(c, &m.Kn, N)) || (rc = alloc_n(c, &m.Gn, N)) || (rc = alloc_n(c, &m.dn, N))) return rc;
       } else {
         if ((rc = alloc_n(c, &m.Ke, N)) || (rc = alloc_n(c, &m.Ge, N))) return rc;
       }
@@ -837,10 +838,32 @@ MixCoefT<P> mixcoef(hpcr_ctx* c, int l) {
   k.fine = (l == c->L) ? 1 : 0;
   k.mfma = c->mix_mfma;
   k.lm = m.lm;
+  if constexpr (std::is_same<P, double>::value) {
+    k.Kn = c->lev[l].K;
+    k.Gn = c->lev[l].G;
+    k.dn = c->lev[l].d;
+  } else {
+    k.Kn = m.Kn;
+    k.Gn = m.Gn;
+    k.dn = m.dn;
+  }
   k.Ke = m.Ke;
   k.Ge = m.Ge;
   k.m = mcoef(c, l);
   return k;
+}
+
+// the FP32 Gauss-point moduli of the matrix-core form, allocated on first use
+// (64 B per finest element beyond the mixed solve's estimate)
+int mix_lm(hpcr_ctx* c) {
+  MixLev<float>& M = c->mix[c->L];
+  Lev& F = c->lev[c->L];
+  if (!M.lm) {
+    int rc = alloc_n(c, &M.lm, 16 * F.N);
+    if (rc) return rc;
+  }
+  launch_mix_lame(c->stream, F.g, F.K, F.G, F.d, F.flag, c->par.small_k, M.lm);
+  return check_launch(c, "matrix-core moduli");
 }
 
 // preconditioner coefficients and diagonals (FP32 copies; the FP64 form
@@ -858,7 +881,10 @@ int mix_setup(hpcr_ctx* c, const hpcr_mixed_params& mp) {
     if constexpr (!F64) {
       launch_convert(c->stream, 3 * v.N, v.invDu, M.invD);
       if (l == c->L) {
-        launch_mix_lame(c->stream, v.g, v.K, v.G, v.d, v.flag, c->par.small_k, M.lm);
+        launch_convert(c->stream, v.N, v.K, M.Kn);
+        launch_convert(c->stream, v.N, v.G, M.Gn);
+        launch_convert(c->stream, v.N, v.d, M.dn);
+        if (mp.element_mfma && (rc = mix_lm(c))) return rc;
       } else {
         launch_convert(c->stream, v.N, v.Ke, M.Ke);
         launch_convert(c->stream, v.N, v.Ge, M.Ge);
@@ -1364,7 +1390,7 @@ long long estimate_bytes(const hpcr_geometry& G, int mixed) {
     if (passb_blocks((size_t)N) > pb) pb = passb_blocks((size_t)N);
     if (mixed == 1) {
       total += 2 * 3 * N * 4;
-      if (l > 0) total += 3 * 3 * N * 4 + (l == L ? 16 * N * 4 : 2 * N * 4);
+      if (l > 0) total += 3 * 3 * N * 4 + (l == L ? 3 * N * 4 : 2 * N * 4);
       if (l == L) total += 3 * N * 8;
     } else if (mixed == 2) {
       total += 2 * 3 * N * 8;
@@ -1954,16 +1980,19 @@ int hpcr_internal_force_f32(hpcr_ctx* c, int src, int dst, int mfma) {
   Lev& F = c->lev[L];
   MixLev<float>& M = c->mix[L];
   const size_t n = 3 * F.N;
-  // the Gauss-point moduli as the mixed solve's set-up forms them (split flag
-  // and d as they stand), s = src in FP32, r = 0, then r -= A s (masked)
-  launch_mix_lame(c->stream, F.g, F.K, F.G, F.d, F.flag, c->par.small_k, M.lm);
+  // the coefficients as the mixed solve's set-up forms them (split flag and
+  // d as they stand), s = src in FP32, r = 0, then r -= A s (masked)
+  launch_convert(c->stream, F.N, F.K, M.Kn);
+  launch_convert(c->stream, F.N, F.G, M.Gn);
+  launch_convert(c->stream, F.N, F.d, M.dn);
+  if (mfma && (rc = mix_lm(c))) return rc;
   launch_convert(c->stream, n, (const double*)field_ptr(c, src, L), M.dA);
   HIPCHK(c, hipMemsetAsync(M.r, 0, n * sizeof(float), c->stream));
   MixCoefT<float> C = mixcoef<float>(c, L);
   C.mfma = mfma;
   launch_mix_res(c->stream, F.g, C, (const float*)M.dA, M.r);
   launch_convert(c->stream, n, (const float*)M.r, field_ptr(c, dst, L));
-  c->mix_epoch++;  // lm was rewritten: a mixed-form batch graph must not replay
+  c->mix_epoch++;  // the coefficients were rewritten: a mixed-form batch graph must not replay
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c, "internal_force_f32");
 }

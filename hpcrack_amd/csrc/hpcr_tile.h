@@ -116,6 +116,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void* base, size_t
 __device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, unsigned boff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
 }
+__device__ __forceinline__ float bldf(__amdgpu_buffer_rsrc_t r, unsigned boff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+}
+template <class P>
+__device__ __forceinline__ P bldp(__amdgpu_buffer_rsrc_t r, unsigned boff) {
+  if constexpr (std::is_same<P, double>::value)
+    return bld(r, boff);
+  else
+    return bldf(r, boff);
+}
 __device__ __forceinline__ signed char bld_i8(__amdgpu_buffer_rsrc_t r, unsigned boff) {
   return (signed char)__builtin_amdgcn_raw_buffer_load_b8(r, boff, 0, 0);
 }
