@@ -144,7 +144,7 @@ def cpu_baseline(workload, n_sample, cycles, threads, ref_loops=False):
 def serve_stream(mixed, requests, rate, timeout):
     """The request server on the serving stream; its JSON summary line."""
     exe = os.path.join(ROOT, "hpcrack_amd", "hpcrack_serve")
-    cmd = [exe, "--requests", str(requests), "--rate", repr(float(rate))] + (["--mixed"] if mixed else [])
+    cmd = [exe, "--requests", str(requests), "--rate", repr(float(rate))] + (["--mixed"] if mixed else ["--fp64"])
     t = time.perf_counter()
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:

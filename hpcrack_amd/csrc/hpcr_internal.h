@@ -170,14 +170,15 @@ struct PfCoef {
 
 // z-chunk of the tile kernels: the longest chunk that still gives >= 1024
 // blocks, so that small levels are not serialised along z.  A level that
-// gives fewer blocks than HPCR_ZC_ONE_BELOW (the GPU's compute units) at two
-// node planes per block takes one: its blocks then march over two element
-// layers instead of three, which is what bounds such a pass.
+// gives fewer than HPCR_ZC_ONE_BELOW blocks at two node planes per block
+// takes one: its blocks then march over two element layers instead of three,
+// which is what bounds such a pass while most compute units idle (17^3 and
+// 9^3; on 33^3, 170 blocks, the extra element layers cost more than they save).
 #ifndef HPCR_ZC_MIN_BLOCKS
 #define HPCR_ZC_MIN_BLOCKS 1024
 #endif
 #ifndef HPCR_ZC_ONE_BELOW
-#define HPCR_ZC_ONE_BELOW 256
+#define HPCR_ZC_ONE_BELOW 128
 #endif
 inline int choose_zchunk(int nx, int ny, int nz) {
   const long xy = (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY);

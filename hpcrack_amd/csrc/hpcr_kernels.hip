@@ -1446,31 +1446,27 @@ __device__ __forceinline__ void prolong_body(const LevelGeom& Lf, const LevelGeo
   int x, y, z;
   node_ijk(Lf, qn, x, y, z);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
-  const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
-  const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
-  const double wx = (x & 1) ? 0.5 : 1.0, wy = (y & 1) ? 0.5 : 1.0, wz = (z & 1) ? 0.5 : 1.0;
-  const size_t qf = nidx(Lf, x, y, z);
+  // separable trilinear weights (0.5 on each odd axis); identical to the
+  // reference's averages of 2, 4 or 8 corrections up to rounding order
+  const Prolong8<double> Pr(Lc, x, y, z);
+  const unsigned of = nid32(Lf, x, y, z) * 8u;
   const int dm = apply_bc ? dir_mask(Lf, x, y, z) : 0;
-  const size_t q00 = nidx(Lc, 0, j0, k0), q10 = nidx(Lc, 0, j1, k0), q01 = nidx(Lc, 0, j0, k1),
-               q11 = nidx(Lc, 0, j1, k1);
-  for (int c = 0; c < ncomp; c++) {
-    const double* ec = e + c * Nc;
-    double corr;
-    if (!(x & 1) && !(y & 1) && !(z & 1)) {
-      corr = ec[q00 + i0];
-    } else {
-      // separable trilinear weights (0.5 on each odd axis); identical to the
-      // reference's averages of 2, 4 or 8 corrections up to rounding order
-      const double a00 = ec[q00 + i0] + ((x & 1) ? ec[q00 + i1] : 0.0);
-      const double a10 = (y & 1) ? ec[q10 + i0] + ((x & 1) ? ec[q10 + i1] : 0.0) : 0.0;
-      const double a01 = (z & 1) ? ec[q01 + i0] + ((x & 1) ? ec[q01 + i1] : 0.0) : 0.0;
-      const double a11 = ((y & 1) && (z & 1)) ? ec[q11 + i0] + ((x & 1) ? ec[q11 + i1] : 0.0) : 0.0;
-      corr = wx * wy * wz * (a00 + a10 + a01 + a11);
+  auto comps = [&](auto nc) {
+    constexpr int NC = decltype(nc)::value;
+    double corr[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) corr[c] = Pr(e + c * Nc);
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      double v = ldo(uf + c * Nf, of) + corr[c];
+      if ((dm >> c) & 1) v = (c == 2 && (dm & 8)) ? Ut : 0.0;
+      sto(uf + c * Nf, of, v);
     }
-    double v = uf[c * Nf + qf] + corr;
-    if ((dm >> c) & 1) v = (c == 2 && (dm & 8)) ? Ut : 0.0;
-    uf[c * Nf + qf] = v;
-  }
+  };
+  if (ncomp == 3)
+    comps(std::integral_constant<int, 3>());
+  else
+    comps(std::integral_constant<int, 1>());
 }
 __global__ __launch_bounds__(256) void prolong_kernel(const LevelGeom Lf, const LevelGeom Lc, int ncomp,
                                                       double* __restrict__ uf, const double* __restrict__ e,

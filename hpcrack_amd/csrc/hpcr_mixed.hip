@@ -525,6 +525,24 @@ struct FcgOpA {
     fine_lame(Kn, Gn, dn, e[0], C.small_k, lam, mu);
     mech_element<false, !FOLD>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
   }
+  // each element takes its bottom node plane's interpolated moduli from the
+  // element below (the gradient parts stay per element: carrying them too
+  // leaves this pass one wave per SIMD)
+  using Carry = std::conditional_t<kDelta<double>, FineCarry<double>, void>;
+  template <class C_>
+  __device__ __forceinline__ void element_planes(const LevelGeom& L, const double (&sN)[NSL][NV][PLANE], int s0,
+                                                 int tx, int ty, int ex, int ey, int ez, const State&,
+                                                 const int (&e)[1], double (&f)[3][2][2][2], C_& cr,
+                                                 bool first) const {
+    double u[3][2][2][2], Kn[2][2][2], Gn[2][2][2], dn[2][2][2], lam[2][2][2], mu[2][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) gather8(sN, s0, tx, ty, c, u[c]);
+    gather8(sN, s0, tx, ty, 3, Kn);
+    gather8(sN, s0, tx, ty, 4, Gn);
+    gather8(sN, s0, tx, ty, 5, dn);
+    fine_lame_planes(Kn, Gn, dn, e[0], C.small_k, cr, first, lam, mu);
+    mech_element<false, !FOLD>(u, lam, mu, 0.0, 0.0, sx, sy, sz, tx_, ty_, tz_, f);
+  }
   struct NData {
     double r[3];
     double u[LAG ? 3 : 1], p[LAG ? 3 : 1];
@@ -888,24 +906,17 @@ __device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const Leve
   int x, y, z;
   mix_ijk(Lf, qn, x, y, z);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
-  const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
-  const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
-  const P w = ((x & 1) ? P(0.5) : P(1)) * ((y & 1) ? P(0.5) : P(1)) * ((z & 1) ? P(0.5) : P(1));
-  const size_t qf = nidx(Lf, x, y, z);
+  const Prolong8<P> Pr(Lc, x, y, z);
+  const unsigned of = nid32(Lf, x, y, z) * (unsigned)sizeof(P);
   const int dm = dir_mask(Lf, x, y, z);
-  const size_t q00 = nidx(Lc, 0, j0, k0), q10 = nidx(Lc, 0, j1, k0), q01 = nidx(Lc, 0, j0, k1),
-               q11 = nidx(Lc, 0, j1, k1);
+  P corr[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) corr[c] = Pr(e + c * Nc);
+#pragma unroll
   for (int c = 0; c < 3; c++) {
-    const P* ec = e + c * Nc;
-    const P a00 = ec[q00 + i0] + ((x & 1) ? ec[q00 + i1] : P(0));
-    const P a10 = (y & 1) ? ec[q10 + i0] + ((x & 1) ? ec[q10 + i1] : P(0)) : P(0);
-    const P a01 = (z & 1) ? ec[q01 + i0] + ((x & 1) ? ec[q01 + i1] : P(0)) : P(0);
-    const P a11 = ((y & 1) && (z & 1)) ? ec[q11 + i0] + ((x & 1) ? ec[q11 + i1] : P(0)) : P(0);
-    P corr = w * (a00 + a10 + a01 + a11);
-    if ((dm >> c) & 1) corr = P(0);
-    const size_t o = c * Nf + qf;
-    xf[o] += corr;
-    cf[o] = corr;
+    const P cv = ((dm >> c) & 1) ? P(0) : corr[c];
+    sto(xf + c * Nf, of, ldo(xf + c * Nf, of) + cv);
+    sto(cf + c * Nf, of, cv);
   }
 }
 template <class P>

@@ -100,6 +100,48 @@ __device__ __forceinline__ void sto(T* base, unsigned boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
+// Trilinear prolongation of a coarse correction to fine node (x, y, z)
+// (multigrid_intergrid.cpp:416-520) in its separable form: the coarse
+// values at the corners of the coarse cell, weight 1/2 on each odd axis.  The
+// 8 loads are unconditional (on an even axis the upper corner index equals
+// the lower one); a term that the sum does not take is multiplied by 0 and
+// added, which leaves each sum as it is (up to the sign of an exact zero).
+// The sum's order and rounding are those of the form that skips the terms.
+template <class P>
+struct Prolong8 {
+  unsigned o[2][2][2];  // coarse node index of corner [dz][dy][dx]
+  P fx, fy, fz, w;
+  __device__ __forceinline__ Prolong8(const LevelGeom& Lc, int x, int y, int z) {
+    const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
+    const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) o[dz][dy][dx] = nid32(Lc, dx ? i1 : i0, dy ? j1 : j0, dz ? k1 : k0);
+    fx = (x & 1) ? P(1) : P(0);
+    fy = (y & 1) ? P(1) : P(0);
+    fz = (z & 1) ? P(1) : P(0);
+    w = ((x & 1) ? P(0.5) : P(1)) * ((y & 1) ? P(0.5) : P(1)) * ((z & 1) ? P(0.5) : P(1));
+  }
+  // the correction from one component of the coarse field e (N_c values)
+  __device__ __forceinline__ P operator()(const P* __restrict__ e) const {
+    P v[2][2][2];
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) v[dz][dy][dx] = e[o[dz][dy][dx]];
+    const P a00 = v[0][0][0] + fx * v[0][0][1];
+    const P a10 = fy * (v[0][1][0] + fx * v[0][1][1]);
+    const P a01 = fz * (v[1][0][0] + fx * v[1][0][1]);
+    const P a11 = (fy * fz) * (v[1][1][0] + fx * v[1][1][1]);
+    return w * (a00 + a10 + a01 + a11);
+  }
+};
+
 // Loads and stores through buffer resources.  The instruction takes the
 // base from the resource in scalar registers and a 32-bit byte offset from one
 // vector register, so no per-lane 64-bit address is formed, and the hardware's

@@ -25,7 +25,10 @@
 //   --vf lo,hi          inclusion volume fraction range (default 0.10,0.30)
 //   --slots K           concurrent requests (default 8)
 //   --budget-gb B       HBM admission budget (default 200)
-//   --mixed             mechanical solve in the mixed-precision form
+//   --mixed             mechanical solve in the mixed-precision form (default:
+//                       it reaches the tolerance where the FAS cycle stops at
+//                       its cycle cap)
+//   --fp64              mechanical solve with the reference's FAS cycle
 //   --cheb D[,R[,U]]    mixed form: Chebyshev degree D, smoothing range R and
 //                       safety factor U on the largest eigenvalue (default 0:
 //                       chosen per request from its modulus contrast)
@@ -101,7 +104,7 @@ struct Options {
   double vf_lo = 0.10, vf_hi = 0.30;
   int slots = 8;
   double budget_gb = 200.0;
-  bool mixed = false;
+  bool mixed = true;
   int cheb_degree = 0;
   double cheb_ratio = 6.0, cheb_upper = 1.1;
   int image_threads = 0;
@@ -290,6 +293,7 @@ int main(int argc, char** argv) {
     } else if (s == "--slots") o.slots = std::atoi(next());
     else if (s == "--budget-gb") o.budget_gb = std::atof(next());
     else if (s == "--mixed") o.mixed = true;
+    else if (s == "--fp64") o.mixed = false;
     else if (s == "--cheb") {
       const char* v = next();
       o.cheb_degree = std::atoi(v);

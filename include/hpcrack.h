@@ -193,7 +193,8 @@ int hpcr_phase_field(hpcr_ctx* ctx, int t, double* fd_norm, hpcr_stats* stats);
  * recomputed from u, or after max_iter iterations; the residual is also
  * recomputed every replace_every iterations (0: only at the end).  stats
  * counts outer iterations in `cycles`.  Allocates its FP32 buffers (about
- * 64 B per finest node plus 48 B FP64) on the first call. */
+ * 72 B per finest node plus 24 B FP64; element_mfma another 64 B per finest
+ * element) on the first call. */
 typedef struct hpcr_mixed_params {
   int max_iter;
   double tol;

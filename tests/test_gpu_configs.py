@@ -220,7 +220,7 @@ def test_c5_short_stream_budget_bound(tmp_path, HA, mixed):
     slots would hold up to 10 GB, so admissions wait on memory.  Each
     request's result is identical when it runs alone."""
     base = ["--requests", "24", "--rate", "20", "--sizes", "33,65,129,257", "--seed", "3000", "--budget-gb", "6"]
-    base += ["--mixed"] if mixed else []
+    base += ["--mixed"] if mixed else ["--fp64"]
     say(f"[gpu] C5 short stream, {'mixed' if mixed else 'fp64'}, budget 6 GB")
     mix_sum, mix = serve(base + ["--slots", "8"], tmp_path / "mix.txt")
     say(f"  mix: {mix_sum}")

@@ -31,8 +31,7 @@ def results(path):
 def test_results_identical_alone_and_in_the_mix(tmp_path, mixed):
     require_gpu()
     base = [SERVER, "--requests", "10", "--rate", "400", "--sizes", "17,33", "--seed", "7", "--budget-gb", "50"]
-    if mixed:
-        base.append("--mixed")
+    base.append("--mixed" if mixed else "--fp64")
     runs = {}
     for slots in (4, 1):
         out = tmp_path / f"res_{slots}.txt"
@@ -65,8 +64,7 @@ def test_fused_serving_identical_to_requests_alone(tmp_path, mixed):
     phase-field cycles); every request's result is that of the request alone."""
     require_gpu()
     base = [SERVER, "--requests", "12", "--rate", "400", "--sizes", "17,33", "--seed", "11", "--budget-gb", "50"]
-    if mixed:
-        base.append("--mixed")
+    base.append("--mixed" if mixed else "--fp64")
     runs = {}
     for slots, extra in ((6, ["--fuse", "2000"]), (1, [])):
         out = tmp_path / f"res_{slots}.txt"
