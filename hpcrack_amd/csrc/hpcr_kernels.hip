@@ -889,16 +889,21 @@ void launch_dirichlet_copy(hipStream_t s, const LevelGeom& L, double* f, const d
 // Whole smoother call of a small coarse level in one workgroup
 // (pcg_u, mechanical_pcg.cpp:223-330; pcg_d, phase_field_pcg.cpp:133-214)
 // ---------------------------------------------------------------------------
-// Thread t owns nodes t and t + SMALL_TPB (u, f, r, z, 1/D and p in
+// Thread t owns nodes t and t + (block size) (u, f, r, z, 1/D and p in
 // registers) and element t.  Per operator apply: p -> LDS, barrier, every
 // element writes its 8 corner contributions to LDS, barrier, every node sums
 // the contributions of its (up to) 8 elements in a fixed order.  Reductions
 // are block-wide with a fixed shuffle tree.  The recurrence is that of the
 // two-pass path: p = z + beta p, alpha = r.z / p.Ap, beta = r.z_new / r.z.
 
+//
+// A level of at most SMALL_WAVE_NODES nodes and 64 elements (the
+// 4x4x4-element coarsest level) runs in one wavefront: its barriers and block
+// sums then involve no other wave.
 constexpr int SMALL_TPB = 256;
-constexpr int SMALL_NPT = SMALL_NODES / SMALL_TPB;
-static_assert(SMALL_NODES % SMALL_TPB == 0 && SMALL_ELEMS == SMALL_TPB, "small-level layout");
+constexpr int SMALL_NPT = 2;  // nodes per thread
+constexpr int SMALL_WAVE_NODES = 64 * SMALL_NPT;
+static_assert(SMALL_NODES == SMALL_TPB * SMALL_NPT && SMALL_ELEMS == SMALL_TPB, "small-level layout");
 
 // Block-wide sums, every thread receiving the total: wavefront partials to LDS,
 // one barrier, then every thread adds the partials in wavefront order (the
@@ -948,7 +953,7 @@ struct SmallArgsT {
 using SmallArgs = SmallArgsT<double>;
 
 // TIO: type of the vectors in HBM; the iteration itself is FP64
-template <int NC, class TIO>
+template <int NC, class TIO, int TPB>
 __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
   const LevelGeom& L = a.L;
   __shared__ double sP[NC][SMALL_NODES];
@@ -994,7 +999,7 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
   double xv[SMALL_NPT][NC], fv[SMALL_NPT][NC], iD[SMALL_NPT][NC];
 #pragma unroll
   for (int s = 0; s < SMALL_NPT; s++) {
-    const int q = tid + s * SMALL_TPB;
+    const int q = tid + s * TPB;
     qn[s] = q < N ? q : -1;
     dm[s] = 0;
 #pragma unroll
@@ -1169,17 +1174,31 @@ __device__ __forceinline__ void small_pcg_body(const SmallArgsT<TIO>& a) {
   }
 }
 
-template <int NC, class TIO>
-__global__ __launch_bounds__(SMALL_TPB) void small_pcg_kernel(const SmallArgsT<TIO> a) {
-  small_pcg_body<NC, TIO>(a);
+template <int NC, class TIO, int TPB>
+__global__ __launch_bounds__(TPB) void small_pcg_kernel(const SmallArgsT<TIO> a) {
+  small_pcg_body<NC, TIO, TPB>(a);
 }
-template <int NC, class TIO>
+template <int NC, class TIO, int TPB>
 struct SmallPcgK {
   using Args = SmallArgsT<TIO>;
-  static constexpr int NT = SMALL_TPB, MINW = 1;
-  static dim3 block() { return dim3(SMALL_TPB); }
-  __device__ __forceinline__ static void run(const Args& a, const Blk&) { small_pcg_body<NC, TIO>(a); }
+  static constexpr int NT = TPB, MINW = 1;
+  static dim3 block() { return dim3(TPB); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk&) { small_pcg_body<NC, TIO, TPB>(a); }
 };
+// one wavefront for the smallest levels, else SMALL_TPB threads
+static bool small_wave(const LevelGeom& L) {
+  return (long)L.nx * L.ny * L.nz <= SMALL_WAVE_NODES && (long)(L.nx - 1) * (L.ny - 1) * (L.nz - 1) <= 64;
+}
+template <int NC, class TIO>
+static void launch_small(hipStream_t s, const LevelGeom& L, const SmallArgsT<TIO>& a) {
+  if (small_wave(L)) {
+    if (record<SmallPcgK<NC, TIO, 64>>(dim3(1), a)) return;
+    small_pcg_kernel<NC, TIO, 64><<<dim3(1), dim3(64), 0, s>>>(a);
+  } else {
+    if (record<SmallPcgK<NC, TIO, SMALL_TPB>>(dim3(1), a)) return;
+    small_pcg_kernel<NC, TIO, SMALL_TPB><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
+  }
+}
 
 // coarsest solve of the FP32 preconditioner: zero start, homogeneous
 // Dirichlet values, FP64 iteration on float right-hand side and result
@@ -1195,8 +1214,7 @@ void launch_small_pcg_u_f32(hipStream_t s, const LevelGeom& L, const double* Ke,
   a.Ut = 0.0;
   a.NB = NB;
   a.scal = scal;
-  if (record<SmallPcgK<3, float>>(dim3(1), a)) return;
-  small_pcg_kernel<3, float><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
+  launch_small<3, float>(s, L, a);
 }
 
 bool small_level(const LevelGeom& L) {
@@ -1222,8 +1240,7 @@ void launch_small_pcg_u(hipStream_t s, const LevelGeom& L, const double* Ke, con
   a.Ut = Ut;
   a.NB = NB;
   a.scal = scal;
-  if (record<SmallPcgK<3, double>>(dim3(1), a)) return;
-  small_pcg_kernel<3, double><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
+  launch_small<3, double>(s, L, a);
 }
 
 void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, const double* gc, double lc, double* d,
@@ -1244,8 +1261,7 @@ void launch_small_pcg_d(hipStream_t s, const LevelGeom& L, const double* H, cons
   a.w = w;
   a.NB = NB;
   a.scal = scal;
-  if (record<SmallPcgK<1, double>>(dim3(1), a)) return;
-  small_pcg_kernel<1, double><<<dim3(1), dim3(SMALL_TPB), 0, s>>>(a);
+  launch_small<1, double>(s, L, a);
 }
 
 // ---------------------------------------------------------------------------
