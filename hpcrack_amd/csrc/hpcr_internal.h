@@ -21,7 +21,7 @@ constexpr int TPB = TX * TY;                  // 256 threads
 constexpr int PX = TX + 1;                    // node row length in LDS
 constexpr int PLANE = (TX + 1) * (TY + 1);    // 297 nodes per LDS plane (32x8 tiles)
 static_assert(64 % TX == 0 && PLANE <= 2 * TX * TY, "tile rows must divide the wavefront");
-constexpr int ZCHUNK_MAX = 32;                // node planes per block (fine levels)
+constexpr int ZCHUNK_MAX = 64;                // node planes per block (fine levels)
 // node planes held in LDS by a tile kernel: 3 lets one barrier per layer
 // order the plane and exchange buffers, 2 needs two but less LDS (tuning
 // macros: the FP64 passes, the FP32 preconditioner passes)
@@ -168,21 +168,48 @@ struct PfCoef {
   double rlc;        // 1 / lc
 };
 
-// z-chunk of the tile kernels: the longest chunk that still gives >= 1024
-// blocks, so that small levels are not serialised along z.  A level that
-// gives fewer than HPCR_ZC_ONE_BELOW blocks at two node planes per block
-// takes one: its blocks then march over two element layers instead of three,
-// which is what bounds such a pass while most compute units idle (17^3 and
-// 9^3; on 33^3, 170 blocks, the extra element layers cost more than they save).
+// z-chunk of the tile kernels (node planes per block).
+// Levels of at least ZC_MODEL_TILES tiles per plane (129^3 and up) are bound
+// by throughput: the chunk count n minimises
+//   ceil(blocks(n) / TILE_SLOTS) * (ceil(nz / n) + ZC_BLOCK_LAYERS),
+// the rounds of resident blocks (two 256-thread blocks per CU on 256 CUs, the
+// occupancy of the FP64 tile passes) times the element layers a block marches
+// plus what its start and end cost in layers (the first element layer, the
+// prologue's plane fetches, the reduction).  A last round of a few blocks
+// otherwise leaves most CUs idle: at 257^3, 32-plane chunks give 2997 blocks
+// (eight chunks of 32 planes and one of a single plane) in 6 rounds of 33
+// layers, 43-plane chunks 1998 blocks in 4 rounds of 44.
+// Smaller levels are bound by a block's latency: the longest power-of-two
+// chunk up to ZC_SMALL_MAX that still gives >= HPCR_ZC_MIN_BLOCKS blocks, so
+// that they are not serialised along z, else two planes, or one where that
+// gives fewer than HPCR_ZC_ONE_BELOW blocks: those blocks then march over two
+// element layers instead of three, which is what bounds such a pass while
+// most compute units idle (17^3 and 9^3; on 33^3, 170 blocks, the extra
+// element layers cost more than they save).
 #ifndef HPCR_ZC_MIN_BLOCKS
 #define HPCR_ZC_MIN_BLOCKS 1024
 #endif
 #ifndef HPCR_ZC_ONE_BELOW
 #define HPCR_ZC_ONE_BELOW 128
 #endif
+constexpr int ZC_MODEL_TILES = 64, TILE_SLOTS = 512, ZC_BLOCK_LAYERS = 3, ZC_SMALL_MAX = 32;
 inline int choose_zchunk(int nx, int ny, int nz) {
   const long xy = (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY);
-  for (int zc = ZCHUNK_MAX; zc > 2; zc /= 2)
+  if (xy >= ZC_MODEL_TILES) {
+    long best = -1;
+    int bz = 1;
+    for (int n = (nz + ZCHUNK_MAX - 1) / ZCHUNK_MAX; n <= nz; n++) {
+      const int zc = (nz + n - 1) / n;
+      const long blocks = xy * ((nz + zc - 1) / zc);
+      const long cost = ((blocks + TILE_SLOTS - 1) / TILE_SLOTS) * (zc + ZC_BLOCK_LAYERS);
+      if (best < 0 || cost < best || (cost == best && zc < bz)) {
+        best = cost;
+        bz = zc;
+      }
+    }
+    return bz;
+  }
+  for (int zc = ZC_SMALL_MAX; zc > 2; zc /= 2)
     if (xy * ((nz + zc - 1) / zc) >= HPCR_ZC_MIN_BLOCKS) return zc;
   return xy * ((nz + 1) / 2) < HPCR_ZC_ONE_BELOW ? 1 : 2;
 }
