@@ -1453,30 +1453,39 @@ struct DiffK {
 };
 
 // refine_standard_routine + Dirichlet_BD (multigrid_intergrid.cpp:416-549):
-// trilinear interpolation of the correction e_c, one writer per fine node
+// trilinear interpolation of the correction e_c, one writer per fine node,
+// two x-adjacent fine nodes per thread (ProlongPair)
 __device__ __forceinline__ void prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, int ncomp,
                                              double* __restrict__ uf, const double* __restrict__ e, int apply_bc,
                                              double Ut, const Blk& bk) {
-  const unsigned qn = bk.x * 256 + threadIdx.x;
-  if (qn >= (unsigned)Nof(Lf)) return;
-  int x, y, z;
-  node_ijk(Lf, qn, x, y, z);
+  const unsigned q = bk.x * 256 + threadIdx.x;
+  if (q >= (unsigned)prolong_pairs(Lf)) return;
+  int ip, y, z;
+  bool has1;
+  pair_ijk(Lf, q, ip, has1, y, z);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
   // separable trilinear weights (0.5 on each odd axis); identical to the
   // reference's averages of 2, 4 or 8 corrections up to rounding order
-  const Prolong8<double> Pr(Lc, x, y, z);
+  const ProlongPair<double> Pr(Lc, ip, has1, y, z);
+  const int x = 2 * ip;
   const unsigned of = nid32(Lf, x, y, z) * 8u;
-  const int dm = apply_bc ? dir_mask(Lf, x, y, z) : 0;
+  const int dm0 = apply_bc ? dir_mask(Lf, x, y, z) : 0;
+  const int dm1 = (apply_bc && has1) ? dir_mask(Lf, x + 1, y, z) : 0;
   auto comps = [&](auto nc) {
     constexpr int NC = decltype(nc)::value;
-    double corr[NC];
+    double c0[NC], c1[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) corr[c] = Pr(e + c * Nc);
+    for (int c = 0; c < NC; c++) Pr(e + c * Nc, c0[c], c1[c]);
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      double v = ldo(uf + c * Nf, of) + corr[c];
-      if ((dm >> c) & 1) v = (c == 2 && (dm & 8)) ? Ut : 0.0;
+      double v = ldo(uf + c * Nf, of) + c0[c];
+      if ((dm0 >> c) & 1) v = (c == 2 && (dm0 & 8)) ? Ut : 0.0;
       sto(uf + c * Nf, of, v);
+      if (has1) {
+        double v1 = ldo(uf + c * Nf, of + 8u) + c1[c];
+        if ((dm1 >> c) & 1) v1 = (c == 2 && (dm1 & 8)) ? Ut : 0.0;
+        sto(uf + c * Nf, of + 8u, v1);
+      }
     }
   };
   if (ncomp == 3)
@@ -1509,11 +1518,11 @@ void launch_prolong(hipStream_t s, const LevelGeom& Lf, const LevelGeom& Lc, int
                     const double* uc, const double* ucold, int apply_bc, double Ut, double* scratch) {
   const size_t n = (size_t)ncomp * Nof(Lc);
   if (record<DiffK>(dim3(pw_blocks(n)), {n, uc, ucold, scratch})) {
-    record<ProlongK>(dim3((unsigned)((Nof(Lf) + 255) / 256)), {Lf, Lc, ncomp, uf, scratch, apply_bc, Ut});
+    record<ProlongK>(dim3((unsigned)((prolong_pairs(Lf) + 255) / 256)), {Lf, Lc, ncomp, uf, scratch, apply_bc, Ut});
     return;
   }
   hipLaunchKernelGGL(diff_kernel, dim3(pw_blocks(n)), dim3(256), 0, s, n, uc, ucold, scratch);
-  hipLaunchKernelGGL(prolong_kernel, dim3((unsigned)((Nof(Lf) + 255) / 256)), dim3(256), 0, s, Lf, Lc, ncomp, uf,
+  hipLaunchKernelGGL(prolong_kernel, dim3((unsigned)((prolong_pairs(Lf) + 255) / 256)), dim3(256), 0, s, Lf, Lc, ncomp, uf,
                      scratch, apply_bc, Ut);
 }
 

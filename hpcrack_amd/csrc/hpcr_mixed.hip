@@ -897,26 +897,35 @@ struct MixRestrictK {
 };
 
 // prolongation of the coarse correction: c = P e (zero on Dirichlet DOFs),
-// x_f += c, c kept as the stencil vector of the first post-smoothing step
+// x_f += c, c kept as the stencil vector of the first post-smoothing step;
+// two x-adjacent fine nodes per thread (ProlongPair)
 template <class P>
 __device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, P* __restrict__ xf,
                                                  P* __restrict__ cf, const P* __restrict__ e, const Blk& bk) {
-  const unsigned qn = bk.x * 256 + threadIdx.x;
-  if (qn >= (unsigned)Nof(Lf)) return;
-  int x, y, z;
-  mix_ijk(Lf, qn, x, y, z);
+  const unsigned q = bk.x * 256 + threadIdx.x;
+  if (q >= (unsigned)prolong_pairs(Lf)) return;
+  int ip, y, z;
+  bool has1;
+  pair_ijk(Lf, q, ip, has1, y, z);
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
-  const Prolong8<P> Pr(Lc, x, y, z);
-  const unsigned of = nid32(Lf, x, y, z) * (unsigned)sizeof(P);
-  const int dm = dir_mask(Lf, x, y, z);
-  P corr[3];
+  const ProlongPair<P> Pr(Lc, ip, has1, y, z);
+  const int x = 2 * ip;
+  const unsigned of = nid32(Lf, x, y, z) * (unsigned)sizeof(P), of1 = of + (unsigned)sizeof(P);
+  const int dm0 = dir_mask(Lf, x, y, z);
+  const int dm1 = has1 ? dir_mask(Lf, x + 1, y, z) : 0;
+  P c0[3], c1[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) corr[c] = Pr(e + c * Nc);
+  for (int c = 0; c < 3; c++) Pr(e + c * Nc, c0[c], c1[c]);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const P cv = ((dm >> c) & 1) ? P(0) : corr[c];
-    sto(xf + c * Nf, of, ldo(xf + c * Nf, of) + cv);
-    sto(cf + c * Nf, of, cv);
+    const P cv0 = ((dm0 >> c) & 1) ? P(0) : c0[c];
+    sto(xf + c * Nf, of, ldo(xf + c * Nf, of) + cv0);
+    sto(cf + c * Nf, of, cv0);
+    if (has1) {
+      const P cv1 = ((dm1 >> c) & 1) ? P(0) : c1[c];
+      sto(xf + c * Nf, of1, ldo(xf + c * Nf, of1) + cv1);
+      sto(cf + c * Nf, of1, cv1);
+    }
   }
 }
 template <class P>
@@ -1075,7 +1084,7 @@ void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& L
 }
 template <class P>
 void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, P* xf, P* cf, const P* ec) {
-  const dim3 g((unsigned)((Nof(Lf) + 255) / 256));
+  const dim3 g((unsigned)((prolong_pairs(Lf) + 255) / 256));
   if (record<MixProlongK<P>>(g, {Lf, Lc, xf, cf, ec})) return;
   hipLaunchKernelGGL(mix_prolong_kernel<P>, g, dim3(256), 0, st, Lf, Lc, xf, cf, ec);
 }

@@ -100,33 +100,39 @@ __device__ __forceinline__ void sto(T* base, unsigned boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
-// Trilinear prolongation of a coarse correction to fine node (x, y, z)
-// (multigrid_intergrid.cpp:416-520) in its separable form: the coarse
-// values at the corners of the coarse cell, weight 1/2 on each odd axis.  The
-// 8 loads are unconditional (on an even axis the upper corner index equals
-// the lower one); a term that the sum does not take is multiplied by 0 and
-// added, which leaves each sum as it is (up to the sign of an exact zero).
-// The sum's order and rounding are those of the form that skips the terms.
+// Trilinear prolongation of a coarse correction (multigrid_intergrid.cpp:416-520)
+// in its separable form, for the two fine nodes (2i, y, z) and (2i+1, y, z) of
+// one thread: the coarse values at the corners of each node's coarse cell,
+// weight 1/2 on each odd axis.  The odd node's corners are coarse columns i
+// and i+1, the even node's column i twice, so 8 loads per component serve
+// both nodes.  Every term enters each node's sum: one that the sum does not
+// take (the second column of an even node, the upper corner on an even y or
+// z) is multiplied by 0 and added, which leaves the sum as it is (up to the
+// sign of an exact zero), so the sum's order and rounding are those of the
+// form that skips the terms.  has1 false: the row ends at the even node,
+// whose column-(i+1) loads then read column i again.
 template <class P>
-struct Prolong8 {
-  unsigned o[2][2][2];  // coarse node index of corner [dz][dy][dx]
-  P fx, fy, fz, w;
-  __device__ __forceinline__ Prolong8(const LevelGeom& Lc, int x, int y, int z) {
-    const int i0 = x >> 1, j0 = y >> 1, k0 = z >> 1;
-    const int i1 = i0 + (x & 1), j1 = j0 + (y & 1), k1 = k0 + (z & 1);
+struct ProlongPair {
+  unsigned o[2][2][2];  // coarse node index of [dz][dy][column i, i+1]
+  P fy, fz, w0, w1;     // weights of the even and the odd node
+  __device__ __forceinline__ ProlongPair(const LevelGeom& Lc, int i, bool has1, int y, int z) {
+    const int j0 = y >> 1, k0 = z >> 1;
+    const int i1 = has1 ? i + 1 : i, j1 = j0 + (y & 1), k1 = k0 + (z & 1);
 #pragma unroll
     for (int dz = 0; dz < 2; dz++)
 #pragma unroll
       for (int dy = 0; dy < 2; dy++)
 #pragma unroll
-        for (int dx = 0; dx < 2; dx++) o[dz][dy][dx] = nid32(Lc, dx ? i1 : i0, dy ? j1 : j0, dz ? k1 : k0);
-    fx = (x & 1) ? P(1) : P(0);
+        for (int dx = 0; dx < 2; dx++) o[dz][dy][dx] = nid32(Lc, dx ? i1 : i, dy ? j1 : j0, dz ? k1 : k0);
     fy = (y & 1) ? P(1) : P(0);
     fz = (z & 1) ? P(1) : P(0);
-    w = ((x & 1) ? P(0.5) : P(1)) * ((y & 1) ? P(0.5) : P(1)) * ((z & 1) ? P(0.5) : P(1));
+    const P wy = (y & 1) ? P(0.5) : P(1), wz = (z & 1) ? P(0.5) : P(1);
+    w0 = (P(1) * wy) * wz;
+    w1 = (P(0.5) * wy) * wz;
   }
-  // the correction from one component of the coarse field e (N_c values)
-  __device__ __forceinline__ P operator()(const P* __restrict__ e) const {
+  // the corrections of the even (c0) and the odd (c1) node from one
+  // component of the coarse field e
+  __device__ __forceinline__ void operator()(const P* __restrict__ e, P& c0, P& c1) const {
     P v[2][2][2];
 #pragma unroll
     for (int dz = 0; dz < 2; dz++)
@@ -134,13 +140,30 @@ struct Prolong8 {
       for (int dy = 0; dy < 2; dy++)
 #pragma unroll
         for (int dx = 0; dx < 2; dx++) v[dz][dy][dx] = e[o[dz][dy][dx]];
-    const P a00 = v[0][0][0] + fx * v[0][0][1];
-    const P a10 = fy * (v[0][1][0] + fx * v[0][1][1]);
-    const P a01 = fz * (v[1][0][0] + fx * v[1][0][1]);
-    const P a11 = (fy * fz) * (v[1][1][0] + fx * v[1][1][1]);
-    return w * (a00 + a10 + a01 + a11);
+    auto node = [&](P fx, int cx) {
+      const P a00 = v[0][0][0] + fx * v[0][0][cx];
+      const P a10 = fy * (v[0][1][0] + fx * v[0][1][cx]);
+      const P a01 = fz * (v[1][0][0] + fx * v[1][0][cx]);
+      const P a11 = (fy * fz) * (v[1][1][0] + fx * v[1][1][cx]);
+      return a00 + a10 + a01 + a11;
+    };
+    c0 = w0 * node(P(0), 0);
+    c1 = w1 * node(P(1), 1);
   }
 };
+// threads of a pairwise prolongation onto level Lf, and the pair of thread q:
+// even node x = 2 ip of row (y, z), the odd one x + 1 when has1
+__host__ __device__ __forceinline__ size_t prolong_pairs(const LevelGeom& Lf) {
+  return (size_t)((Lf.nx + 1) / 2) * Lf.ny * Lf.nz;
+}
+__device__ __forceinline__ void pair_ijk(const LevelGeom& Lf, unsigned q, int& ip, bool& has1, int& y, int& z) {
+  const unsigned px = (unsigned)(Lf.nx + 1) / 2;
+  const unsigned row = q / px;
+  ip = (int)(q - row * px);
+  z = (int)(row / (unsigned)Lf.ny);
+  y = (int)(row - (unsigned)z * (unsigned)Lf.ny);
+  has1 = 2 * ip + 1 < Lf.nx;
+}
 
 // Loads and stores through buffer resources.  The instruction takes the
 // base from the resource in scalar registers and a 32-bit byte offset from one
@@ -552,10 +575,13 @@ __device__ __forceinline__ void gp_mp(const T (&v)[2][2][2], T (&o)[2][2][2]) {
         for (int c = 0; c < 2; c++) o[a][b][c] = T(0);
   }
 }
-// (1-k)(1-d)^2 + k with d clamped to [0,1] (material_gauss_point.cpp:50-56)
+// (1-k)(1-d)^2 + k with d clamped to [0,1] (material_gauss_point.cpp:50-56).
+// The clamp is a max and a min: for every d that is not a NaN it equals the
+// reference's two compares (d < 0 -> 0, d > 1 -> 1; a d of -0 squares to the
+// same factor), in two instructions where the compares and selects take six.
 template <class T>
 __device__ __forceinline__ T degr(T dg, T k) {
-  dg = dg < T(0) ? T(0) : (dg > T(1) ? T(1) : dg);
+  dg = fmin(fmax(dg, T(0)), T(1));
   return (T(1) - k) * (T(1) - dg) * (T(1) - dg) + k;
 }
 
