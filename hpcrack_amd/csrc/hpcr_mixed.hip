@@ -897,34 +897,62 @@ struct MixRestrictK {
 };
 
 // prolongation of the coarse correction: c = P e (zero on Dirichlet DOFs),
-// x_f += c, c kept as the stencil vector of the first post-smoothing step;
-// two x-adjacent fine nodes per thread (ProlongPair)
+// x_f += c, c kept as the stencil vector of the first post-smoothing step.
+// FP64: two x-adjacent fine nodes per thread (ProlongPair), which halves the
+// coarse loads.  FP32: one node per thread (the pair's value for that node),
+// since there the pairwise form's half-width stores cost more than the loads
+// it saves (measured 160 -> 186 us per 257^3 prolongation).
+template <class P>
+constexpr bool kProlongPairs = std::is_same<P, double>::value;
+template <class P>
+__host__ __device__ __forceinline__ size_t mix_prolong_threads(const LevelGeom& Lf) {
+  return kProlongPairs<P> ? prolong_pairs(Lf) : Nof(Lf);
+}
 template <class P>
 __device__ __forceinline__ void mix_prolong_body(const LevelGeom& Lf, const LevelGeom& Lc, P* __restrict__ xf,
                                                  P* __restrict__ cf, const P* __restrict__ e, const Blk& bk) {
   const unsigned q = bk.x * 256 + threadIdx.x;
-  if (q >= (unsigned)prolong_pairs(Lf)) return;
-  int ip, y, z;
-  bool has1;
-  pair_ijk(Lf, q, ip, has1, y, z);
+  if (q >= (unsigned)mix_prolong_threads<P>(Lf)) return;
   const size_t Nc = Nof(Lc), Nf = Nof(Lf);
-  const ProlongPair<P> Pr(Lc, ip, has1, y, z);
-  const int x = 2 * ip;
-  const unsigned of = nid32(Lf, x, y, z) * (unsigned)sizeof(P), of1 = of + (unsigned)sizeof(P);
-  const int dm0 = dir_mask(Lf, x, y, z);
-  const int dm1 = has1 ? dir_mask(Lf, x + 1, y, z) : 0;
-  P c0[3], c1[3];
+  if constexpr (kProlongPairs<P>) {
+    int ip, y, z;
+    bool has1;
+    pair_ijk(Lf, q, ip, has1, y, z);
+    const ProlongPair<P> Pr(Lc, ip, has1, y, z);
+    const int x = 2 * ip;
+    const unsigned of = nid32(Lf, x, y, z) * (unsigned)sizeof(P), of1 = of + (unsigned)sizeof(P);
+    const int dm0 = dir_mask(Lf, x, y, z);
+    const int dm1 = has1 ? dir_mask(Lf, x + 1, y, z) : 0;
+    P c0[3], c1[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) Pr(e + c * Nc, c0[c], c1[c]);
+    for (int c = 0; c < 3; c++) Pr(e + c * Nc, c0[c], c1[c]);
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const P cv0 = ((dm0 >> c) & 1) ? P(0) : c0[c];
-    sto(xf + c * Nf, of, ldo(xf + c * Nf, of) + cv0);
-    sto(cf + c * Nf, of, cv0);
-    if (has1) {
-      const P cv1 = ((dm1 >> c) & 1) ? P(0) : c1[c];
-      sto(xf + c * Nf, of1, ldo(xf + c * Nf, of1) + cv1);
-      sto(cf + c * Nf, of1, cv1);
+    for (int c = 0; c < 3; c++) {
+      const P cv0 = ((dm0 >> c) & 1) ? P(0) : c0[c];
+      sto(xf + c * Nf, of, ldo(xf + c * Nf, of) + cv0);
+      sto(cf + c * Nf, of, cv0);
+      if (has1) {
+        const P cv1 = ((dm1 >> c) & 1) ? P(0) : c1[c];
+        sto(xf + c * Nf, of1, ldo(xf + c * Nf, of1) + cv1);
+        sto(cf + c * Nf, of1, cv1);
+      }
+    }
+  } else {
+    int x, y, z;
+    mix_ijk(Lf, q, x, y, z);
+    // an odd x is the odd node of pair x >> 1; an even x the even node of a
+    // pair whose column-(i+1) loads read column i (has1 false)
+    const bool odd = x & 1;
+    const ProlongPair<P> Pr(Lc, x >> 1, odd, y, z);
+    const unsigned of = nid32(Lf, x, y, z) * (unsigned)sizeof(P);
+    const int dm = dir_mask(Lf, x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      P c0, c1;
+      Pr(e + c * Nc, c0, c1);
+      const P cv = ((dm >> c) & 1) ? P(0) : (odd ? c1 : c0);
+      sto(xf + c * Nf, of, ldo(xf + c * Nf, of) + cv);
+      sto(cf + c * Nf, of, cv);
     }
   }
 }
@@ -1084,7 +1112,7 @@ void launch_mix_restrict(hipStream_t st, const LevelGeom& Lc, const LevelGeom& L
 }
 template <class P>
 void launch_mix_prolong(hipStream_t st, const LevelGeom& Lf, const LevelGeom& Lc, P* xf, P* cf, const P* ec) {
-  const dim3 g((unsigned)((prolong_pairs(Lf) + 255) / 256));
+  const dim3 g((unsigned)((mix_prolong_threads<P>(Lf) + 255) / 256));
   if (record<MixProlongK<P>>(g, {Lf, Lc, xf, cf, ec})) return;
   hipLaunchKernelGGL(mix_prolong_kernel<P>, g, dim3(256), 0, st, Lf, Lc, xf, cf, ec);
 }
