@@ -22,6 +22,34 @@ constexpr int PX = TX + 1;                    // node row length in LDS
 constexpr int PLANE = (TX + 1) * (TY + 1);    // 297 nodes per LDS plane (32x8 tiles)
 static_assert(64 % TX == 0 && PLANE <= 2 * TX * TY, "tile rows must divide the wavefront");
 constexpr int ZCHUNK_MAX = 64;                // node planes per block (fine levels)
+// Split-output tiles of the finest level's fused PCG pass A: 64x4 elements
+// per 256-thread block, a wave per 64-element row, every element computed
+// once.  A tile writes all nodes of its 65x5 node region: the 64x4 at its
+// elements' lower corners to w, its right column, top row and their corner
+// (shared with the tiles to its right and above) to side arrays, which pass B
+// adds in a fixed order.
+constexpr int STX = 64, STY = 4;
+constexpr int SPX = STX + 1;                  // node row length in LDS
+constexpr int SPLANE = (STX + 1) * (STY + 1); // 325 nodes per LDS plane
+static_assert(STX * STY == TPB && SPLANE <= 2 * TPB, "one element per thread, two plane nodes at most");
+inline int split_tiles_x(int nx) { return (nx - 1 + STX - 1) / STX; }
+inline int split_tiles_y(int ny) { return (ny - 1 + STY - 1) / STY; }
+// The side arrays of a level with split pass A, 3 components each, one after
+// the other in one allocation ws (element offsets 0, 3 nr, 3 (nr + nt)):
+//   right columns x = 64 (kx + 1):    [c][z][y][kx],  nr = ntx * ny * nz
+//   top rows y = 4 (ky + 1):          [c][z][ky][x],  nt = nx * nty * nz
+//   their corners:                    [c][z][ky][kx], nc = ntx * nty * nz
+// (only entries on the grid are written and read).  Byte offsets into ws fit
+// in 32 bits as a level's own do: 3 (nr + nt + nc) < N on the levels that
+// split (nr ~ N/64, nt ~ N/4), and a level holds at most MAX_LEVEL_NODES
+// nodes.
+struct SplitW {
+  double* ws;
+  int ntx, nty;
+  int zc;  // node planes per block (the rounds model for the split tile count)
+  unsigned nr, nt, nc;
+  __host__ __device__ size_t count() const { return 3 * ((size_t)nr + nt + nc); }
+};
 // node planes held in LDS by a tile kernel: 3 lets one barrier per layer
 // order the plane and exchange buffers, 2 needs two but less LDS (tuning
 // macros: the FP64 passes, the FP32 preconditioner passes)
@@ -193,28 +221,46 @@ struct PfCoef {
 #define HPCR_ZC_ONE_BELOW 128
 #endif
 constexpr int ZC_MODEL_TILES = 64, TILE_SLOTS = 512, ZC_BLOCK_LAYERS = 3, ZC_SMALL_MAX = 32;
-inline int choose_zchunk(int nx, int ny, int nz) {
-  const long xy = (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY);
-  if (xy >= ZC_MODEL_TILES) {
-    long best = -1;
-    int bz = 1;
-    for (int n = (nz + ZCHUNK_MAX - 1) / ZCHUNK_MAX; n <= nz; n++) {
-      const int zc = (nz + n - 1) / n;
-      const long blocks = xy * ((nz + zc - 1) / zc);
-      const long cost = ((blocks + TILE_SLOTS - 1) / TILE_SLOTS) * (zc + ZC_BLOCK_LAYERS);
-      if (best < 0 || cost < best || (cost == best && zc < bz)) {
-        best = cost;
-        bz = zc;
-      }
+inline long halo_tiles(int nx, int ny) { return (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY); }
+// the finest level's pass A runs on split-output tiles in the same regime
+inline bool split_pass_a(int nx, int ny) { return halo_tiles(nx, ny) >= ZC_MODEL_TILES; }
+// the chunk of the rounds model for xy tiles per plane
+inline int model_zchunk(long xy, int nz) {
+  long best = -1;
+  int bz = 1;
+  for (int n = (nz + ZCHUNK_MAX - 1) / ZCHUNK_MAX; n <= nz; n++) {
+    const int zc = (nz + n - 1) / n;
+    const long blocks = xy * ((nz + zc - 1) / zc);
+    const long cost = ((blocks + TILE_SLOTS - 1) / TILE_SLOTS) * (zc + ZC_BLOCK_LAYERS);
+    if (best < 0 || cost < best || (cost == best && zc < bz)) {
+      best = cost;
+      bz = zc;
     }
-    return bz;
   }
+  return bz;
+}
+inline int choose_zchunk(int nx, int ny, int nz) {
+  const long xy = halo_tiles(nx, ny);
+  if (xy >= ZC_MODEL_TILES) return model_zchunk(xy, nz);
   for (int zc = ZC_SMALL_MAX; zc > 2; zc /= 2)
     if (xy * ((nz + zc - 1) / zc) >= HPCR_ZC_MIN_BLOCKS) return zc;
   return xy * ((nz + 1) / 2) < HPCR_ZC_ONE_BELOW ? 1 : 2;
 }
+inline SplitW split_dims(int nx, int ny, int nz) {
+  SplitW S{};
+  S.ntx = split_tiles_x(nx);
+  S.nty = split_tiles_y(ny);
+  S.nr = (unsigned)S.ntx * (unsigned)ny * (unsigned)nz;
+  S.nt = (unsigned)nx * (unsigned)S.nty * (unsigned)nz;
+  S.nc = (unsigned)S.ntx * (unsigned)S.nty * (unsigned)nz;
+  S.zc = model_zchunk((long)S.ntx * S.nty, nz);
+  return S;
+}
 inline long tile_blocks(int nx, int ny, int nz, int zc) {
   return (long)((nx + OWNX - 1) / OWNX) * ((ny + OWNY - 1) / OWNY) * ((nz + zc - 1) / zc);
+}
+inline long split_blocks(const LevelGeom& L, const SplitW& S) {
+  return (long)S.ntx * S.nty * ((L.nz + S.zc - 1) / S.zc);
 }
 
 // ---- launchers (hpcr_kernels.hip); all asynchronous on `s` -------------
@@ -226,15 +272,19 @@ void launch_mech_resid(hipStream_t s, const LevelGeom& L, const MechCoef& C, con
                        double* r, double* z, const double* invD, int t0fine, const Reduce& R);
 // blocks of the pass-B launches over n nodes (sizes the reduction partials)
 int passb_blocks(size_t n);
-// p_new = z + beta p_old (first: p_new = z); w = A p_new; p.w -> alpha
+// p_new = z + beta p_old (first: p_new = z); w = A p_new; p.w -> alpha.
+// S (finest level, with side arrays): split-output tiles, w in parts that
+// pass B (given the same S) adds up
 void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* z,
-                       const double* p_old, double* p_new, double* w, int first, const Reduce& R);
+                       const double* p_old, double* p_new, double* w, int first, const Reduce& R,
+                       const SplitW* S = nullptr);
 // r -= alpha w; mask r; z = r*invD; r.z, r.r -> beta, rz, rr.  x updates:
 // np == 0 keeps alpha in scal[S_AH + slot]; np >= 1 closes a group,
 // u = fma(alpha_m, P.p[m], u) for m < np, alpha_m from scal[S_AH + m] and
 // the last one the current alpha.
 void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
-                       const double* w, const double* invD, int np, int slot, const Reduce& R);
+                       const double* w, const double* invD, int np, int slot, const Reduce& R,
+                       const SplitW* S = nullptr);
 // invD = 1 / diag(A); zero entries counted into scal[S_ZERO]; diag optionally kept
 void launch_mech_diag(hipStream_t s, const LevelGeom& L, const MechCoef& C, double* diag_or_null,
                       double* invD, const Reduce& R);

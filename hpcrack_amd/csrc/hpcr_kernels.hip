@@ -197,8 +197,8 @@ struct MechOp {
   // finest-level operator passes: each element takes its bottom node plane's
   // interpolated moduli and in-plane gradient parts from the element below
   using Carry = std::conditional_t<FINE && MODE != M_DIAG && kDelta<double>, FineCarry<double>, void>;
-  template <class C_>
-  __device__ __forceinline__ void element_planes(const LevelGeom& L, const double (&sN)[NSL][NV][PLANE], int s0,
+  template <class C_, int PL>
+  __device__ __forceinline__ void element_planes(const LevelGeom& L, const double (&sN)[NSL][NV][PL], int s0,
                                                  int tx, int ty, int ex, int ey, int ez, const State& st,
                                                  const TE (&e)[NE], double (&f)[3][2][2][2], C_& cr,
                                                  bool first) const {
@@ -268,6 +268,37 @@ struct MechOp {
         if (out2) sto(out2 + c * N, ob, v[c]);
         sto(out + c * N, ob, 1.0 / v[c]);
         acc.a[0] += (v[c] == 0.0) ? 1.0 : 0.0;
+      }
+    }
+  }
+
+  // split-output pass A (tile_body_split): node (i, j, k) of the tile's own
+  // region, written to p_new and w as by epilogue()
+  __device__ __forceinline__ void epilogue_split(const LevelGeom& L, const double (&p)[3], int i, int j, int k,
+                                                 const double (&v)[3], Acc& acc) const {
+    static_assert(MODE == M_PCGA, "split outputs are pass A's");
+    const unsigned ob = nid32(L, i, j, k) * 8u;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      sto(out2 + c * N, ob, p[c]);
+      sto(out + c * N, ob, v[c]);
+      acc.a[0] += p[c] * v[c];
+    }
+  }
+  // a side node's part, at byte offset so of the side arrays (components
+  // scomp bytes apart); where no tile has the node in its own region (own_p:
+  // the last node column or row of the grid) also p_new, and w = 0, at the
+  // node's byte offset ob, so that pass B reads w at every node
+  __device__ __forceinline__ void side(const SplitW& S, unsigned so, unsigned scomp, const double (&p)[3],
+                                       const double (&v)[3], bool own_p, unsigned ob, Acc& acc) const {
+    static_assert(MODE == M_PCGA, "split outputs are pass A's");
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      sto(S.ws, so + c * scomp, v[c]);
+      acc.a[0] += p[c] * v[c];
+      if (own_p) {
+        sto(out2 + c * N, ob, p[c]);
+        sto(out + c * N, ob, 0.0);
       }
     }
   }
@@ -460,30 +491,40 @@ static MechOp<MODE, FINE, FOLD> mech_op(const LevelGeom& L, const MechCoef& C) {
 
 template <int MODE, bool FINE, bool FOLD>
 static void run_mech_op(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, const double* invD,
-                        const double* p_old, double* out, double* out2, double* out3, int flag, const Reduce& R) {
+                        const double* p_old, double* out, double* out2, double* out3, int flag, const Reduce& R,
+                        const SplitW* S) {
   auto op = mech_op<MODE, FINE, FOLD>(L, C);
   op.x = x; op.invD = invD; op.p_old = p_old; op.out = out; op.out2 = out2; op.out3 = out3;
   op.flag = flag; op.R = R;
   using Op = MechOp<MODE, FINE, FOLD>;
+  if constexpr (MODE == M_PCGA && FINE) {
+    if (S) {
+      const dim3 g((unsigned)split_blocks(L, *S));
+      if (record<TileSplitK<Op>>(g, {L, op, *S})) return;
+      tile_split_kernel<Op><<<g, dim3(TPB), 0, s>>>(L, op, *S);
+      return;
+    }
+  }
   if (record<TileK<Op>>(tile_grid(L), {L, op})) return;
   tile_kernel<Op><<<tile_grid(L), dim3(TPB), 0, s>>>(L, op);
 }
 
 template <int MODE>
 static void run_mech(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* x, const double* invD,
-                     const double* p_old, double* out, double* out2, double* out3, int flag, const Reduce& R) {
+                     const double* p_old, double* out, double* out2, double* out3, int flag, const Reduce& R,
+                     const SplitW* S = nullptr) {
   constexpr bool CAN_FOLD = (MODE != M_DIAG);
   const bool fold = CAN_FOLD && cubic_grid(L);
   if (C.fine) {
     if (fold)
-      run_mech_op<MODE, true, CAN_FOLD>(s, L, C, x, invD, p_old, out, out2, out3, flag, R);
+      run_mech_op<MODE, true, CAN_FOLD>(s, L, C, x, invD, p_old, out, out2, out3, flag, R, S);
     else
-      run_mech_op<MODE, true, false>(s, L, C, x, invD, p_old, out, out2, out3, flag, R);
+      run_mech_op<MODE, true, false>(s, L, C, x, invD, p_old, out, out2, out3, flag, R, S);
   } else {
     if (fold)
-      run_mech_op<MODE, false, CAN_FOLD>(s, L, C, x, invD, p_old, out, out2, out3, flag, R);
+      run_mech_op<MODE, false, CAN_FOLD>(s, L, C, x, invD, p_old, out, out2, out3, flag, R, S);
     else
-      run_mech_op<MODE, false, false>(s, L, C, x, invD, p_old, out, out2, out3, flag, R);
+      run_mech_op<MODE, false, false>(s, L, C, x, invD, p_old, out, out2, out3, flag, R, S);
   }
 }
 
@@ -496,8 +537,9 @@ void launch_mech_resid(hipStream_t s, const LevelGeom& L, const MechCoef& C, con
   run_mech<M_RESID>(s, L, C, u, invD, nullptr, r, f, z, t0fine, R);
 }
 void launch_mech_pcg_a(hipStream_t s, const LevelGeom& L, const MechCoef& C, const double* z,
-                       const double* p_old, double* p_new, double* w, int first, const Reduce& R) {
-  run_mech<M_PCGA>(s, L, C, z, nullptr, p_old, w, p_new, nullptr, first, R);
+                       const double* p_old, double* p_new, double* w, int first, const Reduce& R,
+                       const SplitW* S) {
+  run_mech<M_PCGA>(s, L, C, z, nullptr, p_old, w, p_new, nullptr, first, R, (S && S->ws && C.fine) ? S : nullptr);
 }
 void launch_mech_diag(hipStream_t s, const LevelGeom& L, const MechCoef& C, double* diag_or_null,
                       double* invD, const Reduce& R) {
@@ -617,11 +659,39 @@ struct GroupAlpha {
   }
 };
 
-template <bool LDSF, int NP>
+// w of node (i, j, k), component c, from the parts of a split-output pass A
+// (SplitW), added in a fixed order: the part of the tile whose own region
+// holds the node, of the tile below (top row), of the tile to its left (right
+// column), of the tile below-left (corner).  A part is taken where that
+// tile exists: none has the last node column x = 64 ntx or row y = 4 nty in
+// its own region, nor a right column in row 4 nty, nor a top row in column
+// 64 ntx.
+__device__ __forceinline__ double split_w(const LevelGeom& L, const SplitW& S, const double* __restrict__ w, int c,
+                                          size_t o, unsigned i, unsigned j, unsigned k) {
+  const bool col = i < (unsigned)(STX * S.ntx), row = j < (unsigned)(STY * S.nty);
+  const bool xe = i > 0 && i % STX == 0, ye = j > 0 && j % STY == 0;
+  const unsigned kx = i / STX - 1, ky = j / STY - 1;
+  // the own-region part is loaded unconditionally (pass A stores 0 at the
+  // nodes past every own region); the right-column and corner parts, which
+  // one lane in 64 needs, without a branch: an absent one from an offset past
+  // the side arrays' resource, which reads 0 (and v + 0 = v)
+  const auto rs = brsrc(S.ws, S.count() * 8);
+  const unsigned ot = (3u * S.nr + c * S.nt + (k * (unsigned)S.nty + ky) * (unsigned)L.nx + i) * 8u;
+  const unsigned orr = (c * S.nr + (k * (unsigned)L.ny + j) * (unsigned)S.ntx + kx) * 8u;
+  const unsigned oc = (3u * (S.nr + S.nt) + c * S.nc + (k * (unsigned)S.nty + ky) * (unsigned)S.ntx + kx) * 8u;
+  const double v = PB_LD(w[o]);
+  double t = 0.0;
+  if (ye && col) t = ldo(S.ws, ot);  // whole rows: a branch that waves mostly take or skip together
+  const double r = bld(rs, (xe && row) ? orr : BOFF_NONE);
+  const double q = bld(rs, (xe && ye) ? oc : BOFF_NONE);
+  return ((v + t) + r) + q;
+}
+
+template <bool LDSF, int NP, bool WPARTS = false>
 __device__ __forceinline__ void mech_pcg_b_body(const LevelGeom& L, double* __restrict__ u, double* __restrict__ r,
                                                 double* __restrict__ z, const PGroup& P,
                                                 const double* __restrict__ w, const double* __restrict__ invD,
-                                                int slot, const Reduce& R, const Blk& bk) {
+                                                int slot, const Reduce& R, const SplitW& S, const Blk& bk) {
   __shared__ unsigned char sflags[LDSF ? DIRF_MAX : 1];
   const DirFlags F{sflags};
   if constexpr (LDSF) {
@@ -637,20 +707,29 @@ __device__ __forceinline__ void mech_pcg_b_body(const LevelGeom& L, double* __re
   auto node = [&](unsigned q) {
     const unsigned k = q / plane;
     int dm = 0;
-    if constexpr (LDSF) {
+    unsigned i = 0, j = 0;
+    if constexpr (LDSF || WPARTS) {
       const unsigned rem = q - k * plane;
-      const unsigned j = rem / (unsigned)L.nx, i = rem - j * (unsigned)L.nx;
+      j = rem / (unsigned)L.nx;
+      i = rem - j * (unsigned)L.nx;
+    }
+    if constexpr (LDSF) {
       dm = F.mask(L, (int)i, (int)j, (int)k);
     } else if (L.fz[k]) {
       const unsigned rem = q - k * plane;
-      const unsigned j = rem / (unsigned)L.nx, i = rem - j * (unsigned)L.nx;
-      dm = dir_mask(L, (int)i, (int)j, (int)k);
+      const unsigned jj = rem / (unsigned)L.nx, ii = rem - jj * (unsigned)L.nx;
+      dm = dir_mask(L, (int)ii, (int)jj, (int)k);
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t o = (size_t)c * N + q;
       if constexpr (NP > 0) PB_ST(ga.apply(P, o, PB_LD(u[o])), u[o]);
-      double rv = PB_LD(r[o]) - alpha * PB_LD(w[o]);
+      double wv;
+      if constexpr (WPARTS)
+        wv = split_w(L, S, w, c, o, i, j, k);
+      else
+        wv = PB_LD(w[o]);
+      double rv = PB_LD(r[o]) - alpha * wv;
       if ((dm >> c) & 1) rv = 0.0;
       PB_ST(rv, r[o]);
       const double zv = rv * PB_LD(invD[o]);
@@ -681,13 +760,13 @@ __device__ __forceinline__ void mech_pcg_b_body(const LevelGeom& L, double* __re
 #endif
 }
 
-template <bool LDSF, int NP>
+template <bool LDSF, int NP, bool WPARTS>
 __global__ __launch_bounds__(PW_TPB) void mech_pcg_b_kernel(const LevelGeom L, double* __restrict__ u,
                                                             double* __restrict__ r, double* __restrict__ z,
                                                             const PGroup P, const double* __restrict__ w,
                                                             const double* __restrict__ invD, int slot,
-                                                            const Reduce R) {
-  mech_pcg_b_body<LDSF, NP>(L, u, r, z, P, w, invD, slot, R, Blk::hw());
+                                                            const Reduce R, const SplitW S) {
+  mech_pcg_b_body<LDSF, NP, WPARTS>(L, u, r, z, P, w, invD, slot, R, S, Blk::hw());
 }
 
 struct PassBArgs {
@@ -701,14 +780,15 @@ struct PassBArgs {
   const double* invD;
   int slot;
   Reduce R;
+  SplitW S;   // mechanics: the parts of a split-output pass A (S.ws null: w whole)
 };
-template <bool LDSF, int NP>
+template <bool LDSF, int NP, bool WPARTS>
 struct MechPassBK {
   using Args = PassBArgs;
   static constexpr int NT = PW_TPB, MINW = 1;
   static dim3 block() { return dim3(PW_TPB); }
   __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
-    mech_pcg_b_body<LDSF, NP>(a.L, a.x, a.r, a.z, a.P, a.w, a.invD, a.slot, a.R, bk);
+    mech_pcg_b_body<LDSF, NP, WPARTS>(a.L, a.x, a.r, a.z, a.P, a.w, a.invD, a.slot, a.R, a.S, bk);
   }
 };
 
@@ -756,34 +836,46 @@ struct PfPassBK {
   }
 };
 
-template <bool LDSF, int NP>
+template <bool LDSF, int NP, bool WPARTS>
 static void mech_pcg_b_np(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
-                          const double* w, const double* invD, int slot, const Reduce& R) {
+                          const double* w, const double* invD, int slot, const Reduce& R, const SplitW& S) {
 #ifndef HPCR_PASSB_SPLIT
-  if (record<MechPassBK<LDSF, NP>>(dim3(passb_blocks(Nof(L))), PassBArgs{L, Nof(L), u, r, z, P, w, invD, slot, R}))
+  if (record<MechPassBK<LDSF, NP, WPARTS>>(dim3(passb_blocks(Nof(L))),
+                                          PassBArgs{L, Nof(L), u, r, z, P, w, invD, slot, R, S}))
     return;
 #endif
-  hipLaunchKernelGGL((mech_pcg_b_kernel<LDSF, NP>), dim3(passb_blocks(Nof(L))), dim3(PW_TPB), 0, s, L, u, r, z, P, w,
-                     invD, slot, R);
+  hipLaunchKernelGGL((mech_pcg_b_kernel<LDSF, NP, WPARTS>), dim3(passb_blocks(Nof(L))), dim3(PW_TPB), 0, s, L, u, r,
+                     z, P, w, invD, slot, R, S);
 }
-template <bool LDSF>
+template <bool LDSF, bool WPARTS>
 static void mech_pcg_b_ldsf(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
-                            const double* w, const double* invD, int np, int slot, const Reduce& R) {
+                            const double* w, const double* invD, int np, int slot, const Reduce& R,
+                            const SplitW& S) {
   static_assert(CG_GROUP == 4, "one instantiation per group position");
   switch (np) {
-    case 0: mech_pcg_b_np<LDSF, 0>(s, L, u, r, z, P, w, invD, slot, R); break;
-    case 1: mech_pcg_b_np<LDSF, 1>(s, L, u, r, z, P, w, invD, slot, R); break;
-    case 2: mech_pcg_b_np<LDSF, 2>(s, L, u, r, z, P, w, invD, slot, R); break;
-    case 3: mech_pcg_b_np<LDSF, 3>(s, L, u, r, z, P, w, invD, slot, R); break;
-    default: mech_pcg_b_np<LDSF, 4>(s, L, u, r, z, P, w, invD, slot, R); break;
+    case 0: mech_pcg_b_np<LDSF, 0, WPARTS>(s, L, u, r, z, P, w, invD, slot, R, S); break;
+    case 1: mech_pcg_b_np<LDSF, 1, WPARTS>(s, L, u, r, z, P, w, invD, slot, R, S); break;
+    case 2: mech_pcg_b_np<LDSF, 2, WPARTS>(s, L, u, r, z, P, w, invD, slot, R, S); break;
+    case 3: mech_pcg_b_np<LDSF, 3, WPARTS>(s, L, u, r, z, P, w, invD, slot, R, S); break;
+    default: mech_pcg_b_np<LDSF, 4, WPARTS>(s, L, u, r, z, P, w, invD, slot, R, S); break;
   }
 }
 void launch_mech_pcg_b(hipStream_t s, const LevelGeom& L, double* u, double* r, double* z, const PGroup& P,
-                       const double* w, const double* invD, int np, int slot, const Reduce& R) {
-  if (L.nx + L.ny + L.nz <= DIRF_MAX)
-    mech_pcg_b_ldsf<true>(s, L, u, r, z, P, w, invD, np, slot, R);
-  else
-    mech_pcg_b_ldsf<false>(s, L, u, r, z, P, w, invD, np, slot, R);
+                       const double* w, const double* invD, int np, int slot, const Reduce& R, const SplitW* S) {
+  const bool parts = S && S->ws;
+  const SplitW none{};
+  const SplitW& SW = parts ? *S : none;
+  if (L.nx + L.ny + L.nz <= DIRF_MAX) {
+    if (parts)
+      mech_pcg_b_ldsf<true, true>(s, L, u, r, z, P, w, invD, np, slot, R, SW);
+    else
+      mech_pcg_b_ldsf<true, false>(s, L, u, r, z, P, w, invD, np, slot, R, SW);
+  } else {
+    if (parts)
+      mech_pcg_b_ldsf<false, true>(s, L, u, r, z, P, w, invD, np, slot, R, SW);
+    else
+      mech_pcg_b_ldsf<false, false>(s, L, u, r, z, P, w, invD, np, slot, R, SW);
+  }
 #ifdef HPCR_PASSB_SPLIT
   hipLaunchKernelGGL(passb_finalize_kernel, dim3(1), dim3(256), 0, s, R, (unsigned)passb_blocks(Nof(L)),
                      np == 0 ? slot : -1);

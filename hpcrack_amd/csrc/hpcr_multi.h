@@ -95,5 +95,18 @@ struct TileK {
   static dim3 block() { return dim3(TPB); }
   __device__ __forceinline__ static void run(const Args& a, const Blk& bk) { tile_body<Op>(a.L, a.op, bk); }
 };
+template <class Op>
+struct TileSplitK {
+  struct Args {
+    LevelGeom L;
+    Op op;
+    SplitW S;
+  };
+  static constexpr int NT = TPB, MINW = Op::MIN_WAVES;
+  static dim3 block() { return dim3(TPB); }
+  __device__ __forceinline__ static void run(const Args& a, const Blk& bk) {
+    tile_body_split<Op>(a.L, a.op, a.S, bk);
+  }
+};
 
 }  // namespace hpcr

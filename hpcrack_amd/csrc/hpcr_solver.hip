@@ -48,6 +48,7 @@ struct Lev {
   double* dprev = nullptr;
   unsigned char *fx = nullptr, *fy = nullptr, *fz = nullptr;
   std::vector<int> kplanes;
+  SplitW sw{};  // finest level: the side arrays of split-output pass A (ws null: not split)
 };
 
 // State of one level of the mixed-precision solve's preconditioner, in its
@@ -256,13 +257,13 @@ int pcg_u(hpcr_ctx* c, int l, int NB, double Ut, int t) {
     {
       TBracket tb(c, HPCR_K_MECH_PCG_A, l);
       launch_mech_pcg_a(c->stream, v.g, mcoef(c, l), v.zu, pb[(m + CG_GROUP - 1) % CG_GROUP], pb[m], v.w, i == 0,
-                        red(c));
+                        red(c), &v.sw);
     }
     {
       // u += alpha_i p_i waits for the pass B that closes the group
       const int np = (m == CG_GROUP - 1 || i == NB - 1) ? m + 1 : 0;
       TBracket tb(c, HPCR_K_MECH_PCG_B, l);
-      launch_mech_pcg_b(c->stream, v.g, v.u, v.ru, v.zu, P, v.w, v.invDu, np, m, red(c));
+      launch_mech_pcg_b(c->stream, v.g, v.u, v.ru, v.zu, P, v.w, v.invDu, np, m, red(c), &v.sw);
     }
     c->smoother_iters++;
   }
@@ -1314,6 +1315,11 @@ int build_levels(hpcr_ctx* c) {
     ii *= 2; jj *= 2; kk *= 2;
     size_t tb = (size_t)tile_blocks(v.g.nx, v.g.ny, v.g.nz, v.g.zc);
     if (tb > max_tile_blocks) max_tile_blocks = tb;
+    if (l == L && split_pass_a(v.g.nx, v.g.ny)) {
+      v.sw = split_dims(v.g.nx, v.g.ny, v.g.nz);
+      tb = (size_t)split_blocks(v.g, v.sw);
+      if (tb > max_tile_blocks) max_tile_blocks = tb;
+    }
   }
   int rc;
   for (int l = 0; l <= L; l++) {
@@ -1331,6 +1337,7 @@ int build_levels(hpcr_ctx* c) {
     if (l == L) {
       if ((rc = alloc_n(c, &v.flag, N))) return rc;
       if ((rc = alloc_n(c, &v.dprev, N))) return rc;
+      if (v.sw.ntx > 0 && (rc = alloc_n(c, &v.sw.ws, v.sw.count()))) return rc;
     }
     // Dirichlet axis flags by coordinate equality (boundary_conditions.cpp:28-66)
     std::vector<unsigned char> fx(v.g.nx), fy(v.g.ny), fz(v.g.nz);
@@ -1385,8 +1392,15 @@ long long estimate_bytes(const hpcr_geometry& G, int mixed) {
     total += 10 * 3 * N * 8 + (l < L ? 3 * N * 8 : 0) + 16 * N * 8 + (l < L ? N * 8 : 0);
     if (l == L) total += N + N * 8;
     total += nx + ny + nz;
-    const long tb = tile_blocks((int)nx, (int)ny, (int)nz, choose_zchunk((int)nx, (int)ny, (int)nz));
+    const int zc = choose_zchunk((int)nx, (int)ny, (int)nz);
+    const long tb = tile_blocks((int)nx, (int)ny, (int)nz, zc);
     if (tb > pb) pb = tb;
+    if (l == L && split_pass_a((int)nx, (int)ny)) {
+      const SplitW S = split_dims((int)nx, (int)ny, (int)nz);
+      total += (long long)(S.count() * 8);
+      const long sb = (long)S.ntx * S.nty * ((nz + S.zc - 1) / S.zc);
+      if (sb > pb) pb = sb;
+    }
     if (passb_blocks((size_t)N) > pb) pb = passb_blocks((size_t)N);
     if (mixed == 1) {
       total += 2 * 3 * N * 4;

@@ -1150,21 +1150,242 @@ __global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_kernel
   tile_body<Op>(L, op, Blk::hw());
 }
 
+// row length of an LDS node plane of PL nodes: the 33x9 planes of the 32x8
+// halo tiles, the 65x5 planes of the 64x4 split tiles
+template <int PL>
+__host__ __device__ constexpr int plane_row() {
+  static_assert(PL == PLANE || PL == SPLANE, "a tile node plane");
+  return PL == PLANE ? PX : SPX;
+}
+
 // the 8 corner values of element (tx, ty) from the LDS planes of its bottom
 // (slot s0) and top (the next slot) nodes
-template <class T, int NS, int NV>
-__device__ __forceinline__ void gather8(const T (&sN)[NS][NV][PLANE], int s0, int tx, int ty, int k,
+template <class T, int NS, int NV, int PL>
+__device__ __forceinline__ void gather8(const T (&sN)[NS][NV][PL], int s0, int tx, int ty, int k,
                                         T (&v)[2][2][2]) {
-  const int q = tx + PX * ty;
+  constexpr int RX = plane_row<PL>();
+  const int q = tx + RX * ty;
   const int s1 = (NS == 3) ? (s0 == 2 ? 0 : s0 + 1) : (s0 ^ 1);
 #pragma unroll
   for (int iz = 0; iz < 2; iz++) {
     const int s = iz ? s1 : s0;
     v[iz][0][0] = sN[s][k][q];
     v[iz][0][1] = sN[s][k][q + 1];
-    v[iz][1][0] = sN[s][k][q + PX];
-    v[iz][1][1] = sN[s][k][q + PX + 1];
+    v[iz][1][0] = sN[s][k][q + RX];
+    v[iz][1][1] = sN[s][k][q + RX + 1];
   }
+}
+
+// The z-march of the finest level's fused PCG pass A on split-output tiles
+// (STX x STY elements; see SplitW).  Each element is computed once.  A wave
+// is one 64-element row: the x-neighbour is the adjacent lane (DPP
+// wave_shr:1, lane 0 receives 0: the element left of the tile is another
+// tile's), the y-neighbour the next wave, through LDS.  Node (tx, ty) of the
+// tile's region (x = ox + tx, y = oy + ty, tx < 64, ty < 4) takes the
+// contributions of the tile's elements (tx-1..tx, ty-1..ty) and goes to w
+// through the op's epilogue; the right column (tx = 64, from lane 63's x = 1
+// corners), the top row (ty = 4, from row 3's iy = 1 halves) and their corner
+// go to the side arrays through the op's side().  The side nodes' values of
+// p and their z-carries stay in LDS (one writer and reader each), so that no
+// registers are held for them across the layer's barrier.  A node's parts add
+// up in pass B in a fixed order; the dot p.w is the sum of p times every part
+// the block writes.
+template <class Op>
+__device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op, const SplitW& S, const Blk& bk) {
+  constexpr int NC = Op::NC;
+  constexpr int NV = Op::NV;
+  using T = typename Op::T;
+  constexpr int NE = Op::NE;
+  using TE = typename Op::TE;
+  static_assert(NC == 3 && Op::NEEDS_OWN && CarryOf<Op>::value, "the fused PCG pass A of the finest level");
+  constexpr int NS = Op::NSL;
+  static_assert(NS == 3, "three plane slots: one barrier per layer");
+  __shared__ T sN[NS][NV][SPLANE];
+  __shared__ TE sE[NS][NE][TPB];
+  __shared__ T sX[2][2 * NC][TPB];          // a row's iy = 1 halves, for the row above
+  __shared__ T sR[2][2][2 * NC][STY];       // lane 63's x = 1 corners [iy][c, iz][row]
+  __shared__ T sP[NC][STX + STY + 1];       // p of the side nodes: top row, then right column (rows 0..4)
+  __shared__ T sC[NC][STX + STY + 1];       // their z-carries, same order
+  const int tid = threadIdx.x;
+  // a wave is one row: ty is uniform across it (a scalar, so that the row
+  // tests below are scalar branches)
+  const int tx = tid % STX, ty = __builtin_amdgcn_readfirstlane(tid / STX);
+  const int t = xcd_tile(bk.x, bk.gx);
+  const int kx = t % S.ntx, ky = (t / S.ntx) % S.nty;
+  const int ox = kx * STX, oy = ky * STY;
+  const int z0 = (t / (S.ntx * S.nty)) * S.zc;
+  const int zend = min(z0 + S.zc, L.nz);
+  const int ex = ox + tx, ey = oy + ty;
+  const bool exy = (ex <= L.nx - 2) && (ey <= L.ny - 2);
+  const typename Op::State st = op.state();
+
+  const bool has2 = tid + TPB < SPLANE;
+  const int q1 = tid, q2 = tid + TPB;
+  const int gx1 = ox + q1 % SPX, gy1 = oy + q1 / SPX;
+  const int gx2 = ox + q2 % SPX, gy2 = oy + q2 / SPX;
+  const bool in1 = gx1 < L.nx && gy1 < L.ny;
+  const bool in2 = has2 && gx2 < L.nx && gy2 < L.ny;
+  typename Op::Raw r1, r2;
+  TE re[NE];
+  auto fetch = [&](int pz) {
+    const bool zok = pz >= 0 && pz < L.nz && pz <= zend;
+    op.efetch(L, ex, ey, pz, exy && zok && pz < zend && pz <= L.nz - 2, re);
+    op.fetch(L, gx1, gy1, pz, in1 && zok, st, r1);
+    op.fetch(L, gx2, gy2, pz, in2 && zok, st, r2);
+  };
+  auto commit = [&](int slot) {
+#pragma unroll
+    for (int k = 0; k < NE; k++) sE[slot][k][tid] = re[k];
+    T v[NV];
+    op.commit(r1, st, v);
+#pragma unroll
+    for (int k = 0; k < NV; k++) sN[slot][k][q1] = v[k];
+    if (has2) {
+      op.commit(r2, st, v);
+#pragma unroll
+      for (int k = 0; k < NV; k++) sN[slot][k][q2] = v[k];
+    }
+  };
+  auto slot = [](int p) { return (p + 3) % 3; };
+
+  // which side node this thread handles: the top row (tx, 4) by row 3's
+  // lanes, the right column (64, r) and the corner (64, 4) by lanes r = 0..4
+  // of row 0; lane 63 of each row hands its x = 1 corners over through LDS.
+  // The node's offsets (side arrays, node fields) at plane 0 and their steps
+  // per plane are set here; a store at plane kz adds kz steps.
+  const bool right = (tx == STX - 1);
+  const int X = ox + tx, Y = oy + ty;
+  const bool own_in = X < L.nx && Y < L.ny;
+  const bool top_lane = ty == STY - 1, rc_lane = ty == 0 && tx <= STY;
+  const int sx = top_lane ? X : ox + STX, sy = top_lane ? oy + STY : oy + tx;
+  const bool side_in = (top_lane || rc_lane) && sx < L.nx && sy < L.ny;
+  const bool side_own_p = top_lane ? sy == L.ny - 1
+                                   : (tx < STY ? sx == L.nx - 1 : (sx == L.nx - 1 && sy == L.ny - 1));
+  unsigned s_base, s_step, s_comp;  // element units: offset at plane 0, per plane, per component
+  if (top_lane) {
+    s_base = 3u * S.nr + (unsigned)ky * L.nx + X;
+    s_step = (unsigned)S.nty * L.nx;
+    s_comp = S.nt;
+  } else if (tx < STY) {
+    s_base = (unsigned)(oy + tx) * S.ntx + kx;
+    s_step = (unsigned)L.ny * S.ntx;
+    s_comp = S.nr;
+  } else {
+    s_base = 3u * (S.nr + S.nt) + (unsigned)ky * S.ntx + kx;
+    s_step = (unsigned)S.nty * S.ntx;
+    s_comp = S.nc;
+  }
+  unsigned s_ob = nid32(L, sx, sy, 0) * 8u;  // plane 0; plus kz planes at the store
+  const unsigned plane8 = (unsigned)L.nx * L.ny * 8u;
+
+  typename Op::Acc acc;
+  typename CarryOf<Op>::type up{};
+  T carry[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) carry[c] = T(0);
+
+  fetch(z0 - 1);
+  commit(slot(z0 - 1));
+  fetch(z0);
+  commit(slot(z0));
+  fetch(z0 + 1);
+  commit(slot(z0 + 1));
+  fetch(z0 + 2);
+  __syncthreads();
+  auto layer = [&](int kz, auto first) {
+    const int s0 = slot(kz);
+    const int xb = kz & 1;
+    T f[NC][2][2][2];
+    {
+      TE e[NE];
+#pragma unroll
+      for (int k = 0; k < NE; k++) e[k] = sE[s0][k][tid];
+      op.element_planes(L, sN, s0, tx, ty, ex, ey, kz, st, e, f, up, decltype(first)::value);
+    }
+    T g[NC][2][2];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+        for (int iy = 0; iy < 2; iy++) g[c][iz][iy] = f[c][iz][iy][0] + wave_shr1(f[c][iz][iy][1]);
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int iz = 0; iz < 2; iz++) sX[xb][c * 2 + iz][tid] = g[c][iz][1];
+    if (right) {
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+          for (int iy = 0; iy < 2; iy++) sR[xb][iy][c * 2 + iz][ty] = f[c][iz][iy][1];
+    }
+    // p of this thread's nodes of plane kz, read before the slot is reused
+    T own[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) own[c] = sN[s0][c][tx + SPX * ty];
+    if (ty == STY - 1) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) sP[c][tx] = sN[s0][c][tx + SPX * STY];
+    }
+    if (ty == 0 && tx <= STY) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) sP[c][STX + tx] = sN[s0][c][STX + SPX * tx];
+    }
+    __syncthreads();
+    commit(s0);
+    fetch(kz + NS + 1);
+    // node (tx, ty): the row below's iy = 1 halves come through LDS (row 0
+    // has none in this tile)
+    {
+      T vb[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const T b0 = ty > 0 ? sX[xb][c * 2 + 0][tid - STX] : T(0);
+        const T b1 = ty > 0 ? sX[xb][c * 2 + 1][tid - STX] : T(0);
+        vb[c] = carry[c] + g[c][0][0] + b0;
+        carry[c] = g[c][1][0] + b1;
+      }
+      if (own_in && kz >= z0) op.epilogue_split(L, own, X, Y, kz, vb, acc);
+    }
+    // side nodes: the top row (tx, 4) from row 3's iy = 1 halves,
+    // x-combined; the right column (64, r) from lane 63's x = 1 corners of row
+    // r (iy = 0) and row r - 1 (iy = 1), the corner (64, 4) from row 3's
+    if (top_lane || rc_lane) {
+      const int r = tx;
+      T v[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        if (top_lane) {
+          v[c] = sC[c][tx] + sX[xb][c * 2 + 0][tid];
+          sC[c][tx] = sX[xb][c * 2 + 1][tid];
+        } else {
+          const T a0 = r < STY ? sR[xb][0][c * 2 + 0][r] : T(0), a1 = r < STY ? sR[xb][0][c * 2 + 1][r] : T(0);
+          const T b0 = r > 0 ? sR[xb][1][c * 2 + 0][r - 1] : T(0), b1 = r > 0 ? sR[xb][1][c * 2 + 1][r - 1] : T(0);
+          v[c] = sC[c][STX + r] + a0 + b0;
+          sC[c][STX + r] = a1 + b1;
+        }
+      }
+      if (side_in && kz >= z0) {
+        const int pi = top_lane ? tx : STX + r;
+        T p[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) p[c] = sP[c][pi];
+        op.side(S, (s_base + (unsigned)kz * s_step) * 8u, s_comp * 8u, p, v, side_own_p,
+                s_ob + (unsigned)kz * plane8, acc);
+      }
+    }
+  };
+  layer(z0 - 1, std::true_type());
+  for (int kz = z0; kz < zend; ++kz) layer(kz, std::false_type());
+  op.finish(acc, bk);
+}
+
+template <class Op>
+__global__ HPCR_TILE_ATTR __launch_bounds__(TPB, Op::MIN_WAVES) void tile_split_kernel(const LevelGeom L, const Op op,
+                                                                                      const SplitW S) {
+  tile_body_split<Op>(L, op, S, Blk::hw());
 }
 
 }  // namespace hpcr

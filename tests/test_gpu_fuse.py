@@ -6,7 +6,9 @@ the results must be identical bit for bit: u, d, the norms, the cycle counts
 and the residual histories.  The mix holds problems with the same level
 structure (they share launches: 17^3, 17x17x33, 33x17x17 and a second 17^3)
 and problems that must not be grouped with them (33^3 has another level count;
-17^3 over a 9^3 coarsest level runs its coarsest smoother as tile passes).
+17^3 over a 9^3 coarsest level runs its coarsest smoother as tile passes;
+129x129x17 runs its finest pass A on split-output tiles, whose w parts pass B
+adds up).
 In the mixed-precision form every outer iteration of the flexible CG (FP32
 V-cycle, FCG passes A and B) goes through the fuser as one unit."""
 import threading
@@ -30,6 +32,7 @@ PROBLEMS = [
     (2, (4, 4, 4), 63),
     (3, (4, 4, 4), 64),
     (1, (8, 8, 8), 65),
+    (2, (32, 32, 4), 66),
 ]
 
 
