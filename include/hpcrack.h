@@ -77,7 +77,10 @@ enum hpcr_field {
   HPCR_RU = 2,   /* mechanical residual (3)                             */
   HPCR_UOLD = 3, /* coarse solution before the coarse solve (3)         */
   HPCR_P = 4,    /* vector work field (3): operator input in tests      */
-  HPCR_W = 5,    /* vector work field (3): operator output in tests     */
+  HPCR_W = 5,    /* vector work field (3): operator output in tests;     
+                    after a smoother on a finest level of 64 or more     
+                    tiles per plane it holds the split pass A's own-tile 
+                    parts, not the whole A p                             */
   HPCR_D = 6,    /* phase field                                         */
   HPCR_FD = 7,   /* phase-field right-hand side                         */
   HPCR_RD = 8,   /* phase-field residual                                */
