@@ -306,3 +306,32 @@ def test_pcg_smoother_on_a_long_grid(HA):
     assert fs == pytest.approx(fo, rel=1e-12)
     assert rel_l2(s.get(O.U, L), o.vec(O.U, L)) < 1e-11
     assert rel_l2(s.get(O.RU, L), o.vec(O.RU, L)) < 1e-10
+
+
+@pytest.mark.parametrize("case", [
+    (1, (64, 64, 4), (0.0, 128.0, 0.0, 128.0, 0.0, 8.0)),   # 129x129x9: x - 1 and y - 1 multiples of 64 and 4
+    (1, (80, 65, 4), (0.0, 160.0, 0.0, 130.0, 0.0, 8.0)),   # 161x131x9: partial last tiles in x and y
+], ids=["129x129x9", "161x131x9"])
+def test_pcg_smoother_on_split_tiles(HA, case):
+    """Finest levels of 64 or more tiles per plane run pass A on split-output
+    tiles (64x4 elements, side parts added up by pass B).  Both edge shapes:
+    the last node column and row past every tile's own region (129), and
+    last tiles that reach past the grid (161 x 131)."""
+    ml, enb, dom = case
+    say(f"[gpu] pcg on split tiles {enb}")
+    o, s, _ = make_pair(HA, ml, enb, dom, seed=17, lc=2.0)
+    L = ml
+    random_state(o, s, L, 18, d_zero_frac=0.0)
+    rng = np.random.default_rng(19)
+    dims = o.dims(L)
+    u0 = rng.standard_normal((3,) + dims[::-1]) * 0.1
+    f0 = rng.standard_normal((3,) + dims[::-1]) * 1e3
+    for obj in (o, s):
+        obj.set(O.U, L, u0)
+        obj.set(O.FU, L, f0)
+    fo = o.pcg_u(L, 4, 0.3, 0, 0.0)
+    fs = s.pcg_u(L, 4, 0.3, 0, 0.0)
+    assert fs == pytest.approx(fo, rel=1e-12)
+    assert rel_l2(s.get(O.U, L), o.vec(O.U, L)) < 1e-11
+    assert rel_l2(s.get(O.RU, L), o.vec(O.RU, L)) < 1e-10
+    assert rel_l2(s.get(O.FU, L), o.vec(O.FU, L)) < 1e-12
