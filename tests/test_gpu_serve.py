@@ -87,7 +87,8 @@ def test_fused_serving_identical_to_requests_alone(tmp_path, mixed):
 
 def test_byte_estimate_matches_context():
     HA = require_gpu()
-    for ml, enb in ((2, (4, 4, 4)), (3, (4, 8, 4)), (1, (2, 6, 4))):
+    # the last one has a finest level of split-output pass A tiles (side arrays)
+    for ml, enb in ((2, (4, 4, 4)), (3, (4, 8, 4)), (1, (2, 6, 4)), (1, (64, 64, 4))):
         dom = (0.0, enb[0] * 2.0 ** ml, 0.0, enb[1] * 2.0 ** ml, 0.0, enb[2] * 2.0 ** ml)
         s = HA.Solver(ml, enb, dom, tol_u=1e-16)
         assert s.device_bytes() == HA.estimate_bytes(ml, enb, dom)
