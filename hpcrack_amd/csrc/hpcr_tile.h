@@ -1250,13 +1250,14 @@ __device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op
 
   // which side node this thread handles: the top row (tx, 4) by row 3's
   // lanes, the right column (64, r) and the corner (64, 4) by lanes r = 0..4
-  // of row 0; lane 63 of each row hands its x = 1 corners over through LDS.
+  // of row 2 (rows 0 and 1 commit the plane's nodes past the first 256); lane
+  // 63 of each row hands its x = 1 corners over through LDS.
   // The node's offsets (side arrays, node fields) at plane 0 and their steps
   // per plane are set here; a store at plane kz adds kz steps.
   const bool right = (tx == STX - 1);
   const int X = ox + tx, Y = oy + ty;
   const bool own_in = X < L.nx && Y < L.ny;
-  const bool top_lane = ty == STY - 1, rc_lane = ty == 0 && tx <= STY;
+  const bool top_lane = ty == STY - 1, rc_lane = ty == 2 && tx <= STY;
   const int sx = top_lane ? X : ox + STX, sy = top_lane ? oy + STY : oy + tx;
   const bool side_in = (top_lane || rc_lane) && sx < L.nx && sy < L.ny;
   const bool side_own_p = top_lane ? sy == L.ny - 1
@@ -1329,7 +1330,7 @@ __device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op
 #pragma unroll
       for (int c = 0; c < NC; c++) sP[c][tx] = sN[s0][c][tx + SPX * STY];
     }
-    if (ty == 0 && tx <= STY) {
+    if (rc_lane) {
 #pragma unroll
       for (int c = 0; c < NC; c++) sP[c][STX + tx] = sN[s0][c][STX + SPX * tx];
     }
