@@ -335,3 +335,29 @@ def test_pcg_smoother_on_split_tiles(HA, case):
     assert rel_l2(s.get(O.U, L), o.vec(O.U, L)) < 1e-11
     assert rel_l2(s.get(O.RU, L), o.vec(O.RU, L)) < 1e-10
     assert rel_l2(s.get(O.FU, L), o.vec(O.FU, L)) < 1e-12
+
+
+def test_pcg_smoother_on_a_large_coarse_level(HA):
+    """A coarse level of 64 or more tiles per plane (129x129x3 under a
+    257x257x5 finest level, whose pass A runs on split-output tiles): its
+    smoother, with per-element moduli K_e, G_e, on the halo tiles and their
+    round-sized z-chunks."""
+    ml, enb, dom = 2, (64, 64, 1), (0.0, 256.0, 0.0, 256.0, 0.0, 4.0)
+    say(f"[gpu] pcg on a large coarse level {enb}")
+    o, s, _ = make_pair(HA, ml, enb, dom, seed=22, lc=2.0)
+    rng = np.random.default_rng(23)
+    l = 1
+    dims = o.dims(l)
+    assert dims == (129, 129, 3)
+    Ke = rng.uniform(1e3, 5e4, dims[::-1])
+    Ge = rng.uniform(1e3, 3e4, dims[::-1])
+    u0 = rng.standard_normal((3,) + dims[::-1]) * 0.1
+    f0 = rng.standard_normal((3,) + dims[::-1]) * 1e3
+    for obj in (o, s):
+        for f, a in ((O.KE, Ke), (O.GE, Ge), (O.U, u0), (O.FU, f0)):
+            obj.set(f, l, a)
+    fo = o.pcg_u(l, 4, 0.3, 0, 0.0)
+    fs = s.pcg_u(l, 4, 0.3, 0, 0.0)
+    assert fs == fo
+    assert rel_l2(s.get(O.U, l), o.vec(O.U, l)) < 1e-11
+    assert rel_l2(s.get(O.RU, l), o.vec(O.RU, l)) < 1e-10
