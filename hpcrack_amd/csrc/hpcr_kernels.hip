@@ -285,21 +285,18 @@ struct MechOp {
       acc.a[0] += p[c] * v[c];
     }
   }
-  // a side node's part, at byte offset so of the side arrays (components
-  // scomp bytes apart); where no tile has the node in its own region (own_p:
-  // the last node column or row of the grid) also p_new, and w = 0, at the
-  // node's byte offset ob, so that pass B reads w at every node
-  __device__ __forceinline__ void side(const SplitW& S, unsigned so, unsigned scomp, const double (&p)[3],
-                                       const double (&v)[3], bool own_p, unsigned ob, Acc& acc) const {
+  // one component c of a side node's part, at byte offset so of the side
+  // arrays; where no tile has the node in its own region (own_p: the last
+  // node column or row of the grid) also p_new, and w = 0, at the node's byte
+  // offset ob, so that pass B reads w at every node
+  __device__ __forceinline__ void side1(const SplitW& S, unsigned so, double p, double v, bool own_p, unsigned ob,
+                                        int c, Acc& acc) const {
     static_assert(MODE == M_PCGA, "split outputs are pass A's");
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      sto(S.ws, so + c * scomp, v[c]);
-      acc.a[0] += p[c] * v[c];
-      if (own_p) {
-        sto(out2 + c * N, ob, p[c]);
-        sto(out + c * N, ob, 0.0);
-      }
+    sto(S.ws, so, v);
+    acc.a[0] += p * v;
+    if (own_p) {
+      sto(out2 + c * N, ob, p);
+      sto(out + c * N, ob, 0.0);
     }
   }
 

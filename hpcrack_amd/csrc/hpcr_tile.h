@@ -1248,35 +1248,31 @@ __device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op
   };
   auto slot = [](int p) { return (p + 3) % 3; };
 
-  // which side node this thread handles: the top row (tx, 4) by row 3's
-  // lanes, the right column (64, r) and the corner (64, 4) by lanes r = 0..4
-  // of row 2 (rows 0 and 1 commit the plane's nodes past the first 256); lane
-  // 63 of each row hands its x = 1 corners over through LDS.
-  // The node's offsets (side arrays, node fields) at plane 0 and their steps
-  // per plane are set here; a store at plane kz adds kz steps.
+  // Side nodes: rows 1..3's waves each take one component c = ty - 1 of all
+  // of the tile's side nodes, the top row (tx, 4) on every lane and the right
+  // column (64, r) and the corner (64, 4) on lanes r = 0..4, so that the side
+  // work is spread over three waves (row 0's wave also commits the plane's
+  // nodes past the first 256 and takes none).  Lane 63 of each row hands its
+  // x = 1 corners over through LDS.  The offsets (side arrays, with the
+  // component folded in, in elements; node fields in bytes) are set here at
+  // plane 0 with their steps per plane; a store at plane kz adds kz steps.
   const bool right = (tx == STX - 1);
   const int X = ox + tx, Y = oy + ty;
   const bool own_in = X < L.nx && Y < L.ny;
-  const bool top_lane = ty == STY - 1, rc_lane = ty == 2 && tx <= STY;
-  const int sx = top_lane ? X : ox + STX, sy = top_lane ? oy + STY : oy + tx;
-  const bool side_in = (top_lane || rc_lane) && sx < L.nx && sy < L.ny;
-  const bool side_own_p = top_lane ? sy == L.ny - 1
-                                   : (tx < STY ? sx == L.nx - 1 : (sx == L.nx - 1 && sy == L.ny - 1));
-  unsigned s_base, s_step, s_comp;  // element units: offset at plane 0, per plane, per component
-  if (top_lane) {
-    s_base = 3u * S.nr + (unsigned)ky * L.nx + X;
-    s_step = (unsigned)S.nty * L.nx;
-    s_comp = S.nt;
-  } else if (tx < STY) {
-    s_base = (unsigned)(oy + tx) * S.ntx + kx;
-    s_step = (unsigned)L.ny * S.ntx;
-    s_comp = S.nr;
-  } else {
-    s_base = 3u * (S.nr + S.nt) + (unsigned)ky * S.ntx + kx;
-    s_step = (unsigned)S.nty * S.ntx;
-    s_comp = S.nc;
-  }
-  unsigned s_ob = nid32(L, sx, sy, 0) * 8u;  // plane 0; plus kz planes at the store
+  const int sc = ty - 1;  // this wave's component (row 0: none)
+  const bool side_wave = ty >= 1, rc_lane = side_wave && tx <= STY;
+  const bool t_in = side_wave && X < L.nx && oy + STY < L.ny;
+  const bool t_own_p = oy + STY == L.ny - 1;
+  const unsigned t_base = 3u * S.nr + (unsigned)sc * S.nt + (unsigned)ky * L.nx + X;
+  const unsigned t_step = (unsigned)S.nty * L.nx;
+  const unsigned t_ob = nid32(L, X, oy + STY, 0) * 8u;
+  const int rx = ox + STX, ry = oy + tx;  // rc lanes: (64, tx), tx = 4 the corner
+  const bool r_in = rc_lane && rx < L.nx && ry < L.ny;
+  const bool r_own_p = tx < STY ? rx == L.nx - 1 : (rx == L.nx - 1 && ry == L.ny - 1);
+  const unsigned r_base = tx < STY ? (unsigned)sc * S.nr + (unsigned)ry * S.ntx + kx
+                                   : 3u * (S.nr + S.nt) + (unsigned)sc * S.nc + (unsigned)ky * S.ntx + kx;
+  const unsigned r_step = tx < STY ? (unsigned)L.ny * S.ntx : (unsigned)S.nty * S.ntx;
+  const unsigned r_ob = nid32(L, rx, ry, 0) * 8u;
   const unsigned plane8 = (unsigned)L.nx * L.ny * 8u;
 
   typename Op::Acc acc;
@@ -1326,13 +1322,9 @@ __device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op
     T own[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) own[c] = sN[s0][c][tx + SPX * ty];
-    if (ty == STY - 1) {
-#pragma unroll
-      for (int c = 0; c < NC; c++) sP[c][tx] = sN[s0][c][tx + SPX * STY];
-    }
-    if (rc_lane) {
-#pragma unroll
-      for (int c = 0; c < NC; c++) sP[c][STX + tx] = sN[s0][c][STX + SPX * tx];
+    if (side_wave) {
+      sP[sc][tx] = sN[s0][sc][tx + SPX * STY];
+      if (rc_lane) sP[sc][STX + tx] = sN[s0][sc][STX + SPX * tx];
     }
     __syncthreads();
     commit(s0);
@@ -1350,31 +1342,26 @@ __device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op
       }
       if (own_in && kz >= z0) op.epilogue_split(L, own, X, Y, kz, vb, acc);
     }
-    // side nodes: the top row (tx, 4) from row 3's iy = 1 halves,
-    // x-combined; the right column (64, r) from lane 63's x = 1 corners of row
-    // r (iy = 0) and row r - 1 (iy = 1), the corner (64, 4) from row 3's
-    if (top_lane || rc_lane) {
-      const int r = tx;
-      T v[NC];
-#pragma unroll
-      for (int c = 0; c < NC; c++) {
-        if (top_lane) {
-          v[c] = sC[c][tx] + sX[xb][c * 2 + 0][tid];
-          sC[c][tx] = sX[xb][c * 2 + 1][tid];
-        } else {
-          const T a0 = r < STY ? sR[xb][0][c * 2 + 0][r] : T(0), a1 = r < STY ? sR[xb][0][c * 2 + 1][r] : T(0);
-          const T b0 = r > 0 ? sR[xb][1][c * 2 + 0][r - 1] : T(0), b1 = r > 0 ? sR[xb][1][c * 2 + 1][r - 1] : T(0);
-          v[c] = sC[c][STX + r] + a0 + b0;
-          sC[c][STX + r] = a1 + b1;
-        }
-      }
-      if (side_in && kz >= z0) {
-        const int pi = top_lane ? tx : STX + r;
-        T p[NC];
-#pragma unroll
-        for (int c = 0; c < NC; c++) p[c] = sP[c][pi];
-        op.side(S, (s_base + (unsigned)kz * s_step) * 8u, s_comp * 8u, p, v, side_own_p,
-                s_ob + (unsigned)kz * plane8, acc);
+    // side nodes, component sc: the top row (tx, 4) from row 3's iy = 1
+    // halves, x-combined; the right column (64, r) from lane 63's x = 1
+    // corners of row r (iy = 0) and row r - 1 (iy = 1), the corner (64, 4)
+    // from row 3's
+    if (side_wave) {
+      const int c = sc;
+      const T vt = sC[c][tx] + sX[xb][c * 2 + 0][(STY - 1) * STX + tx];
+      sC[c][tx] = sX[xb][c * 2 + 1][(STY - 1) * STX + tx];
+      if (t_in && kz >= z0)
+        op.side1(S, (t_base + (unsigned)kz * t_step) * 8u, sP[c][tx], vt, t_own_p, t_ob + (unsigned)kz * plane8, c,
+                 acc);
+      if (rc_lane) {
+        const int r = tx;
+        const T a0 = r < STY ? sR[xb][0][c * 2 + 0][r] : T(0), a1 = r < STY ? sR[xb][0][c * 2 + 1][r] : T(0);
+        const T b0 = r > 0 ? sR[xb][1][c * 2 + 0][r - 1] : T(0), b1 = r > 0 ? sR[xb][1][c * 2 + 1][r - 1] : T(0);
+        const T vr = sC[c][STX + r] + a0 + b0;
+        sC[c][STX + r] = a1 + b1;
+        if (r_in && kz >= z0)
+          op.side1(S, (r_base + (unsigned)kz * r_step) * 8u, sP[c][STX + r], vr, r_own_p,
+                   r_ob + (unsigned)kz * plane8, c, acc);
       }
     }
   };
