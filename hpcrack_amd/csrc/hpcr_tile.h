@@ -1248,30 +1248,36 @@ __device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op
   };
   auto slot = [](int p) { return (p + 3) % 3; };
 
-  // Side nodes: rows 1..3's waves each take one component c = ty - 1 of all
-  // of the tile's side nodes, the top row (tx, 4) on every lane and the right
-  // column (64, r) and the corner (64, 4) on lanes r = 0..4, so that the side
-  // work is spread over three waves (row 0's wave also commits the plane's
-  // nodes past the first 256 and takes none).  Lane 63 of each row hands its
-  // x = 1 corners over through LDS.  The offsets (side arrays, with the
-  // component folded in, in elements; node fields in bytes) are set here at
-  // plane 0 with their steps per plane; a store at plane kz adds kz steps.
+  // Side nodes: rows 1..3's waves each take one component c = ty - 1 of the
+  // tile's top row (tx, 4), on every lane; row 3's wave also takes the right
+  // column (64, r) and the corner (64, 4), all three components in one pass
+  // over lanes 8 c + r (r = 0..4).  Row 0's wave commits the plane's nodes
+  // past the first 256 (row 1's its last 5) and takes no side work.  So every
+  // wave's work per layer is about one full-width pass beside its element.
+  // Lane 63 of each row hands its x = 1 corners over through LDS.  The
+  // offsets (side arrays, with the component folded in, in elements; node
+  // fields in bytes) are set here at plane 0 with their steps per plane; a
+  // store at plane kz adds kz steps.
   const bool right = (tx == STX - 1);
   const int X = ox + tx, Y = oy + ty;
   const bool own_in = X < L.nx && Y < L.ny;
-  const int sc = ty - 1;  // this wave's component (row 0: none)
-  const bool side_wave = ty >= 1, rc_lane = side_wave && tx <= STY;
+  const int sc = ty - 1;  // this wave's top-row component (row 0: none)
+  const bool side_wave = ty >= 1;
+  // right column and corner: row 3's wave, all three components in one
+  // instruction stream, component rcc on lanes 8 rcc .. 8 rcc + 4
+  const int rr = tx & 7, rcc = tx >> 3;
+  const bool rc_lane = ty == STY - 1 && rr <= STY && rcc < NC;
   const bool t_in = side_wave && X < L.nx && oy + STY < L.ny;
   const bool t_own_p = oy + STY == L.ny - 1;
   const unsigned t_base = 3u * S.nr + (unsigned)sc * S.nt + (unsigned)ky * L.nx + X;
   const unsigned t_step = (unsigned)S.nty * L.nx;
   const unsigned t_ob = nid32(L, X, oy + STY, 0) * 8u;
-  const int rx = ox + STX, ry = oy + tx;  // rc lanes: (64, tx), tx = 4 the corner
+  const int rx = ox + STX, ry = oy + rr;  // rc lanes: (64, rr), rr = 4 the corner
   const bool r_in = rc_lane && rx < L.nx && ry < L.ny;
-  const bool r_own_p = tx < STY ? rx == L.nx - 1 : (rx == L.nx - 1 && ry == L.ny - 1);
-  const unsigned r_base = tx < STY ? (unsigned)sc * S.nr + (unsigned)ry * S.ntx + kx
-                                   : 3u * (S.nr + S.nt) + (unsigned)sc * S.nc + (unsigned)ky * S.ntx + kx;
-  const unsigned r_step = tx < STY ? (unsigned)L.ny * S.ntx : (unsigned)S.nty * S.ntx;
+  const bool r_own_p = rr < STY ? rx == L.nx - 1 : (rx == L.nx - 1 && ry == L.ny - 1);
+  const unsigned r_base = rr < STY ? (unsigned)rcc * S.nr + (unsigned)ry * S.ntx + kx
+                                   : 3u * (S.nr + S.nt) + (unsigned)rcc * S.nc + (unsigned)ky * S.ntx + kx;
+  const unsigned r_step = rr < STY ? (unsigned)L.ny * S.ntx : (unsigned)S.nty * S.ntx;
   const unsigned r_ob = nid32(L, rx, ry, 0) * 8u;
   const unsigned plane8 = (unsigned)L.nx * L.ny * 8u;
 
@@ -1322,10 +1328,8 @@ __device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op
     T own[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) own[c] = sN[s0][c][tx + SPX * ty];
-    if (side_wave) {
-      sP[sc][tx] = sN[s0][sc][tx + SPX * STY];
-      if (rc_lane) sP[sc][STX + tx] = sN[s0][sc][STX + SPX * tx];
-    }
+    if (side_wave) sP[sc][tx] = sN[s0][sc][tx + SPX * STY];
+    if (rc_lane) sP[rcc][STX + rr] = sN[s0][rcc][STX + SPX * rr];
     __syncthreads();
     commit(s0);
     fetch(kz + NS + 1);
@@ -1353,16 +1357,16 @@ __device__ __forceinline__ void tile_body_split(const LevelGeom& L, const Op& op
       if (t_in && kz >= z0)
         op.side1(S, (t_base + (unsigned)kz * t_step) * 8u, sP[c][tx], vt, t_own_p, t_ob + (unsigned)kz * plane8, c,
                  acc);
-      if (rc_lane) {
-        const int r = tx;
-        const T a0 = r < STY ? sR[xb][0][c * 2 + 0][r] : T(0), a1 = r < STY ? sR[xb][0][c * 2 + 1][r] : T(0);
-        const T b0 = r > 0 ? sR[xb][1][c * 2 + 0][r - 1] : T(0), b1 = r > 0 ? sR[xb][1][c * 2 + 1][r - 1] : T(0);
-        const T vr = sC[c][STX + r] + a0 + b0;
-        sC[c][STX + r] = a1 + b1;
-        if (r_in && kz >= z0)
-          op.side1(S, (r_base + (unsigned)kz * r_step) * 8u, sP[c][STX + r], vr, r_own_p,
-                   r_ob + (unsigned)kz * plane8, c, acc);
-      }
+    }
+    if (rc_lane) {
+      const int c = rcc, r = rr;
+      const T a0 = r < STY ? sR[xb][0][c * 2 + 0][r] : T(0), a1 = r < STY ? sR[xb][0][c * 2 + 1][r] : T(0);
+      const T b0 = r > 0 ? sR[xb][1][c * 2 + 0][r - 1] : T(0), b1 = r > 0 ? sR[xb][1][c * 2 + 1][r - 1] : T(0);
+      const T vr = sC[c][STX + r] + a0 + b0;
+      sC[c][STX + r] = a1 + b1;
+      if (r_in && kz >= z0)
+        op.side1(S, (r_base + (unsigned)kz * r_step) * 8u, sP[c][STX + r], vr, r_own_p,
+                 r_ob + (unsigned)kz * plane8, c, acc);
     }
   };
   layer(z0 - 1, std::true_type());
